@@ -1,0 +1,121 @@
+/*
+ * pypde_amd C-ABI: MI355X (gfx950) kernels behind pypde's spectral-Galerkin
+ * operator surface.  All entry points are extern "C", take plain pointers to
+ * device memory (row-major float64 arrays owned by the caller), run
+ * asynchronously on the given HIP stream (NULL = default stream), and return
+ * 0 on success or a negative code; pypde_last_error() then holds the message.
+ *
+ * Array geometry used by every along-axis entry point (pypde_lines):
+ *   a batch of 2-D row-major arrays, entry b at  ptr + b*bs,  element (i,j) at
+ *   ptr + b*bs + i*ld + j.  "axis 0" lines are columns (element stride ld),
+ *   "axis 1" lines are rows (element stride 1).  len_in / len_out are the line
+ *   lengths along the axis before / after the operation, nlines the extent of
+ *   the other axis.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * preiter93/pypde).
+ */
+#ifndef PYPDE_AMD_H
+#define PYPDE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pypde_lines {
+    const double* src;
+    double* dst;
+    long long src_ld, dst_ld; /* elements between rows of one 2-D array */
+    long long src_bs, dst_bs; /* elements between batch entries          */
+    int len_in, len_out;      /* line length before / after the op        */
+    int nlines;               /* number of lines per batch entry          */
+    int batch;                /* number of batch entries                  */
+    int axis;                 /* 0: columns, 1: rows                      */
+    int lines_per_block;      /* 0 = library default (autotuner hint)     */
+} pypde_lines;
+
+/* ---- library ---------------------------------------------------------- */
+int pypde_version(void);
+const char* pypde_last_error(void);
+/* Number of visible HIP devices (0 when none); never throws. */
+int pypde_device_count(void);
+
+/* ---- dense operators (MFMA FP64 GEMM) ---------------------------------- */
+/* C[b] = alpha * op(A[b]) @ op(B[b]) + beta * D[b]   (row-major, op = T if trans)
+ * Replaces the dense / csr products of MatrixBase.dot (pypde/solver/matrix.py:41-56)
+ * and carries the Chebyshev transforms as dense contractions against the
+ * collocation matrices (pypde/bases/chebyshev.py:67-93 forward_fft/backward_fft/dctn). */
+int pypde_dgemm(int transA, int transB, int m, int n, int k, double alpha,
+                const double* A, long long lda, long long strideA,
+                const double* B, long long ldb, long long strideB,
+                double beta, const double* D, long long ldd, long long strideD,
+                double* C, long long ldc, long long strideC, int batch, void* stream);
+
+/* ---- line operators (one lane per line, LDS-staged panels) ------------ */
+/* Galerkin -> Chebyshev stencil u = S v, S(i,i)=1, S(i+2,i)=s[i]; len_out = len_in+2.
+ * Replaces GalerkinChebyshev.to_chebyshev (pypde/bases/chebyshev.py:287-293). */
+int pypde_stencil_apply(const pypde_lines* g, const double* s, void* stream);
+
+/* Chebyshev -> Galerkin v = (S^T S)^-1 S^T u via the offset-2 tridiagonal solve;
+ * a,b,c are the -2,0,+2 diagonals of S^T S, w/den the plan-time sweep factors.
+ * Replaces GalerkinChebyshev.from_chebyshev/_solve_stencil_inv
+ * (pypde/bases/chebyshev.py:295-337) -> tdma.solve_tdma_2d (tdma.f90:56-107). */
+int pypde_stencil_inverse(const pypde_lines* g, const double* s, const double* a,
+                          const double* b, const double* w, const double* den,
+                          void* stream);
+
+/* Chebyshev-coefficient derivative of the given order, optionally preceded by
+ * the stencil (s != NULL, Galerkin input), result divided by scale_pow.
+ * Replaces Chebyshev.derivative -> differentiate_cheby.diff_2d
+ * (pypde/bases/chebyshev.py:117-131, differentiate_cheby.f90:28-53) and the
+ * rescale in grad (pypde/field_operations.py:40-45). */
+int pypde_cheb_deriv(const pypde_lines* g, const double* s, int order, double scale_pow,
+                     void* stream);
+
+/* Banded matrix-vector product along an axis, y = A x with A given by ndiag
+ * diagonals (offsets ascending, vals[d*len_out + r] = A(r, r+off[d])), summed in
+ * ascending column order.  Replaces PlanRHS with a sparse operator
+ * (pypde/solver/plans.py:54-74, matrix.py:41-56). */
+int pypde_banded_apply(const pypde_lines* g, int ndiag, const int* offsets_host,
+                       const double* vals, void* stream);
+
+/* Pre-factored four-diagonal (-2,0,2,4) solve, in place (src == dst allowed).
+ * Tables l,d,u1,u2 per batch entry at offset b*table_bs (0 = shared).
+ * Replaces Plan_fdma.solve -> fdma.solve_fdma_2d (plans.py:184-224, fdma.f90:40-98). */
+int pypde_fdma_solve(const pypde_lines* g, const double* l, const double* d,
+                     const double* u1, const double* u2, long long table_bs,
+                     void* stream);
+
+/* Transposed four-diagonal solve x = A^-T b with the same LU tables (adjoint).
+ * No reference counterpart: used by the discrete adjoint of the Helmholtz step. */
+int pypde_fdma_solve_t(const pypde_lines* g, const double* l, const double* d,
+                       const double* u1, const double* u2, long long table_bs,
+                       void* stream);
+
+/* Poisson eigen-split solve: line j solves (A + lam_j C) x_j = b_j with the
+ * factor tables of line j at  tab + j*4*len  (d | u1 | u2 | l, each len long);
+ * lines with sing[j] != 0 solve the trailing (len-1) block and set x_0 = 0.
+ * Replaces Plan_Poisson.solve_fortran -> fdma.solve_fdma_type2
+ * (plans.py:310-314, fdma.f90:146-195); the per-call dense formation and
+ * factorisation of the reference is done once at plan time. */
+int pypde_poisson_solve(const pypde_lines* g, const double* tab, const int* sing,
+                        int transpose, void* stream);
+
+/* Upper two-band (0,+2) back substitution, in place.
+ * Replaces Plan_twodma.solve -> twodma.solve_twodma_2d (plans.py:137-181, twodma.f90:28-62). */
+int pypde_twodma_solve(const pypde_lines* g, const double* d, const double* u, void* stream);
+
+/* Pentadiagonal (-4,-2,0,2,4) solve with PDMA_LU tables a,b,d,e,f, in place.
+ * Replaces PDMA.solve (pypde/solver/linalg/pdma.py:74-93). */
+int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, const double* d,
+                     const double* e, const double* f, void* stream);
+
+/* ---- pointwise --------------------------------------------------------- */
+/* y = alpha*x + beta*y over n elements. */
+int pypde_axpby(long long n, double alpha, const double* x, double beta, double* y,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYPDE_AMD_H */

@@ -1,0 +1,102 @@
+"""ctypes binding of libpypde_hip.so (the C-ABI in include/pypde_amd.h).
+
+The product path has no CPU fallback: importing this module without the built
+library, or calling a kernel without a visible GPU, raises.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpypde_hip.so")
+
+
+class PypdeError(RuntimeError):
+    pass
+
+
+class Lines(ctypes.Structure):
+    """Mirror of `pypde_lines` (include/pypde_amd.h)."""
+
+    _fields_ = [
+        ("src", ctypes.c_void_p),
+        ("dst", ctypes.c_void_p),
+        ("src_ld", ctypes.c_longlong),
+        ("dst_ld", ctypes.c_longlong),
+        ("src_bs", ctypes.c_longlong),
+        ("dst_bs", ctypes.c_longlong),
+        ("len_in", ctypes.c_int),
+        ("len_out", ctypes.c_int),
+        ("nlines", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("axis", ctypes.c_int),
+        ("lines_per_block", ctypes.c_int),
+    ]
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "pypde_amd: %s is missing -- build the HIP library first "
+        "(python -m pypde_amd.build or __graft_entry__.build())" % LIB_PATH)
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_vp = ctypes.c_void_p
+_d = ctypes.c_double
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_lp = ctypes.POINTER(Lines)
+
+_SIGS = {
+    "pypde_version": ([], _i),
+    "pypde_last_error": ([], ctypes.c_char_p),
+    "pypde_device_count": ([], _i),
+    "pypde_dgemm": ([_i, _i, _i, _i, _i, _d, _vp, _ll, _ll, _vp, _ll, _ll, _d, _vp, _ll, _ll,
+                     _vp, _ll, _ll, _i, _vp], _i),
+    "pypde_stencil_apply": ([_lp, _vp, _vp], _i),
+    "pypde_stencil_inverse": ([_lp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pypde_cheb_deriv": ([_lp, _vp, _i, _d, _vp], _i),
+    "pypde_banded_apply": ([_lp, _i, ctypes.POINTER(_i), _vp, _vp], _i),
+    "pypde_fdma_solve": ([_lp, _vp, _vp, _vp, _vp, _ll, _vp], _i),
+    "pypde_fdma_solve_t": ([_lp, _vp, _vp, _vp, _vp, _ll, _vp], _i),
+    "pypde_poisson_solve": ([_lp, _vp, _vp, _i, _vp], _i),
+    "pypde_twodma_solve": ([_lp, _vp, _vp, _vp], _i),
+    "pypde_pdma_solve": ([_lp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pypde_axpby": ([_ll, _d, _vp, _d, _vp, _vp], _i),
+}
+
+for _name, (_args, _res) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.argtypes = _args
+    _fn.restype = _res
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib.pypde_last_error()
+        raise PypdeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+_gpu_checked = False
+
+
+def require_gpu():
+    """Raise unless a HIP device is visible; the kernels have no CPU path."""
+    global _gpu_checked
+    if not _gpu_checked:
+        if not torch.cuda.is_available():
+            raise PypdeError("pypde_amd needs a HIP (ROCm) GPU; none is visible")
+        _gpu_checked = True
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None

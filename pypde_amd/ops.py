@@ -1,0 +1,244 @@
+"""Along-axis operators on device tensors, each one call into libpypde_hip.so.
+
+Tensors are float64 CUDA (HIP) tensors of shape [n] , [n0, n1] or
+[batch, n0, n1] with contiguous rows.  `axis` counts within the trailing two
+dimensions (0 = columns, 1 = rows), like the reference's numpy axis argument.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import Lines, check, lib, ptr, stream_ptr
+
+DTYPE = torch.float64
+
+
+def device():
+    _lib.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_dev(a):
+    """numpy / list / tensor -> contiguous float64 device tensor."""
+    if isinstance(a, torch.Tensor):
+        t = a.to(device=device(), dtype=DTYPE)
+    else:
+        t = torch.as_tensor(np.asarray(a, dtype=np.float64), device=device())
+    return t.contiguous()
+
+
+def _as3(t):
+    """View a 1-D/2-D/3-D tensor as [batch, n0, n1] (1-D -> [1, n, 1])."""
+    if t.dim() == 1:
+        return t.reshape(1, t.shape[0], 1)
+    if t.dim() == 2:
+        return t.unsqueeze(0)
+    if t.dim() == 3:
+        return t
+    raise NotImplementedError("ndim must be less than 4")
+
+
+def _geom(src3, dst3, axis, len_in, len_out, lpb=0):
+    if src3.stride(2) != 1 or dst3.stride(2) != 1:
+        raise ValueError("rows must be contiguous")
+    g = Lines()
+    g.src = src3.data_ptr()
+    g.dst = dst3.data_ptr()
+    g.src_ld = src3.stride(1)
+    g.dst_ld = dst3.stride(1)
+    g.src_bs = src3.stride(0) if src3.shape[0] > 1 else 0
+    g.dst_bs = dst3.stride(0) if dst3.shape[0] > 1 else 0
+    g.len_in = len_in
+    g.len_out = len_out
+    g.nlines = src3.shape[2] if axis == 0 else src3.shape[1]
+    g.batch = src3.shape[0]
+    g.axis = axis
+    g.lines_per_block = lpb
+    return g
+
+
+def _out_shape(x, axis, n_out):
+    shape = list(x.shape)
+    if x.dim() == 1:
+        shape[0] = n_out
+    else:
+        shape[x.dim() - 2 + axis] = n_out
+    return shape
+
+
+def _len(x, axis):
+    return x.shape[0] if x.dim() == 1 else x.shape[x.dim() - 2 + axis]
+
+
+def _check_axis(x, axis):
+    if x.dim() == 1 and axis != 0:
+        raise ValueError("1-D input only has axis 0")
+    if axis not in (0, 1):
+        raise ValueError("axis must be 0 or 1")
+
+
+# ---------------------------------------------------------------------------
+def matmul_axis(mat, x, axis=0, out=None, alpha=1.0, beta=0.0, add=None):
+    """out = alpha * (mat applied along axis of x) + beta * add.
+
+    axis 0: out[b] = mat @ x[b];  axis 1: out[b] = x[b] @ mat.T"""
+    _check_axis(x, axis)
+    m, k = mat.shape
+    if _len(x, axis) != k:
+        raise AssertionError("Shape mismatch: operator has %d columns, input axis has %d"
+                             % (k, _len(x, axis)))
+    if out is None:
+        out = torch.empty(_out_shape(x, axis, m), dtype=DTYPE, device=x.device)
+    x3, o3 = _as3(x), _as3(out)
+    a3 = _as3(add) if add is not None else None
+    B = x3.shape[0]
+    s = stream_ptr()
+    matc = mat if mat.stride(1) == 1 else mat.contiguous()
+    if axis == 0:
+        rc = lib.pypde_dgemm(0, 0, m, x3.shape[2], k, alpha, ptr(matc), matc.stride(0), 0,
+                             ptr(x3), x3.stride(1), x3.stride(0), beta,
+                             ptr(a3) if a3 is not None else None,
+                             a3.stride(1) if a3 is not None else 0,
+                             a3.stride(0) if a3 is not None else 0,
+                             ptr(o3), o3.stride(1), o3.stride(0), B, s)
+    else:
+        rc = lib.pypde_dgemm(0, 1, x3.shape[1], m, k, alpha, ptr(x3), x3.stride(1), x3.stride(0),
+                             ptr(matc), matc.stride(0), 0, beta,
+                             ptr(a3) if a3 is not None else None,
+                             a3.stride(1) if a3 is not None else 0,
+                             a3.stride(0) if a3 is not None else 0,
+                             ptr(o3), o3.stride(1), o3.stride(0), B, s)
+    check(rc, "pypde_dgemm")
+    return out
+
+
+def gemm(A, B, transA=False, transB=False, alpha=1.0, beta=0.0, D=None, out=None):
+    """Plain (batched) row-major GEMM on 2-D or 3-D tensors."""
+    A3 = A if A.dim() == 3 else A.unsqueeze(0)
+    B3 = B if B.dim() == 3 else B.unsqueeze(0)
+    m = A3.shape[2] if transA else A3.shape[1]
+    k = A3.shape[1] if transA else A3.shape[2]
+    kb = B3.shape[2] if transB else B3.shape[1]
+    n = B3.shape[1] if transB else B3.shape[2]
+    if k != kb:
+        raise AssertionError("GEMM inner dimensions differ: %d vs %d" % (k, kb))
+    batch = max(A3.shape[0], B3.shape[0])
+    if out is None:
+        shape = (batch, m, n) if (A.dim() == 3 or B.dim() == 3) else (m, n)
+        out = torch.empty(shape, dtype=DTYPE, device=A.device)
+    o3 = out if out.dim() == 3 else out.unsqueeze(0)
+    D3 = None if D is None else (D if D.dim() == 3 else D.unsqueeze(0))
+    rc = lib.pypde_dgemm(int(transA), int(transB), m, n, k, alpha,
+                         ptr(A3), A3.stride(1), A3.stride(0) if A3.shape[0] > 1 else 0,
+                         ptr(B3), B3.stride(1), B3.stride(0) if B3.shape[0] > 1 else 0,
+                         beta, ptr(D3) if D3 is not None else None,
+                         D3.stride(1) if D3 is not None else 0,
+                         (D3.stride(0) if D3.shape[0] > 1 else 0) if D3 is not None else 0,
+                         ptr(o3), o3.stride(1), o3.stride(0), batch, stream_ptr())
+    check(rc, "pypde_dgemm")
+    return out
+
+
+# ---------------------------------------------------------------------------
+def stencil_apply(s, x, axis=0, lpb=0):
+    """u = S v along axis (Galerkin M -> Chebyshev M+2)."""
+    _check_axis(x, axis)
+    M = _len(x, axis)
+    out = torch.empty(_out_shape(x, axis, M + 2), dtype=DTYPE, device=x.device)
+    g = _geom(_as3(x), _as3(out), axis, M, M + 2, lpb)
+    check(lib.pypde_stencil_apply(ctypes.byref(g), ptr(s), stream_ptr()), "pypde_stencil_apply")
+    return out
+
+
+def stencil_inverse(tabs, x, axis=0, lpb=0):
+    """v = (S^T S)^-1 S^T u along axis (Chebyshev N -> Galerkin N-2).
+    tabs = (s, a, b, w, den) device tensors."""
+    _check_axis(x, axis)
+    N = _len(x, axis)
+    out = torch.empty(_out_shape(x, axis, N - 2), dtype=DTYPE, device=x.device)
+    g = _geom(_as3(x), _as3(out), axis, N, N - 2, lpb)
+    s, a, b, w, den = tabs
+    check(lib.pypde_stencil_inverse(ctypes.byref(g), ptr(s), ptr(a), ptr(b), ptr(w), ptr(den),
+                                    stream_ptr()), "pypde_stencil_inverse")
+    return out
+
+
+def cheb_deriv(x, axis=0, order=1, scale_pow=1.0, s=None, lpb=0):
+    """Chebyshev derivative along axis (stencil first if s is given), / scale_pow."""
+    _check_axis(x, axis)
+    n_in = _len(x, axis)
+    n = n_in + 2 if s is not None else n_in
+    out = torch.empty(_out_shape(x, axis, n), dtype=DTYPE, device=x.device)
+    g = _geom(_as3(x), _as3(out), axis, n_in, n, lpb)
+    check(lib.pypde_cheb_deriv(ctypes.byref(g), ptr(s) if s is not None else None, int(order),
+                               float(scale_pow), stream_ptr()), "pypde_cheb_deriv")
+    return out
+
+
+def banded_apply(offsets, vals, x, axis=0, len_out=None, lpb=0):
+    """y = A x along axis for a banded A stored as diagonals vals[q, r] = A[r, r+offsets[q]]."""
+    _check_axis(x, axis)
+    n_in = _len(x, axis)
+    n_out = vals.shape[1] if len_out is None else len_out
+    out = torch.empty(_out_shape(x, axis, n_out), dtype=DTYPE, device=x.device)
+    g = _geom(_as3(x), _as3(out), axis, n_in, n_out, lpb)
+    offs = (ctypes.c_int * len(offsets))(*[int(o) for o in offsets])
+    check(lib.pypde_banded_apply(ctypes.byref(g), len(offsets), offs, ptr(vals), stream_ptr()),
+          "pypde_banded_apply")
+    return out
+
+
+def fdma_solve(tabs, x, axis=0, transpose=False, table_bs=0, lpb=0):
+    """In-place pre-factored four-diagonal solve; tabs = (l, d, u1, u2)."""
+    _check_axis(x, axis)
+    n = _len(x, axis)
+    x3 = _as3(x)
+    g = _geom(x3, x3, axis, n, n, lpb)
+    l, d, u1, u2 = tabs
+    fn = lib.pypde_fdma_solve_t if transpose else lib.pypde_fdma_solve
+    check(fn(ctypes.byref(g), ptr(l), ptr(d), ptr(u1), ptr(u2), int(table_bs), stream_ptr()),
+          "pypde_fdma_solve")
+    return x
+
+
+def poisson_solve(tab, sing, x, axis=0, transpose=False, lpb=0):
+    """In-place eigen-split Poisson line solves with per-line factor tables."""
+    _check_axis(x, axis)
+    n = _len(x, axis)
+    x3 = _as3(x)
+    g = _geom(x3, x3, axis, n, n, lpb)
+    check(lib.pypde_poisson_solve(ctypes.byref(g), ptr(tab), ptr(sing), int(bool(transpose)),
+                                  stream_ptr()), "pypde_poisson_solve")
+    return x
+
+
+def twodma_solve(d, u, x, axis=0, lpb=0):
+    _check_axis(x, axis)
+    n = _len(x, axis)
+    x3 = _as3(x)
+    g = _geom(x3, x3, axis, n, n, lpb)
+    check(lib.pypde_twodma_solve(ctypes.byref(g), ptr(d), ptr(u), stream_ptr()),
+          "pypde_twodma_solve")
+    return x
+
+
+def pdma_solve(tabs, x, axis=0, lpb=0):
+    _check_axis(x, axis)
+    n = _len(x, axis)
+    x3 = _as3(x)
+    g = _geom(x3, x3, axis, n, n, lpb)
+    a, b, d, e, f = tabs
+    check(lib.pypde_pdma_solve(ctypes.byref(g), ptr(a), ptr(b), ptr(d), ptr(e), ptr(f),
+                               stream_ptr()), "pypde_pdma_solve")
+    return x
+
+
+def axpby(alpha, x, beta, y):
+    """y = alpha*x + beta*y in place (same numel, contiguous)."""
+    if x.numel() != y.numel():
+        raise AssertionError("size mismatch")
+    check(lib.pypde_axpby(y.numel(), float(alpha), ptr(x), float(beta), ptr(y), stream_ptr()),
+          "pypde_axpby")
+    return y

@@ -1,0 +1,69 @@
+"""Pin the oracle's RBC2D step against vectors written from the reference
+itself (tests/golden/make_golden.py): these are the functions no closed form
+pins (NavierStokes.update and its trajectories)."""
+import numpy as np
+import pytest
+
+from oracle import pypde_oracle as o
+from conftest import rel
+
+CASES = {
+    "linear_eu": dict(case="linear", integrator="eu", beta=1.0),
+    "rbc_eu": dict(case="rbc", integrator="eu", beta=1.0),
+    "rbc_rk3": dict(case="rbc", integrator="rk3", beta=1.0),
+    "rbc_eu_beta": dict(case="rbc", integrator="eu", beta=0.5),
+    "zero_eu": dict(case="zero", integrator="eu", beta=1.0),
+}
+
+
+def load_state(ns, golden, prefix):
+    ns.T.vhat = golden[prefix + "T"].copy()
+    ns.U.vhat = golden[prefix + "U"].copy()
+    ns.V.vhat = golden[prefix + "V"].copy()
+    ns.P.vhat = golden[prefix + "P"].copy()
+    ns.pres.vhat = golden[prefix + "pres"].copy()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_single_steps_match_reference(golden, name):
+    kw = dict(CASES[name])
+    case = kw.pop("case")
+    ns = o.NavierStokes(case=case, shape=(20, 16), ra=1e5, pr=1.0, dt=0.01, tsave=None,
+                        dealias=True, aspect=1.0, **kw)
+    load_state(ns, golden, "step_%s_in_" % name)
+    for n in range(3):
+        ns.update()
+        if n == 0:
+            for f in ("T", "U", "V", "P", "pres"):
+                assert rel(getattr(ns, f).vhat, golden["step_%s_out1_%s" % (name, f)]) < 1e-14, f
+    for f in ("T", "U", "V", "P", "pres"):
+        assert rel(getattr(ns, f).vhat, golden["step_%s_out3_%s" % (name, f)]) < 1e-14, f
+
+
+@pytest.mark.parametrize("case", ["linear", "rbc"])
+def test_c1_trajectory_prefix_matches_reference(golden, case):
+    """First 200 steps of the 64x64, Ra=1e5 run (the full 1000 steps run in the
+    GPU tests against the same fixture)."""
+    ns = o.NavierStokes(case=case, shape=(64, 64), ra=1e5, pr=1.0, dt=0.01, tsave=None,
+                        dealias=True, integrator="eu", beta=1.0, aspect=1.0)
+    o.seeded_initial_state(ns, seed=0)
+    nsteps = 200
+    nu = np.zeros(nsteps)
+    ke = np.zeros(nsteps)
+    for n in range(nsteps):
+        ns.update()
+        nu[n] = ns.eval_Nu()[0]
+        ke[n] = ns.eval_KE()
+    assert rel(nu, golden["c1_%s_nu_plate" % case][:nsteps]) < 1e-13
+    assert rel(ke, golden["c1_%s_ke" % case][:nsteps]) < 1e-11
+
+
+def test_steady_residual_matches_reference(golden):
+    ns = o.NavierStokes(case="rbc", shape=(20, 16), ra=5e4, pr=1.0, dt=0.005, tsave=None)
+    X = golden["steady_X"]
+    mt, mu = ns.T.vhat.size, ns.U.vhat.size
+    ns.T.vhat = X[:mt].reshape(ns.T.vhat.shape).copy()
+    ns.U.vhat = X[mt:mt + mu].reshape(ns.U.vhat.shape).copy()
+    ns.V.vhat = X[mt + mu:].reshape(ns.V.vhat.shape).copy()
+    r = ns.steady_residual()
+    assert rel(r, golden["steady_res"]) < 1e-13
