@@ -51,6 +51,32 @@ int pypde_dgemm(int transA, int transB, int m, int n, int k, double alpha,
                 double beta, const double* D, long long ldd, long long strideD,
                 double* C, long long ldc, long long strideC, int batch, void* stream);
 
+/* Batch index of one GEMM operand: entry e starts at
+ *   (e/d1 % m1)*s1 + (e/d2 % m2)*s2   elements   (m = 0: no modulus, d >= 1). */
+typedef struct pypde_bidx {
+    long long s1, s2;
+    int d1, m1, d2, m2;
+} pypde_bidx;
+
+typedef struct pypde_gemm_desc {
+    int transA, transB, m, n, k, batch;
+    double alpha, beta;
+    const double* A;
+    const double* B;
+    const double* D; /* may be NULL (beta ignored) or alias C */
+    double* C;
+    long long lda, ldb, ldd, ldc;
+    pypde_bidx iA, iB, iD, iC;
+    const double* alpha_vec; /* NULL, or per-entry scale alpha_vec[e / alpha_div] */
+    int alpha_div;
+} pypde_gemm_desc;
+
+/* Batched GEMM  C[e] = alpha_e * op(A[e]) @ op(B[e]) + beta * D[e]  with the
+ * operand batch indexing above: one launch applies an operator stack to the
+ * U, V, T fields of many RBC instances (per-instance time steps via
+ * alpha_vec).  Same reference interfaces as pypde_dgemm. */
+int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream);
+
 /* ---- line operators (one lane per line, LDS-staged panels) ------------ */
 /* Galerkin -> Chebyshev stencil u = S v, S(i,i)=1, S(i+2,i)=s[i]; len_out = len_in+2.
  * Replaces GalerkinChebyshev.to_chebyshev (pypde/bases/chebyshev.py:287-293). */
@@ -114,6 +140,23 @@ int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, con
 /* y = alpha*x + beta*y over n elements. */
 int pypde_axpby(long long n, double alpha, const double* x, double beta, double* y,
                 void* stream);
+
+/* ---- fused RBC2D step helpers ------------------------------------------ */
+/* Dealiased convective products of the U, V, T fields, with the lifted
+ * boundary term uz*dTbcdz1 added to the temperature product.  Y is
+ * [batch, 3, 2*L0, 2*L1] (value, d/dy, d/dx blocks per field), P is
+ * [batch, 3, L0, L1].  Replaces the products of conv_term / convective_term
+ * (pypde/field_operations.py:124-129,160-164) and rbc2d.py:342. */
+int pypde_rbc_products(const double* Y, const double* dtbc, double* P, int L0, int L1,
+                       int batch, void* stream);
+
+/* y[b] -= (coef[b] * x[b]) * c2 over per elements of each batch entry:
+ * the pressure update pres -= nu*div*beta of rbc2d.py:390-394, per instance. */
+int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,
+                     const double* x, double* y, void* stream);
+
+/* p[e*stride] = value for e < n: the gauge P[0,0] = 0 of rbc2d.py:384-385. */
+int pypde_set_entries(double* p, long long stride, int n, double value, void* stream);
 
 #ifdef __cplusplus
 }

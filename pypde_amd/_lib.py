@@ -35,10 +35,30 @@ class Lines(ctypes.Structure):
     ]
 
 
+class BIdx(ctypes.Structure):
+    """Mirror of `pypde_bidx`."""
+
+    _fields_ = [("s1", ctypes.c_longlong), ("s2", ctypes.c_longlong), ("d1", ctypes.c_int),
+                ("m1", ctypes.c_int), ("d2", ctypes.c_int), ("m2", ctypes.c_int)]
+
+
+class GemmDesc(ctypes.Structure):
+    """Mirror of `pypde_gemm_desc`."""
+
+    _fields_ = [("transA", ctypes.c_int), ("transB", ctypes.c_int), ("m", ctypes.c_int),
+                ("n", ctypes.c_int), ("k", ctypes.c_int), ("batch", ctypes.c_int),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("D", ctypes.c_void_p),
+                ("C", ctypes.c_void_p), ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong),
+                ("ldd", ctypes.c_longlong), ("ldc", ctypes.c_longlong), ("iA", BIdx), ("iB", BIdx),
+                ("iD", BIdx), ("iC", BIdx), ("alpha_vec", ctypes.c_void_p),
+                ("alpha_div", ctypes.c_int)]
+
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         "pypde_amd: %s is missing -- build the HIP library first "
-        "(python -m pypde_amd.build or __graft_entry__.build())" % LIB_PATH)
+        "(python pypde_amd/build.py or __graft_entry__.build())" % LIB_PATH)
 
 lib = ctypes.CDLL(LIB_PATH)
 
@@ -64,6 +84,10 @@ _SIGS = {
     "pypde_twodma_solve": ([_lp, _vp, _vp, _vp], _i),
     "pypde_pdma_solve": ([_lp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pypde_axpby": ([_ll, _d, _vp, _d, _vp, _vp], _i),
+    "pypde_dgemm_batched": ([ctypes.c_void_p, _vp], _i),
+    "pypde_rbc_products": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
+    "pypde_scaled_sub": ([_ll, _i, _vp, _d, _vp, _vp, _vp], _i),
+    "pypde_set_entries": ([_vp, _ll, _i, _d, _vp], _i),
 }
 
 for _name, (_args, _res) in _SIGS.items():

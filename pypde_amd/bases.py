@@ -354,6 +354,13 @@ class _BCBase(GalerkinChebyshev):
         assert ops._len(vhat, axis) == 2
         return ops.matmul_axis(self._dev.get("S", self._stencil), vhat, axis)
 
+    def derivative(self, vhat, deriv, out_cheby=True, axis=0, scale_pow=1.0):
+        u = self.to_chebyshev(vhat, axis)
+        duhat = self.family.derivative(u, deriv, axis=axis, scale_pow=scale_pow)
+        if out_cheby:
+            return duhat
+        return self.from_chebyshev(duhat, axis)
+
     def from_chebyshev(self, c, axis=0):
         c = ops.to_dev(c)
         Sinv = self._dev.get("Sinv2", lambda: np.hstack([np.linalg.inv(self._stencil()[:2, :2]),

@@ -1,6 +1,6 @@
 """Build the pypde_amd HIP library (gfx950) in-tree: pypde_amd/libpypde_hip.so.
 
-    python -m pypde_amd.build        (or __graft_entry__.build())
+    python pypde_amd/build.py        (or __graft_entry__.build())
 
 Each .hip source is compiled to an object with hipcc, then linked into one
 shared library exporting the C-ABI declared in include/pypde_amd.h.

@@ -19,15 +19,33 @@ namespace pypde {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Batch index of one operand: entry e is at  (e/d1 % m1)*s1 + (e/d2 % m2)*s2
+// (m = 0 means no modulus).  A plain strided batch is {s, 0, 1, 0, 1, 0}.
+struct BIdx {
+    long long s1, s2;
+    int d1, m1, d2, m2;
+};
+
+__device__ __forceinline__ long long bofs(int e, const BIdx& x)
+{
+    int i1 = e / x.d1;
+    if (x.m1) i1 %= x.m1;
+    int i2 = e / x.d2;
+    if (x.m2) i2 %= x.m2;
+    return (long long)i1 * x.s1 + (long long)i2 * x.s2;
+}
+
 struct GemmArgs {
     const double* A;
     const double* B;
     const double* D;
     double* C;
     long long lda, ldb, ldd, ldc;
-    long long sA, sB, sD, sC;
+    BIdx iA, iB, iD, iC;
     int m, n, k;
     double alpha, beta;
+    const double* alpha_vec;  // alpha * alpha_vec[e / alpha_div] when given
+    int alpha_div;
 };
 
 template <int BM, int BN, bool TA, bool TB>
@@ -53,8 +71,8 @@ __global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
     const int m0 = blockIdx.y * BM;
     const int n0 = blockIdx.x * BN;
 
-    const double* __restrict__ A = g.A + (long long)bz * g.sA;
-    const double* __restrict__ B = g.B + (long long)bz * g.sB;
+    const double* __restrict__ A = g.A + bofs(bz, g.iA);
+    const double* __restrict__ B = g.B + bofs(bz, g.iB);
 
     v4d acc[TM][TN];
 #pragma unroll
@@ -125,19 +143,21 @@ __global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
         }
     }
 
-    // Epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile.
-    double* __restrict__ C = g.C + (long long)bz * g.sC;
-    const double* __restrict__ Dm = g.D ? g.D + (long long)bz * g.sD : nullptr;
+    // Epilogue: for the f64 16x16x4 MFMA, accumulator item r of a lane holds
+    // row (lane/16) + 4*r, column lane%16 of its 16x16 tile.
+    double* __restrict__ C = g.C + bofs(bz, g.iC);
+    const double* __restrict__ Dm = g.D ? g.D + bofs(bz, g.iD) : nullptr;
+    const double alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
                 const int gn = n0 + wn * WN + j * 16 + (lane & 15);
                 if (gm < g.m && gn < g.n) {
-                    double v = g.alpha * acc[i][j][r];
+                    double v = alpha * acc[i][j][r];
                     if (Dm && g.beta != 0.0) v += g.beta * Dm[(long long)gm * g.ldd + gn];
                     C[(long long)gm * g.ldc + gn] = v;
                 }
@@ -155,21 +175,49 @@ static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStrea
     else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, true, true>), grid, block, 0, s, g);
 }
 
+static BIdx bidx(const pypde_bidx& b)
+{
+    BIdx r{b.s1, b.s2, b.d1 < 1 ? 1 : b.d1, b.m1, b.d2 < 1 ? 1 : b.d2, b.m2};
+    return r;
+}
+
+static BIdx strided(long long s)
+{
+    BIdx r{s, 0, 1, 0, 1, 0};
+    return r;
+}
+
+static int launch(int transA, int transB, GemmArgs& g, int batch, hipStream_t stream)
+{
+    PYPDE_REQUIRE(g.m >= 0 && g.n >= 0 && g.k >= 0 && batch >= 0, "negative GEMM size");
+    if (g.m == 0 || g.n == 0 || batch == 0) return OK;
+    PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
+    PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
+    // Small problems use 32x32 tiles so that more CUs get work.
+    const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * batch;
+    if (tiles64 >= 512) launch_gemm<64, 64>(transA, transB, g, batch, stream);
+    else launch_gemm<32, 32>(transA, transB, g, batch, stream);
+    return check_launch("pypde_dgemm");
+}
+
 int dgemm(int transA, int transB, int m, int n, int k, double alpha, const double* A,
           long long lda, long long strideA, const double* B, long long ldb, long long strideB,
           double beta, const double* D, long long ldd, long long strideD, double* C,
           long long ldc, long long strideC, int batch, hipStream_t stream)
 {
-    PYPDE_REQUIRE(m >= 0 && n >= 0 && k >= 0 && batch >= 0, "negative GEMM size");
-    if (m == 0 || n == 0 || batch == 0) return OK;
-    PYPDE_REQUIRE(A && B && C, "null GEMM operand");
-    PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
-    GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strideA, strideB, strideD, strideC, m, n, k, alpha, beta};
-    // Small problems use 32x32 tiles so that more CUs get work.
-    const long long tiles64 = (long long)((m + 63) / 64) * ((n + 63) / 64) * batch;
-    if (tiles64 >= 512) launch_gemm<64, 64>(transA, transB, g, batch, stream);
-    else launch_gemm<32, 32>(transA, transB, g, batch, stream);
-    return check_launch("pypde_dgemm");
+    GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strided(strideA), strided(strideB),
+               strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1};
+    return launch(transA, transB, g, batch, stream);
+}
+
+int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
+{
+    PYPDE_REQUIRE(d, "null GEMM descriptor");
+    PYPDE_REQUIRE(!d->alpha_vec || d->alpha_div >= 1, "alpha_div must be >= 1");
+    GemmArgs g{d->A, d->B, d->D, d->C, d->lda, d->ldb, d->ldd, d->ldc, bidx(d->iA), bidx(d->iB),
+               bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
+               d->alpha_div < 1 ? 1 : d->alpha_div};
+    return launch(d->transA, d->transB, g, d->batch, stream);
 }
 
 }  // namespace pypde
@@ -182,4 +230,9 @@ extern "C" int pypde_dgemm(int transA, int transB, int m, int n, int k, double a
 {
     return pypde::dgemm(transA, transB, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, D,
                         ldd, strideD, C, ldc, strideC, batch, pypde::as_stream(stream));
+}
+
+extern "C" int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream)
+{
+    return pypde::dgemm_batched(d, pypde::as_stream(stream));
 }

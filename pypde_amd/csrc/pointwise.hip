@@ -1,0 +1,101 @@
+// Pointwise kernels of the fused RBC2D step.
+#include "common.h"
+#include "../../include/pypde_amd.h"
+
+namespace pypde {
+
+#pragma clang fp contract(off)
+
+// Convective products on the dealiased grid (field_operations.py:124-129,160-164):
+//   PU = dUdx*ux + dUdy*uz,  PV = dVdx*ux + dVdy*uz,
+//   PT = dTdx*ux + dTdy*uz + uz*dTbcdz1.
+// Y holds, per instance and field f in (U, V, T), a [2*L0, 2*L1] array whose
+// blocks are value (0,0), d/dy (0,1) and d/dx (1,0) on the L0 x L1 grid.
+__global__ void rbc_products_kernel(const double* __restrict__ Y, const double* __restrict__ dtbc,
+                                    double* __restrict__ P, int L0, int L1, int B)
+{
+    const long long per = (long long)L0 * L1;
+    const long long ld = 2LL * L1;
+    const long long fs = 4LL * per;  // field stride in Y
+    const long long total = per * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long b = t / per;
+        const long long r = t - b * per;
+        const int i = (int)(r / L1), j = (int)(r - (long long)i * L1);
+        const double* Yb = Y + b * 3 * fs;
+        const long long v = (long long)i * ld + j;        // value block
+        const long long dy = v + L1;                      // d/dy block
+        const long long dx = (long long)(i + L0) * ld + j;  // d/dx block
+        const double ux = Yb[v];
+        const double uz = Yb[fs + v];
+        const double pu = Yb[dx] * ux + Yb[dy] * uz;
+        const double pv = Yb[fs + dx] * ux + Yb[fs + dy] * uz;
+        const double pt = (Yb[2 * fs + dx] * ux + Yb[2 * fs + dy] * uz) + uz * dtbc[r];
+        double* Pb = P + b * 3 * per;
+        Pb[r] = pu;
+        Pb[per + r] = pv;
+        Pb[2 * per + r] = pt;
+    }
+}
+
+// y[b] = y[b] - (coef[b] * x[b]) * c2   (pres -= nu*div*beta, rbc2d.py:391)
+__global__ void scaled_sub_kernel(long long per, int B, const double* __restrict__ coef, double c2,
+                                  const double* __restrict__ x, double* __restrict__ y)
+{
+    const long long total = per * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const double c = coef[t / per];
+        y[t] = y[t] - (c * x[t]) * c2;
+    }
+}
+
+__global__ void set_entries_kernel(double* p, long long stride, int n, double value)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[(long long)e * stride] = value;
+}
+
+static unsigned grid_for(long long n)
+{
+    long long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace pypde
+
+using namespace pypde;
+
+extern "C" int pypde_rbc_products(const double* Y, const double* dtbc, double* P, int L0, int L1,
+                                  int batch, void* stream)
+{
+    PYPDE_REQUIRE(Y && dtbc && P, "null pointer");
+    PYPDE_REQUIRE(L0 > 0 && L1 > 0 && batch >= 0, "invalid product geometry");
+    if (batch == 0) return OK;
+    hipLaunchKernelGGL(rbc_products_kernel, dim3(grid_for((long long)L0 * L1 * batch)), dim3(256), 0,
+                       as_stream(stream), Y, dtbc, P, L0, L1, batch);
+    return check_launch("pypde_rbc_products");
+}
+
+extern "C" int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,
+                                const double* x, double* y, void* stream)
+{
+    PYPDE_REQUIRE(coef && x && y, "null pointer");
+    PYPDE_REQUIRE(per >= 0 && batch >= 0, "negative size");
+    if (per == 0 || batch == 0) return OK;
+    hipLaunchKernelGGL(scaled_sub_kernel, dim3(grid_for(per * batch)), dim3(256), 0,
+                       as_stream(stream), per, batch, coef, c2, x, y);
+    return check_launch("pypde_scaled_sub");
+}
+
+extern "C" int pypde_set_entries(double* p, long long stride, int n, double value, void* stream)
+{
+    PYPDE_REQUIRE(p, "null pointer");
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(set_entries_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), p,
+                       stride, n, value);
+    return check_launch("pypde_set_entries");
+}

@@ -1,0 +1,457 @@
+"""Fused RBC2D time step on the GPU: a batch of same-shape instances with
+per-instance Rayleigh / Prandtl numbers and time steps.
+
+The Euler IMEX step of navier/rbc2d.py:396-434 is regrouped into 21 kernel
+launches whatever the batch size: every separable 1-D operator chain of the
+reference (stencil -> derivative -> zero-pad -> DCT, forward DCT -> truncate ->
+pseudo-inverse B, ...) is folded into one dense operator per axis and applied
+as a batched MFMA GEMM; the Helmholtz and Poisson solves stay banded line
+solves with the reference's LU factors; the convective products and the
+pressure update are pointwise kernels.
+
+Data layout in HBM (float64, row-major):
+  X    [B, 3, M0, M1]      Galerkin coefficients of U, V (ChebDirichlet^2) and
+                           T (ChebNeumann or ChebDirichlet in x, ChebDirichlet in y)
+  pres [B, N0, N1]         Chebyshev coefficients of the accumulated pressure
+  P    [B, M0, M1]         pressure-correction potential (ChebNeumann^2)
+with N = grid size, M = N-2, L = int(1.5 N) the dealiased grid.
+
+Step (per instance; s = domain scale, dt/nu/kappa per instance):
+   1  Y1_f = [G; Gd]_x(f) X_f                     x: values / x-derivatives on L0
+   2  Y2_f = Y1_f [G; Gd]_y^T                     y: values / y-derivatives on L1
+   3  P_f  = conv products (+ uz dTbc/dz for T)   pointwise on [L0, L1]
+   4  Z_f[0]   = -dt P_f (B F)_y^T                 forward DCT + truncation + B, y
+   5  Z_f[1]   = -dt pres (B | B D/s)_y^T          pressure gradient (U, V)
+   6  Z_f[2]   = X_f (B S)_y^T                     old-value term
+   7  X_f = [B F | B D/s or B | B S]_x Z_f + const_f   Helmholtz right-hand side
+   8  X_V += dt (B S_T)_x Z_T[2]                   buoyancy
+   9-10 Helmholtz ADI: four-diagonal LU sweeps along x then y (per-instance tables)
+  11-12 div = (D S/s)_x U* S_y^T + S_x V* (D S/s)_y^T
+  13-16 Poisson: B_x div H_y^T, per-line eigen-split solves, Q_y^T
+  17    P[0,0] = 0
+  18    W = P [S; T S; T D S/s]_y^T               (T = (S^T S)^-1 S^T)
+  19-20 pres = pres - nu div beta + S_x W_0 / dt
+  21    U = U* - (T D S/s)_x W_1,  V = V* - (T S)_x W_2
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import BIdx, GemmDesc, Lines, check, lib
+from .bases import Base, cheb_backward_matrix, cheb_forward_matrix, pseudoinverse_d2
+from .solver import fdma_lu, poisson_line_tables
+from .templates import hholtz_matrices, poisson_matrices
+
+
+# ---------------------------------------------------------------------------
+# 1-D operator matrices (host, float64)
+# ---------------------------------------------------------------------------
+def deriv_matrix(N):
+    """Dense Chebyshev-coefficient derivative, equal to the recurrence of
+    differentiate_cheby.f90:28-53: (D c)_n = (2/c_n) sum_{p>n, p+n odd} p c_p."""
+    D = np.zeros((N, N))
+    for n in range(N):
+        D[n, n + 1::2] = 2.0 * np.arange(n + 1, N, 2)
+    D[0, :] *= 0.5
+    return D
+
+
+def stencil(key, N):
+    return Base(N, key).stencil() if key != "CH" else np.eye(N)
+
+
+def stencil_pinv(key, N):
+    """(S^T S)^-1 S^T, the Galerkin projection of chebyshev.py:295-337."""
+    S = stencil(key, N)
+    return np.linalg.solve(S.T @ S, S.T)
+
+
+def pinv_B(N):
+    return pseudoinverse_d2(N)[2:, :]
+
+
+class AxisOps:
+    """All dense 1-D operators of one axis (grid size N, dealiased size L)."""
+
+    def __init__(self, N, L, scale, side="CD"):
+        self.N, self.M, self.L, self.s = N, N - 2, L, scale
+        Bk = cheb_backward_matrix(L, N)
+        F = cheb_forward_matrix(L, N)
+        D1 = deriv_matrix(N)
+        S = stencil("CD", N)
+        St = stencil(side, N)
+        Sn = stencil("CN", N)
+        B = pinv_B(N)
+        T = stencil_pinv("CD", N)
+        self.G = Bk @ S                    # values on the L grid from CD coefficients
+        self.Gd = Bk @ (D1 @ S) / scale    # derivative values
+        self.GT = Bk @ St                  # the same for the temperature basis
+        self.GdT = Bk @ (D1 @ St) / scale
+        self.BF = B @ F                    # forward DCT, truncation to N, then B
+        self.B = B
+        self.BD = B @ D1 / scale           # pressure gradient, then B
+        self.BS = B @ S
+        self.BST = B @ St
+        self.C = S                         # divergence pieces
+        self.Cd = D1 @ S / scale
+        self.Sn = Sn                       # P (ChebNeumann) -> Chebyshev
+        self.TSn = T @ Sn                  # velocity correction pieces
+        self.TDSn = T @ D1 @ Sn / scale
+
+
+# ---------------------------------------------------------------------------
+# launch records
+# ---------------------------------------------------------------------------
+def _bidx(s1=0, d1=1, m1=0, s2=0, d2=1, m2=0):
+    b = BIdx()
+    b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s1, d1, m1, s2, d2, m2
+    return b
+
+
+class _Gemm:
+    def __init__(self, name, **kw):
+        self.name = name
+        d = GemmDesc()
+        d.transA = kw.get("ta", 0)
+        d.transB = kw.get("tb", 0)
+        d.m, d.n, d.k, d.batch = kw["m"], kw["n"], kw["k"], kw["batch"]
+        d.alpha = kw.get("alpha", 1.0)
+        d.beta = kw.get("beta", 0.0)
+        d.A, d.B, d.C = kw["A"], kw["B"], kw["C"]
+        d.D = kw.get("D", None)
+        d.lda, d.ldb, d.ldc = kw["lda"], kw["ldb"], kw["ldc"]
+        d.ldd = kw.get("ldd", 0)
+        d.iA = kw.get("iA", _bidx())
+        d.iB = kw.get("iB", _bidx())
+        d.iC = kw.get("iC", _bidx())
+        d.iD = kw.get("iD", _bidx())
+        d.alpha_vec = kw.get("alpha_vec", None)
+        d.alpha_div = kw.get("alpha_div", 1)
+        self.desc = d
+        self.flops = 2.0 * d.m * d.n * d.k * d.batch
+
+    def __call__(self, stream):
+        check(lib.pypde_dgemm_batched(ctypes.byref(self.desc), stream), self.name)
+
+
+class _Call:
+    def __init__(self, name, fn, *args):
+        self.name, self.fn, self.args = name, fn, args
+
+    def __call__(self, stream):
+        check(self.fn(*self.args, stream), self.name)
+
+
+def _lines(src, dst, ld, bs, n, nlines, batch, axis):
+    g = Lines()
+    g.src, g.dst = src, dst
+    g.src_ld = g.dst_ld = ld
+    g.src_bs = g.dst_bs = bs
+    g.len_in = g.len_out = n
+    g.nlines, g.batch, g.axis = nlines, batch, axis
+    g.lines_per_block = 0
+    return g
+
+
+# ---------------------------------------------------------------------------
+class RBCBatch:
+    """Euler IMEX RBC2D step for `B` instances of one shape.
+
+    params: list of dicts with keys ra, pr, dt (one per instance).
+    case: "rbc" (adiabatic sidewalls, T in ChebNeumann x ChebDirichlet),
+          "linear" or "zero" (T in ChebDirichlet^2).
+    """
+
+    def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0):
+        if case not in ("rbc", "linear", "zero"):
+            raise ValueError("Specified case is not available: ", ["rbc", "linear", "zero"])
+        self.shape = tuple(shape)
+        self.case, self.dealias, self.aspect, self.beta = case, dealias, aspect, float(beta)
+        self.params = [dict(p) for p in params]
+        B = self.B = len(params)
+        N0, N1 = self.shape
+        M0, M1 = N0 - 2, N1 - 2
+        L0 = int(N0 * 1.5) if dealias else N0
+        L1 = int(N1 * 1.5) if dealias else N1
+        self.N0, self.N1, self.M0, self.M1, self.L0, self.L1 = N0, N1, M0, M1, L0, L1
+        self.scale = (aspect * 0.5, 0.5)
+        side = "CN" if case == "rbc" else "CD"
+        self.side = side
+        ax = AxisOps(N0, L0, self.scale[0], side)
+        ay = AxisOps(N1, L1, self.scale[1], "CD")
+        self.ax, self.ay = ax, ay
+        dev = ops.device()
+        f64 = dict(dtype=ops.DTYPE, device=dev)
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a), **f64)
+
+        # per-instance scalars
+        ra = np.array([p["ra"] for p in params], float)
+        pr = np.array([p.get("pr", 1.0) for p in params], float)
+        dt = np.array([p["dt"] for p in params], float)
+        nu = np.sqrt(pr / (ra / 1.0 ** 3.0))
+        kappa = np.sqrt(1 / pr / (ra / 1.0 ** 3.0))
+        self.dt, self.nu, self.kappa = dt, nu, kappa
+        self.t_mdt = T(-dt)
+        self.t_dt = T(dt)
+        self.t_idt = T(1.0 / dt)
+        self.t_nu = T(nu)
+
+        # ---- operator stacks (device)
+        self.GGx = T(np.stack([np.vstack([ax.G, ax.Gd]), np.vstack([ax.G, ax.Gd]),
+                               np.vstack([ax.GT, ax.GdT])]))                   # [3, 2L0, M0]
+        self.GGy = T(np.vstack([ay.G, ay.Gd]))                                # [2L1, M1]
+        self.BFy = T(ay.BF)                                                   # [M1, L1]
+        self.PBy = T(np.stack([ay.B, ay.BD]))                                 # [2, M1, N1]
+        self.BSy = T(ay.BS)                                                   # [M1, M1]
+        K = L0 + N0 + M0
+        self.K = K
+        zero_blk = np.zeros((M0, N0))
+        self.Arhs = T(np.stack([np.hstack([ax.BF, ax.BD, ax.BS]),
+                                np.hstack([ax.BF, ax.B, ax.BS]),
+                                np.hstack([ax.BF, zero_blk, ax.BST])]))        # [3, M0, K]
+        self.BSTx = T(ax.BST)                                                 # [M0, M0]
+        self.Cy = T(np.stack([ay.C, ay.Cd]))                                  # [2, N1, M1]
+        self.Cx = T(np.hstack([ax.Cd, ax.C]))                                 # [N0, 2M0]
+        pm = poisson_matrices([Base(N0, "CN"), Base(N1, "CN")], singular=True, scale=self.scale)
+        self.Bx = T(ax.B)                                                     # [M0, N0]
+        self.Hy = T(pm["Hy"])                                                 # [M1, N1]
+        self.Qy = T(pm["Qy"])                                                 # [M1, M1]
+        tab, sing = poisson_line_tables(pm["Ax"], pm["Cx"], pm["wy"], True)
+        self.ptab = T(tab)
+        self.psing = torch.as_tensor(sing, device=dev)
+        self.Wy = T(np.vstack([ay.Sn, ay.TSn, ay.TDSn]))                      # [N1+2M1, M1]
+        self.Snx = T(ax.Sn)                                                   # [N0, M0]
+        self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
+
+        # Helmholtz LU tables per instance and field: [B, 3, 4, M]
+        hx = np.zeros((B, 3, 4, M0))
+        hy = np.zeros((B, 3, 4, M1))
+        bx_cd, bx_t, by_cd = Base(N0, "CD"), Base(N0, side), Base(N1, "CD")
+        for b in range(B):
+            lam_u = dt[b] * 1.0 * self.beta * nu[b]
+            lam_t = dt[b] * 1.0 * self.beta * kappa[b]
+            for f, (basis, lam) in enumerate(((bx_cd, lam_u), (bx_cd, lam_u), (bx_t, lam_t))):
+                hx[b, f] = self._lu(hholtz_matrices(basis, lam, self.scale[0])[2])
+            for f, lam in enumerate((lam_u, lam_u, lam_t)):
+                hy[b, f] = self._lu(hholtz_matrices(by_cd, lam, self.scale[1])[2])
+        self.hx, self.hy = T(hx), T(hy)
+
+        # boundary lifting constants (shape-only) and per-instance RHS constants
+        self._setup_bc()
+        cst = np.zeros((B, 3, M0, M1))
+        Bx_, By_ = ax.B, ay.B
+        cv = Bx_ @ self.Tbc_cheby_h @ By_.T
+        ct = Bx_ @ self.dTbcdz2_h @ By_.T
+        for b in range(B):
+            cst[b, 1] = dt[b] * 1.0 * cv
+            cst[b, 2] = dt[b] * 1.0 * kappa[b] * ct
+        self.const = T(cst)
+
+        # ---- state and workspaces
+        self.X = torch.zeros((B, 3, M0, M1), **f64)
+        self.pres = torch.zeros((B, N0, N1), **f64)
+        self.P = torch.zeros((B, M0, M1), **f64)
+        self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
+        self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+        self.Pr = torch.zeros((B, 3, L0, L1), **f64)
+        self.Z = torch.zeros((B, 3, K, M1), **f64)
+        self.Wd = torch.zeros((B, 2, M0, N1), **f64)
+        self.div = torch.zeros((B, N0, N1), **f64)
+        self.Q1 = torch.zeros((B, M0, N1), **f64)
+        self.RP = torch.zeros((B, M0, M1), **f64)
+        self.W1 = torch.zeros((B, M0, N1 + 2 * M1), **f64)
+        self._build_launches()
+        self.graph = None
+
+    @staticmethod
+    def _lu(A):
+        A = np.asarray(A)
+        l, d = np.diag(A, -2).copy(), np.diag(A).copy()
+        u1, u2 = np.diag(A, 2).copy(), np.diag(A, 4).copy()
+        fdma_lu(l, d, u1, u2)
+        M = d.shape[0]
+        out = np.zeros((4, M))
+        out[0, :M - 2], out[1], out[2, :M - 2], out[3, :M - 4] = l, d, u1, u2
+        return out
+
+    def _setup_bc(self):
+        """Lifted temperature boundary field and its derivatives
+        (rbc2d.py:135-178), computed with the drop-in field operations."""
+        from .field import Field, FieldBC
+        from .field_operations import galerkin_to_cheby, grad
+        N0, N1 = self.shape
+        xs = [Base(N0, self.side, dealias=1.5), Base(N1, "CD", dealias=1.5)]
+        deriv_field = Field([Base(N0, "CH", dealias=1.5), Base(N1, "CH", dealias=1.5)])
+        if self.case == "zero":
+            y = deriv_field.y * self.scale[1]
+            bc = np.zeros((2, N1))
+            bc[0, :] = transfer_function(0.5, 0, -0.5, y, k=0.02)
+            bc[1, :] = bc[0, :]
+            Tbc = FieldBC(xs, axis=0)
+        else:
+            bc = np.zeros((N0, 2))
+            bc[:, 0], bc[:, 1] = 0.5, -0.5
+            Tbc = FieldBC(xs, axis=1)
+        Tbc.add_bc(bc)
+        self.Tbc = Tbc
+        self.dTbcdz2 = grad(Tbc, (0, 2), scale=self.scale)
+        vhat = grad(Tbc, (0, 1), scale=self.scale)
+        if self.dealias:
+            self.dTbcdz1 = deriv_field.dealias.backward(vhat).contiguous()
+        else:
+            self.dTbcdz1 = deriv_field.backward(vhat).contiguous()
+        self.Tbc_cheby = galerkin_to_cheby(Tbc.vhat, Tbc)
+        self.Tbc_cheby_h = self.Tbc_cheby.cpu().numpy()
+        self.dTbcdz2_h = self.dTbcdz2.cpu().numpy()
+        self.deriv_field = deriv_field
+
+    # -----------------------------------------------------------------------
+    def _build_launches(self):
+        B, N0, N1, M0, M1, L0, L1, K = (self.B, self.N0, self.N1, self.M0, self.M1, self.L0,
+                                        self.L1, self.K)
+        p = lambda t: t.data_ptr()
+        X, Y1, Y2, Pr, Z = self.X, self.Y1, self.Y2, self.Pr, self.Z
+        sX = M0 * M1
+        sZ = K * M1
+        L = []
+        # 1  x-pass: values and x-derivatives on the L0 grid
+        L.append(_Gemm("xpass", m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
+                       iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
+                       C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1)))
+        # 2  y-pass: values and y-derivatives on the L1 grid
+        L.append(_Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
+                       iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
+                       C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1)))
+        # 3  convective products
+        L.append(_Call("products", lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
+                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
+        # 4  Z[0] = -dt P BF_y^T
+        L.append(_Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
+                       iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
+                       iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3))
+        # 5  Z[1] = -dt pres (B | BD)_y^T   for U, V
+        L.append(_Gemm("z_pres", m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
+                       iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
+                       iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
+                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.t_mdt),
+                       alpha_div=2))
+        # 6  Z[2] = X BS_y^T
+        L.append(_Gemm("z_old", m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1, iA=_bidx(s1=sX),
+                       B=p(self.BSy), ldb=M1, tb=1, C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1,
+                       iC=_bidx(s1=sZ)))
+        # 7  X = A_f Z_f + const_f
+        L.append(_Gemm("rhs", m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
+                       iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
+                       D=p(self.const), ldd=M1, iD=_bidx(s1=sX), beta=1.0,
+                       C=p(X), ldc=M1, iC=_bidx(s1=sX)))
+        # 8  buoyancy  X_V += dt BST_x Z_T[2]
+        L.append(_Gemm("buoyancy", m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx), lda=M0,
+                       B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1, iB=_bidx(s1=3 * sZ),
+                       D=p(X) + 8 * sX, ldd=M1, iD=_bidx(s1=3 * sX), beta=1.0,
+                       C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX), alpha_vec=p(self.t_dt)))
+        # 9-10  Helmholtz ADI sweeps
+        gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
+        gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
+        hx, hy = self.hx, self.hy
+        self._keep = [gx, gy]
+        L.append(_Call("helmholtz_x", lib.pypde_fdma_solve, ctypes.byref(gx),
+                       ctypes.c_void_p(p(hx)), ctypes.c_void_p(p(hx) + 8 * M0),
+                       ctypes.c_void_p(p(hx) + 16 * M0), ctypes.c_void_p(p(hx) + 24 * M0), 4 * M0))
+        L.append(_Call("helmholtz_y", lib.pypde_fdma_solve, ctypes.byref(gy),
+                       ctypes.c_void_p(p(hy)), ctypes.c_void_p(p(hy) + 8 * M1),
+                       ctypes.c_void_p(p(hy) + 16 * M1), ctypes.c_void_p(p(hy) + 24 * M1), 4 * M1))
+        # 11  Wd = [U* C_y^T ; V* Cd_y^T]
+        L.append(_Gemm("div_y", m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
+                       iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.Cy), ldb=M1, tb=1,
+                       iB=_bidx(s1=N1 * M1, m1=2), C=p(self.Wd), ldc=N1, iC=_bidx(s1=M0 * N1)))
+        # 12  div = [Cd | C]_x Wd
+        L.append(_Gemm("div_x", m=N0, n=N1, k=2 * M0, batch=B, A=p(self.Cx), lda=2 * M0,
+                       B=p(self.Wd), ldb=N1, iB=_bidx(s1=2 * M0 * N1), C=p(self.div), ldc=N1,
+                       iC=_bidx(s1=N0 * N1)))
+        # 13-14  Poisson right-hand side  B_x div H_y^T
+        L.append(_Gemm("poisson_bx", m=M0, n=N1, k=N0, batch=B, A=p(self.Bx), lda=N0,
+                       B=p(self.div), ldb=N1, iB=_bidx(s1=N0 * N1), C=p(self.Q1), ldc=N1,
+                       iC=_bidx(s1=M0 * N1)))
+        L.append(_Gemm("poisson_hy", m=M0, n=M1, k=N1, batch=B, A=p(self.Q1), lda=N1,
+                       iA=_bidx(s1=M0 * N1), B=p(self.Hy), ldb=N1, tb=1, C=p(self.RP), ldc=M1,
+                       iC=_bidx(s1=sX)))
+        # 15  per-line eigen-split solves along x
+        gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
+        self._keep.append(gp)
+        L.append(_Call("poisson_lines", lib.pypde_poisson_solve, ctypes.byref(gp),
+                       ctypes.c_void_p(p(self.ptab)), ctypes.c_void_p(p(self.psing)), 0))
+        # 16  P = RP Q_y^T
+        L.append(_Gemm("poisson_qy", m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
+                       iA=_bidx(s1=sX), B=p(self.Qy), ldb=M1, tb=1, C=p(self.P), ldc=M1,
+                       iC=_bidx(s1=sX)))
+        # 17  gauge
+        L.append(_Call("gauge", lib.pypde_set_entries, ctypes.c_void_p(p(self.P)), sX, B, 0.0))
+        # 18  W = P [S; T S; T D S/s]_y^T
+        NW = N1 + 2 * M1
+        L.append(_Gemm("corr_y", m=M0, n=NW, k=M1, batch=B, A=p(self.P), lda=M1,
+                       iA=_bidx(s1=sX), B=p(self.Wy), ldb=M1, tb=1, C=p(self.W1), ldc=NW,
+                       iC=_bidx(s1=M0 * NW)))
+        # 19  pres -= nu div beta
+        L.append(_Call("pres_div", lib.pypde_scaled_sub, N0 * N1, B, ctypes.c_void_p(p(self.t_nu)),
+                       self.beta, ctypes.c_void_p(p(self.div)), ctypes.c_void_p(p(self.pres))))
+        # 20  pres += S_x W_0 / dt
+        L.append(_Gemm("pres_p", m=N0, n=N1, k=M0, batch=B, A=p(self.Snx), lda=M0,
+                       B=p(self.W1), ldb=NW, iB=_bidx(s1=M0 * NW), D=p(self.pres), ldd=N1,
+                       iD=_bidx(s1=N0 * N1), beta=1.0, C=p(self.pres), ldc=N1,
+                       iC=_bidx(s1=N0 * N1), alpha_vec=p(self.t_idt)))
+        # 21  velocity correction
+        L.append(_Gemm("correct", m=M0, n=M1, k=M0, batch=2 * B, A=p(self.Acorr), lda=M0,
+                       iA=_bidx(s1=M0 * M0, m1=2), B=p(self.W1) + 8 * N1, ldb=NW,
+                       iB=_bidx(s1=M0 * NW, d1=2, s2=M1, m2=2), D=p(X), ldd=M1,
+                       iD=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), beta=1.0, alpha=-1.0,
+                       C=p(X), ldc=M1, iC=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2)))
+        self.launches = L
+
+    # -----------------------------------------------------------------------
+    def step_eager(self):
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for f in self.launches:
+            f(s)
+
+    def capture(self, nsteps=1):
+        """Capture `nsteps` consecutive steps into one HIP graph (the state is
+        left unchanged: the warm-up step before capture is undone)."""
+        saved = [t.clone() for t in (self.X, self.pres, self.P)]
+        self.step_eager()  # warm up outside capture
+        torch.cuda.synchronize()
+        for t, v in zip((self.X, self.pres, self.P), saved):
+            t.copy_(v)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(nsteps):
+                self.step_eager()
+        self.graph = g
+        self.graph_steps = nsteps
+
+    def step(self, n=1):
+        """Advance n steps (graph replay when a graph of matching length exists)."""
+        if self.graph is not None and n % self.graph_steps == 0:
+            for _ in range(n // self.graph_steps):
+                self.graph.replay()
+        else:
+            for _ in range(n):
+                self.step_eager()
+
+    def gemm_flops(self):
+        return sum(l.flops for l in self.launches if isinstance(l, _Gemm))
+
+
+def transfer_function(TL, TM, TR, x, k=0.01):
+    """Sidewall temperature profile of the "zero" case (rbc2d.py:437-446)."""
+    arr = np.zeros(x.shape)
+    L = x[-1] - x[0]
+    for i in range(x.size):
+        xs = x[i] * 2.0 / L
+        if xs < 0:
+            arr[i] = -k * xs / (k + xs + 1) * (TL - TM) + TM
+        else:
+            arr[i] = k * xs / (k - xs + 1) * (TR - TM) + TM
+    return arr

@@ -1,0 +1,156 @@
+"""Shared parts of the RBC2D solvers (drop-in for navier/rbc2d_base.py:8-386)."""
+import numpy as np
+import torch
+
+from .. import ops
+from ..bases import Base
+from ..field import Field
+from ..field_operations import avg_vol, avg_x, grad
+
+
+def nu(Ra, Pr, L):
+    return np.sqrt(Pr / (Ra / L ** 3.0))
+
+
+def kappa(Ra, Pr, L):
+    return np.sqrt(1 / Pr / (Ra / L ** 3.0))
+
+
+def Ra(nu, kappa, L):
+    return 1 / (nu * kappa) * L ** 3.0
+
+
+def Pr(nu, kappa):
+    return nu / kappa
+
+
+class NavierStokesBase:
+    """CONFIG handling, nu/kappa/scale, time-step coefficients and diagnostics
+    (rbc2d_base.py:24-131)."""
+
+    def __init__(self, **kwargs):
+        if "Ra" in kwargs or "Pr" in kwargs:
+            raise ValueError("Use small ra/pr!")
+        self.CONFIG = {
+            "shape": (50, 50), "ra": 5e3, "pr": 1.0, "dt": 0.2, "ndim": 2, "tsave": 0.1,
+            "dealias": True, "integrator": "eu", "beta": 1.0, "aspect": 1.0,
+        }
+        self.CONFIG.update(**kwargs)
+        self.__dict__.update(**self.CONFIG)
+        self.normalize = True
+        self.set_nu_kappa()
+        self.deriv_field = Field([Base(self.shape[0], "CH", dealias=3 / 2),
+                                  Base(self.shape[1], "CH", dealias=3 / 2)])
+        self.x = self.deriv_field.x * self.scale[0]
+        self.y = self.deriv_field.y * self.scale[1]
+        self.xx, self.yy = np.meshgrid(self.x, self.y, indexing="ij")
+
+    def set_nu_kappa(self, normalize=None):
+        if normalize is None:
+            normalize = self.normalize
+        if normalize:
+            self.nu = nu(self.ra, self.pr, L=1.0)
+            self.kappa = kappa(self.ra, self.pr, L=1.0)
+            self.scale = (self.aspect * 0.5, 0.5)
+        else:
+            self.nu = nu(self.ra, self.pr, L=2.0)
+            self.kappa = kappa(self.ra, self.pr, L=2.0)
+            self.scale = (self.aspect * 1.0, 1.0)
+
+    def grad(self, field, deriv, return_field=False):
+        return grad(field, deriv=deriv, return_field=return_field, scale=self.scale)
+
+    def set_timestep_coefficients_rk3(self):
+        self.nstage = 3
+        self.a = np.array([8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0])
+        self.b = np.array([8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0])
+        self.c = np.array([0, -17.0 / 60.0, -5.0 / 12.0])
+
+    def set_timestep_coefficients_euler(self):
+        self.nstage = 1
+        self.a = np.array([1.0])
+        self.b = np.array([1.0])
+        self.c = np.array([0])
+
+    def callback(self):
+        self.eval_Nu()
+
+    def eval_Nu(self):
+        """(Nu_plate, Nu_vol) (rbc2d_base.py:196-203)."""
+        Lz = self.y[-1] - self.y[0]
+        Nuz = eval_Nu(self.T, self.deriv_field, Tbc=self.Tbc, Lz=Lz)
+        Nuv = eval_Nuvol(self.T, self.V, self.kappa, self.deriv_field, Tbc=self.Tbc, Lz=Lz)
+        return Nuz, Nuv
+
+    def eval_KE(self):
+        """Kinetic energy 0.5*avg_vol(u^2+v^2) on the N x N Gauss-Lobatto grid.
+        The reference has no kinetic-energy diagnostic; this is the definition
+        used for trajectory comparisons."""
+        u = self.U.backward(self.U.vhat)
+        v = self.V.backward(self.V.vhat)
+        f = self.deriv_field
+        return float(0.5 * avg_vol(u * u + v * v, f.dx, f.dy))
+
+    @staticmethod
+    def fname_from_Ra(Ra):
+        return "Flow_Ra{:3.3e}.h5".format(Ra)
+
+    def save(self):
+        self.field.save()
+
+    # -- steady state (rbc2d_base.py:297-341)
+    def flatten(self):
+        return (self.T.vhat.flatten().clone(), self.U.vhat.flatten().clone(),
+                self.V.vhat.flatten().clone())
+
+    def vectorify(self):
+        return torch.cat(self.flatten())
+
+    def get_masks(self):
+        t, u, v = self.flatten()
+        return (slice(0, t.numel()), slice(t.numel(), t.numel() + u.numel()),
+                slice(t.numel() + u.numel(), t.numel() + u.numel() + v.numel()))
+
+    def reshape(self, X):
+        tm, um, vm = self.get_masks()
+        return (X[tm].reshape(self.T.vhat.shape).clone(), X[um].reshape(self.U.vhat.shape).clone(),
+                X[vm].reshape(self.V.vhat.shape).clone())
+
+    def steady_fun(self, X, NS=None, dt=None):
+        """(Phi_dt(X) - X)/dt for the flattened state [T, U, V]."""
+        NS = self if NS is None else NS
+        X = ops.to_dev(X)
+        T, U, V = NS.reshape(X)
+        NS.set_state(T=T, U=U, V=V)
+        NS.update()
+        Y = NS.vectorify()
+        return (Y - X) / NS.dt
+
+
+def eval_Nu(T, field, Lz=1.0, Tbc=None):
+    """Heat flux at the plates (rbc2d_base.py:344-363)."""
+    Tv = T.backward(T.vhat)
+    if Tbc is not None:
+        Tv = Tv + Tbc.v
+    That = field.forward(Tv)
+    scale = Lz / 2.0
+    dThat = field.derivative(That, 1, axis=1) / scale
+    dT = field.backward(dThat)
+    dTavg = avg_x(dT, field.dx)
+    Nu_bot = -dTavg[0] * Lz
+    Nu_top = -dTavg[-1] * Lz
+    return float((Nu_bot + Nu_top) / 2.0)
+
+
+def eval_Nuvol(T, V, kappa, field, Lz=1.0, Tbc=None):
+    """Heat flux through the volume (rbc2d_base.py:366-386)."""
+    Tv = T.backward(T.vhat)
+    if Tbc is not None:
+        Tv = Tv + Tbc.v
+    Vv = V.backward(V.vhat)
+    That = field.forward(Tv)
+    scale = Lz / 2.0
+    dThat = field.derivative(That, 1, axis=1) / scale
+    dT = field.backward(dThat)
+    Nuvol = (Tv * Vv / kappa - dT) * Lz
+    return float(avg_vol(Nuvol, field.dx, field.dy))
