@@ -1,0 +1,280 @@
+#!/usr/bin/env python
+"""Benchmark: RBC2D FP64 time steps per second at 256x256 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+
+Workload (value): the plain Euler IMEX step of navier.rbc2d.NavierStokes at
+256x256, rbc case, Ra=1e8, Pr=1, dt=1e-3, 3/2 dealiasing, seeded
+conductive-plus-noise initial state; W untimed steps, then K steps timed
+between device synchronisations, replayed from a HIP graph.
+
+Also reported on the same line:
+  roofline       the dominant kernel of the timed step (HIP events around each
+                 launch over an eager pass of the same K steps)
+  batched        64 instances of 256x256 (Ra log-uniform in [1e4, 1e7]) in one
+                 batched step: aggregate instance-steps/s and the ratio to the plain path
+  microbench     512 fields of 256x256: transforms, ChebDirichlet stencil maps,
+                 Helmholtz right-hand side and four-diagonal solves, per-kernel HBM fractions
+  cpu_baseline   the oracle (CPU restatement of the reference) on the host, one
+                 thread, on a bounded sample of the same 256x256 workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def seeded_noise(shape, seed=0, amplitude=1e-3):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-amplitude, amplitude, size=shape)
+
+
+def launch_cost(name, launch, eng):
+    """(bound, algorithmic amount per launch) of one launch of the fused step."""
+    from pypde_amd.engine import _Gemm
+    if isinstance(launch, _Gemm):
+        return "mfma", launch.flops
+    B, N0, N1, M0, M1, L0, L1 = eng.B, eng.N0, eng.N1, eng.M0, eng.M1, eng.L0, eng.L1
+    el = {
+        # read 9 blocks + the lifted term, write 3 products
+        "products": (3 * 3 * L0 * L1 + 3 * L0 * L1) * B + L0 * L1,
+        "helmholtz_x": 2 * 3 * M0 * M1 * B,
+        "helmholtz_y": 2 * 3 * M0 * M1 * B,
+        # in/out plus the per-line factor tables, counted once per launch
+        "poisson_lines": 2 * M0 * M1 * B + 4 * M0 * M1,
+        "pres_div": 3 * N0 * N1 * B,
+        "gauge": B,
+    }.get(name, 0)
+    return "hbm", 8.0 * el
+
+
+def time_plain(args):
+    import torch
+    from pypde_amd.navier import NavierStokes
+    N = 256
+    ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None,
+                      dealias=True, integrator="eu", beta=1.0, aspect=1.0)
+    ns.set_state(T=ns.T.forward(seeded_noise((N, N), seed=0)))
+    ns.update_many(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ns.update_many(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    # kernel pass: the same K steps eagerly with events around each launch
+    eng = ns.engine
+    recs = []
+    for _ in range(args.steps):
+        eng.step_timed(recs)
+    torch.cuda.synchronize()
+    per = {}
+    for name, e0, e1, launch in recs:
+        per.setdefault(name, [0.0, 0, launch])
+        per[name][0] += e0.elapsed_time(e1) * 1e-3
+        per[name][1] += 1
+    dom = max(per, key=lambda k: per[k][0])
+    tot, cnt, launch = per[dom]
+    avg = tot / cnt
+    bound, amount = launch_cost(dom, launch, eng)
+    if bound == "mfma":
+        achieved, peak, unit = amount / avg / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        achieved, peak, unit = amount / avg / 1e9, HBM_PEAK_GBS, "GB/s"
+    kernel_table = {k: dict(avg_us=round(v[0] / v[1] * 1e6, 3), calls=v[1]) for k, v in per.items()}
+    nu = ns.eval_Nu()
+    finite = bool(np.isfinite(nu[0]) and np.isfinite(nu[1]))
+    return dict(sec=sec, steps_per_s=args.steps / sec, dominant=dom, avg_s=avg, bound=bound,
+                amount=amount, achieved=achieved, peak=peak, unit=unit, kernels=kernel_table,
+                nu=nu, finite=finite, eager_sec=sum(v[0] for v in per.values()))
+
+
+def time_batched(args, nb=64, steps=None):
+    import torch
+    from pypde_amd.engine import RBCBatch
+    steps = steps or max(10, args.steps // 4)
+    N = 256
+    rng = np.random.default_rng(2024)
+    ras = 10.0 ** rng.uniform(4, 7, size=nb)
+    dts = np.clip(0.01 * np.sqrt(1e5 / ras), 1e-3, 1e-2)
+    params = [dict(ra=float(r), pr=1.0, dt=float(d)) for r, d in zip(ras, dts)]
+    eng = RBCBatch((N, N), params, case="rbc")
+    from pypde_amd.field import Field
+    from pypde_amd.bases import Base
+    Tf = Field([Base(N, "CN"), Base(N, "CD")])
+    for b in range(nb):
+        eng.X[b, 2].copy_(Tf.forward(seeded_noise((N, N), seed=1000 + b)))
+    eng.capture(1)
+    eng.step(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    finite = bool(torch.isfinite(eng.X).all().item())
+    return dict(instances=nb, steps=steps, sec=sec, inst_steps_per_s=nb * steps / sec,
+                finite=finite)
+
+
+def microbench(nf=512, reps=3):
+    """Kernel chain of 512 fields at 256x256 (per-field algorithmic bytes in the
+    docstring of each entry); returns per-kernel time and HBM fraction."""
+    import torch
+    import pypde_amd
+    from pypde_amd import ops
+    from pypde_amd.templates import hholtz_matrices
+    from pypde_amd.solver import Plan_fdma, PlanRHS
+    N, M = 256, 254
+    dev = ops.device()
+    g = torch.Generator(device="cpu").manual_seed(1)
+    k = torch.arange(N, dtype=torch.float64)
+    decay = torch.exp(-0.05 * (k[:, None] + k[None, :]))
+    c = (torch.randn((nf, N, N), generator=g, dtype=torch.float64) * decay).to(dev)
+    ch = pypde_amd.Base(N, "CH")
+    cd = pypde_amd.Base(N, "CD")
+    Bk, Fw = ch.backward_matrix(), ch.forward_matrix()
+    tabs_inv = cd.stencil_inv_tables()
+    s = cd.stencil_vector()
+    Bm, _, Ax = hholtz_matrices(cd, 0.01, 1.0)
+    rhs_x, rhs_y = PlanRHS(Bm, 2, 0), PlanRHS(Bm, 2, 1)
+    lhs_x, lhs_y = Plan_fdma(Ax, 2, 0), Plan_fdma(Ax, 2, 1)
+    out = {}
+    bufs = {}
+
+    def run(name, fn, bytes_per_field, flops=None):
+        fn()  # warm
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        t = float(np.median(ts))
+        gbs = bytes_per_field * nf / t / 1e9
+        rec = dict(us=round(t * 1e6, 2), GBps=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4))
+        if flops:
+            rec["TFLOPps"] = round(flops * nf / t / 1e12, 2)
+            rec["mfma_frac"] = round(flops * nf / t / 1e12 / FP64_PEAK_TFLOPS, 4)
+        out[name] = rec
+
+    fb = 8 * 2 * N * N
+    bufs["u1"] = ops.matmul_axis(Bk, c, 0)
+    run("K1_backward_x", lambda: ops.matmul_axis(Bk, c, 0, out=bufs["u1"]), fb, 2 * N * N * N)
+    bufs["u2"] = ops.matmul_axis(Bk, bufs["u1"], 1)
+    run("K2_backward_y", lambda: ops.matmul_axis(Bk, bufs["u1"], 1, out=bufs["u2"]), fb, 2 * N * N * N)
+    bufs["c1"] = ops.matmul_axis(Fw, bufs["u2"], 0)
+    run("K3_forward_x", lambda: ops.matmul_axis(Fw, bufs["u2"], 0, out=bufs["c1"]), fb, 2 * N * N * N)
+    bufs["c2"] = ops.matmul_axis(Fw, bufs["c1"], 1)
+    run("K4_forward_y", lambda: ops.matmul_axis(Fw, bufs["c1"], 1, out=bufs["c2"]), fb, 2 * N * N * N)
+    run("K5_from_cheb_x", lambda: bufs.__setitem__("g1", ops.stencil_inverse(tabs_inv, bufs["c2"], 0)),
+        8 * (N * N + M * N))
+    run("K6_from_cheb_y", lambda: bufs.__setitem__("g2", ops.stencil_inverse(tabs_inv, bufs["g1"], 1)),
+        8 * (M * N + M * M))
+    run("K7_to_cheb_x", lambda: bufs.__setitem__("t1", ops.stencil_apply(s, bufs["g2"], 0)),
+        8 * (M * M + N * M))
+    run("K8_to_cheb_y", lambda: bufs.__setitem__("t2", ops.stencil_apply(s, bufs["t1"], 1)),
+        8 * (N * M + N * N))
+    run("K9_rhs_Bx", lambda: bufs.__setitem__("r1", rhs_x.solve(bufs["c2"])), 8 * (N * N + M * N))
+    run("K10_rhs_By", lambda: bufs.__setitem__("r2", rhs_y.solve(bufs["r1"])), 8 * (M * N + M * M))
+    x = bufs["r2"].clone()
+    run("K11_fdma_x", lambda: lhs_x.solve(x), 8 * 2 * M * M)
+    run("K12_fdma_y", lambda: lhs_y.solve(x), 8 * 2 * M * M)
+    total_us = sum(v["us"] for v in out.values())
+    total_bytes = 12501216 * nf
+    return dict(fields=nf, N=N, kernels=out, chain_us=round(total_us, 1),
+                chain_GBps=round(total_bytes / (total_us * 1e-6) / 1e9, 1),
+                chain_hbm_frac=round(total_bytes / (total_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
+
+
+def cpu_baseline(steps=6, warm=1):
+    """Oracle (CPU restatement of the reference, scipy DCT-I + C ports of the
+    Fortran kernels, per-call dense Poisson formation as the reference) at
+    256x256, one thread."""
+    from threadpoolctl import threadpool_limits
+    from oracle import pypde_oracle as o
+    N = 256
+    with threadpool_limits(limits=1):
+        ns = o.NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None)
+        o.seeded_initial_state(ns, seed=0)
+        for _ in range(warm):
+            ns.update()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ns.update()
+        sec = time.perf_counter() - t0
+    return dict(value=round(steps / sec, 3), unit="steps/s", cores=1, kind="port",
+                sample="%d Euler steps of the 256x256 rbc workload (Ra=1e8, dt=1e-3) after %d warm-up" %
+                       (steps, warm), sec=round(sec, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (got --gpus %d); multi-GPU runs are "
+              "not part of it." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    plain = time_plain(args)
+    extra = {}
+    if not args.no_extra:
+        b = time_batched(args)
+        extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
+        extra["microbench"] = microbench()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(steps=args.cpu_steps)
+    line = {
+        "metric": "RBC2D FP64 time-steps/sec @256x256 (plain vs batched vs adjoint), % HBM roofline",
+        "value": round(plain["steps_per_s"], 2),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(plain["sec"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "rbc2d_plain_256x256", "shape": [256, 256], "case": "rbc",
+                   "ra": 1e8, "pr": 1.0, "dt": 1e-3, "integrator": "euler", "dealias": 1.5,
+                   "init": "conductive + uniform noise 1e-3, seed 0", "graph_replay": True},
+        "roofline": {"bound": plain["bound"], "achieved": round(plain["achieved"], 3),
+                     "peak": plain["peak"], "unit": plain["unit"],
+                     "frac": round(plain["achieved"] / plain["peak"], 5), "traffic": None,
+                     "kernel": plain["dominant"], "avg_us": round(plain["avg_s"] * 1e6, 3)},
+        "cpu_baseline": cpu,
+        "kernels_us": plain["kernels"],
+        "eager_ms_per_step": round(plain["eager_sec"] / args.steps * 1e3, 4),
+        "nu_final": plain["nu"],
+        "finite": plain["finite"],
+    }
+    line.update(extra)
+    line["adjoint"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

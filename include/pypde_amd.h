@@ -136,6 +136,36 @@ int pypde_twodma_solve(const pypde_lines* g, const double* d, const double* u, v
 int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, const double* d,
                      const double* e, const double* f, void* stream);
 
+/* ---- register-resident recurrences (the hot path) ---------------------- */
+/* Banded line solve in one normalised form over positions i of each line:
+ *   pre-stencil (pre != NULL)  y_i = u_i + pre_i u_{i+2}     (len_in = len_out + 2)
+ *   forward     (LM  != NULL)  y_i = F_i y_i + LM_i y_{i-2}   ascending  (F NULL: 1)
+ *   backward                   z_i = R_i y_i + MU_i z_{i+2} + NU_i z_{i+4}  descending
+ *                              (R NULL: 1, NU NULL: 0)
+ * Tables are zero-padded so that chain starts/ends are encoded (LM_i = 0 for
+ * i < 2, MU_i = 0 for i >= n-2, NU_i = 0 for i >= n-4) and hold reciprocal
+ * pivots, so the kernel does not divide.  Tables are per batch entry at
+ * entry_stride (0 = shared) or, with per_line = 1, laid out [n][nlines].
+ * Carries: the four-diagonal Helmholtz sweeps (Plan_fdma.solve, plans.py:184-224
+ * -> fdma.f90:40-98), the Poisson line solves (Plan_Poisson.solve_fortran,
+ * plans.py:310-314 -> fdma.f90:146-195, factors formed once at plan time), the
+ * Galerkin projection (chebyshev.py:295-337 -> tdma.f90:56-107) and the
+ * two-band solve (plans.py:137-181 -> twodma.f90:28-62).  Results agree with
+ * the reference to rounding (about 1e-15 relative); the bit-exact
+ * restatements above remain available. */
+typedef struct pypde_recurrence {
+    const double* F;
+    const double* LM;
+    const double* R;
+    const double* MU;
+    const double* NU;
+    const double* pre;
+    long long entry_stride;
+    int per_line;
+} pypde_recurrence;
+
+int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurrence* r, void* stream);
+
 /* ---- pointwise --------------------------------------------------------- */
 /* y = alpha*x + beta*y over n elements. */
 int pypde_axpby(long long n, double alpha, const double* x, double beta, double* y,

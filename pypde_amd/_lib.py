@@ -55,6 +55,14 @@ class GemmDesc(ctypes.Structure):
                 ("alpha_div", ctypes.c_int)]
 
 
+class Recurrence(ctypes.Structure):
+    """Mirror of `pypde_recurrence`."""
+
+    _fields_ = [("F", ctypes.c_void_p), ("LM", ctypes.c_void_p), ("R", ctypes.c_void_p),
+                ("MU", ctypes.c_void_p), ("NU", ctypes.c_void_p), ("pre", ctypes.c_void_p),
+                ("entry_stride", ctypes.c_longlong), ("per_line", ctypes.c_int)]
+
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         "pypde_amd: %s is missing -- build the HIP library first "
@@ -88,6 +96,7 @@ _SIGS = {
     "pypde_rbc_products": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "pypde_scaled_sub": ([_ll, _i, _vp, _d, _vp, _vp, _vp], _i),
     "pypde_set_entries": ([_vp, _ll, _i, _d, _vp], _i),
+    "pypde_recurrence_solve": ([_lp, ctypes.c_void_p, _vp], _i),
 }
 
 for _name, (_args, _res) in _SIGS.items():

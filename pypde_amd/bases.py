@@ -288,11 +288,24 @@ class GalerkinChebyshev(MetaBase):
         assert ops._len(vhat, axis) == self.M, "{} {}".format(ops._len(vhat, axis), self.M)
         return ops.stencil_apply(self.stencil_vector(), vhat, axis)
 
-    def from_chebyshev(self, uhat, axis=0):
+    def projection_tables(self):
+        """Normalised recurrence of (S^T S)^-1 S^T for the register-resident kernel."""
+        if not hasattr(self, "_proj"):
+            from .solver import rec_from_tdma
+            self.stencil_inv_tables()
+            a, b, w, den = self._inv_tabs_host
+            s = np.array([self._stencil()[i + 2, i] for i in range(self.M)])
+            pre, F, LM, MU = rec_from_tdma(s, a, b, w, den)
+            self._proj = ops.RecTables(pre=pre, F=F, LM=LM, MU=MU)
+        return self._proj
+
+    def from_chebyshev(self, uhat, axis=0, exact=False):
         uhat = ops.to_dev(uhat)
         assert ops._len(uhat, axis) == self.N, "Shape mismatch ({:3}) ({:3})".format(
             ops._len(uhat, axis), self.N)
-        return ops.stencil_inverse(self.stencil_inv_tables(), uhat, axis)
+        if exact or self.M < 3:
+            return ops.stencil_inverse(self.stencil_inv_tables(), uhat, axis)
+        return ops.recurrence_solve(self.projection_tables(), uhat, axis)
 
     def eval_inhomogeneous(self, bchat, axis=0):
         return self.bc.backward_fft(bchat, axis=axis)

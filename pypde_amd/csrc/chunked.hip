@@ -1,0 +1,339 @@
+// Register-resident banded line solves along one axis (the hot recurrences of
+// the time step: four-diagonal Helmholtz sweeps, eigen-split Poisson line
+// solves, the Galerkin projection (S^T S)^-1 S^T, two-band solves).
+//
+// Every operator is written in one normalised form over positions i of a line
+// (all offsets are even, so even and odd positions form independent chains):
+//
+//   pre-stencil (optional)  y_i = u_i + pre_i u_{i+2}
+//   forward                 y_i = F_i y_i + LM_i y_{i-2}            (ascending)
+//   backward                z_i = R_i y_i + MU_i z_{i+2} + NU_i z_{i+4}  (descending)
+//
+// with zero-padded tables that encode chain starts and ends, and reciprocals
+// of the pivots precomputed at plan time (no division in the kernel).
+//
+// Parallel layout: a block owns 64 lines (one lane each) and P = ceil(n / CH)
+// wave64s (CH = 32); wave w holds positions [w*CH, (w+1)*CH) of its lane's
+// line in registers.  Each sweep runs twice per chunk: once to obtain the chunk's
+// affine transfer (end value and carry coefficient), then, after the carries
+// are combined across the P chunks through LDS, once more from the exact
+// incoming values.  Loads and stores are coalesced: along columns (axis 0)
+// neighbouring lanes read neighbouring columns; along rows (axis 1) each wave
+// transposes 64 x 8 tiles through LDS.
+#include "common.h"
+#include "../../include/pypde_amd.h"
+
+#include <algorithm>
+
+namespace pypde {
+
+constexpr int CH = 32;    // positions per chunk (per lane)
+constexpr int TW = 8;     // row-tile width for axis-1 transposes
+constexpr int TP = TW + 1;
+
+struct RecArgs {
+    const double* src;
+    double* dst;
+    long long src_ld, dst_ld, src_bs, dst_bs;
+    int n_in, n, nlines, batch, axis;
+    const double *F, *LM, *R, *MU, *NU, *pre;
+    long long tbs;   // table stride between batch entries (shared mode)
+    int per_line;    // tables laid out [n][nlines] per table
+};
+
+// Table access: per-line tables from global memory ([n][nlines], coalesced
+// across lanes); shared tables from the LDS copy of the block's batch entry.
+template <bool PERLINE>
+struct Tab {
+    const double* g;   // per-line: this lane's column of slot 1 (LM); shared: LDS base
+    int nlines, line, n;
+    __device__ __forceinline__ double operator()(int which, int pos) const
+    {
+        // per-line: the offset is uniform across the wave, the base per lane
+        if (PERLINE) return g[(unsigned)(((which - 1) * n + pos) * nlines)];
+        return g[which * n + pos];
+    }
+};
+
+// Sweeps over one chunk held in registers.  GUARD: skip positions >= n (only
+// the last chunk of a line needs it).
+template <bool GUARD, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int NX>
+__device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE>& T, int base, int n,
+                                       int P, int w, int lane, double* xch)
+{
+    // table slots: 0 F, 1 LM, 2 R, 3 MU, 4 NU, 5 pre
+    if (PRE) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+            if (!GUARD || base + k < n) v[k] = v[k] + T(5, base + k) * v[k + 2];
+    }
+    if (FWD) {
+        double ye[2] = {0.0, 0.0}, pe[2] = {1.0, 1.0};
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int p = k & 1;
+            if (!GUARD || base + k < n) {
+                const double f = HASF ? T(0, base + k) : 1.0;
+                const double m = T(1, base + k);
+                ye[p] = f * v[k] + m * ye[p];
+                pe[p] = m * pe[p];
+            }
+            if (PERLINE && (k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+        xch[(w * 4 + 0) * 64 + lane] = ye[0];
+        xch[(w * 4 + 1) * 64 + lane] = ye[1];
+        xch[(w * 4 + 2) * 64 + lane] = pe[0];
+        xch[(w * 4 + 3) * 64 + lane] = pe[1];
+        __syncthreads();
+        double y[2] = {0.0, 0.0};
+        for (int u = 0; u < w; ++u) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+                y[p] = xch[(u * 4 + p) * 64 + lane] + xch[(u * 4 + 2 + p) * 64 + lane] * y[p];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int p = k & 1;
+            if (!GUARD || base + k < n) {
+                const double f = HASF ? T(0, base + k) : 1.0;
+                y[p] = f * v[k] + T(1, base + k) * y[p];
+                v[k] = y[p];
+            }
+            if (PERLINE && (k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // backward: lowest two same-parity values as affine functions of the
+    // incoming (Z1, Z2) = z at the next chunk's lowest positions
+    double s1[2] = {0, 0}, a1[2] = {1, 1}, b1[2] = {0, 0};
+    double s2[2] = {0, 0}, a2[2] = {0, 0}, b2[2] = {1, 1};
+#pragma unroll
+    for (int k = CH - 1; k >= 0; --k) {
+        const int p = k & 1;
+        if (!GUARD || base + k < n) {
+            const double r = HASR ? T(2, base + k) : 1.0;
+            const double mu = T(3, base + k);
+            const double nu = HASNU ? T(4, base + k) : 0.0;
+            const double sv = r * v[k] + mu * s1[p] + nu * s2[p];
+            const double al = mu * a1[p] + nu * a2[p];
+            const double be = mu * b1[p] + nu * b2[p];
+            s2[p] = s1[p]; a2[p] = a1[p]; b2[p] = b1[p];
+            s1[p] = sv; a1[p] = al; b1[p] = be;
+        }
+        if (PERLINE && (k & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        double* o = xch + ((w * 2 + p) * 6) * 64 + lane;
+        o[0] = s1[p]; o[64] = a1[p]; o[128] = b1[p];
+        o[192] = s2[p]; o[256] = a2[p]; o[320] = b2[p];
+    }
+    __syncthreads();
+    double Z1[2] = {0.0, 0.0}, Z2[2] = {0.0, 0.0};
+    for (int u = P - 1; u > w; --u) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double* o = xch + ((u * 2 + p) * 6) * 64 + lane;
+            const double z1 = o[0] + o[64] * Z1[p] + o[128] * Z2[p];
+            const double z2 = o[192] + o[256] * Z1[p] + o[320] * Z2[p];
+            Z1[p] = z1;
+            Z2[p] = z2;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = CH - 1; k >= 0; --k) {
+        const int p = k & 1;
+        if (!GUARD || base + k < n) {
+            const double r = HASR ? T(2, base + k) : 1.0;
+            const double z = r * v[k] + T(3, base + k) * Z1[p] + (HASNU ? T(4, base + k) * Z2[p] : 0.0);
+            v[k] = z;
+            Z2[p] = Z1[p];
+            Z1[p] = z;
+        }
+        if (PERLINE && (k & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+__global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
+{
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int P = blockDim.x >> 6;
+    const int bpe = (a.nlines + 63) / 64;
+    const int b = blockIdx.x / bpe;
+    const int l0 = (blockIdx.x - b * bpe) * 64;
+    const int line = l0 + lane;
+    const bool live = line < a.nlines;
+    const int base = w * CH;
+    const double* __restrict__ src = a.src + (long long)b * a.src_bs;
+    double* __restrict__ dst = a.dst + (long long)b * a.dst_bs;
+    const int n = a.n;
+
+    // LDS: [tables 6*n (shared mode)] [transpose tiles P*64*TP (axis 1), reused
+    // as the carry-exchange area P*12*64 during the sweeps]
+    double* ltab = lds;
+    double* tiles = lds + (PERLINE ? 0 : 6 * n);
+    double* xch = tiles;
+    if (!PERLINE) {
+        const long long eo = (long long)b * a.tbs;
+        const double* src_t[6] = {a.F, a.LM, a.R, a.MU, a.NU, a.pre};
+        for (int t = 0; t < 6; ++t) {
+            if (!src_t[t]) continue;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) ltab[t * n + i] = src_t[t][eo + i];
+        }
+    }
+    Tab<PERLINE> T;
+    T.nlines = a.nlines;
+    T.line = live ? line : 0;
+    T.n = n;
+    T.g = PERLINE ? nullptr : ltab;
+    // per-line tables: one contiguous [6][n][nlines] view is not assumed; the
+    // slots are separate arrays, so route through a small pointer table
+    constexpr int NX = PRE ? CH + 2 : CH;
+    double v[NX];
+
+    if (a.axis == 0) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int pos = base + k;
+            v[k] = (live && pos < a.n_in) ? src[(long long)pos * a.src_ld + line] : 0.0;
+        }
+    } else {
+        double* tile = tiles + w * 64 * TP;
+#pragma unroll
+        for (int t0 = 0; t0 < NX; t0 += TW) {
+#pragma unroll
+            for (int q = 0; q < TW; ++q) {
+                const int r = (lane / TW) + q * (64 / TW);
+                const int c = lane % TW;
+                const int pos = base + t0 + c;
+                const int ln = l0 + r;
+                if (r < 64)
+                    tile[r * TP + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < TW; ++c)
+                if (t0 + c < NX) v[t0 + c] = tile[lane * TP + c];
+            __syncthreads();
+        }
+    }
+    if (!PERLINE) __syncthreads();  // table copy visible
+
+    if constexpr (PERLINE) {
+        // per-line mode only ever uses LM, R, MU, NU (Poisson): address them
+        // through a 4-slot layout [slot][n][nlines] starting at LM
+        Tab<true> TL;
+        TL.nlines = a.nlines;
+        TL.line = live ? line : 0;
+        TL.n = n;
+        TL.g = a.LM + (live ? line : 0);  // slot s of this lane at g + (s-1)*n*nlines
+        if (base + CH <= n) sweeps<false, PRE, FWD, HASF, HASR, HASNU, true>(v, TL, base, n, P, w, lane, xch);
+        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, true>(v, TL, base, n, P, w, lane, xch);
+    } else {
+        if (base + CH <= n) sweeps<false, PRE, FWD, HASF, HASR, HASNU, false>(v, T, base, n, P, w, lane, xch);
+        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, false>(v, T, base, n, P, w, lane, xch);
+    }
+
+    if (a.axis == 0) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int pos = base + k;
+            if (live && pos < a.n) dst[(long long)pos * a.dst_ld + line] = v[k];
+        }
+    } else {
+        double* tile = tiles + w * 64 * TP;
+#pragma unroll
+        for (int t0 = 0; t0 < CH; t0 += TW) {
+#pragma unroll
+            for (int c = 0; c < TW; ++c) tile[lane * TP + c] = v[t0 + c];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < TW; ++q) {
+                const int r = (lane / TW) + q * (64 / TW);
+                const int c = lane % TW;
+                const int pos = base + t0 + c;
+                const int ln = l0 + r;
+                if (r < 64 && ln < a.nlines && pos < a.n) dst[(long long)ln * a.dst_ld + pos] = tile[r * TP + c];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+static int launch_rec_t(const RecArgs& a, hipStream_t s)
+{
+    const int P = (a.n + CH - 1) / CH;
+    const int bpe = (a.nlines + 63) / 64;
+    const long long blocks = (long long)bpe * a.batch;
+    if (blocks == 0) return OK;
+    size_t tabs = PERLINE ? 0 : (size_t)6 * a.n;
+    size_t tiles = a.axis == 1 ? (size_t)P * 64 * TP : 0;
+    size_t xch = (size_t)P * 12 * 64;
+    const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);
+    auto fn = recurrence_kernel<PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    PYPDE_REQUIRE(shmem <= 160 * 1024, "recurrence: LDS %zu too large", shmem);
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(P * 64), shmem, s, a);
+    return check_launch("pypde_recurrence_solve");
+}
+
+// Block size P*64 threads; the launch bound is the smallest that fits so the
+// compiler may give each lane enough registers for its chunk.
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
+static int launch_rec(const RecArgs& a, hipStream_t s)
+{
+    const int P = (a.n + CH - 1) / CH;
+    PYPDE_REQUIRE(P >= 1 && P <= 16, "recurrence: line length %d exceeds %d", a.n, 16 * CH);
+    if (P <= 4) return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
+    if (P <= 8) return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
+    return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+}
+
+}  // namespace pypde
+
+using namespace pypde;
+
+extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurrence* r, void* stream)
+{
+    PYPDE_REQUIRE(g && r, "null argument");
+    PYPDE_REQUIRE(g->src && g->dst, "null src/dst pointer");
+    PYPDE_REQUIRE(g->axis == 0 || g->axis == 1, "axis must be 0 or 1 (got %d)", g->axis);
+    PYPDE_REQUIRE(r->MU, "recurrence needs the backward multiplier table MU");
+    const bool pre = r->pre != nullptr;
+    PYPDE_REQUIRE(g->len_out >= 3 && g->nlines >= 0 && g->batch >= 0, "invalid line geometry");
+    PYPDE_REQUIRE(g->len_in == g->len_out + (pre ? 2 : 0),
+                  "recurrence: len_in must be len_out%s", pre ? " + 2" : "");
+    PYPDE_REQUIRE(!(g->src == g->dst && pre), "pre-stencil cannot run in place");
+    RecArgs a{g->src, g->dst, g->src_ld, g->dst_ld, g->src_bs, g->dst_bs, g->len_in, g->len_out,
+              g->nlines, g->batch, g->axis, r->F, r->LM, r->R, r->MU, r->NU, r->pre,
+              r->entry_stride, r->per_line};
+    hipStream_t s = as_stream(stream);
+    const bool fwd = r->LM != nullptr, hf = r->F != nullptr, hr = r->R != nullptr,
+               hn = r->NU != nullptr;
+    // the combinations used by the plans
+    if (r->per_line) {
+        PYPDE_REQUIRE(!pre && fwd && !hf && hr && hn,
+                      "per-line tables support the (LM, R, MU, NU) form only");
+        PYPDE_REQUIRE(r->R == r->LM + (long long)g->len_out * g->nlines &&
+                      r->MU == r->R + (long long)g->len_out * g->nlines &&
+                      r->NU == r->MU + (long long)g->len_out * g->nlines,
+                      "per-line tables must be contiguous slots LM | R | MU | NU of [n][nlines]");
+        return launch_rec<false, true, false, true, true, true>(a, s);
+    }
+    if (!pre && fwd && !hf && hr && hn) return launch_rec<false, true, false, true, true, false>(a, s);
+    if (pre && fwd && hf && !hr && !hn) return launch_rec<true, true, true, false, false, false>(a, s);
+    if (!pre && !fwd && !hf && hr && !hn) return launch_rec<false, false, false, true, false, false>(a, s);
+    if (!pre && fwd && hf && hr && hn) return launch_rec<false, true, true, true, true, false>(a, s);
+    set_error("pypde_recurrence_solve: unsupported table combination (pre=%d F=%d LM=%d R=%d NU=%d)",
+              (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
+    return ERR_UNSUP;
+}

@@ -1,13 +1,24 @@
 // Line operators along one axis of a batch of row-major 2-D float64 arrays.
 //
-// Every operator is a sequential recurrence (or a short banded stencil) along
-// a line, exactly as the reference's Fortran kernels run it.  A block stages a
-// panel of W lines into LDS with coalesced global accesses (rows of 64
-// adjacent columns for axis 0, whole contiguous rows for axis 1), one lane
-// then runs the operator on its own line in LDS, and the panel is written back
-// coalesced.  The arithmetic of each operator is the reference's, statement by
-// statement, with FMA contraction disabled, so results are bit-identical to
-// the Fortran/C restatement of the reference.
+// Two kernel families:
+//
+// * Banded applies (stencil S, pseudo-inverse B, B S, ...): no recurrence, so
+//   every output element is computed by its own thread from at most nine
+//   inputs of its line.  Neighbouring threads own neighbouring columns (axis
+//   0) or neighbouring entries of a row (axis 1), so all accesses coalesce.
+//
+// * Recurrences (four-diagonal LU sweeps, the offset-2 tridiagonal solve, the
+//   Poisson per-line solves, two-/five-band solves, the derivative recurrence):
+//   a block stages a panel of W lines into LDS with coalesced accesses (rows of
+//   W adjacent columns for axis 0, whole rows for axis 1) together with the
+//   operator tables.  Every operator here couples only indices of equal
+//   parity (all band offsets are even), so each line is swept by two lanes, one
+//   for the even and one for the odd entries, which halves the serial chain.
+//   The sweeps load their inputs a few elements ahead of the chain.
+//
+// The arithmetic of every operator is the reference's, statement by statement,
+// with FMA contraction disabled, so results are bit-identical to the
+// Fortran/C restatement of the reference.
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
@@ -15,16 +26,81 @@
 
 namespace pypde {
 
+#pragma clang fp contract(off)
+
 struct LineGeom {
     const double* src;
     double* dst;
     long long src_ld, dst_ld, src_bs, dst_bs;
     int len_in, len_out, nlines, batch, axis;
-    int W;      // lines per block (power of two)
+    int W;      // lines per block
     int pitch;  // LDS doubles per line (odd)
 };
 
-// One block handles W lines of one batch entry.
+// ---------------------------------------------------------------------------
+// Banded applies: y_r = sum_q vals[q][r] * x_{r + off_q}  (ascending columns)
+// ---------------------------------------------------------------------------
+constexpr int kMaxDiag = 9;
+struct BandSpec {
+    const double* vals;  // [nd][len_out]
+    int off[kMaxDiag];
+    int nd;
+    int stencil;         // 1: stencil form y_i = s_{i-2} x_{i-2} + x_i (vals = s)
+};
+
+__global__ void __launch_bounds__(256) banded_kernel(LineGeom g, BandSpec bs)
+{
+    // thread -> (entry b, output index r, line index c)
+    const long long per = (long long)g.len_out * g.nlines;
+    const long long total = per * g.batch;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(t / per);
+        const long long rem = t - (long long)b * per;
+        int r, c;
+        if (g.axis == 0) { r = (int)(rem / g.nlines); c = (int)(rem - (long long)r * g.nlines); }
+        else             { c = (int)(rem / g.len_out); r = (int)(rem - (long long)c * g.len_out); }
+        const double* src = g.src + (long long)b * g.src_bs;
+        auto X = [&](int k) -> double {
+            return g.axis == 0 ? src[(long long)k * g.src_ld + c] : src[(long long)c * g.src_ld + k];
+        };
+        double y;
+        if (bs.stencil) {
+            // u_i = s_{i-2} v_{i-2} + v_i (csc column order of S); M = len_in
+            if (r >= 2) {
+                const double tt = bs.vals[r - 2] * X(r - 2);
+                y = (r < g.len_in) ? tt + X(r) : tt;
+            } else {
+                y = X(r);
+            }
+        } else {
+            double acc = 0.0;
+            bool first = true;
+            for (int q = 0; q < bs.nd; ++q) {
+                const int k = r + bs.off[q];
+                if (k < 0 || k >= g.len_in) continue;
+                const double a = bs.vals[(long long)q * g.len_out + r];
+                if (a == 0.0) continue;
+                const double tt = a * X(k);
+                acc = first ? tt : acc + tt;
+                first = false;
+            }
+            y = acc;
+        }
+        double* dst = g.dst + (long long)b * g.dst_bs;
+        if (g.axis == 0) dst[(long long)r * g.dst_ld + c] = y;
+        else dst[(long long)c * g.dst_ld + r] = y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Recurrence panel kernel
+// ---------------------------------------------------------------------------
+// Op interface:
+//   static constexpr int kTables;             doubles of shared table per entry (x n)
+//   __device__ void load_tables(double* t, int b, int tid, int nt) const;
+//   __device__ void operator()(double* x, const double* t, int parity, int b, int line) const;
+// `t` points at the LDS copy of the tables of entry b.
 template <class Op>
 __global__ void __launch_bounds__(256) line_panel_kernel(LineGeom g, Op op)
 {
@@ -37,18 +113,14 @@ __global__ void __launch_bounds__(256) line_panel_kernel(LineGeom g, Op op)
     double* __restrict__ dst = g.dst + (long long)b * g.dst_bs;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    double* tab = lds + g.W * g.pitch;
+    op.load_tables(tab, b, tid, nt);
 
     if (g.axis == 0) {
-        // rows of the panel: element (k, c) of the input lives at src[k*ld + l0 + c]
-        const int rows_per_pass = nt / g.W;  // W divides nt or W >= nt
-        if (rows_per_pass >= 1) {
-            const int c = tid & (g.W - 1);
-            for (int k = tid / g.W; k < g.len_in; k += rows_per_pass)
-                if (c < nl) lds[c * g.pitch + k] = src[(long long)k * g.src_ld + l0 + c];
-        } else {
-            for (int k = 0; k < g.len_in; ++k)
-                for (int c = tid; c < nl; c += nt) lds[c * g.pitch + k] = src[(long long)k * g.src_ld + l0 + c];
-        }
+        const int rows_per_pass = nt / g.W;
+        const int c = tid % g.W;
+        for (int k = tid / g.W; k < g.len_in; k += rows_per_pass)
+            if (c < nl) lds[c * g.pitch + k] = src[(long long)k * g.src_ld + l0 + c];
     } else {
         for (int c = 0; c < nl; ++c) {
             const double* row = src + (long long)(l0 + c) * g.src_ld;
@@ -56,18 +128,13 @@ __global__ void __launch_bounds__(256) line_panel_kernel(LineGeom g, Op op)
         }
     }
     __syncthreads();
-    for (int c = tid; c < nl; c += nt) op(lds + c * g.pitch, b, l0 + c);
+    for (int u = tid; u < 2 * nl; u += nt) op(lds + (u >> 1) * g.pitch, tab, u & 1, b, l0 + (u >> 1));
     __syncthreads();
     if (g.axis == 0) {
         const int rows_per_pass = nt / g.W;
-        if (rows_per_pass >= 1) {
-            const int c = tid & (g.W - 1);
-            for (int k = tid / g.W; k < g.len_out; k += rows_per_pass)
-                if (c < nl) dst[(long long)k * g.dst_ld + l0 + c] = lds[c * g.pitch + k];
-        } else {
-            for (int k = 0; k < g.len_out; ++k)
-                for (int c = tid; c < nl; c += nt) dst[(long long)k * g.dst_ld + l0 + c] = lds[c * g.pitch + k];
-        }
+        const int c = tid % g.W;
+        for (int k = tid / g.W; k < g.len_out; k += rows_per_pass)
+            if (c < nl) dst[(long long)k * g.dst_ld + l0 + c] = lds[c * g.pitch + k];
     } else {
         for (int c = 0; c < nl; ++c) {
             double* row = dst + (long long)(l0 + c) * g.dst_ld;
@@ -76,46 +143,179 @@ __global__ void __launch_bounds__(256) line_panel_kernel(LineGeom g, Op op)
     }
 }
 
-// ------------------------------------------------------------------------
-// Operators.  Each works in place on one LDS line x[0 .. max(len_in,len_out)).
-// ------------------------------------------------------------------------
-#pragma clang fp contract(off)
+__device__ __forceinline__ void copy_tab(double* dst, const double* src, int n, int tid, int nt)
+{
+    for (int i = tid; i < n; i += nt) dst[i] = src[i];
+}
 
-// u = S v  (chebyshev.py:287-293): u_i = s_{i-2} v_{i-2} + v_i, summed in the
-// column order of the csc stencil; i >= M keeps only the s-term.
-struct StencilOp {
-    const double* s;
-    int M;
-    __device__ __forceinline__ void apply(double* x) const
+// Largest index <= top with the given parity.
+__device__ __forceinline__ int top_of(int top, int parity) { return ((top - parity) & 1) ? top - 1 : top; }
+
+// ---- four-diagonal sweeps (fdma.f90:1-38) on the entries of one parity ----
+// Forward: x_i -= l_{i-2} x_{i-2}; backward as in the reference.
+__device__ __forceinline__ void fdma_par(double* x, int n, int p, const double* l, const double* d,
+                                         const double* u1, const double* u2)
+{
+    // forward elimination along i = p+2, p+4, ...
     {
-        const int N = M + 2;
-        for (int i = N - 1; i >= 0; --i) {
-            double y;
-            if (i >= 2) {
-                const double t = s[i - 2] * x[i - 2];
-                y = (i < M) ? t + x[i] : t;
-            } else {
-                y = x[i];
-            }
+        double prev = x[p];
+        int i = p + 2;
+        for (; i + 6 < n; i += 8) {
+            const double x0 = x[i], x1 = x[i + 2], x2 = x[i + 4], x3 = x[i + 6];
+            const double l0 = l[i - 2], l1 = l[i], l2 = l[i + 2], l3 = l[i + 4];
+            const double y0 = x0 - l0 * prev;
+            const double y1 = x1 - l1 * y0;
+            const double y2 = x2 - l2 * y1;
+            const double y3 = x3 - l3 * y2;
+            x[i] = y0; x[i + 2] = y1; x[i + 4] = y2; x[i + 6] = y3;
+            prev = y3;
+        }
+        for (; i < n; i += 2) {
+            const double y = x[i] - l[i - 2] * prev;
             x[i] = y;
+            prev = y;
         }
     }
-    __device__ void operator()(double* x, int, int) const { apply(x); }
+    // backward substitution
+    const int t = top_of(n - 1, p);
+    double a1 = x[t] / d[t];                                   // x_{i+2}
+    x[t] = a1;
+    double a0 = (x[t - 2] - u1[t - 2] * a1) / d[t - 2];        // x_{i+2} for next, a1 -> x_{i+4}
+    x[t - 2] = a0;
+    double x2 = a0, x4 = a1;
+    int i = t - 4;
+    for (; i - 6 >= 0; i -= 8) {
+        const double b0 = x[i], b1 = x[i - 2], b2 = x[i - 4], b3 = x[i - 6];
+        const double c0 = u1[i], c1 = u1[i - 2], c2 = u1[i - 4], c3 = u1[i - 6];
+        const double e0 = u2[i], e1 = u2[i - 2], e2 = u2[i - 4], e3 = u2[i - 6];
+        const double d0 = d[i], d1 = d[i - 2], d2 = d[i - 4], d3 = d[i - 6];
+        const double y0 = (b0 - c0 * x2 - e0 * x4) / d0;
+        const double y1 = (b1 - c1 * y0 - e1 * x2) / d1;
+        const double y2 = (b2 - c2 * y1 - e2 * y0) / d2;
+        const double y3 = (b3 - c3 * y2 - e3 * y1) / d3;
+        x[i] = y0; x[i - 2] = y1; x[i - 4] = y2; x[i - 6] = y3;
+        x4 = y2;
+        x2 = y3;
+    }
+    for (; i >= 0; i -= 2) {
+        const double y = (x[i] - u1[i] * x2 - u2[i] * x4) / d[i];
+        x[i] = y;
+        x4 = x2;
+        x2 = y;
+    }
+}
+
+// Transposed: U^T z = b ascending, then L^T x = z descending.
+__device__ __forceinline__ void fdma_t_par(double* x, int n, int p, const double* l, const double* d,
+                                           const double* u1, const double* u2)
+{
+    double z2 = 0.0, z4 = 0.0;  // z_{i-2}, z_{i-4}
+    for (int i = p; i < n; i += 2) {
+        double v = x[i];
+        if (i >= 2) v = v - u1[i - 2] * z2;
+        if (i >= 4) v = v - u2[i - 4] * z4;
+        v = v / d[i];
+        x[i] = v;
+        z4 = z2;
+        z2 = v;
+    }
+    const int t = top_of(n - 1, p);
+    double xp = x[t];
+    for (int j = t - 2; j >= 0; j -= 2) {
+        const double v = x[j] - l[j] * xp;
+        x[j] = v;
+        xp = v;
+    }
+}
+
+struct FdmaOp {
+    const double *l, *d, *u1, *u2;
+    long long tbs;
+    int n;
+    int transpose;
+    __device__ void load_tables(double* t, int b, int tid, int nt) const
+    {
+        const long long o = (long long)b * tbs;
+        copy_tab(t, l + o, n - 2, tid, nt);
+        copy_tab(t + n, d + o, n, tid, nt);
+        copy_tab(t + 2 * n, u1 + o, n - 2, tid, nt);
+        copy_tab(t + 3 * n, u2 + o, n - 4, tid, nt);
+    }
+    __device__ void operator()(double* x, const double* t, int p, int, int) const
+    {
+        if (transpose) fdma_t_par(x, n, p, t, t + n, t + 2 * n, t + 3 * n);
+        else fdma_par(x, n, p, t, t + n, t + 2 * n, t + 3 * n);
+    }
 };
 
-// Chebyshev derivative recurrence (differentiate_cheby.f90:28-53) applied
-// `order` times, then divided by scale_pow (field_operations.py:40-45).
+// Poisson per-line tables (d | u1 | u2 | l), read from global memory.
+struct PoissonOp {
+    const double* tab;
+    const int* sing;
+    int n;
+    int transpose;
+    __device__ void load_tables(double*, int, int, int) const {}
+    __device__ void operator()(double* x, const double*, int p, int, int line) const
+    {
+        const double* t = tab + (long long)line * 4 * n;
+        const double *d = t, *u1 = t + n, *u2 = t + 2 * n, *l = t + 3 * n;
+        if (sing[line]) {
+            if (transpose) fdma_t_par(x + 1, n - 1, p, l, d, u1, u2);
+            else fdma_par(x + 1, n - 1, p, l, d, u1, u2);
+            if (p == 1) x[0] = 0.0;
+        } else {
+            if (transpose) fdma_t_par(x, n, p, l, d, u1, u2);
+            else fdma_par(x, n, p, l, d, u1, u2);
+        }
+    }
+};
+
+// v = (S^T S)^-1 S^T u, offset-2 tridiagonal solve (tdma.f90:56-107).
+struct StencilInvOp {
+    const double *s, *a, *b, *w, *den;
+    int M;
+    __device__ void load_tables(double* t, int, int tid, int nt) const
+    {
+        copy_tab(t, s, M, tid, nt);
+        copy_tab(t + M, a, M - 2, tid, nt);
+        copy_tab(t + 2 * M, b, M, tid, nt);
+        copy_tab(t + 3 * M, w, M - 2, tid, nt);
+        copy_tab(t + 4 * M, den, M, tid, nt);
+    }
+    __device__ void operator()(double* x, const double* t, int p, int, int) const
+    {
+        const double *s_ = t, *a_ = t + M, *b_ = t + 2 * M, *w_ = t + 3 * M, *den_ = t + 4 * M;
+        // y_k = u_k + s_k u_{k+2}   (csc column order of S^T)
+        for (int k = p; k < M; k += 2) x[k] = x[k] + s_[k] * x[k + 2];
+        // forward sweep g
+        double g = x[p] / b_[p];
+        x[p] = g;
+        for (int i = p + 2; i < M; i += 2) {
+            g = (x[i] - a_[i - 2] * g) / den_[i];
+            x[i] = g;
+        }
+        // back substitution x_j = g_j - w_j x_{j+2}
+        const int t0 = top_of(M - 1, p);
+        double xp = x[t0];
+        for (int j = t0 - 2; j >= 0; j -= 2) {
+            xp = x[j] - w_[j] * xp;
+            x[j] = xp;
+        }
+    }
+};
+
+// Chebyshev derivative recurrence (differentiate_cheby.f90:28-53), `order`
+// times, optionally after the stencil, then / scale_pow.  The recurrence reads
+// the other parity's input, so one lane does the whole line.
 __device__ __forceinline__ void cheb_deriv_inplace(double* x, int n)
 {
-    double cn = x[n - 1];       // c_{j+1} (input value) for the next j
-    double dp2 = 0.0;           // dc_{j+2}
-    double dp1 = 0.0;           // dc_{j+1}
+    double cn = x[n - 1];
     x[n - 1] = 0.0;
+    double dp2 = 0.0, dp1;
     {
         const double cj = x[n - 2];
         const double dj = (double)(2 * (n - 1)) * cn;
         x[n - 2] = dj;
-        dp2 = 0.0;
         dp1 = dj;
         cn = cj;
     }
@@ -127,149 +327,35 @@ __device__ __forceinline__ void cheb_deriv_inplace(double* x, int n)
         dp1 = dj;
         cn = cj;
     }
-    // dc_0 = dc_2 / 2 + c_1   (dp2 holds dc_2 here, cn holds c_1)
     x[0] = dp2 / 2.0 + cn;
 }
 
 struct DerivOp {
-    const double* s;  // stencil (nullptr for Chebyshev input)
-    int len_in;       // M for Galerkin input, N for Chebyshev
-    int n;            // Chebyshev length
-    int order;
+    const double* s;
+    int len_in, n, order;
     double scale_pow;
-    __device__ void operator()(double* x, int, int) const
+    __device__ void load_tables(double* t, int, int tid, int nt) const
     {
-        if (s) StencilOp{s, len_in}.apply(x);
+        if (s) copy_tab(t, s, len_in, tid, nt);
+    }
+    __device__ void operator()(double* x, const double* t, int p, int, int) const
+    {
+        if (p) return;
+        if (s) {
+            const int M = len_in, N = M + 2;
+            for (int i = N - 1; i >= 0; --i) {
+                double y;
+                if (i >= 2) {
+                    const double tt = t[i - 2] * x[i - 2];
+                    y = (i < M) ? tt + x[i] : tt;
+                } else {
+                    y = x[i];
+                }
+                x[i] = y;
+            }
+        }
         for (int o = 0; o < order; ++o) cheb_deriv_inplace(x, n);
         for (int i = 0; i < n; ++i) x[i] = x[i] / scale_pow;
-    }
-};
-
-// v = (S^T S)^-1 S^T u with the offset-2 tridiagonal solve (tdma.f90:56-107).
-struct StencilInvOp {
-    const double *s, *a, *b, *w, *den;
-    int M;
-    __device__ void operator()(double* x, int, int) const
-    {
-        // y_k = u_k + s_k u_{k+2}  (csc column order of S^T)
-        for (int k = 0; k < M; ++k) x[k] = x[k] + s[k] * x[k + 2];
-        // forward sweep g
-        double g2 = x[0] / b[0];
-        double g1 = x[1] / b[1];
-        x[0] = g2;
-        x[1] = g1;
-        for (int i = 2; i < M; ++i) {
-            const double gi = (x[i] - a[i - 2] * g2) / den[i];
-            x[i] = gi;
-            g2 = g1;
-            g1 = gi;
-        }
-        // back substitution  x_j = g_j - w_j x_{j+2}
-        double xp2 = x[M - 1];
-        double xp1 = x[M - 2];
-        for (int j = M - 3; j >= 0; --j) {
-            const double xj = x[j] - w[j] * xp2;
-            x[j] = xj;
-            xp2 = xp1;
-            xp1 = xj;
-        }
-    }
-};
-
-// Pre-factored four-diagonal solve (fdma.f90:1-38), tables per batch entry.
-__device__ __forceinline__ void fdma_inplace(double* x, int n, const double* __restrict__ l,
-                                             const double* __restrict__ d,
-                                             const double* __restrict__ u1,
-                                             const double* __restrict__ u2)
-{
-    double a2 = x[0], a1 = x[1];
-    for (int i = 2; i < n; ++i) {
-        const double v = x[i] - l[i - 2] * a2;
-        x[i] = v;
-        a2 = a1;
-        a1 = v;
-    }
-    double p1 = x[n - 1] / d[n - 1];
-    double p2 = x[n - 2] / d[n - 2];
-    x[n - 1] = p1;
-    x[n - 2] = p2;
-    double p3 = (x[n - 3] - u1[n - 3] * p1) / d[n - 3];
-    x[n - 3] = p3;
-    double p4 = (x[n - 4] - u1[n - 4] * p2) / d[n - 4];
-    x[n - 4] = p4;
-    // rolling window: x_{i+1}=p4? keep x_{i+2}, x_{i+4}
-    double xi1 = p4, xi2 = p3, xi3 = p2, xi4 = p1;  // x_{i+1..i+4} for i = n-5
-    for (int i = n - 5; i >= 0; --i) {
-        const double v = (x[i] - u1[i] * xi2 - u2[i] * xi4) / d[i];
-        x[i] = v;
-        xi4 = xi3;
-        xi3 = xi2;
-        xi2 = xi1;
-        xi1 = v;
-    }
-}
-
-// Transposed solve A^T x = b with A = L U from the same tables.
-__device__ __forceinline__ void fdma_t_inplace(double* x, int n, const double* __restrict__ l,
-                                               const double* __restrict__ d,
-                                               const double* __restrict__ u1,
-                                               const double* __restrict__ u2)
-{
-    // U^T z = b : z_i = (b_i - u1_{i-2} z_{i-2} - u2_{i-4} z_{i-4}) / d_i
-    double z1 = 0.0, z2 = 0.0, z3 = 0.0, z4 = 0.0;  // z_{i-1..i-4}
-    for (int i = 0; i < n; ++i) {
-        double v = x[i];
-        if (i >= 2) v = v - u1[i - 2] * z2;
-        if (i >= 4) v = v - u2[i - 4] * z4;
-        v = v / d[i];
-        x[i] = v;
-        z4 = z3;
-        z3 = z2;
-        z2 = z1;
-        z1 = v;
-    }
-    // L^T x = z : x_j = z_j - l_j x_{j+2}
-    double xp2 = x[n - 1], xp1 = x[n - 2];
-    for (int j = n - 3; j >= 0; --j) {
-        const double v = x[j] - l[j] * xp2;
-        x[j] = v;
-        xp2 = xp1;
-        xp1 = v;
-    }
-}
-
-struct FdmaOp {
-    const double *l, *d, *u1, *u2;
-    long long tbs;
-    int n;
-    int transpose;
-    __device__ void operator()(double* x, int b, int) const
-    {
-        const long long o = (long long)b * tbs;
-        if (transpose) fdma_t_inplace(x, n, l + o, d + o, u1 + o, u2 + o);
-        else fdma_inplace(x, n, l + o, d + o, u1 + o, u2 + o);
-    }
-};
-
-// Poisson line solve with per-line tables (d | u1 | u2 | l) of length n each.
-struct PoissonOp {
-    const double* tab;
-    const int* sing;
-    int n;
-    int transpose;
-    __device__ void operator()(double* x, int, int line) const
-    {
-        const double* t = tab + (long long)line * 4 * n;
-        const double *d = t, *u1 = t + n, *u2 = t + 2 * n, *l = t + 3 * n;
-        if (sing[line]) {
-            // reduced system on x[1:], tables stored from index 0 for length n-1
-            if (transpose) fdma_t_inplace(x + 1, n - 1, l, d, u1, u2);
-            else fdma_inplace(x + 1, n - 1, l, d, u1, u2);
-            x[0] = 0.0;
-        } else {
-            if (transpose) fdma_t_inplace(x, n, l, d, u1, u2);
-            else fdma_inplace(x, n, l, d, u1, u2);
-        }
     }
 };
 
@@ -277,17 +363,20 @@ struct PoissonOp {
 struct TwodmaOp {
     const double *d, *u;
     int n;
-    __device__ void operator()(double* x, int, int) const
+    __device__ void load_tables(double* t, int, int tid, int nt) const
     {
-        double p1 = x[n - 1] / d[n - 1];
-        double p2 = x[n - 2] / d[n - 2];
-        x[n - 1] = p1;
-        x[n - 2] = p2;
-        for (int i = n - 3; i >= 0; --i) {
-            const double v = (x[i] - u[i] * p1) / d[i];
-            x[i] = v;
-            p1 = p2;
-            p2 = v;
+        copy_tab(t, d, n, tid, nt);
+        copy_tab(t + n, u, n - 2, tid, nt);
+    }
+    __device__ void operator()(double* x, const double* t, int p, int, int) const
+    {
+        const double *d_ = t, *u_ = t + n;
+        const int t0 = top_of(n - 1, p);
+        double xp = x[t0] / d_[t0];
+        x[t0] = xp;
+        for (int i = t0 - 2; i >= 0; i -= 2) {
+            xp = (x[i] - u_[i] * xp) / d_[i];
+            x[i] = xp;
         }
     }
 };
@@ -296,76 +385,39 @@ struct TwodmaOp {
 struct PdmaOp {
     const double *a, *b, *d, *e, *f;
     int n;
-    __device__ void operator()(double* x, int, int) const
+    __device__ void load_tables(double* t, int, int tid, int nt) const
     {
-        x[2] -= b[0] * x[0];
-        x[3] -= b[1] * x[1];
-        for (int k = 4; k < n; ++k) x[k] -= b[k - 2] * x[k - 2] + a[k - 4] * x[k - 4];
-        x[n - 1] /= d[n - 1];
-        x[n - 2] /= d[n - 2];
-        x[n - 3] = (x[n - 3] - e[n - 3] * x[n - 1]) / d[n - 3];
-        x[n - 4] = (x[n - 4] - e[n - 4] * x[n - 2]) / d[n - 4];
-        for (int k = n - 5; k >= 0; --k) x[k] = (x[k] - e[k] * x[k + 2] - f[k] * x[k + 4]) / d[k];
+        copy_tab(t, a, n - 4, tid, nt);
+        copy_tab(t + n, b, n - 2, tid, nt);
+        copy_tab(t + 2 * n, d, n, tid, nt);
+        copy_tab(t + 3 * n, e, n - 2, tid, nt);
+        copy_tab(t + 4 * n, f, n - 4, tid, nt);
+    }
+    __device__ void operator()(double* x, const double* t, int p, int, int) const
+    {
+        const double *a_ = t, *b_ = t + n, *d_ = t + 2 * n, *e_ = t + 3 * n, *f_ = t + 4 * n;
+        x[2 + p] -= b_[p] * x[p];
+        for (int k = 4 + p; k < n; k += 2) x[k] -= b_[k - 2] * x[k - 2] + a_[k - 4] * x[k - 4];
+        const int t0 = top_of(n - 1, p);
+        x[t0] /= d_[t0];
+        x[t0 - 2] = (x[t0 - 2] - e_[t0 - 2] * x[t0]) / d_[t0 - 2];
+        for (int k = t0 - 4; k >= 0; k -= 2) x[k] = (x[k] - e_[k] * x[k + 2] - f_[k] * x[k + 4]) / d_[k];
     }
 };
-
-// Banded y = A x, diagonals with offsets in [-4, 4], ascending column order.
-constexpr int kMaxDiag = 9;
-struct BandedOp {
-    const double* vals;
-    int off[kMaxDiag];
-    int nd;
-    int len_in, len_out;
-    __device__ void operator()(double* x, int, int) const
-    {
-        double old1 = 0.0, old2 = 0.0, old3 = 0.0, old4 = 0.0;  // input x_{r-1..r-4}
-        for (int r = 0; r < len_out; ++r) {
-            const double xr = (r < len_in) ? x[r] : 0.0;
-            double acc = 0.0;
-            bool first = true;
-            for (int q = 0; q < nd; ++q) {
-                const int c = r + off[q];
-                if (c < 0 || c >= len_in) continue;
-                const double a = vals[(long long)q * len_out + r];
-                if (a == 0.0) continue;
-                double xc;
-                const int o = off[q];
-                if (o == 0) xc = xr;
-                else if (o == -1) xc = old1;
-                else if (o == -2) xc = old2;
-                else if (o == -3) xc = old3;
-                else if (o == -4) xc = old4;
-                else xc = x[c];
-                const double t = a * xc;
-                acc = first ? t : acc + t;
-                first = false;
-            }
-            x[r] = acc;
-            old4 = old3;
-            old3 = old2;
-            old2 = old1;
-            old1 = xr;
-        }
-    }
-};
-
-#pragma clang fp contract(on)
 
 // ------------------------------------------------------------------------
 // Launch helpers
 // ------------------------------------------------------------------------
-static int pick_lines_per_block(int len, int hint, int nlines)
+constexpr int kLdsBudget = 64 * 1024;
+
+static int pick_lines_per_block(int pitch, int tab_doubles, int hint, int nlines)
 {
-    if (hint > 0) {
-        int w = 1;
-        while (w * 2 <= hint) w *= 2;
-        return w;
-    }
-    const int pitch = (len | 1);
-    const int budget = 64 * 1024;  // bytes of LDS per block
     int w = 1;
-    while (w * 2 <= 64 && (w * 2) * pitch * 8 <= budget) w *= 2;
-    // do not stage far more lines than exist
+    if (hint > 0) {
+        while (w * 2 <= hint && w * 2 <= 128) w *= 2;
+    } else {
+        while (w * 2 <= 64 && ((long long)(w * 2) * pitch + tab_doubles) * 8 <= kLdsBudget) w *= 2;
+    }
     while (w > 1 && w / 2 >= nlines) w /= 2;
     return w;
 }
@@ -389,25 +441,36 @@ static int make_geom(const pypde_lines* in, LineGeom& g)
     g.nlines = in->nlines;
     g.batch = in->batch;
     g.axis = in->axis;
-    const int len = std::max(in->len_in, in->len_out);
-    g.pitch = len | 1;
-    g.W = pick_lines_per_block(len, in->lines_per_block, in->nlines);
-    PYPDE_REQUIRE((long long)g.W * g.pitch * 8 <= 160 * 1024,
-                  "line length %d too long for an LDS panel", len);
+    g.pitch = std::max(in->len_in, in->len_out) | 1;
+    g.W = 1;
     return OK;
 }
 
 template <class Op>
-static int launch_lines(const LineGeom& g, const Op& op, void* stream, const char* name)
+static int launch_lines(LineGeom g, const Op& op, int ntab, int hint, void* stream, const char* name)
 {
     if (g.nlines == 0 || g.batch == 0) return OK;
+    const int tab_doubles = ntab;
+    g.W = pick_lines_per_block(g.pitch, tab_doubles, hint, g.nlines);
+    const size_t shmem = ((size_t)g.W * g.pitch + tab_doubles) * sizeof(double);
+    PYPDE_REQUIRE(shmem <= (size_t)kLdsBudget, "%s: line length %d too long for an LDS panel",
+                  name, g.pitch);
     const int bpe = (g.nlines + g.W - 1) / g.W;
     const long long blocks = (long long)bpe * g.batch;
     PYPDE_REQUIRE(blocks < (1LL << 31), "%s: grid too large", name);
-    const int threads = std::min(256, std::max(64, g.W));
-    const size_t shmem = (size_t)g.W * g.pitch * sizeof(double);
+    const int threads = std::min(256, std::max(64, 2 * g.W));
     hipLaunchKernelGGL(line_panel_kernel<Op>, dim3((unsigned)blocks), dim3(threads), shmem,
                        as_stream(stream), g, op);
+    return check_launch(name);
+}
+
+static int launch_banded(const LineGeom& g, const BandSpec& bs, void* stream, const char* name)
+{
+    const long long total = (long long)g.len_out * g.nlines * g.batch;
+    if (total == 0) return OK;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(banded_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), g, bs);
     return check_launch(name);
 }
 
@@ -422,7 +485,31 @@ extern "C" int pypde_stencil_apply(const pypde_lines* in, const double* s, void*
     if (rc) return rc;
     PYPDE_REQUIRE(s, "null stencil");
     PYPDE_REQUIRE(g.len_out == g.len_in + 2, "stencil apply needs len_out == len_in + 2");
-    return launch_lines(g, StencilOp{s, g.len_in}, stream, "pypde_stencil_apply");
+    PYPDE_REQUIRE(g.src != g.dst, "stencil apply cannot run in place");
+    BandSpec bs{};
+    bs.vals = s;
+    bs.stencil = 1;
+    return launch_banded(g, bs, stream, "pypde_stencil_apply");
+}
+
+extern "C" int pypde_banded_apply(const pypde_lines* in, int ndiag, const int* offsets_host,
+                                  const double* vals, void* stream)
+{
+    LineGeom g;
+    int rc = make_geom(in, g);
+    if (rc) return rc;
+    PYPDE_REQUIRE(ndiag >= 1 && ndiag <= kMaxDiag, "ndiag must be 1..%d", kMaxDiag);
+    PYPDE_REQUIRE(vals && offsets_host, "null diagonal pointer");
+    PYPDE_REQUIRE(g.src != g.dst, "banded apply cannot run in place");
+    BandSpec bs{};
+    bs.vals = vals;
+    bs.nd = ndiag;
+    for (int q = 0; q < ndiag; ++q) {
+        PYPDE_REQUIRE(offsets_host[q] >= -4 && offsets_host[q] <= 4, "offset out of [-4,4]");
+        if (q) PYPDE_REQUIRE(offsets_host[q] > offsets_host[q - 1], "offsets must ascend");
+        bs.off[q] = offsets_host[q];
+    }
+    return launch_banded(g, bs, stream, "pypde_banded_apply");
 }
 
 extern "C" int pypde_stencil_inverse(const pypde_lines* in, const double* s, const double* a,
@@ -435,7 +522,8 @@ extern "C" int pypde_stencil_inverse(const pypde_lines* in, const double* s, con
     PYPDE_REQUIRE(s && a && b && w && den, "null table pointer");
     PYPDE_REQUIRE(g.len_in == g.len_out + 2 && g.len_out >= 3,
                   "stencil inverse needs len_in == len_out + 2 >= 5");
-    return launch_lines(g, StencilInvOp{s, a, b, w, den, g.len_out}, stream,
+    const int M = g.len_out;
+    return launch_lines(g, StencilInvOp{s, a, b, w, den, M}, 5 * M, in->lines_per_block, stream,
                         "pypde_stencil_inverse");
 }
 
@@ -449,28 +537,8 @@ extern "C" int pypde_cheb_deriv(const pypde_lines* in, const double* s, int orde
     const int n = s ? g.len_in + 2 : g.len_in;
     PYPDE_REQUIRE(g.len_out == n, "derivative output length must be %d", n);
     PYPDE_REQUIRE(n >= 3 || order == 0, "derivative needs n >= 3");
-    return launch_lines(g, DerivOp{s, g.len_in, n, order, scale_pow}, stream, "pypde_cheb_deriv");
-}
-
-extern "C" int pypde_banded_apply(const pypde_lines* in, int ndiag, const int* offsets_host,
-                                  const double* vals, void* stream)
-{
-    LineGeom g;
-    int rc = make_geom(in, g);
-    if (rc) return rc;
-    PYPDE_REQUIRE(ndiag >= 1 && ndiag <= kMaxDiag, "ndiag must be 1..%d", kMaxDiag);
-    PYPDE_REQUIRE(vals && offsets_host, "null diagonal pointer");
-    BandedOp op{};
-    op.vals = vals;
-    op.nd = ndiag;
-    op.len_in = g.len_in;
-    op.len_out = g.len_out;
-    for (int q = 0; q < ndiag; ++q) {
-        PYPDE_REQUIRE(offsets_host[q] >= -4 && offsets_host[q] <= 4, "offset out of [-4,4]");
-        if (q) PYPDE_REQUIRE(offsets_host[q] > offsets_host[q - 1], "offsets must ascend");
-        op.off[q] = offsets_host[q];
-    }
-    return launch_lines(g, op, stream, "pypde_banded_apply");
+    return launch_lines(g, DerivOp{s, g.len_in, n, order, scale_pow}, s ? g.len_in : 0,
+                        in->lines_per_block, stream, "pypde_cheb_deriv");
 }
 
 static int fdma_common(const pypde_lines* in, const double* l, const double* d, const double* u1,
@@ -480,9 +548,10 @@ static int fdma_common(const pypde_lines* in, const double* l, const double* d, 
     int rc = make_geom(in, g);
     if (rc) return rc;
     PYPDE_REQUIRE(l && d && u1 && u2, "null fdma table");
-    PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 5, "fdma needs len_in == len_out >= 5");
-    return launch_lines(g, FdmaOp{l, d, u1, u2, tbs, g.len_in, transpose}, stream,
-                        transpose ? "pypde_fdma_solve_t" : "pypde_fdma_solve");
+    PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 6, "fdma needs len_in == len_out >= 6");
+    const int n = g.len_in;
+    return launch_lines(g, FdmaOp{l, d, u1, u2, tbs, n, transpose}, 4 * n, in->lines_per_block,
+                        stream, transpose ? "pypde_fdma_solve_t" : "pypde_fdma_solve");
 }
 
 extern "C" int pypde_fdma_solve(const pypde_lines* in, const double* l, const double* d,
@@ -506,8 +575,9 @@ extern "C" int pypde_poisson_solve(const pypde_lines* in, const double* tab, con
     int rc = make_geom(in, g);
     if (rc) return rc;
     PYPDE_REQUIRE(tab && sing, "null poisson table");
-    PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 6, "poisson needs len_in == len_out >= 6");
-    return launch_lines(g, PoissonOp{tab, sing, g.len_in, transpose}, stream, "pypde_poisson_solve");
+    PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 7, "poisson needs len_in == len_out >= 7");
+    return launch_lines(g, PoissonOp{tab, sing, g.len_in, transpose}, 0, in->lines_per_block, stream,
+                        "pypde_poisson_solve");
 }
 
 extern "C" int pypde_twodma_solve(const pypde_lines* in, const double* d, const double* u,
@@ -518,7 +588,8 @@ extern "C" int pypde_twodma_solve(const pypde_lines* in, const double* d, const 
     if (rc) return rc;
     PYPDE_REQUIRE(d && u, "null twodma table");
     PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 3, "twodma needs len_in == len_out >= 3");
-    return launch_lines(g, TwodmaOp{d, u, g.len_in}, stream, "pypde_twodma_solve");
+    return launch_lines(g, TwodmaOp{d, u, g.len_in}, 2 * g.len_in, in->lines_per_block, stream,
+                        "pypde_twodma_solve");
 }
 
 extern "C" int pypde_pdma_solve(const pypde_lines* in, const double* a, const double* b,
@@ -529,5 +600,6 @@ extern "C" int pypde_pdma_solve(const pypde_lines* in, const double* a, const do
     if (rc) return rc;
     PYPDE_REQUIRE(a && b && d && e && f, "null pdma table");
     PYPDE_REQUIRE(g.len_in == g.len_out && g.len_in >= 6, "pdma needs len_in == len_out >= 6");
-    return launch_lines(g, PdmaOp{a, b, d, e, f, g.len_in}, stream, "pypde_pdma_solve");
+    return launch_lines(g, PdmaOp{a, b, d, e, f, g.len_in}, 5 * g.len_in, in->lines_per_block,
+                        stream, "pypde_pdma_solve");
 }

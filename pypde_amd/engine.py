@@ -41,7 +41,8 @@ import torch
 from . import ops
 from ._lib import BIdx, GemmDesc, Lines, check, lib
 from .bases import Base, cheb_backward_matrix, cheb_forward_matrix, pseudoinverse_d2
-from .solver import fdma_lu, poisson_line_tables
+from ._lib import Recurrence
+from .solver import fdma_lu, poisson_line_tables, rec_from_fdma, rec_from_poisson
 from .templates import hholtz_matrices, poisson_matrices
 
 
@@ -155,6 +156,15 @@ def _lines(src, dst, ld, bs, n, nlines, batch, axis):
     return g
 
 
+def _rec(base, M, entry_stride):
+    """Recurrence tables stacked [entry][LM | R | MU | NU][M] at `base`."""
+    r = Recurrence()
+    r.LM, r.R, r.MU, r.NU = base, base + 8 * M, base + 16 * M, base + 24 * M
+    r.F = r.pre = None
+    r.entry_stride, r.per_line = entry_stride, 0
+    return r
+
+
 # ---------------------------------------------------------------------------
 class RBCBatch:
     """Euler IMEX RBC2D step for `B` instances of one shape.
@@ -221,6 +231,7 @@ class RBCBatch:
         tab, sing = poisson_line_tables(pm["Ax"], pm["Cx"], pm["wy"], True)
         self.ptab = T(tab)
         self.psing = torch.as_tensor(sing, device=dev)
+        self.prec = T(np.stack(rec_from_poisson(tab, sing)))  # [LM | R | MU | NU][M0][M1]
         self.Wy = T(np.vstack([ay.Sn, ay.TSn, ay.TDSn]))                      # [N1+2M1, M1]
         self.Snx = T(ax.Sn)                                                   # [N0, M0]
         self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
@@ -237,6 +248,11 @@ class RBCBatch:
             for f, lam in enumerate((lam_u, lam_u, lam_t)):
                 hy[b, f] = self._lu(hholtz_matrices(by_cd, lam, self.scale[1])[2])
         self.hx, self.hy = T(hx), T(hy)
+        # normalised (reciprocal-pivot) form for the register-resident kernel
+        self.rhx = T(np.stack(rec_from_fdma(hx[:, :, 0, :M0 - 2], hx[:, :, 1], hx[:, :, 2, :M0 - 2],
+                                            hx[:, :, 3, :M0 - 4]), axis=2))
+        self.rhy = T(np.stack(rec_from_fdma(hy[:, :, 0, :M1 - 2], hy[:, :, 1], hy[:, :, 2, :M1 - 2],
+                                            hy[:, :, 3, :M1 - 4]), axis=2))
 
         # boundary lifting constants (shape-only) and per-instance RHS constants
         self._setup_bc()
@@ -351,17 +367,14 @@ class RBCBatch:
                        B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1, iB=_bidx(s1=3 * sZ),
                        D=p(X) + 8 * sX, ldd=M1, iD=_bidx(s1=3 * sX), beta=1.0,
                        C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX), alpha_vec=p(self.t_dt)))
-        # 9-10  Helmholtz ADI sweeps
+        # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
-        hx, hy = self.hx, self.hy
-        self._keep = [gx, gy]
-        L.append(_Call("helmholtz_x", lib.pypde_fdma_solve, ctypes.byref(gx),
-                       ctypes.c_void_p(p(hx)), ctypes.c_void_p(p(hx) + 8 * M0),
-                       ctypes.c_void_p(p(hx) + 16 * M0), ctypes.c_void_p(p(hx) + 24 * M0), 4 * M0))
-        L.append(_Call("helmholtz_y", lib.pypde_fdma_solve, ctypes.byref(gy),
-                       ctypes.c_void_p(p(hy)), ctypes.c_void_p(p(hy) + 8 * M1),
-                       ctypes.c_void_p(p(hy) + 16 * M1), ctypes.c_void_p(p(hy) + 24 * M1), 4 * M1))
+        rx = _rec(p(self.rhx), M0, 4 * M0)
+        ry = _rec(p(self.rhy), M1, 4 * M1)
+        self._keep = [gx, gy, rx, ry]
+        L.append(_Call("helmholtz_x", lib.pypde_recurrence_solve, ctypes.byref(gx), ctypes.byref(rx)))
+        L.append(_Call("helmholtz_y", lib.pypde_recurrence_solve, ctypes.byref(gy), ctypes.byref(ry)))
         # 11  Wd = [U* C_y^T ; V* Cd_y^T]
         L.append(_Gemm("div_y", m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
                        iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.Cy), ldb=M1, tb=1,
@@ -379,9 +392,11 @@ class RBCBatch:
                        iC=_bidx(s1=sX)))
         # 15  per-line eigen-split solves along x
         gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
-        self._keep.append(gp)
-        L.append(_Call("poisson_lines", lib.pypde_poisson_solve, ctypes.byref(gp),
-                       ctypes.c_void_p(p(self.ptab)), ctypes.c_void_p(p(self.psing)), 0))
+        rp = Recurrence()
+        rp.LM, rp.R, rp.MU, rp.NU = (self.prec[i].data_ptr() for i in range(4))
+        rp.per_line, rp.entry_stride = 1, 0
+        self._keep += [gp, rp]
+        L.append(_Call("poisson_lines", lib.pypde_recurrence_solve, ctypes.byref(gp), ctypes.byref(rp)))
         # 16  P = RP Q_y^T
         L.append(_Gemm("poisson_qy", m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Qy), ldb=M1, tb=1, C=p(self.P), ldc=M1,
@@ -414,6 +429,19 @@ class RBCBatch:
         s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         for f in self.launches:
             f(s)
+
+    def step_timed(self, records):
+        """One eager step with a HIP event pair around every launch, on the
+        current stream; appends (name, start, end, launch) to records."""
+        stream = torch.cuda.current_stream()
+        s = ctypes.c_void_p(stream.cuda_stream)
+        for f in self.launches:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            f(s)
+            e1.record(stream)
+            records.append((f.name, e0, e1, f))
 
     def capture(self, nsteps=1):
         """Capture `nsteps` consecutive steps into one HIP graph (the state is

@@ -242,3 +242,49 @@ def axpby(alpha, x, beta, y):
     check(lib.pypde_axpby(y.numel(), float(alpha), ptr(x), float(beta), ptr(y), stream_ptr()),
           "pypde_axpby")
     return y
+
+
+# ---------------------------------------------------------------------------
+# register-resident recurrences (normalised tables, see include/pypde_amd.h)
+# ---------------------------------------------------------------------------
+class RecTables:
+    """Device copy of a normalised recurrence (F, LM, R, MU, NU, pre tables)."""
+
+    def __init__(self, F=None, LM=None, R=None, MU=None, NU=None, pre=None, per_line=False,
+                 entry_stride=0):
+        if per_line:
+            # per-line tables live in one [LM | R | MU | NU][n][nlines] block
+            self.block = to_dev(np.stack([LM, R, MU, NU]))
+            self.t = dict(F=None, LM=self.block[0], R=self.block[1], MU=self.block[2],
+                          NU=self.block[3], pre=None)
+        else:
+            self.t = {k: (to_dev(v) if v is not None else None)
+                      for k, v in dict(F=F, LM=LM, R=R, MU=MU, NU=NU, pre=pre).items()}
+        self.per_line = bool(per_line)
+        self.entry_stride = int(entry_stride)
+        self.has_pre = pre is not None
+
+    def struct(self):
+        from ._lib import Recurrence
+        r = Recurrence()
+        for k, v in self.t.items():
+            setattr(r, k, v.data_ptr() if v is not None else None)
+        r.entry_stride = self.entry_stride
+        r.per_line = int(self.per_line)
+        return r
+
+
+def recurrence_solve(tabs, x, axis=0, out=None, lpb=0):
+    """Banded line solve in normalised form along axis (in place when out is x
+    and there is no pre-stencil)."""
+    _check_axis(x, axis)
+    n_in = _len(x, axis)
+    n = n_in - 2 if tabs.has_pre else n_in
+    if out is None:
+        out = x if not tabs.has_pre else torch.empty(_out_shape(x, axis, n), dtype=DTYPE,
+                                                     device=x.device)
+    g = _geom(_as3(x), _as3(out), axis, n_in, n, lpb)
+    r = tabs.struct()
+    check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(r), stream_ptr()),
+          "pypde_recurrence_solve")
+    return out

@@ -17,6 +17,64 @@ import torch
 from . import ops
 
 
+# ---------------------------------------------------------------------------
+# normalised recurrence tables (reciprocal pivots, zero-padded chain ends)
+# ---------------------------------------------------------------------------
+def rec_from_fdma(l, d, u1, u2):
+    """Four-diagonal LU tables (plans.py:226-234) -> (LM, R, MU, NU)."""
+    n = d.shape[-1]
+    lead = d.shape[:-1]
+    LM = np.zeros(lead + (n,))
+    MU = np.zeros(lead + (n,))
+    NU = np.zeros(lead + (n,))
+    LM[..., 2:] = -l[..., : n - 2]
+    R = 1.0 / d
+    MU[..., : n - 2] = -u1[..., : n - 2] / d[..., : n - 2]
+    NU[..., : n - 4] = -u2[..., : n - 4] / d[..., : n - 4]
+    return LM, R, MU, NU
+
+
+def rec_from_poisson(tab, sing):
+    """Per-line Poisson factors (poisson_line_tables) -> per-line normalised
+    tables laid out [n][nlines]; singular lines act on positions 1.. only and
+    return x_0 = 0 (fdma.f90:176-179)."""
+    nl, _, n = tab.shape
+    LM, R, MU, NU = (np.zeros((nl, n)) for _ in range(4))
+    for j in range(nl):
+        d, u1, u2, l = tab[j]
+        if sing[j]:
+            m = n - 1
+            a, b, c, e = rec_from_fdma(l[: m - 2], d[:m], u1[: m - 2], u2[: m - 4])
+            LM[j, 1:], R[j, 1:], MU[j, 1:], NU[j, 1:] = a, b, c, e
+            R[j, 0] = 0.0
+        else:
+            a, b, c, e = rec_from_fdma(l[: n - 2], d[:n], u1[: n - 2], u2[: n - 4])
+            LM[j], R[j], MU[j], NU[j] = a, b, c, e
+    return tuple(np.ascontiguousarray(t.T) for t in (LM, R, MU, NU))
+
+
+def rec_from_tdma(s, a, b, w, den):
+    """Galerkin projection (S^T S)^-1 S^T (tdma.f90:56-107) -> (pre, F, LM, MU)."""
+    M = b.shape[0]
+    pre = np.array(s, dtype=float)
+    F = np.empty(M)
+    LM = np.zeros(M)
+    F[:2] = 1.0 / b[:2]
+    F[2:] = 1.0 / den[2:]
+    LM[2:] = -a[: M - 2] / den[2:]
+    MU = np.zeros(M)
+    MU[: M - 2] = -w[: M - 2]
+    return pre, F, LM, MU
+
+
+def rec_from_twodma(d, u):
+    n = d.shape[0]
+    R = 1.0 / d
+    MU = np.zeros(n)
+    MU[: n - 2] = -u[: n - 2] / d[: n - 2]
+    return R, MU
+
+
 class Flags:
     def __init__(self, **kwargs):
         self.flags = dict(**kwargs)
@@ -59,11 +117,14 @@ class MetaPlan:
         return self._flags.flags
 
     def _check_b(self, b):
+        """Shape checks of plans.py:40-51; a leading batch dimension is allowed
+        (axis then counts within the trailing ndim dimensions)."""
         assert isinstance(b, torch.Tensor)
-        assert self.ndim == b.dim(), "Dimensionality mismatch: ndim {:4d} | b.ndim {:4d}. Check ndim.".format(
-            self.ndim, b.dim())
-        assert self.N == b.shape[self.axis], "Shape mismatch. : N {:4d} | b.shape[axis] {:4d}. Check axis.".format(
-            self.N, b.shape[self.axis])
+        assert b.dim() in (self.ndim, self.ndim + 1), (
+            "Dimensionality mismatch: ndim {:4d} | b.ndim {:4d}. Check ndim.".format(self.ndim, b.dim()))
+        n = b.shape[b.dim() - self.ndim + self.axis]
+        assert self.N == n, "Shape mismatch. : N {:4d} | b.shape[axis] {:4d}. Check axis.".format(
+            self.N, n)
 
 
 class PlanRHS(MetaPlan):
@@ -120,13 +181,19 @@ class Plan_fdma(MetaPlan):
         fdma_lu(l, d, u1, u2)
         self.l, self.d, self.u1, self.u2 = l, d, u1, u2
         self.tabs = tuple(ops.to_dev(t) for t in (l, d, u1, u2))
+        LM, R, MU, NU = rec_from_fdma(l, d, u1, u2)
+        self.rec = ops.RecTables(LM=LM, R=R, MU=MU, NU=NU)
 
-    def solve(self, b):
+    def solve(self, b, exact=False):
+        """In-place solve; exact=True runs the reference's sweep statement by
+        statement (bit-identical), the default the register-resident kernel."""
         b = ops.to_dev(b)
         self._check_b(b)
         if self.ndim > 2:
             raise NotImplementedError("Plan_fdma supports only ndim<3.")
-        return ops.fdma_solve(self.tabs, b, self.axis)
+        if exact:
+            return ops.fdma_solve(self.tabs, b, self.axis)
+        return ops.recurrence_solve(self.rec, b, self.axis)
 
 
 class Plan_twodma(MetaPlan):
@@ -136,15 +203,20 @@ class Plan_twodma(MetaPlan):
         MetaPlan.__init__(self, A=A, ndim=ndim, axis=axis)
         self.flags.update({"method": "twodma"})
         A = _dense(A)
-        self.d = ops.to_dev(np.diag(A, 0).copy())
-        self.u = ops.to_dev(np.diag(A, 2).copy())
+        d, u = np.diag(A, 0).copy(), np.diag(A, 2).copy()
+        self.d = ops.to_dev(d)
+        self.u = ops.to_dev(u)
+        R, MU = rec_from_twodma(d, u)
+        self.rec = ops.RecTables(R=R, MU=MU)
 
-    def solve(self, b):
+    def solve(self, b, exact=False):
         b = ops.to_dev(b)
         self._check_b(b)
         if self.ndim > 2:
             raise NotImplementedError("Plan_twodma supports only ndim<3.")
-        return ops.twodma_solve(self.d, self.u, b, self.axis)
+        if exact or b.shape[b.dim() - self.ndim + self.axis] < 3:
+            return ops.twodma_solve(self.d, self.u, b, self.axis)
+        return ops.recurrence_solve(self.rec, b, self.axis)
 
 
 class Plan_numpy(MetaPlan):
@@ -213,14 +285,18 @@ class Plan_Poisson(MetaPlan):
         tab, sing = poisson_line_tables(A, C, self.alpha, singular and axis == 0)
         self.tab = ops.to_dev(tab)
         self.sing = torch.as_tensor(sing, device=ops.device())
+        LM, R, MU, NU = rec_from_poisson(tab, sing)
+        self.rec = ops.RecTables(LM=LM, R=R, MU=MU, NU=NU, per_line=True)
 
-    def solve(self, b):
+    def solve(self, b, exact=False):
         b = ops.to_dev(b)
-        other = b.shape[1 - self.axis]
+        other = b.shape[b.dim() - 1 - self.axis]
         if self.alpha.size != other:
             raise ValueError("Size of eigenvalue {:3} array does not match to b {:3}!".format(
                 self.alpha.size, other))
-        return ops.poisson_solve(self.tab, self.sing, b, self.axis)
+        if exact:
+            return ops.poisson_solve(self.tab, self.sing, b, self.axis)
+        return ops.recurrence_solve(self.rec, b, self.axis)
 
 
 def PlanLHS(A, ndim, axis, method, **kwargs):
