@@ -180,6 +180,15 @@ int pypde_axpby(long long n, double alpha, const double* x, double beta, double*
 int pypde_rbc_products(const double* Y, const double* dtbc, double* P, int L0, int L1,
                        int batch, void* stream);
 
+/* Discrete adjoint of pypde_rbc_products: from the cotangents Pb
+ * [batch, 3, L0, L1] of the three products and the forward blocks Y, write the
+ * cotangents Yb [batch, 3, 2*L0, 2*L1] of the value, d/dy and d/dx blocks.
+ * No reference counterpart: the adjoint of the convective step used by the
+ * gradient of the time step (the reference's navier/rbc2d_adj.py:162-187 forms
+ * a continuous adjoint instead). */
+int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const double* Pb, double* Yb,
+                           int L0, int L1, int batch, void* stream);
+
 /* y[b] -= (coef[b] * x[b]) * c2 over per elements of each batch entry:
  * the pressure update pres -= nu*div*beta of rbc2d.py:390-394, per instance. */
 int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,

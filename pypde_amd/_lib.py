@@ -97,6 +97,7 @@ _SIGS = {
     "pypde_scaled_sub": ([_ll, _i, _vp, _d, _vp, _vp, _vp], _i),
     "pypde_set_entries": ([_vp, _ll, _i, _d, _vp], _i),
     "pypde_recurrence_solve": ([_lp, ctypes.c_void_p, _vp], _i),
+    "pypde_rbc_products_vjp": ([_vp, _vp, _vp, _vp, _i, _i, _i, _vp], _i),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -1,0 +1,272 @@
+"""Discrete adjoint of the fused RBC2D Euler step and gradients of trajectory
+objectives with respect to the initial temperature.
+
+The step of pypde_amd.engine.RBCBatch maps the state s = (X = [U, V, T], pres)
+to s'.  It is linear except for the convective products, so its
+vector-Jacobian product is the reverse sequence of the transposed operators:
+transposed MFMA GEMMs, transposed four-diagonal and Poisson line solves
+(pypde_fdma_solve_t, pypde_poisson_solve with transpose=1) and the adjoint of
+the products (pypde_rbc_products_vjp), for which the two forward GEMMs that
+produce the dealiased velocity and gradient fields are recomputed from the
+step's input state.
+
+Objectives (no reference counterpart; the reference's navier/rbc2d_adj.py:
+321-356 is an adjoint-descent iteration toward steady states, a different
+object, compared by direction in the tests):
+  J_nu  = (1/n) sum_{k=1..n} Nu_plate(T_k)   time-averaged plate Nusselt number
+  J_res = 0.5 || (Phi_dt(X) - X) / dt ||^2    steady-state residual (X = [T, U, V])
+Long horizons use checkpoints every `ckpt` steps and recompute each segment
+during the backward sweep.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import check, lib
+from .engine import AxisOps, RBCBatch, cheb_backward_matrix, cheb_forward_matrix, deriv_matrix, stencil
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class StepAdjoint:
+    """VJP of one fused Euler step of an RBCBatch (all instances at once)."""
+
+    def __init__(self, eng: RBCBatch):
+        self.e = eng
+        e = eng
+        B, N0, N1, M0, M1, L0, L1 = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1
+        dev = e.X.device
+        f64 = dict(dtype=ops.DTYPE, device=dev)
+        self.Xb_buf = torch.zeros((B, 3, M0, M1), **f64)
+        self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
+        self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+        self.Y2b = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+        self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
+
+    def forward_fields(self, X):
+        """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients)."""
+        e = self.e
+        for f in range(3):
+            ops.bgemm(e.GGx[f], X[:, f], out=self.Y1[:, f])
+            ops.bgemm(self.Y1[:, f], e.GGy, transB=True, out=self.Y2[:, f])
+
+    def vjp(self, X, pres, Xb, pb):
+        """Cotangents of the step input (X, pres) from those of its output."""
+        e = self.e
+        B, N0, N1, M0, M1, L0, L1, K = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1, e.K
+        f64 = dict(dtype=ops.DTYPE, device=X.device)
+        g = ops.bgemm
+        # 21  velocity correction: W1b = -A_c0^T Ub, W2b = -A_c1^T Vb
+        NW = N1 + 2 * M1
+        Wb = torch.empty((B, M0, NW), **f64)
+        g(e.Snx, pb, transA=True, alpha_vec=e.t_idt, out=Wb[:, :, :N1])          # 20
+        g(e.Acorr[0], Xb[:, 0], transA=True, alpha=-1.0, out=Wb[:, :, N1:N1 + M1])
+        g(e.Acorr[1], Xb[:, 1], transA=True, alpha=-1.0, out=Wb[:, :, N1 + M1:])
+        divb = torch.zeros((B, N0, N1), **f64)
+        check(lib.pypde_scaled_sub(N0 * N1, B, _vp(e.t_nu), e.beta, _vp(pb), _vp(divb),
+                                   ops.stream_ptr()), "pres_div_vjp")
+        # 18  P^ = W^ Wy
+        Pb = g(Wb, e.Wy)
+        # 17  gauge
+        check(lib.pypde_set_entries(_vp(Pb), M0 * M1, B, 0.0, ops.stream_ptr()), "gauge_vjp")
+        # 16  XP^ = P^ Qy ; 15 transposed Poisson line solves ; 14 Q1^ = RP^ Hy
+        XPb = g(Pb, e.Qy)
+        ops.poisson_solve(e.ptab, e.psing, XPb, 0, transpose=True)
+        Q1b = g(XPb, e.Hy)
+        # 13  div^ += Bx^T Q1^
+        g(e.Bx, Q1b, transA=True, beta=1.0, D=divb, out=divb)
+        # 12  Wd^ = Cx^T div^   [B, 2M0, N1]
+        Wdb = g(e.Cx, divb, transA=True)
+        # 11  U*^ = Ub + Wd^_U C_y, V*^ = Vb + Wd^_V Cd_y, T*^ = Tb
+        Rb = torch.empty((B, 3, M0, M1), **f64)
+        g(Wdb[:, :M0], e.Cy[0], beta=1.0, D=Xb[:, 0], out=Rb[:, 0])
+        g(Wdb[:, M0:], e.Cy[1], beta=1.0, D=Xb[:, 1], out=Rb[:, 1])
+        Rb[:, 2].copy_(Xb[:, 2])
+        # 9-10  transposed Helmholtz ADI: R^ = Hx^-T X*^ Hy^-1
+        # tables [B, 3, (l | d | u1 | u2), M] with entry stride 4M; the views
+        # below only provide the base pointer of each slot
+        hx, hy = e.hx, e.hy
+        R3 = Rb.reshape(3 * B, M0, M1)
+        ops.fdma_solve((hy[:, :, 0], hy[:, :, 1], hy[:, :, 2], hy[:, :, 3]), R3, axis=1,
+                       transpose=True, table_bs=4 * M1)
+        ops.fdma_solve((hx[:, :, 0], hx[:, :, 1], hx[:, :, 2], hx[:, :, 3]), R3, axis=0,
+                       transpose=True, table_bs=4 * M0)
+        # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt BST_x^T R^_V
+        Zb = torch.empty((B, 3, K, M1), **f64)
+        for f in range(3):
+            g(e.Arhs[f], Rb[:, f], transA=True, out=Zb[:, f])
+        zt = Zb[:, 2, L0 + N0:]
+        g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.t_dt, beta=1.0, D=zt, out=zt)
+        # 6  X^ (input) = Z^_f[2] BS_y
+        Xin = torch.empty((B, 3, M0, M1), **f64)
+        for f in range(3):
+            g(Zb[:, f, L0 + N0:], e.BSy, out=Xin[:, f])
+        # 5  pres^ (input) = pres^ - dt (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
+        pin = pb.clone()
+        g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
+        g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
+        # 4  P^r_f = -dt Z^_f[0] BF_y
+        Prb = torch.empty((B, 3, L0, L1), **f64)
+        for f in range(3):
+            g(Zb[:, f, :L0], e.BFy, alpha_vec=e.t_mdt, out=Prb[:, f])
+        # 3  products
+        self.forward_fields(X)
+        check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb), _vp(self.Y2b),
+                                         L0, L1, B, ops.stream_ptr()), "products_vjp")
+        # 2  Y1^ = Y2^ GGy ; 1 X^ += GG_f^T Y1^
+        for f in range(3):
+            y1b = g(self.Y2b[:, f], e.GGy)
+            g(e.GGx[f], y1b, transA=True, beta=1.0, D=Xin[:, f], out=Xin[:, f])
+        return Xin, pin
+
+
+class NusseltFunctional:
+    """Plate Nusselt number as an affine functional of the temperature
+    coefficients: Nu(T) = c0 + a^T T b (rbc2d_base.py:344-363 is linear in T)."""
+
+    def __init__(self, eng: RBCBatch):
+        from .bases import Base
+        e = eng
+        N0, N1 = e.N0, e.N1
+        Lz = 1.0  # y spans [-0.5, 0.5]
+        x = Base(N0, "CH").x
+        xm = np.zeros(N0 + 1)
+        xm[0], xm[-1] = x[0], x[-1]
+        xm[1:-1] = (x[1:] + x[:-1]) / 2.0
+        dx = np.diff(xm)
+        w = dx / dx.sum()
+        Gx = cheb_backward_matrix(N0, N0) @ stencil(e.side, N0)     # T coefficients -> grid (x)
+        Gy = cheb_backward_matrix(N1, N1) @ stencil("CD", N1)
+        Dy = cheb_backward_matrix(N1, N1) @ deriv_matrix(N1) @ cheb_forward_matrix(N1, N1)
+        ev = np.zeros(N1)
+        ev[0] = ev[-1] = -0.5 * Lz
+        scale = Lz / 2.0
+        self.a = w @ Gx                                   # [M0]
+        self.b = (ev @ (Dy @ Gy)) / scale                 # [M1]
+        # constant part from the lifted boundary field Tbc
+        tbc = e.Tbc.v.detach().cpu().numpy()
+        self.c0 = float(w @ (tbc @ (Dy.T @ ev)) / scale)
+        self.ta = torch.as_tensor(self.a, dtype=ops.DTYPE, device=e.X.device)
+        self.tb = torch.as_tensor(self.b, dtype=ops.DTYPE, device=e.X.device)
+        self.gmat = torch.outer(self.ta, self.tb)         # dNu/dT  [M0, M1]
+
+    def value(self, T):
+        """Nu for T [..., M0, M1] (one GEMM pair per call)."""
+        tb = ops.bgemm(T if T.dim() == 3 else T.unsqueeze(0), self.tb.reshape(-1, 1))
+        v = ops.bgemm(self.ta.reshape(1, -1), tb)
+        return v.reshape(-1) + self.c0
+
+
+class Trajectory:
+    """Forward trajectory with checkpoints and its reverse (adjoint) sweep."""
+
+    def __init__(self, eng: RBCBatch, ckpt=100):
+        self.e = eng
+        self.adj = StepAdjoint(eng)
+        self.nu = NusseltFunctional(eng)
+        self.ckpt = ckpt
+
+    def _state(self):
+        return self.e.X.clone(), self.e.pres.clone()
+
+    def _set(self, X, pres):
+        self.e.X.copy_(X)
+        self.e.pres.copy_(pres)
+
+    def run(self, X0, pres0, nsteps):
+        """J_nu and the checkpoints; the engine state ends at step n."""
+        self._set(X0, pres0)
+        cks = {0: self._state()}
+        acc = torch.zeros(self.e.B, dtype=ops.DTYPE, device=X0.device)
+        for k in range(1, nsteps + 1):
+            self.e.step_eager()
+            acc += self.nu.value(self.e.X[:, 2])
+            if k % self.ckpt == 0 and k < nsteps:
+                cks[k] = self._state()
+        return acc / nsteps, cks
+
+    def grad_nusselt(self, X0, pres0, nsteps):
+        """dJ_nu / d(X0, pres0) by the reverse sweep over checkpointed segments."""
+        J, cks = self.run(X0, pres0, nsteps)
+        B = self.e.B
+        Xb = torch.zeros_like(X0)
+        pb = torch.zeros_like(pres0)
+        g = self.nu.gmat / nsteps
+        starts = sorted(cks)
+        for si in range(len(starts) - 1, -1, -1):
+            k0 = starts[si]
+            k1 = starts[si + 1] if si + 1 < len(starts) else nsteps
+            # recompute the segment's inputs
+            self._set(*cks[k0])
+            states = []
+            for k in range(k0, k1):
+                states.append(self._state())
+                self.e.step_eager()
+            for k in range(k1, k0, -1):
+                # Nu(T_k) enters J for k >= 1
+                Xb[:, 2] += g
+                X, pres = states[k - 1 - k0]
+                Xb, pb = self.adj.vjp(X, pres, Xb, pb)
+        return J, Xb, pb
+
+
+class TimeAveragedNusselt(torch.autograd.Function):
+    """J = (1/n) sum_k Nu_plate(T_k) over n fused Euler steps from the initial
+    temperature coefficients T0 (velocities and pressure fixed inputs);
+    backward runs the checkpointed adjoint sweep."""
+
+    @staticmethod
+    def forward(ctx, T0, traj, X0, pres0, nsteps):
+        X = X0.clone()
+        X[:, 2] = T0
+        J, _ = traj.run(X, pres0, nsteps)
+        ctx.traj, ctx.nsteps = traj, nsteps
+        ctx.save_for_backward(X, pres0)
+        return J
+
+    @staticmethod
+    def backward(ctx, gJ):
+        X, pres0 = ctx.saved_tensors
+        _, Xb, _ = ctx.traj.grad_nusselt(X, pres0, ctx.nsteps)
+        return Xb[:, 2] * gJ.reshape(-1, 1, 1), None, None, None, None
+
+
+class RBCStep(torch.autograd.Function):
+    """One fused Euler step (X, pres) -> (X', pres') with its hand-written VJP."""
+
+    @staticmethod
+    def forward(ctx, X, pres, adj):
+        e = adj.e
+        e.X.copy_(X)
+        e.pres.copy_(pres)
+        e.step_eager()
+        ctx.adj = adj
+        ctx.save_for_backward(X, pres)
+        return e.X.clone(), e.pres.clone()
+
+    @staticmethod
+    def backward(ctx, Xb, pb):
+        X, pres = ctx.saved_tensors
+        Xb = torch.zeros_like(X) if Xb is None else Xb.contiguous()
+        pb = torch.zeros_like(pres) if pb is None else pb.contiguous()
+        Xin, pin = ctx.adj.vjp(X, pres, Xb, pb)
+        return Xin, pin, None
+
+
+def steady_residual_grad(eng: RBCBatch, X, pres, adj=None):
+    """J_res = 0.5 ||(Phi(X) - X)/dt||^2 over X = [U, V, T] with pres held
+    fixed (rbc2d_base.py:321-341 restated), and dJ/dX."""
+    adj = adj or StepAdjoint(eng)
+    eng.X.copy_(X)
+    eng.pres.copy_(pres)
+    eng.step_eager()
+    dt = torch.as_tensor(eng.dt, dtype=ops.DTYPE, device=X.device).reshape(-1, 1, 1, 1)
+    r = (eng.X - X) / dt
+    J = 0.5 * (r * r).sum(dim=(1, 2, 3))
+    rb = r / dt
+    Xin, _ = adj.vjp(X, pres, rb.contiguous(), torch.zeros_like(pres))
+    return J, Xin - rb

@@ -48,49 +48,60 @@ struct BandSpec {
     int stencil;         // 1: stencil form y_i = s_{i-2} x_{i-2} + x_i (vals = s)
 };
 
-__global__ void __launch_bounds__(256) banded_kernel(LineGeom g, BandSpec bs)
+// Output element r of a line from its input window (ascending columns, zero
+// entries skipped as the reference's csr storage does).
+template <class Get>
+__device__ __forceinline__ double band_row(const BandSpec& bs, int r, int len_in, int len_out,
+                                           const Get& X)
 {
-    // thread -> (entry b, output index r, line index c)
-    const long long per = (long long)g.len_out * g.nlines;
-    const long long total = per * g.batch;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int b = (int)(t / per);
-        const long long rem = t - (long long)b * per;
-        int r, c;
-        if (g.axis == 0) { r = (int)(rem / g.nlines); c = (int)(rem - (long long)r * g.nlines); }
-        else             { c = (int)(rem / g.len_out); r = (int)(rem - (long long)c * g.len_out); }
-        const double* src = g.src + (long long)b * g.src_bs;
-        auto X = [&](int k) -> double {
-            return g.axis == 0 ? src[(long long)k * g.src_ld + c] : src[(long long)c * g.src_ld + k];
-        };
-        double y;
-        if (bs.stencil) {
-            // u_i = s_{i-2} v_{i-2} + v_i (csc column order of S); M = len_in
-            if (r >= 2) {
-                const double tt = bs.vals[r - 2] * X(r - 2);
-                y = (r < g.len_in) ? tt + X(r) : tt;
-            } else {
-                y = X(r);
-            }
-        } else {
-            double acc = 0.0;
-            bool first = true;
-            for (int q = 0; q < bs.nd; ++q) {
-                const int k = r + bs.off[q];
-                if (k < 0 || k >= g.len_in) continue;
-                const double a = bs.vals[(long long)q * g.len_out + r];
-                if (a == 0.0) continue;
-                const double tt = a * X(k);
-                acc = first ? tt : acc + tt;
-                first = false;
-            }
-            y = acc;
+    if (bs.stencil) {
+        if (r >= 2) {
+            const double tt = bs.vals[r - 2] * X(-2);
+            return (r < len_in) ? tt + X(0) : tt;
         }
-        double* dst = g.dst + (long long)b * g.dst_bs;
-        if (g.axis == 0) dst[(long long)r * g.dst_ld + c] = y;
-        else dst[(long long)c * g.dst_ld + r] = y;
+        return X(0);
     }
+    double acc = 0.0;
+    bool first = true;
+    for (int q = 0; q < bs.nd; ++q) {
+        const int k = r + bs.off[q];
+        if (k < 0 || k >= len_in) continue;
+        const double a = bs.vals[(long long)q * len_out + r];
+        if (a == 0.0) continue;
+        const double tt = a * X(bs.off[q]);
+        acc = first ? tt : acc + tt;
+        first = false;
+    }
+    return acc;
+}
+
+// Axis 0: one thread per output element; grid.x over columns (neighbouring
+// threads read neighbouring columns), grid.y over output rows, grid.z over
+// batch entries.  Rows shared by neighbouring outputs are served from cache.
+__global__ void __launch_bounds__(256) banded_axis0_kernel(LineGeom g, BandSpec bs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    const int b = blockIdx.z;
+    if (c >= g.nlines) return;
+    const double* __restrict__ col = g.src + (long long)b * g.src_bs + c;
+    const long long ls = g.src_ld;
+    const double y = band_row(bs, r, g.len_in, g.len_out,
+                              [&](int o) { return col[(long long)(r + o) * ls]; });
+    g.dst[(long long)b * g.dst_bs + (long long)r * g.dst_ld + c] = y;
+}
+
+// Axis 1: one thread per output element of a row; grid.x over the row,
+// grid.y over rows, grid.z over batch entries.
+__global__ void __launch_bounds__(256) banded_axis1_kernel(LineGeom g, BandSpec bs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b = blockIdx.z;
+    if (r >= g.len_out) return;
+    const double* __restrict__ row = g.src + (long long)b * g.src_bs + (long long)c * g.src_ld;
+    const double y = band_row(bs, r, g.len_in, g.len_out, [&](int o) { return row[r + o]; });
+    g.dst[(long long)b * g.dst_bs + (long long)c * g.dst_ld + r] = y;
 }
 
 // ---------------------------------------------------------------------------
@@ -466,11 +477,16 @@ static int launch_lines(LineGeom g, const Op& op, int ntab, int hint, void* stre
 
 static int launch_banded(const LineGeom& g, const BandSpec& bs, void* stream, const char* name)
 {
-    const long long total = (long long)g.len_out * g.nlines * g.batch;
-    if (total == 0) return OK;
-    long long blocks = (total + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(banded_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), g, bs);
+    if ((long long)g.len_out * g.nlines * g.batch == 0) return OK;
+    if (g.axis == 0) {
+        PYPDE_REQUIRE(g.len_out <= 65535 && g.batch <= 65535, "%s: grid too large", name);
+        dim3 grid((g.nlines + 255) / 256, g.len_out, g.batch);
+        hipLaunchKernelGGL(banded_axis0_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
+    } else {
+        PYPDE_REQUIRE(g.nlines <= 65535 && g.batch <= 65535, "%s: grid too large", name);
+        dim3 grid((g.len_out + 255) / 256, g.nlines, g.batch);
+        hipLaunchKernelGGL(banded_axis1_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
+    }
     return check_launch(name);
 }
 

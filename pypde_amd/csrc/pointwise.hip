@@ -39,6 +39,46 @@ __global__ void rbc_products_kernel(const double* __restrict__ Y, const double* 
     }
 }
 
+// Adjoint of rbc_products_kernel: given the cotangents Pb of the three
+// products, the cotangents of the value / d/dy / d/dx blocks of Y (the
+// d2/dxdy blocks and the temperature value block receive zero).
+__global__ void rbc_products_vjp_kernel(const double* __restrict__ Y, const double* __restrict__ dtbc,
+                                        const double* __restrict__ Pb, double* __restrict__ Yb,
+                                        int L0, int L1, int B)
+{
+    const long long per = (long long)L0 * L1;
+    const long long ld = 2LL * L1;
+    const long long fs = 4LL * per;
+    const long long total = per * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long b = t / per;
+        const long long r = t - b * per;
+        const int i = (int)(r / L1), j = (int)(r - (long long)i * L1);
+        const double* Yb_in = Y + b * 3 * fs;
+        double* Yo = Yb + b * 3 * fs;
+        const long long v = (long long)i * ld + j;
+        const long long dy = v + L1;
+        const long long dx = (long long)(i + L0) * ld + j;
+        const long long dxy = dx + L1;
+        const double ux = Yb_in[v], uz = Yb_in[fs + v];
+        const double* Pbb = Pb + b * 3 * per;
+        const double pu = Pbb[r], pv = Pbb[per + r], pt = Pbb[2 * per + r];
+        Yo[dx] = pu * ux;
+        Yo[dy] = pu * uz;
+        Yo[fs + dx] = pv * ux;
+        Yo[fs + dy] = pv * uz;
+        Yo[2 * fs + dx] = pt * ux;
+        Yo[2 * fs + dy] = pt * uz;
+        Yo[v] = pu * Yb_in[dx] + pv * Yb_in[fs + dx] + pt * Yb_in[2 * fs + dx];
+        Yo[fs + v] = pu * Yb_in[dy] + pv * Yb_in[fs + dy] + pt * (Yb_in[2 * fs + dy] + dtbc[r]);
+        Yo[2 * fs + v] = 0.0;
+        Yo[dxy] = 0.0;
+        Yo[fs + dxy] = 0.0;
+        Yo[2 * fs + dxy] = 0.0;
+    }
+}
+
 // y[b] = y[b] - (coef[b] * x[b]) * c2   (pres -= nu*div*beta, rbc2d.py:391)
 __global__ void scaled_sub_kernel(long long per, int B, const double* __restrict__ coef, double c2,
                                   const double* __restrict__ x, double* __restrict__ y)
@@ -78,6 +118,17 @@ extern "C" int pypde_rbc_products(const double* Y, const double* dtbc, double* P
     hipLaunchKernelGGL(rbc_products_kernel, dim3(grid_for((long long)L0 * L1 * batch)), dim3(256), 0,
                        as_stream(stream), Y, dtbc, P, L0, L1, batch);
     return check_launch("pypde_rbc_products");
+}
+
+extern "C" int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const double* Pb,
+                                      double* Yb, int L0, int L1, int batch, void* stream)
+{
+    PYPDE_REQUIRE(Y && dtbc && Pb && Yb, "null pointer");
+    PYPDE_REQUIRE(L0 > 0 && L1 > 0 && batch >= 0, "invalid product geometry");
+    if (batch == 0) return OK;
+    hipLaunchKernelGGL(rbc_products_vjp_kernel, dim3(grid_for((long long)L0 * L1 * batch)), dim3(256),
+                       0, as_stream(stream), Y, dtbc, Pb, Yb, L0, L1, batch);
+    return check_launch("pypde_rbc_products_vjp");
 }
 
 extern "C" int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,

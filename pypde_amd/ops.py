@@ -141,6 +141,55 @@ def gemm(A, B, transA=False, transB=False, alpha=1.0, beta=0.0, D=None, out=None
     return out
 
 
+def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0, D=None,
+          out=None):
+    """Batched GEMM over a leading batch dimension with broadcasting of 2-D
+    operands (one launch of pypde_dgemm_batched); alpha_vec [batch] scales
+    entry e by alpha * alpha_vec[e]."""
+    from ._lib import BIdx, GemmDesc
+
+    def geo(T):
+        if T.dim() == 2:
+            return T, 1, 0
+        return T, T.shape[0], T.stride(0)
+
+    A3, ba, sa = geo(A)
+    B3, bb, sb = geo(B)
+    am, ak = (A3.shape[-1], A3.shape[-2]) if transA else (A3.shape[-2], A3.shape[-1])
+    bk, bn = (B3.shape[-1], B3.shape[-2]) if transB else (B3.shape[-2], B3.shape[-1])
+    if ak != bk:
+        raise AssertionError("GEMM inner dimensions differ: %d vs %d" % (ak, bk))
+    if A3.stride(-1) != 1 or B3.stride(-1) != 1:
+        raise ValueError("operands need contiguous rows")
+    batch = max(ba, bb)
+    if out is None:
+        out = torch.empty((batch, am, bn) if batch > 1 or A.dim() == 3 or B.dim() == 3 else (am, bn),
+                          dtype=DTYPE, device=A.device)
+    o3, bo, so = geo(out)
+    d = GemmDesc()
+    d.transA, d.transB, d.m, d.n, d.k, d.batch = int(transA), int(transB), am, bn, ak, batch
+    d.alpha, d.beta = float(alpha), float(beta)
+    d.A, d.B, d.C = A3.data_ptr(), B3.data_ptr(), o3.data_ptr()
+    d.lda, d.ldb, d.ldc = A3.stride(-2), B3.stride(-2), o3.stride(-2)
+
+    def bi(s):
+        b = BIdx()
+        b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s, 1, 0, 0, 1, 0
+        return b
+    d.iA, d.iB, d.iC = bi(sa if ba > 1 else 0), bi(sb if bb > 1 else 0), bi(so)
+    if D is not None:
+        D3, bd, sd = geo(D)
+        d.D, d.ldd, d.iD = D3.data_ptr(), D3.stride(-2), bi(sd if bd > 1 else 0)
+    else:
+        d.D, d.ldd, d.iD = None, 0, bi(0)
+    if alpha_vec is not None:
+        d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), 1
+    else:
+        d.alpha_vec, d.alpha_div = None, 1
+    check(lib.pypde_dgemm_batched(ctypes.byref(d), stream_ptr()), "pypde_dgemm_batched")
+    return out
+
+
 # ---------------------------------------------------------------------------
 def stencil_apply(s, x, axis=0, lpb=0):
     """u = S v along axis (Galerkin M -> Chebyshev M+2)."""
