@@ -2,6 +2,8 @@
 """Benchmark: RBC2D FP64 time steps per second at 256x256 on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+    python bench.py --micro              # microbench only (one JSON line)
+    python bench.py --copy-calibration   # 256 MiB device copies, for scaling PMC byte counters
 
 Workload (value): the plain Euler IMEX step of navier.rbc2d.NavierStokes at
 256x256, rbc case, Ra=1e8, Pr=1, dt=1e-3, 3/2 dealiasing, seeded
@@ -15,6 +17,10 @@ Also reported on the same line:
                  batched step: aggregate instance-steps/s and the ratio to the plain path
   microbench     512 fields of 256x256: transforms, ChebDirichlet stencil maps,
                  Helmholtz right-hand side and four-diagonal solves, per-kernel HBM fractions
+  adjoint        gradient of the time-averaged plate Nusselt number with respect
+                 to the initial temperature (checkpointed reverse sweep): 200 steps
+                 at 256x256 and 2000 steps at 128x128, Ra=5e4, dt=5e-3, checkpoints
+                 every 100 steps; reported as trajectory steps/s of the whole gradient
   cpu_baseline   the oracle (CPU restatement of the reference) on the host, one
                  thread, on a bounded sample of the same 256x256 workload
 """
@@ -36,6 +42,26 @@ FP64_PEAK_TFLOPS = 78.6
 def seeded_noise(shape, seed=0, amplitude=1e-3):
     rng = np.random.default_rng(seed)
     return rng.uniform(-amplitude, amplitude, size=shape)
+
+
+# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over `python3 bench.py --steps 20 --warmup 2 --no-cpu
+# --no-extra`, each divided by what the same counter reads for a device copy of
+# known size (`python3 bench.py --copy-calibration`); the profile summaries
+# under profiles/ give the commands and raw counter values.
+TRAFFIC_BYTES = {}
+
+
+def copy_calibration(mib=256, reps=5):
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": 8 * n, "bytes_written": 8 * n,
+                                           "launches": reps, "kernel": "torch.mul(x, 1.0, out=y)"}}))
 
 
 def launch_cost(name, launch, eng):
@@ -197,6 +223,32 @@ def microbench(nf=512, reps=3):
                 chain_hbm_frac=round(total_bytes / (total_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
 
 
+def time_adjoint(N, nsteps, ckpt=100, ra=5e4, dt=5e-3):
+    """Whole gradient dJ/dT0 of J = mean_k Nu_plate(T_k) over nsteps: forward run
+    with checkpoints, segment recomputation and the reverse VJP sweep."""
+    import torch
+    from pypde_amd.adjoint import Trajectory
+    from pypde_amd.engine import RBCBatch
+    from pypde_amd.field import Field
+    from pypde_amd.bases import Base
+    eng = RBCBatch((N, N), [dict(ra=ra, pr=1.0, dt=dt)], case="rbc")
+    Tf = Field([Base(N, "CN"), Base(N, "CD")])
+    X0 = torch.zeros_like(eng.X)
+    X0[0, 2].copy_(Tf.forward(seeded_noise((N, N), seed=0)))
+    pres0 = torch.zeros_like(eng.pres)
+    traj = Trajectory(eng, ckpt=ckpt)
+    traj.grad_nusselt(X0, pres0, min(nsteps, 2 * ckpt))  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, Xb, pb = traj.grad_nusselt(X0, pres0, nsteps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return dict(N=N, steps=nsteps, checkpoint_every=ckpt, ra=ra, dt=dt, sec=round(sec, 3),
+                steps_per_s=round(nsteps / sec, 2), J=float(J[0]),
+                grad_norm=float(torch.linalg.norm(Xb[0, 2])),
+                finite=bool(torch.isfinite(Xb).all().item()))
+
+
 def cpu_baseline(steps=6, warm=1):
     """Oracle (CPU restatement of the reference, scipy DCT-I + C ports of the
     Fortran kernels, per-call dense Poisson formation as the reference) at
@@ -226,6 +278,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--micro", action="store_true", help="run the microbench only")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (got --gpus %d); multi-GPU runs are "
@@ -235,12 +289,19 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.micro:
+        print(json.dumps({"microbench": microbench()}))
+        return
     plain = time_plain(args)
     extra = {}
     if not args.no_extra:
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
         extra["microbench"] = microbench()
+        extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(steps=args.cpu_steps)
@@ -262,7 +323,8 @@ def main():
                    "init": "conductive + uniform noise 1e-3, seed 0", "graph_replay": True},
         "roofline": {"bound": plain["bound"], "achieved": round(plain["achieved"], 3),
                      "peak": plain["peak"], "unit": plain["unit"],
-                     "frac": round(plain["achieved"] / plain["peak"], 5), "traffic": None,
+                     "frac": round(plain["achieved"] / plain["peak"], 5),
+                     "traffic": TRAFFIC_BYTES.get(plain["dominant"]),
                      "kernel": plain["dominant"], "avg_us": round(plain["avg_s"] * 1e6, 3)},
         "cpu_baseline": cpu,
         "kernels_us": plain["kernels"],
@@ -270,8 +332,8 @@ def main():
         "nu_final": plain["nu"],
         "finite": plain["finite"],
     }
-    line.update(extra)
     line["adjoint"] = None
+    line.update(extra)
     print(json.dumps(line))
 
 

@@ -69,6 +69,9 @@ typedef struct pypde_gemm_desc {
     pypde_bidx iA, iB, iD, iC;
     const double* alpha_vec; /* NULL, or per-entry scale alpha_vec[e / alpha_div] */
     int alpha_div;
+    int variant; /* kernel choice: 0 size heuristic; 1 LDS-tiled 64x64; 2 LDS-tiled
+                    32x32; 3 LDS-tiled 128x128; 4 the same, double-buffered; 16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)
+                    + 4 (KW == 8) for a (16 TM) x (16 TN) tile over KW waves */
 } pypde_gemm_desc;
 
 /* Batched GEMM  C[e] = alpha_e * op(A[e]) @ op(B[e]) + beta * D[e]  with the

@@ -1,0 +1,103 @@
+"""Kernel-schedule autotuner.
+
+Each GEMM shape (transposes, m, n, k, batch, leading dimensions) is timed once
+per process over the kernel variants of pypde_dgemm_batched (LDS-tiled
+128x128, 64x64 and 32x32, split-K register-direct tiles of 16/32 rows and columns over 4 or 8
+waves); the fastest variant is cached and written into the descriptor that the
+fused step replays.  Recurrence (banded line) launches are tuned the same way
+over the wave count per workgroup.
+
+Timing uses HIP events on the launch stream, `reps` back-to-back launches per
+candidate after one warm launch.  Nothing is tuned while a stream is being
+captured into a graph: captured launches use the cached choice or the size
+heuristic of the library (variant 0).
+
+PYPDE_AUTOTUNE=0 in the environment switches tuning off (heuristics only).
+"""
+import os
+
+import torch
+
+GEMM_CANDIDATES = (1, 2, 3, 4, 16, 17, 18, 19, 20, 21, 22, 23)
+
+_enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
+_cache = {}
+_timings = {}
+
+
+def enabled():
+    return _enabled
+
+
+def set_enabled(flag):
+    global _enabled
+    _enabled = bool(flag)
+
+
+def cache():
+    """{key: chosen variant} of this process."""
+    return dict(_cache)
+
+
+def timings():
+    """{key: {variant: microseconds per launch}} of this process."""
+    return {k: dict(v) for k, v in _timings.items()}
+
+
+def _capturing():
+    try:
+        return torch.cuda.is_current_stream_capturing()
+    except Exception:
+        return False
+
+
+def time_candidates(run, candidates, reps=5):
+    """{candidate: us per launch}; run(c) enqueues one launch on the current stream."""
+    out = {}
+    for c in candidates:
+        run(c)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run(c)
+        e1.record()
+        e1.synchronize()
+        out[c] = e0.elapsed_time(e1) * 1e3 / reps
+    return out
+
+
+def choose(key, run, candidates, default=0):
+    """Cached best candidate for `key`, timing the candidates on first use."""
+    v = _cache.get(key)
+    if v is not None:
+        return v
+    if not _enabled or _capturing():
+        return default
+    t = time_candidates(run, candidates)
+    v = min(t, key=t.get)
+    _cache[key] = v
+    _timings[key] = t
+    return v
+
+
+def gemm_key(d):
+    return ("gemm", d.transA, d.transB, d.m, d.n, d.k, d.batch, d.lda, d.ldb, d.ldc,
+            d.D is not None and d.beta != 0.0)
+
+
+def tune_gemm_desc(d, launch, allow_alias=False):
+    """Fill d.variant; launch(d) enqueues the GEMM with the descriptor's variant.
+    A descriptor whose output is also an input (C aliases A, B or D) is not
+    timed unless the caller restores the output afterwards (allow_alias),
+    since repeated launches change its result; it keeps the size heuristic."""
+    aliased = d.C in (d.A, d.B) or (d.D is not None and d.D == d.C and d.beta != 0.0)
+    if aliased and not allow_alias:
+        d.variant = _cache.get(gemm_key(d), 0)
+        return d.variant
+
+    def run(v):
+        d.variant = v
+        launch(d)
+    d.variant = choose(gemm_key(d), run, GEMM_CANDIDATES)
+    return d.variant

@@ -7,11 +7,29 @@
 // derivative, padding and truncation folded into the matrix), the Poisson
 // eigen-mixing Hy / Qy, and the pseudo-inverse right-hand sides.
 //
-// Block tile BM x BN (64x64 or 32x32), BK = 16, 256 threads = 4 wave64s in a
-// 2x2 arrangement; each wave owns a (BM/2)x(BN/2) sub-tile made of 16x16 MFMA
-// tiles.  Tiles of A and B are staged through LDS (k-major, one double of
-// padding per row), the next K-slab is prefetched into registers while the
-// current one feeds the MFMAs.
+// Two kernel families:
+//
+//  * LDS-tiled (throughput): block tile BM x BN (128x128 over 8 wave64s in a
+//    4x2 arrangement, or 64x64 / 32x32 over 4 waves in 2x2), BK = 16; each
+//    wave owns a (BM/WGM)x(BN/WGN) sub-tile made of 16x16 MFMA tiles.  Tiles
+//    of A and B are staged through LDS (k-major, one double of padding per
+//    row), the next K-slab is prefetched into registers while the current one
+//    feeds the MFMAs; the double-buffered form stores it into the other LDS
+//    buffer and needs one barrier per slab.  128x128 tiles reach about 70% of
+//    the FP64 MFMA peak on batched 256-point transforms, 64x64 about 55%.
+//
+//  * Split-K, register-direct (latency): one workgroup of KW waves owns a
+//    (16 TM) x (16 TN) output tile.  The K range is cut into 4 KW contiguous
+//    chunks, one per (wave, lane group of 16): the f64 MFMA sums over its four
+//    lane groups, so giving each lane group its own chunk makes every MFMA step
+//    consume four chunks at once.  Operands are loaded straight into registers
+//    (no LDS staging, no barriers in the K loop); the KW partial tiles are
+//    reduced through LDS in the epilogue.  Single-instance 256x256 operators
+//    give 64-200 32x32 tiles -- too few for 256 CUs -- and this variant puts
+//    4-16x more waves on the same problem with a K loop 4 KW times shorter.
+//
+// The variant is chosen per call (pypde_gemm_desc.variant, filled by the
+// host-side autotuner) or by a size heuristic when the variant is 0.
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
@@ -48,25 +66,30 @@ struct GemmArgs {
     int alpha_div;
 };
 
-template <int BM, int BN, bool TA, bool TB>
-__global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
+template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 {
     constexpr int BK = 16;
+    constexpr int NT = 64 * WGM * WGN;  // threads
     constexpr int PA = BM + 1;
     constexpr int PB = BN + 1;
-    constexpr int WM = BM / 2;  // wave tile rows
-    constexpr int WN = BN / 2;
-    constexpr int TM = WM / 16; // MFMA tiles per wave (rows)
+    constexpr int WM = BM / WGM;  // wave tile rows
+    constexpr int WN = BN / WGN;
+    constexpr int TM = WM / 16;   // MFMA tiles per wave (rows)
     constexpr int TN = WN / 16;
-    constexpr int LA = BM * BK / 256;  // A elements loaded per thread per slab
-    constexpr int LB = BN * BK / 256;
-    __shared__ double As[BK * PA];
-    __shared__ double Bs[BK * PB];
+    constexpr int LA = BM * BK / NT;  // A elements loaded per thread per slab
+    constexpr int LB = BN * BK / NT;
+    constexpr int NBUF = DBUF ? 2 : 1;
+    static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+    static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "tile / thread mismatch");
+    static_assert(NT % BK == 0 && NT % BM == 0 && NT % BN == 0, "load mapping");
+    __shared__ double As[NBUF][BK * PA];
+    __shared__ double Bs[NBUF][BK * PB];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WGN, wn = wave % WGN;
     const int bz = blockIdx.z;
     const int m0 = blockIdx.y * BM;
     const int n0 = blockIdx.x * BN;
@@ -82,12 +105,21 @@ __global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
 
     double ra[LA], rb[LB];
 
+    // (r, c) of the q-th element a thread moves: consecutive threads walk the
+    // contiguous dimension of the operand in memory.
+    auto a_rc = [&](int q, int& r, int& c) {  // r: row of op(A) in tile, c: k in slab
+        if (!TA) { c = tid % BK; r = tid / BK + q * (NT / BK); }
+        else     { r = tid % BM; c = tid / BM + q * (NT / BM); }
+    };
+    auto b_rc = [&](int q, int& r, int& c) {  // r: k in slab, c: column of op(B) in tile
+        if (!TB) { c = tid % BN; r = tid / BN + q * (NT / BN); }
+        else     { r = tid % BK; c = tid / BK + q * (NT / BK); }
+    };
     auto load_slab = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
-            int r, c;  // r: row of op(A) within tile (m), c: k within slab
-            if (!TA) { c = tid % BK; r = tid / BK + q * (256 / BK); }
-            else     { r = tid % BM; c = tid / BM + q * (256 / BM); }
+            int r, c;
+            a_rc(q, r, c);
             const int gm = m0 + r, gk = k0 + c;
             double v = 0.0;
             if (gm < g.m && gk < g.k) v = TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + gk];
@@ -95,51 +127,67 @@ __global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
-            int r, c;  // r: k within slab, c: column of op(B) within tile (n)
-            if (!TB) { c = tid % BN; r = tid / BN + q * (256 / BN); }
-            else     { r = tid % BK; c = tid / BK + q * (256 / BK); }
+            int r, c;
+            b_rc(q, r, c);
             const int gk = k0 + r, gn = n0 + c;
             double v = 0.0;
             if (gk < g.k && gn < g.n) v = TB ? B[(long long)gn * g.ldb + gk] : B[(long long)gk * g.ldb + gn];
             rb[q] = v;
         }
     };
-    auto store_slab = [&]() {
+    auto store_slab = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
-            if (!TA) { c = tid % BK; r = tid / BK + q * (256 / BK); }
-            else     { r = tid % BM; c = tid / BM + q * (256 / BM); }
-            As[c * PA + r] = ra[q];
+            a_rc(q, r, c);
+            As[buf][c * PA + r] = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             int r, c;
-            if (!TB) { c = tid % BN; r = tid / BN + q * (256 / BN); }
-            else     { r = tid % BK; c = tid / BK + q * (256 / BK); }
-            Bs[r * PB + c] = rb[q];
+            b_rc(q, r, c);
+            Bs[buf][r * PB + c] = rb[q];
         }
     };
-
-    load_slab(0);
-    for (int k0 = 0; k0 < g.k; k0 += BK) {
-        __syncthreads();
-        store_slab();
-        __syncthreads();
-        if (k0 + BK < g.k) load_slab(k0 + BK);
+    auto compute = [&](int buf) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4);
             double af[TM], bf[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + wm * WM + i * 16 + (lane & 15)];
+            for (int i = 0; i < TM; ++i) af[i] = As[buf][kr * PA + wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[buf][kr * PB + wn * WN + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    if (DBUF) {
+        // one barrier per slab: the next slab is stored into the other buffer
+        load_slab(0);
+        store_slab(0);
+        __syncthreads();
+        int cur = 0;
+        for (int k0 = 0; k0 < g.k; k0 += BK) {
+            const bool more = k0 + BK < g.k;
+            if (more) load_slab(k0 + BK);
+            compute(cur);
+            if (more) store_slab(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+    } else {
+        load_slab(0);
+        for (int k0 = 0; k0 < g.k; k0 += BK) {
+            __syncthreads();
+            store_slab(0);
+            __syncthreads();
+            if (k0 + BK < g.k) load_slab(k0 + BK);
+            compute(0);
         }
     }
 
@@ -164,15 +212,146 @@ __global__ void __launch_bounds__(256) dgemm_mfma_kernel(GemmArgs g)
             }
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int WGM, int WGN, bool DBUF>
 static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
-    dim3 block(256);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, false, false>), grid, block, 0, s, g);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, false, true>), grid, block, 0, s, g);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, true, false>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, true, true>), grid, block, 0, s, g);
+    dim3 block(64 * WGM * WGN);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, false, false>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, false, true>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, true, false>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, true, true>), grid, block, 0, s, g);
+}
+
+// ---------------------------------------------------------------------------
+// Split-K register-direct kernel.
+template <int TM, int TN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
+{
+    constexpr int U = 4;  // MFMA steps per register block
+    __shared__ double red[KW * TM * TN * 4 * 64];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int r16 = lane & 15;
+    const int q = lane >> 4;
+    const int bz = blockIdx.z;
+    const int m0 = blockIdx.y * (16 * TM);
+    const int n0 = blockIdx.x * (16 * TN);
+
+    const double* __restrict__ A = g.A + bofs(bz, g.iA);
+    const double* __restrict__ B = g.B + bofs(bz, g.iB);
+
+    // this lane group's K chunk [kb, ke)
+    const int P = 4 * KW;
+    const int c = (g.k + P - 1) / P;
+    const int kb = (wave * 4 + q) * c;
+    const int ke = min(kb + c, g.k);
+
+    // row / column of op(A) / op(B) each lane feeds, with clamped addresses
+    const double* pa[TM];
+    bool va[TM];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int gm = m0 + i * 16 + r16;
+        va[i] = gm < g.m;
+        const int gmc = va[i] ? gm : 0;
+        pa[i] = TA ? A + gmc : A + (long long)gmc * g.lda;
+    }
+    const double* pb[TN];
+    bool vb[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int gn = n0 + j * 16 + r16;
+        vb[j] = gn < g.n;
+        const int gnc = vb[j] ? gn : 0;
+        pb[j] = TB ? B + (long long)gnc * g.ldb : B + gnc;
+    }
+    const long long sa = TA ? g.lda : 1;  // stride of one k step
+    const long long sb = TB ? 1 : g.ldb;
+
+    v4d acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[2][U][TM], rb[2][U][TN];
+    auto load = [&](int buf, int k0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k0 + u;
+            const bool ok = kk < ke;
+#pragma unroll
+            for (int i = 0; i < TM; ++i) ra[buf][u][i] = (ok && va[i]) ? pa[i][kk * sa] : 0.0;
+#pragma unroll
+            for (int j = 0; j < TN; ++j) rb[buf][u][j] = (ok && vb[j]) ? pb[j][kk * sb] : 0.0;
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][u][i], rb[buf][u][j],
+                                                                      acc[i][j], 0, 0, 0);
+    };
+    // The trip count is uniform over the workgroup (c steps per lane group);
+    // lanes past the end of their chunk feed zeros.
+    const int nblk = (c + U - 1) / U;
+    if (nblk > 0) load(0, kb);
+    int blk = 0;
+    for (; blk + 2 <= nblk; blk += 2) {
+        load(1, kb + (blk + 1) * U);
+        compute(0);
+        if (blk + 2 < nblk) load(0, kb + (blk + 2) * U);
+        compute(1);
+    }
+    if (blk < nblk) compute(0);
+
+    // reduce the KW partial tiles
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                red[((wave * TM * TN + i * TN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+    __syncthreads();
+
+    double* __restrict__ C = g.C + bofs(bz, g.iC);
+    const double* __restrict__ Dm = g.D ? g.D + bofs(bz, g.iD) : nullptr;
+    const double alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
+    constexpr int E = TM * TN * 4 * 64;
+    for (int e = tid; e < E; e += 64 * KW) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) v += red[w * E + e];
+        const int l = e & 63;
+        const int r = (e >> 6) & 3;
+        const int t = e >> 8;
+        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        const int gn = n0 + (t % TN) * 16 + (l & 15);
+        if (gm < g.m && gn < g.n) {
+            v *= alpha;
+            if (Dm && g.beta != 0.0) v += g.beta * Dm[(long long)gm * g.ldd + gn];
+            C[(long long)gm * g.ldc + gn] = v;
+        }
+    }
+}
+
+template <int TM, int TN, int KW>
+static void launch_splitk(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
+{
+    dim3 grid((g.n + 16 * TN - 1) / (16 * TN), (g.m + 16 * TM - 1) / (16 * TM), batch);
+    dim3 block(64 * KW);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, false, false>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, false, true>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, true, false>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, true, true>), grid, block, 0, s, g);
 }
 
 static BIdx bidx(const pypde_bidx& b)
@@ -187,16 +366,46 @@ static BIdx strided(long long s)
     return r;
 }
 
-static int launch(int transA, int transB, GemmArgs& g, int batch, hipStream_t stream)
+// Variant codes (pypde_gemm_desc.variant):
+//   0 heuristic, 1 LDS 64x64 (4 waves), 2 LDS 32x32 (4 waves),
+//   3 LDS 128x128 (8 waves), 4 LDS 128x128 (8 waves, double-buffered LDS),
+//   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
+static int heuristic_variant(const GemmArgs& g, int batch)
+{
+    const long long tiles128 = (long long)((g.m + 127) / 128) * ((g.n + 127) / 128) * batch;
+    if (tiles128 >= 512) return 3;
+    const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * batch;
+    if (tiles64 >= 512) return 1;
+    const long long tiles32 = (long long)((g.m + 31) / 32) * ((g.n + 31) / 32) * batch;
+    if (tiles32 >= 1024) return 2;
+    // latency-bound: 32x32 split-K tiles when they still give >= 256 workgroups
+    const int big = (tiles32 >= 256) ? 3 : 0;
+    return 16 + big + (g.k > 512 ? 4 : 0);
+}
+
+static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, hipStream_t stream)
 {
     PYPDE_REQUIRE(g.m >= 0 && g.n >= 0 && g.k >= 0 && batch >= 0, "negative GEMM size");
     if (g.m == 0 || g.n == 0 || batch == 0) return OK;
     PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
-    // Small problems use 32x32 tiles so that more CUs get work.
-    const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * batch;
-    if (tiles64 >= 512) launch_gemm<64, 64>(transA, transB, g, batch, stream);
-    else launch_gemm<32, 32>(transA, transB, g, batch, stream);
+    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 16 && variant < 24),
+                  "unknown GEMM variant %d", variant);
+    if (variant == 0) variant = heuristic_variant(g, batch);
+    switch (variant) {
+    case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
+    case 2: launch_gemm<32, 32, 2, 2, false>(transA, transB, g, batch, stream); break;
+    case 3: launch_gemm<128, 128, 4, 2, false>(transA, transB, g, batch, stream); break;
+    case 4: launch_gemm<128, 128, 4, 2, true>(transA, transB, g, batch, stream); break;
+    case 16: launch_splitk<1, 1, 4>(transA, transB, g, batch, stream); break;
+    case 17: launch_splitk<2, 1, 4>(transA, transB, g, batch, stream); break;
+    case 18: launch_splitk<1, 2, 4>(transA, transB, g, batch, stream); break;
+    case 19: launch_splitk<2, 2, 4>(transA, transB, g, batch, stream); break;
+    case 20: launch_splitk<1, 1, 8>(transA, transB, g, batch, stream); break;
+    case 21: launch_splitk<2, 1, 8>(transA, transB, g, batch, stream); break;
+    case 22: launch_splitk<1, 2, 8>(transA, transB, g, batch, stream); break;
+    default: launch_splitk<2, 2, 8>(transA, transB, g, batch, stream); break;
+    }
     return check_launch("pypde_dgemm");
 }
 
@@ -207,7 +416,7 @@ int dgemm(int transA, int transB, int m, int n, int k, double alpha, const doubl
 {
     GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strided(strideA), strided(strideB),
                strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1};
-    return launch(transA, transB, g, batch, stream);
+    return launch(transA, transB, g, batch, 0, stream);
 }
 
 int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
@@ -217,7 +426,7 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
     GemmArgs g{d->A, d->B, d->D, d->C, d->lda, d->ldb, d->ldd, d->ldc, bidx(d->iA), bidx(d->iB),
                bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
                d->alpha_div < 1 ? 1 : d->alpha_div};
-    return launch(d->transA, d->transB, g, d->batch, stream);
+    return launch(d->transA, d->transB, g, d->batch, d->variant, stream);
 }
 
 }  // namespace pypde

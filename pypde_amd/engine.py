@@ -279,6 +279,7 @@ class RBCBatch:
         self.RP = torch.zeros((B, M0, M1), **f64)
         self.W1 = torch.zeros((B, M0, N1 + 2 * M1), **f64)
         self._build_launches()
+        self.tune()
         self.graph = None
 
     @staticmethod
@@ -442,6 +443,27 @@ class RBCBatch:
             f(s)
             e1.record(stream)
             records.append((f.name, e0, e1, f))
+
+    def tune(self):
+        """Autotune the kernel variant of every GEMM launch of the step (the
+        state is restored afterwards, so in-place launches can be timed)."""
+        from . import autotune
+        if not autotune.enabled():
+            return
+        saved = [t.clone() for t in (self.X, self.pres, self.P)]
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for launch in self.launches:
+            if isinstance(launch, _Gemm):
+                autotune.tune_gemm_desc(launch.desc, lambda d: check(
+                    lib.pypde_dgemm_batched(ctypes.byref(d), s), launch.name), allow_alias=True)
+        torch.cuda.synchronize()
+        for t, v in zip((self.X, self.pres, self.P), saved):
+            t.copy_(v)
+        torch.cuda.synchronize()
+
+    def variants(self):
+        """{launch name: GEMM kernel variant} chosen for this step."""
+        return {l.name: l.desc.variant for l in self.launches if isinstance(l, _Gemm)}
 
     def capture(self, nsteps=1):
         """Capture `nsteps` consecutive steps into one HIP graph (the state is

@@ -80,6 +80,39 @@ def _check_axis(x, axis):
 
 
 # ---------------------------------------------------------------------------
+def _bi(s1, d1=1, m1=0, s2=0, d2=1, m2=0):
+    from ._lib import BIdx
+    b = BIdx()
+    b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s1, d1, m1, s2, d2, m2
+    return b
+
+
+def _launch_desc(d):
+    """Launch a GEMM descriptor with the autotuned kernel variant for its shape."""
+    from . import autotune
+    from ._lib import GemmDesc
+    s = stream_ptr()
+    autotune.tune_gemm_desc(d, lambda dd: check(lib.pypde_dgemm_batched(ctypes.byref(dd), s),
+                                                "pypde_dgemm_batched"))
+    check(lib.pypde_dgemm_batched(ctypes.byref(d), s), "pypde_dgemm_batched")
+
+
+def _desc(ta, tb, m, n, k, batch, alpha, beta, A, lda, sA, B, ldb, sB, D, ldd, sD, C, ldc, sC,
+          alpha_vec=None):
+    from ._lib import GemmDesc
+    d = GemmDesc()
+    d.transA, d.transB, d.m, d.n, d.k, d.batch = int(ta), int(tb), m, n, k, batch
+    d.alpha, d.beta = float(alpha), float(beta)
+    d.A, d.B, d.C = A, B, C
+    d.D = D
+    d.lda, d.ldb, d.ldd, d.ldc = lda, ldb, ldd, ldc
+    d.iA, d.iB, d.iD, d.iC = _bi(sA), _bi(sB), _bi(sD), _bi(sC)
+    d.alpha_vec = alpha_vec
+    d.alpha_div = 1
+    d.variant = 0
+    return d
+
+
 def matmul_axis(mat, x, axis=0, out=None, alpha=1.0, beta=0.0, add=None):
     """out = alpha * (mat applied along axis of x) + beta * add.
 
@@ -94,23 +127,18 @@ def matmul_axis(mat, x, axis=0, out=None, alpha=1.0, beta=0.0, add=None):
     x3, o3 = _as3(x), _as3(out)
     a3 = _as3(add) if add is not None else None
     B = x3.shape[0]
-    s = stream_ptr()
     matc = mat if mat.stride(1) == 1 else mat.contiguous()
+    Dp = ptr(a3) if a3 is not None else None
+    ldd = a3.stride(1) if a3 is not None else 0
+    sd = a3.stride(0) if a3 is not None else 0
     if axis == 0:
-        rc = lib.pypde_dgemm(0, 0, m, x3.shape[2], k, alpha, ptr(matc), matc.stride(0), 0,
-                             ptr(x3), x3.stride(1), x3.stride(0), beta,
-                             ptr(a3) if a3 is not None else None,
-                             a3.stride(1) if a3 is not None else 0,
-                             a3.stride(0) if a3 is not None else 0,
-                             ptr(o3), o3.stride(1), o3.stride(0), B, s)
+        d = _desc(0, 0, m, x3.shape[2], k, B, alpha, beta, ptr(matc), matc.stride(0), 0,
+                  ptr(x3), x3.stride(1), x3.stride(0), Dp, ldd, sd, ptr(o3), o3.stride(1),
+                  o3.stride(0))
     else:
-        rc = lib.pypde_dgemm(0, 1, x3.shape[1], m, k, alpha, ptr(x3), x3.stride(1), x3.stride(0),
-                             ptr(matc), matc.stride(0), 0, beta,
-                             ptr(a3) if a3 is not None else None,
-                             a3.stride(1) if a3 is not None else 0,
-                             a3.stride(0) if a3 is not None else 0,
-                             ptr(o3), o3.stride(1), o3.stride(0), B, s)
-    check(rc, "pypde_dgemm")
+        d = _desc(0, 1, x3.shape[1], m, k, B, alpha, beta, ptr(x3), x3.stride(1), x3.stride(0),
+                  ptr(matc), matc.stride(0), 0, Dp, ldd, sd, ptr(o3), o3.stride(1), o3.stride(0))
+    _launch_desc(d)
     return out
 
 
@@ -130,14 +158,13 @@ def gemm(A, B, transA=False, transB=False, alpha=1.0, beta=0.0, D=None, out=None
         out = torch.empty(shape, dtype=DTYPE, device=A.device)
     o3 = out if out.dim() == 3 else out.unsqueeze(0)
     D3 = None if D is None else (D if D.dim() == 3 else D.unsqueeze(0))
-    rc = lib.pypde_dgemm(int(transA), int(transB), m, n, k, alpha,
-                         ptr(A3), A3.stride(1), A3.stride(0) if A3.shape[0] > 1 else 0,
-                         ptr(B3), B3.stride(1), B3.stride(0) if B3.shape[0] > 1 else 0,
-                         beta, ptr(D3) if D3 is not None else None,
-                         D3.stride(1) if D3 is not None else 0,
-                         (D3.stride(0) if D3.shape[0] > 1 else 0) if D3 is not None else 0,
-                         ptr(o3), o3.stride(1), o3.stride(0), batch, stream_ptr())
-    check(rc, "pypde_dgemm")
+    d = _desc(transA, transB, m, n, k, batch, alpha, beta,
+              ptr(A3), A3.stride(1), A3.stride(0) if A3.shape[0] > 1 else 0,
+              ptr(B3), B3.stride(1), B3.stride(0) if B3.shape[0] > 1 else 0,
+              ptr(D3) if D3 is not None else None, D3.stride(1) if D3 is not None else 0,
+              (D3.stride(0) if D3.shape[0] > 1 else 0) if D3 is not None else 0,
+              ptr(o3), o3.stride(1), o3.stride(0))
+    _launch_desc(d)
     return out
 
 
@@ -146,7 +173,6 @@ def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0,
     """Batched GEMM over a leading batch dimension with broadcasting of 2-D
     operands (one launch of pypde_dgemm_batched); alpha_vec [batch] scales
     entry e by alpha * alpha_vec[e]."""
-    from ._lib import BIdx, GemmDesc
 
     def geo(T):
         if T.dim() == 2:
@@ -166,27 +192,17 @@ def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0,
         out = torch.empty((batch, am, bn) if batch > 1 or A.dim() == 3 or B.dim() == 3 else (am, bn),
                           dtype=DTYPE, device=A.device)
     o3, bo, so = geo(out)
-    d = GemmDesc()
-    d.transA, d.transB, d.m, d.n, d.k, d.batch = int(transA), int(transB), am, bn, ak, batch
-    d.alpha, d.beta = float(alpha), float(beta)
-    d.A, d.B, d.C = A3.data_ptr(), B3.data_ptr(), o3.data_ptr()
-    d.lda, d.ldb, d.ldc = A3.stride(-2), B3.stride(-2), o3.stride(-2)
-
-    def bi(s):
-        b = BIdx()
-        b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s, 1, 0, 0, 1, 0
-        return b
-    d.iA, d.iB, d.iC = bi(sa if ba > 1 else 0), bi(sb if bb > 1 else 0), bi(so)
     if D is not None:
         D3, bd, sd = geo(D)
-        d.D, d.ldd, d.iD = D3.data_ptr(), D3.stride(-2), bi(sd if bd > 1 else 0)
+        Dp, ldd, sD = D3.data_ptr(), D3.stride(-2), sd if bd > 1 else 0
     else:
-        d.D, d.ldd, d.iD = None, 0, bi(0)
-    if alpha_vec is not None:
-        d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), 1
-    else:
-        d.alpha_vec, d.alpha_div = None, 1
-    check(lib.pypde_dgemm_batched(ctypes.byref(d), stream_ptr()), "pypde_dgemm_batched")
+        Dp, ldd, sD = None, 0, 0
+    d = _desc(transA, transB, am, bn, ak, batch, alpha, beta,
+              A3.data_ptr(), A3.stride(-2), sa if ba > 1 else 0,
+              B3.data_ptr(), B3.stride(-2), sb if bb > 1 else 0,
+              Dp, ldd, sD, o3.data_ptr(), o3.stride(-2), so,
+              alpha_vec.data_ptr() if alpha_vec is not None else None)
+    _launch_desc(d)
     return out
 
 

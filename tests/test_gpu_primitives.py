@@ -38,6 +38,37 @@ def test_mfma_gemm_layout(amd):
                 assert rel(host(C), ref) < 1e-14, (m, n, k, ta, tb)
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 16, 17, 18, 19, 20, 21, 22, 23])
+def test_gemm_every_variant(amd, variant):
+    """Each kernel variant of pypde_dgemm_batched on ragged shapes (m, n not
+    multiples of the tile, k not a multiple of the split), all transposes,
+    batch offsets with two-term indexing, per-entry alpha and beta * D."""
+    import ctypes
+    _, ops, torch = amd
+    from pypde_amd._lib import lib
+    rng = np.random.default_rng(variant)
+    for (m, n, k, batch) in [(16, 16, 4, 1), (37, 53, 29, 3), (130, 70, 91, 2), (254, 254, 894, 3),
+                             (5, 3, 1, 2), (64, 64, 700, 1)]:
+        for ta in (0, 1):
+            for tb in (0, 1):
+                A = rng.standard_normal((batch, k, m) if ta else (batch, m, k))
+                B = rng.standard_normal((k, n) if not tb else (n, k))  # broadcast over batch
+                D = rng.standard_normal((batch, m, n))
+                av = rng.standard_normal(batch)
+                dA, dB, dD, dav = (ops.to_dev(t) for t in (A, B, D, av))
+                C = torch.empty((batch, m, n), dtype=torch.float64, device=dA.device)
+                d = ops._desc(ta, tb, m, n, k, batch, 0.5, -2.0, dA.data_ptr(), dA.stride(1),
+                              dA.stride(0), dB.data_ptr(), dB.stride(0), 0, dD.data_ptr(),
+                              dD.stride(1), dD.stride(0), C.data_ptr(), C.stride(1), C.stride(0),
+                              dav.data_ptr())
+                d.variant = variant
+                assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+                opA = np.swapaxes(A, 1, 2) if ta else A
+                opB = B.T if tb else B
+                ref = 0.5 * av[:, None, None] * (opA @ opB) - 2.0 * D
+                assert rel(host(C), ref) < 1e-14, (variant, m, n, k, batch, ta, tb)
+
+
 def test_gemm_batched_strided(amd):
     _, ops, torch = amd
     rng = np.random.default_rng(1)
