@@ -4,8 +4,9 @@ Each GEMM shape (transposes, m, n, k, batch, leading dimensions) is timed once
 per process over the kernel variants of pypde_dgemm_batched (LDS-tiled
 128x128, 64x64 and 32x32, split-K register-direct tiles of 16/32 rows and columns over 4 or 8
 waves); the fastest variant is cached and written into the descriptor that the
-fused step replays.  Recurrence (banded line) launches are tuned the same way
-over the wave count per workgroup.
+fused step replays.  Banded applies (stencil maps, sparse plan right-hand
+sides) are tuned the same way over the rows each thread (axis 0) or block
+(axis 1) covers.
 
 Timing uses HIP events on the launch stream, `reps` back-to-back launches per
 candidate after one warm launch.  Nothing is tuned while a stream is being
@@ -18,6 +19,7 @@ import os
 
 import torch
 
+ROWS_CANDIDATES = (1, 2, 4, 8)
 GEMM_CANDIDATES = (1, 2, 3, 4, 16, 17, 18, 19, 20, 21, 22, 23)
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"

@@ -19,7 +19,8 @@
 // are combined across the P chunks through LDS, once more from the exact
 // incoming values.  Loads and stores are coalesced: along columns (axis 0)
 // neighbouring lanes read neighbouring columns; along rows (axis 1) each wave
-// transposes 64 x 8 tiles through LDS.
+// transposes 64 x 16 tiles (128-byte row segments) through its own LDS tile,
+// synchronising only within the wave.
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
@@ -28,7 +29,7 @@
 namespace pypde {
 
 constexpr int CH = 32;    // positions per chunk (per lane)
-constexpr int TW = 8;     // row-tile width for axis-1 transposes
+constexpr int TW = 16;    // row-tile width for axis-1 transposes
 constexpr int TP = TW + 1;
 
 struct RecArgs {
@@ -41,17 +42,27 @@ struct RecArgs {
     int per_line;    // tables laid out [n][nlines] per table
 };
 
+// Orders LDS traffic of one wavefront (a wave-local tile written by some lanes
+// and read by others); waves of the block are not synchronised.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Table access: per-line tables from global memory ([n][nlines], coalesced
 // across lanes); shared tables from the LDS copy of the block's batch entry.
 template <bool PERLINE>
 struct Tab {
     const double* g;   // per-line: this lane's column of slot 1 (LM); shared: LDS base
     int nlines, line, n;
+    int off[6];        // shared: LDS offset of each table (only present tables are stored)
     __device__ __forceinline__ double operator()(int which, int pos) const
     {
         // per-line: the offset is uniform across the wave, the base per lane
         if (PERLINE) return g[(unsigned)(((which - 1) * n + pos) * nlines)];
-        return g[which * n + pos];
+        return g[off[which] + pos];
     }
 };
 
@@ -172,17 +183,21 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
     double* __restrict__ dst = a.dst + (long long)b * a.dst_bs;
     const int n = a.n;
 
-    // LDS: [tables 6*n (shared mode)] [transpose tiles P*64*TP (axis 1), reused
+    // LDS: [present tables, n each (shared mode)] [transpose tiles P*64*TP (axis 1), reused
     // as the carry-exchange area P*12*64 during the sweeps]
     double* ltab = lds;
-    double* tiles = lds + (PERLINE ? 0 : 6 * n);
+    const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
+                     (a.NU != nullptr) + (a.pre != nullptr);
+    double* tiles = lds + (PERLINE ? 0 : ntab * n);
     double* xch = tiles;
     if (!PERLINE) {
         const long long eo = (long long)b * a.tbs;
         const double* src_t[6] = {a.F, a.LM, a.R, a.MU, a.NU, a.pre};
+        int slot = 0;
         for (int t = 0; t < 6; ++t) {
             if (!src_t[t]) continue;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) ltab[t * n + i] = src_t[t][eo + i];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) ltab[slot * n + i] = src_t[t][eo + i];
+            ++slot;
         }
     }
     Tab<PERLINE> T;
@@ -190,6 +205,15 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
     T.line = live ? line : 0;
     T.n = n;
     T.g = PERLINE ? nullptr : ltab;
+    {
+        const bool present[6] = {a.F != nullptr, a.LM != nullptr, a.R != nullptr, a.MU != nullptr,
+                                 a.NU != nullptr, a.pre != nullptr};
+        int slot = 0;
+        for (int t = 0; t < 6; ++t) {
+            T.off[t] = slot * n;
+            slot += present[t] ? 1 : 0;
+        }
+    }
     // per-line tables: one contiguous [6][n][nlines] view is not assumed; the
     // slots are separate arrays, so route through a small pointer table
     constexpr int NX = PRE ? CH + 2 : CH;
@@ -204,24 +228,30 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
     } else {
         double* tile = tiles + w * 64 * TP;
 #pragma unroll
-        for (int t0 = 0; t0 < NX; t0 += TW) {
+        for (int t0 = 0; t0 < CH; t0 += TW) {
 #pragma unroll
             for (int q = 0; q < TW; ++q) {
                 const int r = (lane / TW) + q * (64 / TW);
                 const int c = lane % TW;
                 const int pos = base + t0 + c;
                 const int ln = l0 + r;
-                if (r < 64)
-                    tile[r * TP + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
+                tile[r * TP + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
             }
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
-            for (int c = 0; c < TW; ++c)
-                if (t0 + c < NX) v[t0 + c] = tile[lane * TP + c];
-            __syncthreads();
+            for (int c = 0; c < TW; ++c) v[t0 + c] = tile[lane * TP + c];
+            wave_lds_sync();
+        }
+        // the pre-stencil's two look-ahead positions: direct loads
+#pragma unroll
+        for (int k = CH; k < NX; ++k) {
+            const int pos = base + k;
+            v[k] = (live && pos < a.n_in) ? src[(long long)line * a.src_ld + pos] : 0.0;
         }
     }
-    if (!PERLINE) __syncthreads();  // table copy visible
+    // table copy visible; every wave is done with its tile before the area is
+    // reused for the carry exchange
+    __syncthreads();
 
     if constexpr (PERLINE) {
         // per-line mode only ever uses LM, R, MU, NU (Poisson): address them
@@ -250,16 +280,16 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
         for (int t0 = 0; t0 < CH; t0 += TW) {
 #pragma unroll
             for (int c = 0; c < TW; ++c) tile[lane * TP + c] = v[t0 + c];
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int q = 0; q < TW; ++q) {
                 const int r = (lane / TW) + q * (64 / TW);
                 const int c = lane % TW;
                 const int pos = base + t0 + c;
                 const int ln = l0 + r;
-                if (r < 64 && ln < a.nlines && pos < a.n) dst[(long long)ln * a.dst_ld + pos] = tile[r * TP + c];
+                if (ln < a.nlines && pos < a.n) dst[(long long)ln * a.dst_ld + pos] = tile[r * TP + c];
             }
-            __syncthreads();
+            wave_lds_sync();
         }
     }
 }
@@ -271,7 +301,9 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
     const int bpe = (a.nlines + 63) / 64;
     const long long blocks = (long long)bpe * a.batch;
     if (blocks == 0) return OK;
-    size_t tabs = PERLINE ? 0 : (size_t)6 * a.n;
+    const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
+                     (a.NU != nullptr) + (a.pre != nullptr);
+    size_t tabs = PERLINE ? 0 : (size_t)ntab * a.n;
     size_t tiles = a.axis == 1 ? (size_t)P * 64 * TP : 0;
     size_t xch = (size_t)P * 12 * 64;
     const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);

@@ -75,33 +75,47 @@ __device__ __forceinline__ double band_row(const BandSpec& bs, int r, int len_in
     return acc;
 }
 
-// Axis 0: one thread per output element; grid.x over columns (neighbouring
-// threads read neighbouring columns), grid.y over output rows, grid.z over
+// Axis 0: one thread per column and `rows` consecutive output rows (rows =
+// g.W, 1..8, the lines_per_block hint); grid.x over columns (neighbouring
+// threads read neighbouring columns), grid.y over row groups, grid.z over
 // batch entries.  Rows shared by neighbouring outputs are served from cache.
+constexpr int kMaxRowsPerThread = 8;
+
 __global__ void __launch_bounds__(256) banded_axis0_kernel(LineGeom g, BandSpec bs)
 {
+    const int rows = g.W;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
+    const int r0 = blockIdx.y * rows;
     const int b = blockIdx.z;
     if (c >= g.nlines) return;
     const double* __restrict__ col = g.src + (long long)b * g.src_bs + c;
+    double* __restrict__ out = g.dst + (long long)b * g.dst_bs + c;
     const long long ls = g.src_ld;
-    const double y = band_row(bs, r, g.len_in, g.len_out,
-                              [&](int o) { return col[(long long)(r + o) * ls]; });
-    g.dst[(long long)b * g.dst_bs + (long long)r * g.dst_ld + c] = y;
+    for (int j = 0; j < rows; ++j) {
+        const int r = r0 + j;
+        if (r >= g.len_out) break;
+        const double y = band_row(bs, r, g.len_in, g.len_out,
+                                  [&](int o) { return col[(long long)(r + o) * ls]; });
+        out[(long long)r * g.dst_ld] = y;
+    }
 }
 
-// Axis 1: one thread per output element of a row; grid.x over the row,
-// grid.y over rows, grid.z over batch entries.
+// Axis 1: one thread per output element of a row, `rows` rows per block
+// (rows = g.W, 1..8); grid.x over the row, grid.y over row groups, grid.z over
+// batch entries.
 __global__ void __launch_bounds__(256) banded_axis1_kernel(LineGeom g, BandSpec bs)
 {
+    const int rows = g.W;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
     const int b = blockIdx.z;
     if (r >= g.len_out) return;
-    const double* __restrict__ row = g.src + (long long)b * g.src_bs + (long long)c * g.src_ld;
-    const double y = band_row(bs, r, g.len_in, g.len_out, [&](int o) { return row[r + o]; });
-    g.dst[(long long)b * g.dst_bs + (long long)c * g.dst_ld + r] = y;
+    for (int j = 0; j < rows; ++j) {
+        const int c = blockIdx.y * rows + j;
+        if (c >= g.nlines) break;
+        const double* __restrict__ row = g.src + (long long)b * g.src_bs + (long long)c * g.src_ld;
+        const double y = band_row(bs, r, g.len_in, g.len_out, [&](int o) { return row[r + o]; });
+        g.dst[(long long)b * g.dst_bs + (long long)c * g.dst_ld + r] = y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -475,16 +489,22 @@ static int launch_lines(LineGeom g, const Op& op, int ntab, int hint, void* stre
     return check_launch(name);
 }
 
-static int launch_banded(const LineGeom& g, const BandSpec& bs, void* stream, const char* name)
+// rows per thread (axis 0) / per block (axis 1): the lines_per_block hint, 1..8
+// (0 = 1), chosen per shape by the host-side autotuner.
+static int launch_banded(LineGeom g, const BandSpec& bs, int hint, void* stream, const char* name)
 {
     if ((long long)g.len_out * g.nlines * g.batch == 0) return OK;
+    g.W = std::min(kMaxRowsPerThread, std::max(1, hint));
+    const int kRowsPerThread = g.W, kRowsPerBlock1 = g.W;
     if (g.axis == 0) {
-        PYPDE_REQUIRE(g.len_out <= 65535 && g.batch <= 65535, "%s: grid too large", name);
-        dim3 grid((g.nlines + 255) / 256, g.len_out, g.batch);
+        PYPDE_REQUIRE(g.batch <= 65535 && (g.len_out + kRowsPerThread - 1) / kRowsPerThread <= 65535,
+                      "%s: grid too large", name);
+        dim3 grid((g.nlines + 255) / 256, (g.len_out + kRowsPerThread - 1) / kRowsPerThread, g.batch);
         hipLaunchKernelGGL(banded_axis0_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
     } else {
-        PYPDE_REQUIRE(g.nlines <= 65535 && g.batch <= 65535, "%s: grid too large", name);
-        dim3 grid((g.len_out + 255) / 256, g.nlines, g.batch);
+        PYPDE_REQUIRE(g.batch <= 65535 && (g.nlines + kRowsPerBlock1 - 1) / kRowsPerBlock1 <= 65535,
+                      "%s: grid too large", name);
+        dim3 grid((g.len_out + 255) / 256, (g.nlines + kRowsPerBlock1 - 1) / kRowsPerBlock1, g.batch);
         hipLaunchKernelGGL(banded_axis1_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
     }
     return check_launch(name);
@@ -505,7 +525,7 @@ extern "C" int pypde_stencil_apply(const pypde_lines* in, const double* s, void*
     BandSpec bs{};
     bs.vals = s;
     bs.stencil = 1;
-    return launch_banded(g, bs, stream, "pypde_stencil_apply");
+    return launch_banded(g, bs, in->lines_per_block, stream, "pypde_stencil_apply");
 }
 
 extern "C" int pypde_banded_apply(const pypde_lines* in, int ndiag, const int* offsets_host,
@@ -525,7 +545,7 @@ extern "C" int pypde_banded_apply(const pypde_lines* in, int ndiag, const int* o
         if (q) PYPDE_REQUIRE(offsets_host[q] > offsets_host[q - 1], "offsets must ascend");
         bs.off[q] = offsets_host[q];
     }
-    return launch_banded(g, bs, stream, "pypde_banded_apply");
+    return launch_banded(g, bs, in->lines_per_block, stream, "pypde_banded_apply");
 }
 
 extern "C" int pypde_stencil_inverse(const pypde_lines* in, const double* s, const double* a,

@@ -213,7 +213,16 @@ def stencil_apply(s, x, axis=0, lpb=0):
     M = _len(x, axis)
     out = torch.empty(_out_shape(x, axis, M + 2), dtype=DTYPE, device=x.device)
     g = _geom(_as3(x), _as3(out), axis, M, M + 2, lpb)
-    check(lib.pypde_stencil_apply(ctypes.byref(g), ptr(s), stream_ptr()), "pypde_stencil_apply")
+    st = stream_ptr()
+
+    def run(rows):
+        g.lines_per_block = rows
+        check(lib.pypde_stencil_apply(ctypes.byref(g), ptr(s), st), "pypde_stencil_apply")
+    if lpb == 0:
+        from . import autotune
+        g.lines_per_block = autotune.choose(("stencil", axis, M, g.nlines, g.batch), run,
+                                            autotune.ROWS_CANDIDATES, default=1)
+    run(g.lines_per_block)
     return out
 
 
@@ -250,8 +259,18 @@ def banded_apply(offsets, vals, x, axis=0, len_out=None, lpb=0):
     out = torch.empty(_out_shape(x, axis, n_out), dtype=DTYPE, device=x.device)
     g = _geom(_as3(x), _as3(out), axis, n_in, n_out, lpb)
     offs = (ctypes.c_int * len(offsets))(*[int(o) for o in offsets])
-    check(lib.pypde_banded_apply(ctypes.byref(g), len(offsets), offs, ptr(vals), stream_ptr()),
-          "pypde_banded_apply")
+    s = stream_ptr()
+
+    def run(rows):
+        g.lines_per_block = rows
+        check(lib.pypde_banded_apply(ctypes.byref(g), len(offsets), offs, ptr(vals), s),
+              "pypde_banded_apply")
+    if lpb == 0:
+        from . import autotune
+        g.lines_per_block = autotune.choose(
+            ("banded", axis, n_in, n_out, g.nlines, g.batch, tuple(int(o) for o in offsets)),
+            run, autotune.ROWS_CANDIDATES, default=1)
+    run(g.lines_per_block)
     return out
 
 
