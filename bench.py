@@ -15,6 +15,9 @@ Also reported on the same line:
                  launch over an eager pass of the same K steps)
   batched        64 instances of 256x256 (Ra log-uniform in [1e4, 1e7]) in one
                  batched step: aggregate instance-steps/s and the ratio to the plain path
+  hetero         256 instances of mixed sizes (N per axis in {64, 96, 128, 192}) and
+                 Ra, one graph launch per step, 500 steps: aggregate instance-steps/s and
+                 the ratio to looping the plain single-instance step over them
   microbench     512 fields of 256x256: transforms, ChebDirichlet stencil maps,
                  Helmholtz right-hand side and four-diagonal solves, per-kernel HBM fractions
   adjoint        gradient of the time-averaged plate Nusselt number with respect
@@ -149,6 +152,55 @@ def time_batched(args, nb=64, steps=None):
     finite = bool(torch.isfinite(eng.X).all().item())
     return dict(instances=nb, steps=steps, sec=sec, inst_steps_per_s=nb * steps / sec,
                 finite=finite)
+
+
+def time_hetero(steps=500, warm=5, n=256, seed=7):
+    """256 independent instances, N drawn per axis from {64, 96, 128, 192},
+    Ra log-uniform in [1e4, 1e7], Pr = 1, dt = clip(0.01 sqrt(1e5/Ra), 1e-3, 1e-2),
+    instance i seeded with noise 1e-3 from seed 1000 + i; one graph launch per
+    step for all size classes.  Compared with looping the plain single-instance
+    graph-replayed step over the same instances."""
+    import torch
+    from pypde_amd.batch import HeteroBatch, default_dt
+    from pypde_amd.engine import RBCBatch
+    rng = np.random.default_rng(seed)
+    sizes = np.array([64, 96, 128, 192])
+    nx = rng.choice(sizes, size=n)
+    ny = rng.choice(sizes, size=n)
+    ras = 10.0 ** rng.uniform(4, 7, size=n)
+    specs = [dict(shape=(int(a), int(b)), ra=float(r), pr=1.0, dt=default_dt(r))
+             for a, b, r in zip(nx, ny, ras)]
+    t0 = time.perf_counter()
+    hb = HeteroBatch(specs, case="rbc")
+    setup = time.perf_counter() - t0
+    hb.seed([1000 + i for i in range(n)], amplitude=1e-3)
+    hb.capture(1)
+    hb.step(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hb.step(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    finite = hb.finite()
+    # plain path: one graph-replayed single instance per shape, timed per step
+    per_shape = {}
+    for shape in hb.shapes:
+        e = RBCBatch(shape, [dict(ra=1e5, pr=1.0, dt=0.01)], case="rbc")
+        e.capture(1)
+        e.step(5)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        e.step(100)
+        torch.cuda.synchronize()
+        per_shape[shape] = (time.perf_counter() - t1) / 100
+        del e
+    loop_sec_per_step = sum(per_shape[tuple(s["shape"])] for s in specs)
+    agg = n * steps / sec
+    return dict(instances=n, size_classes=len(hb.shapes), steps=steps, sec=round(sec, 3),
+                inst_steps_per_s=round(agg, 1), ms_per_step=round(sec / steps * 1e3, 3),
+                plain_loop_inst_steps_per_s=round(n / loop_sec_per_step, 1),
+                ratio_vs_plain_loop=round(agg / (n / loop_sec_per_step), 2),
+                finite=finite, setup_sec=round(setup, 1))
 
 
 def microbench(nf=512, reps=3):
@@ -300,6 +352,7 @@ def main():
     if not args.no_extra:
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
+        extra["hetero"] = time_hetero()
         extra["microbench"] = microbench()
         extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000)}
     cpu = None

@@ -52,24 +52,28 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // Table access: per-line tables from global memory ([n][nlines], coalesced
-// across lanes); shared tables from the LDS copy of the block's batch entry.
-template <bool PERLINE>
+// across lanes); shared tables from the LDS copy of the block's batch entry,
+// which stores only the present tables in the order F, LM, R, MU, NU, pre (the
+// slot of each is fixed by the template flags).
+template <bool PERLINE, bool HASF, bool FWD, bool HASR, bool HASNU>
 struct Tab {
     const double* g;   // per-line: this lane's column of slot 1 (LM); shared: LDS base
     int nlines, line, n;
-    int off[6];        // shared: LDS offset of each table (only present tables are stored)
     __device__ __forceinline__ double operator()(int which, int pos) const
     {
         // per-line: the offset is uniform across the wave, the base per lane
         if (PERLINE) return g[(unsigned)(((which - 1) * n + pos) * nlines)];
-        return g[off[which] + pos];
+        const int slot = (which > 0 && HASF) + (which > 1 && FWD) + (which > 2 && HASR) + (which > 3) +
+                         (which > 4 && HASNU);
+        return g[slot * n + pos];
     }
 };
 
 // Sweeps over one chunk held in registers.  GUARD: skip positions >= n (only
 // the last chunk of a line needs it).
 template <bool GUARD, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int NX>
-__device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE>& T, int base, int n,
+__device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF, FWD, HASR, HASNU>& T,
+                                       int base, int n,
                                        int P, int w, int lane, double* xch)
 {
     // table slots: 0 F, 1 LM, 2 R, 3 MU, 4 NU, 5 pre
@@ -166,8 +170,11 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE>& T, i
     }
 }
 
-template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
-__global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
+// Occupancy hint: the pre-stencil (tri-diagonal projection) form runs best
+// capped at 128 VGPRs (4 waves per SIMD, two 8-wave blocks per CU); the other
+// forms spill under that cap and run best uncapped (measured at n = 254).
+template <int AXIS, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+__global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs a)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63;
@@ -200,26 +207,17 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
             ++slot;
         }
     }
-    Tab<PERLINE> T;
+    Tab<PERLINE, HASF, FWD, HASR, HASNU> T;
     T.nlines = a.nlines;
     T.line = live ? line : 0;
     T.n = n;
     T.g = PERLINE ? nullptr : ltab;
-    {
-        const bool present[6] = {a.F != nullptr, a.LM != nullptr, a.R != nullptr, a.MU != nullptr,
-                                 a.NU != nullptr, a.pre != nullptr};
-        int slot = 0;
-        for (int t = 0; t < 6; ++t) {
-            T.off[t] = slot * n;
-            slot += present[t] ? 1 : 0;
-        }
-    }
     // per-line tables: one contiguous [6][n][nlines] view is not assumed; the
     // slots are separate arrays, so route through a small pointer table
     constexpr int NX = PRE ? CH + 2 : CH;
     double v[NX];
 
-    if (a.axis == 0) {
+    if constexpr (AXIS == 0) {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int pos = base + k;
@@ -256,7 +254,7 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
     if constexpr (PERLINE) {
         // per-line mode only ever uses LM, R, MU, NU (Poisson): address them
         // through a 4-slot layout [slot][n][nlines] starting at LM
-        Tab<true> TL;
+        Tab<true, HASF, FWD, HASR, HASNU> TL;
         TL.nlines = a.nlines;
         TL.line = live ? line : 0;
         TL.n = n;
@@ -268,7 +266,7 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
         else sweeps<true, PRE, FWD, HASF, HASR, HASNU, false>(v, T, base, n, P, w, lane, xch);
     }
 
-    if (a.axis == 0) {
+    if constexpr (AXIS == 0) {
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
             const int pos = base + k;
@@ -294,7 +292,7 @@ __global__ void __launch_bounds__(MAXT) recurrence_kernel(RecArgs a)
     }
 }
 
-template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+template <int AXIS, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
 static int launch_rec_t(const RecArgs& a, hipStream_t s)
 {
     const int P = (a.n + CH - 1) / CH;
@@ -304,13 +302,13 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
     const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
                      (a.NU != nullptr) + (a.pre != nullptr);
     size_t tabs = PERLINE ? 0 : (size_t)ntab * a.n;
-    size_t tiles = a.axis == 1 ? (size_t)P * 64 * TP : 0;
+    size_t tiles = AXIS == 1 ? (size_t)P * 64 * TP : 0;
     size_t xch = (size_t)P * 12 * 64;
     const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);
-    auto fn = recurrence_kernel<PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
+    auto fn = recurrence_kernel<AXIS, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
     static bool attr = false;
     if (!attr) {
-        hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
     PYPDE_REQUIRE(shmem <= 160 * 1024, "recurrence: LDS %zu too large", shmem);
@@ -325,9 +323,14 @@ static int launch_rec(const RecArgs& a, hipStream_t s)
 {
     const int P = (a.n + CH - 1) / CH;
     PYPDE_REQUIRE(P >= 1 && P <= 16, "recurrence: line length %d exceeds %d", a.n, 16 * CH);
-    if (P <= 4) return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
-    if (P <= 8) return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
-    return launch_rec_t<PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+    if (a.axis == 0) {
+        if (P <= 4) return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
+        if (P <= 8) return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
+        return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+    }
+    if (P <= 4) return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
+    if (P <= 8) return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
+    return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
 }
 
 }  // namespace pypde

@@ -76,14 +76,16 @@ __device__ __forceinline__ double band_row(const BandSpec& bs, int r, int len_in
 }
 
 // Axis 0: one thread per column and `rows` consecutive output rows (rows =
-// g.W, 1..8, the lines_per_block hint); grid.x over columns (neighbouring
-// threads read neighbouring columns), grid.y over row groups, grid.z over
-// batch entries.  Rows shared by neighbouring outputs are served from cache.
+// g.W, 1..8, the lines_per_block hint; MULTI = false is the one-row form);
+// grid.x over columns (neighbouring threads read neighbouring columns), grid.y
+// over row groups, grid.z over batch entries.  Rows shared by neighbouring
+// outputs are served from cache.
 constexpr int kMaxRowsPerThread = 8;
 
+template <bool MULTI>
 __global__ void __launch_bounds__(256) banded_axis0_kernel(LineGeom g, BandSpec bs)
 {
-    const int rows = g.W;
+    const int rows = MULTI ? g.W : 1;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r0 = blockIdx.y * rows;
     const int b = blockIdx.z;
@@ -91,6 +93,12 @@ __global__ void __launch_bounds__(256) banded_axis0_kernel(LineGeom g, BandSpec 
     const double* __restrict__ col = g.src + (long long)b * g.src_bs + c;
     double* __restrict__ out = g.dst + (long long)b * g.dst_bs + c;
     const long long ls = g.src_ld;
+    if (!MULTI) {
+        const double y = band_row(bs, r0, g.len_in, g.len_out,
+                                  [&](int o) { return col[(long long)(r0 + o) * ls]; });
+        out[(long long)r0 * g.dst_ld] = y;
+        return;
+    }
     for (int j = 0; j < rows; ++j) {
         const int r = r0 + j;
         if (r >= g.len_out) break;
@@ -101,14 +109,22 @@ __global__ void __launch_bounds__(256) banded_axis0_kernel(LineGeom g, BandSpec 
 }
 
 // Axis 1: one thread per output element of a row, `rows` rows per block
-// (rows = g.W, 1..8); grid.x over the row, grid.y over row groups, grid.z over
-// batch entries.
+// (rows = g.W, 1..8; MULTI = false is the one-row form); grid.x over the row,
+// grid.y over row groups, grid.z over batch entries.
+template <bool MULTI>
 __global__ void __launch_bounds__(256) banded_axis1_kernel(LineGeom g, BandSpec bs)
 {
-    const int rows = g.W;
+    const int rows = MULTI ? g.W : 1;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.z;
     if (r >= g.len_out) return;
+    if (!MULTI) {
+        const int c = blockIdx.y;
+        const double* __restrict__ row = g.src + (long long)b * g.src_bs + (long long)c * g.src_ld;
+        const double y = band_row(bs, r, g.len_in, g.len_out, [&](int o) { return row[r + o]; });
+        g.dst[(long long)b * g.dst_bs + (long long)c * g.dst_ld + r] = y;
+        return;
+    }
     for (int j = 0; j < rows; ++j) {
         const int c = blockIdx.y * rows + j;
         if (c >= g.nlines) break;
@@ -500,12 +516,14 @@ static int launch_banded(LineGeom g, const BandSpec& bs, int hint, void* stream,
         PYPDE_REQUIRE(g.batch <= 65535 && (g.len_out + kRowsPerThread - 1) / kRowsPerThread <= 65535,
                       "%s: grid too large", name);
         dim3 grid((g.nlines + 255) / 256, (g.len_out + kRowsPerThread - 1) / kRowsPerThread, g.batch);
-        hipLaunchKernelGGL(banded_axis0_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
+        if (g.W == 1) hipLaunchKernelGGL(banded_axis0_kernel<false>, grid, dim3(256), 0, as_stream(stream), g, bs);
+        else hipLaunchKernelGGL(banded_axis0_kernel<true>, grid, dim3(256), 0, as_stream(stream), g, bs);
     } else {
         PYPDE_REQUIRE(g.batch <= 65535 && (g.nlines + kRowsPerBlock1 - 1) / kRowsPerBlock1 <= 65535,
                       "%s: grid too large", name);
         dim3 grid((g.len_out + 255) / 256, (g.nlines + kRowsPerBlock1 - 1) / kRowsPerBlock1, g.batch);
-        hipLaunchKernelGGL(banded_axis1_kernel, grid, dim3(256), 0, as_stream(stream), g, bs);
+        if (g.W == 1) hipLaunchKernelGGL(banded_axis1_kernel<false>, grid, dim3(256), 0, as_stream(stream), g, bs);
+        else hipLaunchKernelGGL(banded_axis1_kernel<true>, grid, dim3(256), 0, as_stream(stream), g, bs);
     }
     return check_launch(name);
 }

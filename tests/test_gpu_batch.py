@@ -1,0 +1,59 @@
+"""Heterogeneous batches: instances of different shapes, Rayleigh numbers and
+time steps advanced by one graph launch per step must equal the oracle's
+single-instance runs (reference navier/rbc2d.py:396-434) of each instance."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def test_hetero_batch_matches_oracle_per_instance():
+    from pypde_amd.batch import HeteroBatch, default_dt
+    from oracle import pypde_oracle as o
+    specs = [dict(shape=(20, 16), ra=1e4, pr=1.0), dict(shape=(24, 20), ra=3e5, pr=1.0),
+             dict(shape=(16, 24), ra=2e6, pr=0.7), dict(shape=(20, 16), ra=5e5, pr=1.0),
+             dict(shape=(24, 20), ra=1e5, pr=1.0)]
+    for s in specs:
+        s["dt"] = default_dt(s["ra"])
+    hb = HeteroBatch(specs, case="rbc")
+    assert len(hb.classes) == 3
+    seeds = [100 + i for i in range(len(specs))]
+    hb.seed(seeds, amplitude=0.05)
+    refs = []
+    for s, sd in zip(specs, seeds):
+        r = o.NavierStokes(case="rbc", shape=s["shape"], ra=s["ra"], pr=s["pr"], dt=s["dt"],
+                           tsave=None)
+        o.seeded_initial_state(r, seed=sd, amplitude=0.05)
+        refs.append(r)
+    for i, r in enumerate(refs):
+        assert rel(host(hb.state(i)["T"]), r.T.vhat) < 1e-13, i
+    hb.capture(1)
+    nsteps = 5
+    hb.step(nsteps)
+    for r in refs:
+        for _ in range(nsteps):
+            r.update()
+    for i, r in enumerate(refs):
+        st = hb.state(i)
+        for f in ("U", "V", "T"):
+            e = rel(host(st[f]), getattr(r, f).vhat)
+            assert e < 1e-11, (i, f, e)
+        assert rel(host(st["pres"]), r.pres.vhat) < 1e-11, i
+    nu = hb.nusselt()
+    for i, r in enumerate(refs):
+        assert abs(nu[i] - r.eval_Nu()[0]) < 1e-10 * max(1.0, abs(nu[i])), i
+    assert hb.finite()
+
+
+def test_default_dt_rule():
+    from pypde_amd.batch import default_dt
+    assert default_dt(1e5) == pytest.approx(0.01)
+    assert default_dt(1e7) == pytest.approx(1e-3)
+    assert default_dt(1e8) == pytest.approx(1e-3)
+    assert default_dt(1e4) == pytest.approx(1e-2)
