@@ -80,8 +80,6 @@ def launch_cost(name, launch, eng):
         "helmholtz_y": 2 * 3 * M0 * M1 * B,
         # in/out plus the per-line factor tables, counted once per launch
         "poisson_lines": 2 * M0 * M1 * B + 4 * M0 * M1,
-        "pres_div": 3 * N0 * N1 * B,
-        "gauge": B,
     }.get(name, 0)
     return "hbm", 8.0 * el
 

@@ -70,8 +70,19 @@ typedef struct pypde_gemm_desc {
     const double* alpha_vec; /* NULL, or per-entry scale alpha_vec[e / alpha_div] */
     int alpha_div;
     int variant; /* kernel choice: 0 size heuristic; 1 LDS-tiled 64x64; 2 LDS-tiled
-                    32x32; 3 LDS-tiled 128x128; 4 the same, double-buffered; 16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)
+                    32x32; 3 LDS-tiled 128x128; 4 the same, double-buffered;
+                    8..11 split-K LDS-staged (32x32 over 4 / 8 waves, 64x32, 32x64); 16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)
                     + 4 (KW == 8) for a (16 TM) x (16 TN) tile over KW waves */
+    /* optional second addend and gauge (fused pressure update of rbc2d.py:384-394):
+     *   C[e] = (beta D[e] - (gamma_vec[e / gamma_div] E[e]) gamma) + alpha_e op(A) op(B)
+     * E NULL: no E term (gamma_vec NULL: factor 1); zero_first = 1 sets C[e](0,0) = 0 last. */
+    const double* E;
+    long long lde;
+    pypde_bidx iE;
+    const double* gamma_vec;
+    double gamma;
+    int gamma_div;
+    int zero_first;
 } pypde_gemm_desc;
 
 /* Batched GEMM  C[e] = alpha_e * op(A[e]) @ op(B[e]) + beta * D[e]  with the

@@ -52,7 +52,10 @@ class GemmDesc(ctypes.Structure):
                 ("C", ctypes.c_void_p), ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong),
                 ("ldd", ctypes.c_longlong), ("ldc", ctypes.c_longlong), ("iA", BIdx), ("iB", BIdx),
                 ("iD", BIdx), ("iC", BIdx), ("alpha_vec", ctypes.c_void_p),
-                ("alpha_div", ctypes.c_int), ("variant", ctypes.c_int)]
+                ("alpha_div", ctypes.c_int), ("variant", ctypes.c_int),
+                ("E", ctypes.c_void_p), ("lde", ctypes.c_longlong), ("iE", BIdx),
+                ("gamma_vec", ctypes.c_void_p), ("gamma", ctypes.c_double),
+                ("gamma_div", ctypes.c_int), ("zero_first", ctypes.c_int)]
 
 
 class Recurrence(ctypes.Structure):

@@ -18,6 +18,10 @@
 //    buffer and needs one barrier per slab.  128x128 tiles reach about 70% of
 //    the FP64 MFMA peak on batched 256-point transforms, 64x64 about 55%.
 //
+//  * Split-K, LDS-staged (latency): KW waves share one output tile, each
+//    staging its own K range through a private LDS slab with coalesced loads;
+//    partial tiles are summed through LDS.
+//
 //  * Split-K, register-direct (latency): one workgroup of KW waves owns a
 //    (16 TM) x (16 TN) output tile.  The K range is cut into 4 KW contiguous
 //    chunks, one per (wave, lane group of 16): the f64 MFMA sums over its four
@@ -36,6 +40,14 @@
 namespace pypde {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Orders LDS traffic within one wavefront (other waves are not synchronised).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Batch index of one operand: entry e is at  (e/d1 % m1)*s1 + (e/d2 % m2)*s2
 // (m = 0 means no modulus).  A plain strided batch is {s, 0, 1, 0, 1, 0}.
@@ -64,6 +76,52 @@ struct GemmArgs {
     double alpha, beta;
     const double* alpha_vec;  // alpha * alpha_vec[e / alpha_div] when given
     int alpha_div;
+    const double* E;          // optional second addend
+    long long lde;
+    BIdx iE;
+    const double* gamma_vec;  // gamma * gamma_vec[e / gamma_div] scales E when given
+    double gamma;
+    int gamma_div;
+    int zero_first;           // set element (0,0) of every C[e] to 0 (pressure gauge)
+};
+
+// Output update shared by every kernel family:
+//   C = (beta D - (g_e E) gamma) + alpha_e acc      (E term and D term optional)
+// in the order of the reference's pressure update (pres -= nu div beta, then
+// pres += ...), with the gauge C[0,0] = 0 applied last when requested.
+struct Epilogue {
+    double* C;
+    const double* D;
+    const double* E;
+    long long ldc, ldd, lde;
+    double alpha, beta, ge;
+    bool zero_first;
+    __device__ __forceinline__ Epilogue(const GemmArgs& g, int bz)
+    {
+        C = g.C + bofs(bz, g.iC);
+        D = (g.D && g.beta != 0.0) ? g.D + bofs(bz, g.iD) : nullptr;
+        E = g.E ? g.E + bofs(bz, g.iE) : nullptr;
+        ldc = g.ldc;
+        ldd = g.ldd;
+        lde = g.lde;
+        alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
+        beta = g.beta;
+        ge = g.gamma_vec ? g.gamma_vec[bz / g.gamma_div] : 1.0;
+        beta_e_gamma = g.gamma;
+        zero_first = g.zero_first != 0;
+    }
+    __device__ __forceinline__ void store(int gm, int gn, double acc) const
+    {
+        double v = alpha * acc;
+        if (D || E) {
+            double d = D ? beta * D[(long long)gm * ldd + gn] : 0.0;
+            if (E) d = d - (ge * E[(long long)gm * lde + gn]) * beta_e_gamma;
+            v = d + v;
+        }
+        if (zero_first && gm == 0 && gn == 0) v = 0.0;
+        C[(long long)gm * ldc + gn] = v;
+    }
+    double beta_e_gamma;
 };
 
 template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
@@ -193,9 +251,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 
     // Epilogue: for the f64 16x16x4 MFMA, accumulator item r of a lane holds
     // row (lane/16) + 4*r, column lane%16 of its 16x16 tile.
-    double* __restrict__ C = g.C + bofs(bz, g.iC);
-    const double* __restrict__ Dm = g.D ? g.D + bofs(bz, g.iD) : nullptr;
-    const double alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
+    const Epilogue ep(g, bz);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -204,11 +260,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
                 const int gn = n0 + wn * WN + j * 16 + (lane & 15);
-                if (gm < g.m && gn < g.n) {
-                    double v = alpha * acc[i][j][r];
-                    if (Dm && g.beta != 0.0) v += g.beta * Dm[(long long)gm * g.ldd + gn];
-                    C[(long long)gm * g.ldc + gn] = v;
-                }
+                if (gm < g.m && gn < g.n) ep.store(gm, gn, acc[i][j][r]);
             }
 }
 
@@ -322,9 +374,7 @@ __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
                 red[((wave * TM * TN + i * TN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
     __syncthreads();
 
-    double* __restrict__ C = g.C + bofs(bz, g.iC);
-    const double* __restrict__ Dm = g.D ? g.D + bofs(bz, g.iD) : nullptr;
-    const double alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
+    const Epilogue ep(g, bz);
     constexpr int E = TM * TN * 4 * 64;
     for (int e = tid; e < E; e += 64 * KW) {
         double v = 0.0;
@@ -335,11 +385,7 @@ __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
         const int t = e >> 8;
         const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
         const int gn = n0 + (t % TN) * 16 + (l & 15);
-        if (gm < g.m && gn < g.n) {
-            v *= alpha;
-            if (Dm && g.beta != 0.0) v += g.beta * Dm[(long long)gm * g.ldd + gn];
-            C[(long long)gm * g.ldc + gn] = v;
-        }
+        if (gm < g.m && gn < g.n) ep.store(gm, gn, v);
     }
 }
 
@@ -352,6 +398,144 @@ static void launch_splitk(bool ta, bool tb, const GemmArgs& g, int batch, hipStr
     else if (!ta && tb) hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, false, true>), grid, block, 0, s, g);
     else if (ta && !tb) hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, true, false>), grid, block, 0, s, g);
     else hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, true, true>), grid, block, 0, s, g);
+}
+
+// ---------------------------------------------------------------------------
+// Split-K, LDS-staged kernel (latency): KW waves share one BM x BN output tile;
+// wave w owns a contiguous K range and stages its own 16-deep slabs of A and B
+// through a private LDS region with coalesced loads (consecutive lanes walk the
+// contiguous dimension of each operand), so the K loop has no block barriers.
+// The KW partial tiles are summed through LDS at the end.
+template <int BM, int BN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
+{
+    constexpr int BK = 16;
+    constexpr int PA = BM + 1, PB = BN + 1;
+    constexpr int SLAB = BK * PA + BK * PB;  // doubles per wave
+    constexpr int TM = BM / 16, TN = BN / 16;
+    constexpr int LA = BM * BK / 64, LB = BN * BK / 64;
+    constexpr int E = BM * BN;
+    constexpr int AREA = (KW * SLAB > KW * E) ? KW * SLAB : KW * E;
+    static_assert(64 % BK == 0 && 64 % BM == 0 && 64 % BN == 0, "load mapping: BM, BN divide 64");
+    __shared__ double lds[AREA];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int bz = blockIdx.z;
+    const int m0 = blockIdx.y * BM;
+    const int n0 = blockIdx.x * BN;
+    const double* __restrict__ A = g.A + bofs(bz, g.iA);
+    const double* __restrict__ B = g.B + bofs(bz, g.iB);
+    double* As = lds + wave * SLAB;
+    double* Bs = As + BK * PA;
+
+    // K range of this wave, in whole slabs
+    const int kc = ((g.k + KW - 1) / KW + BK - 1) / BK * BK;
+    const int kb = wave * kc;
+    const int ke = min(kb + kc, g.k);
+
+    v4d acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[LA], rb[LB];
+    auto a_rc = [&](int q, int& r, int& c) {
+        if (!TA) { c = lane % BK; r = lane / BK + q * (64 / BK); }
+        else     { r = lane % BM; c = lane / BM + q * (64 / BM); }
+    };
+    auto b_rc = [&](int q, int& r, int& c) {
+        if (!TB) { c = lane % BN; r = lane / BN + q * (64 / BN); }
+        else     { r = lane % BK; c = lane / BK + q * (64 / BK); }
+    };
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            const int gm = m0 + r, gk = k0 + c;
+            ra[q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + gk]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            const int gk = k0 + r, gn = n0 + c;
+            rb[q] = (gk < ke && gn < g.n) ? (TB ? B[(long long)gn * g.ldb + gk] : B[(long long)gk * g.ldb + gn]) : 0.0;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            As[c * PA + r] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            Bs[r * PB + c] = rb[q];
+        }
+    };
+
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        wave_lds_sync();  // previous slab fully read
+        store();
+        wave_lds_sync();
+        if (k0 + BK < ke) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    __syncthreads();  // every wave is done with its slabs; the area becomes the reduction buffer
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                lds[wave * E + ((i * TN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+    __syncthreads();
+
+    const Epilogue ep(g, bz);
+    for (int e = tid; e < E; e += 64 * KW) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) v += lds[w * E + e];
+        const int l = e & 63;
+        const int r = (e >> 6) & 3;
+        const int t = e >> 8;
+        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        const int gn = n0 + (t % TN) * 16 + (l & 15);
+        if (gm < g.m && gn < g.n) ep.store(gm, gn, v);
+    }
+}
+
+template <int BM, int BN, int KW>
+static void launch_wsplit(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
+{
+    dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
+    dim3 block(64 * KW);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, false>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, true>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, false>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, true>), grid, block, 0, s, g);
 }
 
 static BIdx bidx(const pypde_bidx& b)
@@ -369,6 +553,8 @@ static BIdx strided(long long s)
 // Variant codes (pypde_gemm_desc.variant):
 //   0 heuristic, 1 LDS 64x64 (4 waves), 2 LDS 32x32 (4 waves),
 //   3 LDS 128x128 (8 waves), 4 LDS 128x128 (8 waves, double-buffered LDS),
+//   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
+//   4 waves, 11 32x64 over 4 waves,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
 static int heuristic_variant(const GemmArgs& g, int batch)
 {
@@ -389,7 +575,8 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     if (g.m == 0 || g.n == 0 || batch == 0) return OK;
     PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
-    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 16 && variant < 24),
+    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 8 && variant <= 11) ||
+                      (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
@@ -397,6 +584,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 2: launch_gemm<32, 32, 2, 2, false>(transA, transB, g, batch, stream); break;
     case 3: launch_gemm<128, 128, 4, 2, false>(transA, transB, g, batch, stream); break;
     case 4: launch_gemm<128, 128, 4, 2, true>(transA, transB, g, batch, stream); break;
+    case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
+    case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
+    case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
+    case 11: launch_wsplit<32, 64, 4>(transA, transB, g, batch, stream); break;
     case 16: launch_splitk<1, 1, 4>(transA, transB, g, batch, stream); break;
     case 17: launch_splitk<2, 1, 4>(transA, transB, g, batch, stream); break;
     case 18: launch_splitk<1, 2, 4>(transA, transB, g, batch, stream); break;
@@ -415,7 +606,8 @@ int dgemm(int transA, int transB, int m, int n, int k, double alpha, const doubl
           long long ldc, long long strideC, int batch, hipStream_t stream)
 {
     GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strided(strideA), strided(strideB),
-               strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1};
+               strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1,
+               nullptr, 0, strided(0), nullptr, 0.0, 1, 0};
     return launch(transA, transB, g, batch, 0, stream);
 }
 
@@ -423,9 +615,11 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
 {
     PYPDE_REQUIRE(d, "null GEMM descriptor");
     PYPDE_REQUIRE(!d->alpha_vec || d->alpha_div >= 1, "alpha_div must be >= 1");
+    PYPDE_REQUIRE(!d->gamma_vec || d->gamma_div >= 1, "gamma_div must be >= 1");
     GemmArgs g{d->A, d->B, d->D, d->C, d->lda, d->ldb, d->ldd, d->ldc, bidx(d->iA), bidx(d->iB),
                bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
-               d->alpha_div < 1 ? 1 : d->alpha_div};
+               d->alpha_div < 1 ? 1 : d->alpha_div, d->E, d->lde, bidx(d->iE), d->gamma_vec,
+               d->gamma, d->gamma_div < 1 ? 1 : d->gamma_div, d->zero_first};
     return launch(d->transA, d->transB, g, d->batch, d->variant, stream);
 }
 

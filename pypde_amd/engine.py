@@ -1,7 +1,7 @@
 """Fused RBC2D time step on the GPU: a batch of same-shape instances with
 per-instance Rayleigh / Prandtl numbers and time steps.
 
-The Euler IMEX step of navier/rbc2d.py:396-434 is regrouped into 21 kernel
+The Euler IMEX step of navier/rbc2d.py:396-434 is regrouped into 19 kernel
 launches whatever the batch size: every separable 1-D operator chain of the
 reference (stencil -> derivative -> zero-pad -> DCT, forward DCT -> truncate ->
 pseudo-inverse B, ...) is folded into one dense operator per axis and applied
@@ -28,9 +28,9 @@ Step (per instance; s = domain scale, dt/nu/kappa per instance):
    9-10 Helmholtz ADI: four-diagonal LU sweeps along x then y (per-instance tables)
   11-12 div = (D S/s)_x U* S_y^T + S_x V* (D S/s)_y^T
   13-16 Poisson: B_x div H_y^T, per-line eigen-split solves, Q_y^T
-  17    P[0,0] = 0
+  17    P[0,0] = 0 (in the epilogue of 16)
   18    W = P [S; T S; T D S/s]_y^T               (T = (S^T S)^-1 S^T)
-  19-20 pres = pres - nu div beta + S_x W_0 / dt
+  19-20 pres = pres - nu div beta + S_x W_0 / dt (one GEMM with a second addend)
   21    U = U* - (T D S/s)_x W_1,  V = V* - (T S)_x W_2
 """
 import ctypes
@@ -130,6 +130,13 @@ class _Gemm:
         d.iD = kw.get("iD", _bidx())
         d.alpha_vec = kw.get("alpha_vec", None)
         d.alpha_div = kw.get("alpha_div", 1)
+        d.E = kw.get("E", None)
+        d.lde = kw.get("lde", 0)
+        d.iE = kw.get("iE", _bidx())
+        d.gamma_vec = kw.get("gamma_vec", None)
+        d.gamma = kw.get("gamma", 0.0)
+        d.gamma_div = kw.get("gamma_div", 1)
+        d.zero_first = kw.get("zero_first", 0)
         self.desc = d
         self.flops = 2.0 * d.m * d.n * d.k * d.batch
 
@@ -333,6 +340,16 @@ class RBCBatch:
         sX = M0 * M1
         sZ = K * M1
         L = []
+        # 5  Z[1] = -dt pres (B | BD)_y^T   for U, V      (needs only the state)
+        L.append(_Gemm("z_pres", m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
+                       iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
+                       iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
+                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.t_mdt),
+                       alpha_div=2))
+        # 6  Z[2] = X BS_y^T
+        L.append(_Gemm("z_old", m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1, iA=_bidx(s1=sX),
+                       B=p(self.BSy), ldb=M1, tb=1, C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1,
+                       iC=_bidx(s1=sZ)))
         # 1  x-pass: values and x-derivatives on the L0 grid
         L.append(_Gemm("xpass", m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
                        iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
@@ -348,16 +365,6 @@ class RBCBatch:
         L.append(_Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
                        iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
                        iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3))
-        # 5  Z[1] = -dt pres (B | BD)_y^T   for U, V
-        L.append(_Gemm("z_pres", m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
-                       iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
-                       iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
-                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.t_mdt),
-                       alpha_div=2))
-        # 6  Z[2] = X BS_y^T
-        L.append(_Gemm("z_old", m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1, iA=_bidx(s1=sX),
-                       B=p(self.BSy), ldb=M1, tb=1, C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1,
-                       iC=_bidx(s1=sZ)))
         # 7  X = A_f Z_f + const_f
         L.append(_Gemm("rhs", m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
                        iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
@@ -399,24 +406,22 @@ class RBCBatch:
         self._keep += [gp, rp]
         L.append(_Call("poisson_lines", lib.pypde_recurrence_solve, ctypes.byref(gp), ctypes.byref(rp)))
         # 16  P = RP Q_y^T
+        # 17  ... with the gauge P[0,0] = 0 applied in the GEMM epilogue
         L.append(_Gemm("poisson_qy", m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Qy), ldb=M1, tb=1, C=p(self.P), ldc=M1,
-                       iC=_bidx(s1=sX)))
-        # 17  gauge
-        L.append(_Call("gauge", lib.pypde_set_entries, ctypes.c_void_p(p(self.P)), sX, B, 0.0))
+                       iC=_bidx(s1=sX), zero_first=1))
         # 18  W = P [S; T S; T D S/s]_y^T
         NW = N1 + 2 * M1
         L.append(_Gemm("corr_y", m=M0, n=NW, k=M1, batch=B, A=p(self.P), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Wy), ldb=M1, tb=1, C=p(self.W1), ldc=NW,
                        iC=_bidx(s1=M0 * NW)))
-        # 19  pres -= nu div beta
-        L.append(_Call("pres_div", lib.pypde_scaled_sub, N0 * N1, B, ctypes.c_void_p(p(self.t_nu)),
-                       self.beta, ctypes.c_void_p(p(self.div)), ctypes.c_void_p(p(self.pres))))
-        # 20  pres += S_x W_0 / dt
+        # 19-20  pres = (pres - nu div beta) + S_x W_0 / dt   (one GEMM, fused epilogue)
         L.append(_Gemm("pres_p", m=N0, n=N1, k=M0, batch=B, A=p(self.Snx), lda=M0,
                        B=p(self.W1), ldb=NW, iB=_bidx(s1=M0 * NW), D=p(self.pres), ldd=N1,
                        iD=_bidx(s1=N0 * N1), beta=1.0, C=p(self.pres), ldc=N1,
-                       iC=_bidx(s1=N0 * N1), alpha_vec=p(self.t_idt)))
+                       iC=_bidx(s1=N0 * N1), alpha_vec=p(self.t_idt),
+                       E=p(self.div), lde=N1, iE=_bidx(s1=N0 * N1), gamma_vec=p(self.t_nu),
+                       gamma=self.beta))
         # 21  velocity correction
         L.append(_Gemm("correct", m=M0, n=M1, k=M0, batch=2 * B, A=p(self.Acorr), lda=M0,
                        iA=_bidx(s1=M0 * M0, m1=2), B=p(self.W1) + 8 * N1, ldb=NW,

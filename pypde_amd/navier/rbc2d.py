@@ -4,7 +4,7 @@ navier/rbc2d.py:28-434).
 Same constructor (`NavierStokes(case, shape=, ra=, pr=, dt=, tsave=, dealias=,
 integrator=, beta=, aspect=)`), fields (T, U, V, P, pres with .vhat / .v),
 `update()`, `iterate()`, `eval_Nu()`.  The Euler step with beta = 1 runs on
-the fused engine (pypde_amd.engine.RBCBatch, 21 launches per step, optionally
+the fused engine (pypde_amd.engine.RBCBatch, 19 launches per step, optionally
 replayed from a HIP graph); RK3 and beta != 1 run the reference's operation
 sequence on the drop-in GPU operators.
 """

@@ -275,5 +275,36 @@ def main():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def c3(nsteps=2000):
+    """Configuration c3: 512x512, rbc, Ra=1e8, Pr=1, dt=1e-3, Euler, seed 0, noise
+    1e-3, 2000 steps: Nusselt / kinetic-energy series, the leading 32x32 block
+    of the final temperature coefficients and the reference's wall time per step
+    (one thread)."""
+    NavierStokes, _, avg_vol = import_reference()
+    ns = NavierStokes(case="rbc", shape=(512, 512), ra=1e8, pr=1.0, dt=1e-3, tsave=None,
+                      dealias=True, integrator="eu", beta=1.0, aspect=1.0)
+    seeded(ns, seed=0)
+    nu_plate, nu_vol, ke, wall = [], [], [], []
+    for n in range(nsteps):
+        t0 = time.perf_counter()
+        ns.update()
+        ns.update_time()
+        wall.append(time.perf_counter() - t0)
+        a, b = quiet_nu(ns)
+        nu_plate.append(a)
+        nu_vol.append(b)
+        ke.append(kinetic_energy(ns, avg_vol))
+        if n % 50 == 0:
+            print("c3 step %d  %.3f s/step  Nu %.12f  KE %.6e" % (n, wall[-1], a, ke[-1]), flush=True)
+    out = dict(c3_nu_plate=np.array(nu_plate), c3_nu_vol=np.array(nu_vol), c3_ke=np.array(ke),
+               c3_T_final_32=ns.T.vhat[:32, :32].copy(), c3_sec_per_step=np.array(wall))
+    path = os.path.join(HERE, "rbc2d_c3_512.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "c3":
+        c3()
+    else:
+        main()
