@@ -201,7 +201,7 @@ def time_hetero(steps=500, warm=5, n=256, seed=7):
                 finite=finite, setup_sec=round(setup, 1))
 
 
-def microbench(nf=512, reps=3):
+def microbench(nf=512, reps=10):
     """Kernel chain of 512 fields at 256x256 (per-field algorithmic bytes in the
     docstring of each entry); returns per-kernel time and HBM fraction."""
     import torch
@@ -218,6 +218,9 @@ def microbench(nf=512, reps=3):
     ch = pypde_amd.Base(N, "CH")
     cd = pypde_amd.Base(N, "CD")
     Bk, Fw = ch.backward_matrix(), ch.forward_matrix()
+    # parity-folded transforms (the cosine matrices are mirror-symmetric)
+    opB, opF = ops.MirrorOperator(Bk), ops.MirrorOperator(Fw)
+    assert opB.mode and opF.mode
     s = cd.stencil_vector()
     Bm, _, Ax = hholtz_matrices(cd, 0.01, 1.0)
     rhs_x, rhs_y = PlanRHS(Bm, 2, 0), PlanRHS(Bm, 2, 1)
@@ -245,14 +248,16 @@ def microbench(nf=512, reps=3):
         out[name] = rec
 
     fb = 8 * 2 * N * N
-    bufs["u1"] = ops.matmul_axis(Bk, c, 0)
-    run("K1_backward_x", lambda: ops.matmul_axis(Bk, c, 0, out=bufs["u1"]), fb, 2 * N * N * N)
-    bufs["u2"] = ops.matmul_axis(Bk, bufs["u1"], 1)
-    run("K2_backward_y", lambda: ops.matmul_axis(Bk, bufs["u1"], 1, out=bufs["u2"]), fb, 2 * N * N * N)
-    bufs["c1"] = ops.matmul_axis(Fw, bufs["u2"], 0)
-    run("K3_forward_x", lambda: ops.matmul_axis(Fw, bufs["u2"], 0, out=bufs["c1"]), fb, 2 * N * N * N)
-    bufs["c2"] = ops.matmul_axis(Fw, bufs["c1"], 1)
-    run("K4_forward_y", lambda: ops.matmul_axis(Fw, bufs["c1"], 1, out=bufs["c2"]), fb, 2 * N * N * N)
+    # MFMA work issued by a folded transform: half the dense count 2 N^3 per field
+    fl = N * N * N
+    bufs["u1"] = opB.apply(c, 0)
+    run("K1_backward_x", lambda: opB.apply(c, 0, out=bufs["u1"]), fb, fl)
+    bufs["u2"] = opB.apply(bufs["u1"], 1)
+    run("K2_backward_y", lambda: opB.apply(bufs["u1"], 1, out=bufs["u2"]), fb, fl)
+    bufs["c1"] = opF.apply(bufs["u2"], 0)
+    run("K3_forward_x", lambda: opF.apply(bufs["u2"], 0, out=bufs["c1"]), fb, fl)
+    bufs["c2"] = opF.apply(bufs["c1"], 1)
+    run("K4_forward_y", lambda: opF.apply(bufs["c1"], 1, out=bufs["c2"]), fb, fl)
     run("K5_from_cheb_x", lambda: bufs.__setitem__("g1", cd.from_chebyshev(bufs["c2"], 0)),
         8 * (N * N + M * N))
     run("K6_from_cheb_y", lambda: bufs.__setitem__("g2", cd.from_chebyshev(bufs["g1"], 1)),

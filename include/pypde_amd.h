@@ -83,6 +83,26 @@ typedef struct pypde_gemm_desc {
     double gamma;
     int gamma_div;
     int zero_first;
+    long long csn; /* column stride of C, D and E (0 or 1: contiguous) */
+    /* parity folds of mirror-symmetric operators (halve the MFMA work), variants 1-3:
+     *   fold 1 / 2: output rows / columns of C come in blocks of fold_len mirrored
+     *     pairs; m (fold 1) or n (fold 2) counts the first halves of the blocks;
+     *     E = A B over k (first product), O = A1 B1 over k1 (second product) and
+     *     C[j] = E + O, C[fold_len-1-j] = sign_b (E - O), sign_b = -1 where bit b
+     *     of fold_sign is set (b = block index);
+     *   fold 3 / 4: the k-th element of A (3) or B (4) is X[k] + s_e X[fold_len-1-k]
+     *     with k < fold_len / 2 = k and s_e = mirror_sign * (-1)^((e / mirror_div) % 2).
+     * ska / skb: element stride along k of A (not transposed) / B (transposed). */
+    int fold;
+    const double* A1;
+    const double* B1;
+    long long lda1, ldb1;
+    int k1;
+    long long ska, skb;
+    int fold_len;
+    unsigned fold_sign;
+    double mirror_sign;
+    int mirror_div;
 } pypde_gemm_desc;
 
 /* Batched GEMM  C[e] = alpha_e * op(A[e]) @ op(B[e]) + beta * D[e]  with the

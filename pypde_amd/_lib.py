@@ -55,7 +55,12 @@ class GemmDesc(ctypes.Structure):
                 ("alpha_div", ctypes.c_int), ("variant", ctypes.c_int),
                 ("E", ctypes.c_void_p), ("lde", ctypes.c_longlong), ("iE", BIdx),
                 ("gamma_vec", ctypes.c_void_p), ("gamma", ctypes.c_double),
-                ("gamma_div", ctypes.c_int), ("zero_first", ctypes.c_int)]
+                ("gamma_div", ctypes.c_int), ("zero_first", ctypes.c_int),
+                ("csn", ctypes.c_longlong), ("fold", ctypes.c_int), ("A1", ctypes.c_void_p),
+                ("B1", ctypes.c_void_p), ("lda1", ctypes.c_longlong), ("ldb1", ctypes.c_longlong),
+                ("k1", ctypes.c_int), ("ska", ctypes.c_longlong), ("skb", ctypes.c_longlong),
+                ("fold_len", ctypes.c_int), ("fold_sign", ctypes.c_uint),
+                ("mirror_sign", ctypes.c_double), ("mirror_div", ctypes.c_int)]
 
 
 class Recurrence(ctypes.Structure):

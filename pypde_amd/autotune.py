@@ -20,6 +20,7 @@ import os
 import torch
 
 ROWS_CANDIDATES = (1, 2, 4, 8)
+FOLD_CANDIDATES = (1, 2, 3)
 GEMM_CANDIDATES = (1, 2, 3, 4, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
@@ -85,7 +86,7 @@ def choose(key, run, candidates, default=0):
 
 def gemm_key(d):
     return ("gemm", d.transA, d.transB, d.m, d.n, d.k, d.batch, d.lda, d.ldb, d.ldc,
-            d.D is not None and d.beta != 0.0)
+            d.D is not None and d.beta != 0.0, d.fold, d.csn, d.ska)
 
 
 def tune_gemm_desc(d, launch, allow_alias=False):
@@ -101,5 +102,5 @@ def tune_gemm_desc(d, launch, allow_alias=False):
     def run(v):
         d.variant = v
         launch(d)
-    d.variant = choose(gemm_key(d), run, GEMM_CANDIDATES)
+    d.variant = choose(gemm_key(d), run, FOLD_CANDIDATES if d.fold else GEMM_CANDIDATES)
     return d.variant

@@ -106,6 +106,7 @@ class MetaBase:
         self.name = self.__class__.__name__
         self.id = None
         self._dev = _DeviceCache()
+        self._mops = {}
         if dealias is not None:
             self.create_dealiased_base(self.N * dealias)
 
@@ -123,6 +124,13 @@ class MetaBase:
 
     def slice(self):
         return slice(0, self.N)
+
+    def _mirror(self, key, mat):
+        """Cached ops.MirrorOperator of a device operator table."""
+        op = self._mops.get(key)
+        if op is None:
+            op = self._mops[key] = ops.MirrorOperator(mat)
+        return op
 
     def create_dealiased_base(self, size):
         self.dealias = self.__class__(int(size), dealias=None)
@@ -175,7 +183,7 @@ class Chebyshev(MetaBase):
         if not mass:
             raise NotImplementedError("forward_fft(mass=False) is not provided")
         f = ops.to_dev(f)
-        return ops.matmul_axis(self.forward_matrix(n_coef), f, axis)
+        return self._mirror(("fwd", n_coef), self.forward_matrix(n_coef)).apply(f, axis)
 
     def backward_fft(self, c, axis=0):
         """Values at the grid of a series with c.shape[axis] <= N coefficients."""
@@ -183,7 +191,7 @@ class Chebyshev(MetaBase):
         n = ops._len(c, axis)
         if n > self.N:
             raise AssertionError("Shape mismatch (%d) (%d)" % (n, self.N))
-        return ops.matmul_axis(self.backward_matrix(n), c, axis)
+        return self._mirror(("bwd", n), self.backward_matrix(n)).apply(c, axis)
 
     def derivative(self, fhat, deriv, out_cheby=True, axis=0, scale_pow=1.0):
         fhat = ops.to_dev(fhat)
@@ -278,7 +286,7 @@ class GalerkinChebyshev(MetaBase):
         n = ops._len(c, axis)
         if n > self.M:
             raise AssertionError("Shape mismatch (%d) (%d)" % (n, self.M))
-        v = ops.matmul_axis(self.backward_matrix(n), c, axis)
+        v = self._mirror(("gbwd", n), self.backward_matrix(n)).apply(c, axis)
         if bc is not None:
             v = v + self.bc.backward_fft(bc, axis=axis)
         return v

@@ -83,6 +83,7 @@ struct GemmArgs {
     double gamma;
     int gamma_div;
     int zero_first;           // set element (0,0) of every C[e] to 0 (pressure gauge)
+    long long csn;            // column stride of C, D and E (1: contiguous rows)
 };
 
 // Output update shared by every kernel family:
@@ -93,7 +94,7 @@ struct Epilogue {
     double* C;
     const double* D;
     const double* E;
-    long long ldc, ldd, lde;
+    long long ldc, ldd, lde, csn;
     double alpha, beta, ge;
     bool zero_first;
     __device__ __forceinline__ Epilogue(const GemmArgs& g, int bz)
@@ -104,6 +105,7 @@ struct Epilogue {
         ldc = g.ldc;
         ldd = g.ldd;
         lde = g.lde;
+        csn = g.csn;
         alpha = g.alpha_vec ? g.alpha * g.alpha_vec[bz / g.alpha_div] : g.alpha;
         beta = g.beta;
         ge = g.gamma_vec ? g.gamma_vec[bz / g.gamma_div] : 1.0;
@@ -113,13 +115,14 @@ struct Epilogue {
     __device__ __forceinline__ void store(int gm, int gn, double acc) const
     {
         double v = alpha * acc;
+        const long long cn = (long long)gn * csn;
         if (D || E) {
-            double d = D ? beta * D[(long long)gm * ldd + gn] : 0.0;
-            if (E) d = d - (ge * E[(long long)gm * lde + gn]) * beta_e_gamma;
+            double d = D ? beta * D[(long long)gm * ldd + cn] : 0.0;
+            if (E) d = d - (ge * E[(long long)gm * lde + cn]) * beta_e_gamma;
             v = d + v;
         }
         if (zero_first && gm == 0 && gn == 0) v = 0.0;
-        C[(long long)gm * ldc + gn] = v;
+        C[(long long)gm * ldc + cn] = v;
     }
     double beta_e_gamma;
 };
@@ -538,6 +541,221 @@ static void launch_wsplit(bool ta, bool tb, const GemmArgs& g, int batch, hipStr
     else hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, true>), grid, block, 0, s, g);
 }
 
+// ---------------------------------------------------------------------------
+// Parity-folded GEMMs (LDS-tiled, BK = 16, single LDS buffer).
+//
+// The collocation operators of the spectral step are mirror-symmetric: for a
+// backward operator G (grid point j, coefficient k) G[L-1-j, k] =
+// s (-1)^k G[j, k], and for a forward operator F, F[k, L-1-j] = s (-1)^k F[k, j].
+// Each half of the work is then a GEMM of half the size:
+//   MODE 1 / 2 (output fold, rows / columns of C): with E = A0 B0 (even k) and
+//     O = A1 B1 (odd k) over the first half of each length-Lb block of output
+//     rows (columns), C[j] = E + O and C[Lb-1-j] = sign_b (E - O).  One
+//     workgroup runs both products (interleaved slab by slab) into two sets of
+//     accumulators and writes both mirrored outputs.
+//   MODE 3 / 4 (input fold of A's / B's k): the operand element k of a GEMM
+//     over the first half of a length-Kf range is  X[k] + s_e X[Kf-1-k]; the
+//     sign s_e alternates with the output parity, which the caller maps to
+//     batch entries (even / odd output rows or columns, written with row or
+//     column stride 2).
+// Both halve the MFMA work of a transform.
+struct FoldArgs {
+    const double* A1;  // second product (modes 1, 2)
+    const double* B1;
+    long long lda1, ldb1;
+    int k1;
+    long long ska, skb;   // k-stride of A (not transposed) and of B (transposed)
+    int fold_len;         // modes 1, 2: block length Lb; modes 3, 4: Kf
+    unsigned fold_sign;   // modes 1, 2: bit b set: sign_b = -1
+    double mirror_sign;   // modes 3, 4: s_e = mirror_sign * (-1)^((e / mirror_div) % 2)
+    int mirror_div;
+};
+
+template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, FoldArgs f)
+{
+    constexpr int BK = 16;
+    constexpr int NT = 64 * WGM * WGN;
+    constexpr int PA = BM + 1;
+    constexpr int PB = BN + 1;
+    constexpr int WM = BM / WGM;
+    constexpr int WN = BN / WGN;
+    constexpr int TM = WM / 16;
+    constexpr int TN = WN / 16;
+    constexpr int LA = BM * BK / NT;
+    constexpr int LB = BN * BK / NT;
+    constexpr bool DUAL = (MODE == 1 || MODE == 2);
+    constexpr int NACC = DUAL ? 2 : 1;
+    static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+    static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "tile / thread mismatch");
+    static_assert(NT % BK == 0 && NT % BM == 0 && NT % BN == 0, "load mapping");
+    __shared__ double As[BK * PA];
+    __shared__ double Bs[BK * PB];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave / WGN, wn = wave % WGN;
+    const int bz = blockIdx.z;
+    const int m0 = blockIdx.y * BM;
+    const int n0 = blockIdx.x * BN;
+    const long long oA = bofs(bz, g.iA), oB = bofs(bz, g.iB);
+    const double se = (MODE >= 3) ? (((bz / f.mirror_div) & 1) ? -f.mirror_sign : f.mirror_sign) : 0.0;
+    const int kf1 = f.fold_len - 1;
+
+    v4d acc[NACC][TM][TN];
+#pragma unroll
+    for (int a = 0; a < NACC; ++a)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[a][i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[LA], rb[LB];
+    auto a_rc = [&](int q, int& r, int& c) {
+        if (!TA) { c = tid % BK; r = tid / BK + q * (NT / BK); }
+        else     { r = tid % BM; c = tid / BM + q * (NT / BM); }
+    };
+    auto b_rc = [&](int q, int& r, int& c) {
+        if (!TB) { c = tid % BN; r = tid / BN + q * (NT / BN); }
+        else     { r = tid % BK; c = tid / BK + q * (NT / BK); }
+    };
+    // slab v: product p = v & 1 (dual modes; else 0), k0 = (v >> DUAL) * BK
+    auto load_slab = [&](int v) {
+        const int p = DUAL ? (v & 1) : 0;
+        const int k0 = (DUAL ? (v >> 1) : v) * BK;
+        const double* __restrict__ A = (p ? f.A1 : g.A) + oA;
+        const double* __restrict__ B = (p ? f.B1 : g.B) + oB;
+        const long long lda = p ? f.lda1 : g.lda, ldb = p ? f.ldb1 : g.ldb;
+        const int K = p ? f.k1 : g.k;
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            const int gm = m0 + r, gk = k0 + c;
+            double v = 0.0;
+            if (gm < g.m && gk < K) {
+                v = TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + (long long)gk * f.ska];
+                if (MODE == 3) {
+                    const int mk = kf1 - gk;
+                    v += se * (TA ? A[(long long)mk * lda + gm] : A[(long long)gm * lda + (long long)mk * f.ska]);
+                }
+            }
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            const int gk = k0 + r, gn = n0 + c;
+            double v = 0.0;
+            if (gk < K && gn < g.n) {
+                v = TB ? B[(long long)gn * ldb + (long long)gk * f.skb] : B[(long long)gk * ldb + gn];
+                if (MODE == 4) {
+                    const int mk = kf1 - gk;
+                    v += se * (TB ? B[(long long)gn * ldb + (long long)mk * f.skb] : B[(long long)mk * ldb + gn]);
+                }
+            }
+            rb[q] = v;
+        }
+    };
+    auto store_slab = [&]() {
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            As[c * PA + r] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            Bs[r * PB + c] = rb[q];
+        }
+    };
+    auto compute = [&](int a) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[a][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[a][i][j], 0, 0, 0);
+        }
+    };
+
+    const int kmax = DUAL ? max(g.k, f.k1) : g.k;
+    const int nslab = (kmax + BK - 1) / BK;
+    const int nv = DUAL ? 2 * nslab : nslab;
+    if (nv > 0) load_slab(0);
+    for (int v = 0; v < nv; ++v) {
+        __syncthreads();
+        store_slab();
+        __syncthreads();
+        if (v + 1 < nv) load_slab(v + 1);
+        if (DUAL && (v & 1)) compute(NACC - 1);
+        else compute(0);
+    }
+
+    const Epilogue ep(g, bz);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
+                const int gn = n0 + wn * WN + j * 16 + (lane & 15);
+                if (gm >= g.m || gn >= g.n) continue;
+                if (!DUAL) {
+                    ep.store(gm, gn, acc[0][i][j][r]);
+                    continue;
+                }
+                const double e = acc[0][i][j][r], o = acc[NACC - 1][i][j][r];
+                const int Lb = f.fold_len, Lh = (Lb + 1) / 2;
+                const int x = (MODE == 1) ? gm : gn;
+                const int b = x / Lh, jj = x - b * Lh;
+                const int x1 = b * Lb + jj, x2 = b * Lb + Lb - 1 - jj;
+                const double sg = ((f.fold_sign >> b) & 1u) ? -1.0 : 1.0;
+                if (MODE == 1) {
+                    ep.store(x1, gn, e + o);
+                    if (x2 != x1) ep.store(x2, gn, sg * (e - o));
+                } else {
+                    ep.store(gm, x1, e + o);
+                    if (x2 != x1) ep.store(gm, x2, sg * (e - o));
+                }
+            }
+}
+
+template <int BM, int BN, int WGM, int WGN, int MODE>
+static void launch_fold_m(bool ta, bool tb, const GemmArgs& g, const FoldArgs& f, int batch, hipStream_t s)
+{
+    dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
+    dim3 block(64 * WGM * WGN);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, false>), grid, block, 0, s, g, f);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, true>), grid, block, 0, s, g, f);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, false>), grid, block, 0, s, g, f);
+    else hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, true>), grid, block, 0, s, g, f);
+}
+
+template <int BM, int BN, int WGM, int WGN>
+static void launch_fold(int mode, bool ta, bool tb, const GemmArgs& g, const FoldArgs& f, int batch,
+                        hipStream_t s)
+{
+    switch (mode) {
+    case 1: launch_fold_m<BM, BN, WGM, WGN, 1>(ta, tb, g, f, batch, s); break;
+    case 2: launch_fold_m<BM, BN, WGM, WGN, 2>(ta, tb, g, f, batch, s); break;
+    case 3: launch_fold_m<BM, BN, WGM, WGN, 3>(ta, tb, g, f, batch, s); break;
+    default: launch_fold_m<BM, BN, WGM, WGN, 4>(ta, tb, g, f, batch, s); break;
+    }
+}
+
 static BIdx bidx(const pypde_bidx& b)
 {
     BIdx r{b.s1, b.s2, b.d1 < 1 ? 1 : b.d1, b.m1, b.d2 < 1 ? 1 : b.d2, b.m2};
@@ -607,7 +825,7 @@ int dgemm(int transA, int transB, int m, int n, int k, double alpha, const doubl
 {
     GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strided(strideA), strided(strideB),
                strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1,
-               nullptr, 0, strided(0), nullptr, 0.0, 1, 0};
+               nullptr, 0, strided(0), nullptr, 0.0, 1, 0, 1};
     return launch(transA, transB, g, batch, 0, stream);
 }
 
@@ -619,8 +837,46 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
     GemmArgs g{d->A, d->B, d->D, d->C, d->lda, d->ldb, d->ldd, d->ldc, bidx(d->iA), bidx(d->iB),
                bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
                d->alpha_div < 1 ? 1 : d->alpha_div, d->E, d->lde, bidx(d->iE), d->gamma_vec,
-               d->gamma, d->gamma_div < 1 ? 1 : d->gamma_div, d->zero_first};
-    return launch(d->transA, d->transB, g, d->batch, d->variant, stream);
+               d->gamma, d->gamma_div < 1 ? 1 : d->gamma_div, d->zero_first,
+               d->csn < 1 ? 1 : d->csn};
+    if (d->fold == 0) return launch(d->transA, d->transB, g, d->batch, d->variant, stream);
+
+    // parity-folded forms
+    PYPDE_REQUIRE(d->fold >= 1 && d->fold <= 4, "unknown fold mode %d", d->fold);
+    PYPDE_REQUIRE(g.m >= 0 && g.n >= 0 && g.k >= 0 && d->batch >= 0, "negative GEMM size");
+    if (g.m == 0 || g.n == 0 || d->batch == 0) return OK;
+    PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
+    PYPDE_REQUIRE(d->batch <= 65535, "GEMM batch %d exceeds 65535", d->batch);
+    PYPDE_REQUIRE(d->fold_len >= 1, "fold_len must be >= 1");
+    FoldArgs f{d->A1, d->B1, d->lda1, d->ldb1, d->k1, d->ska < 1 ? 1 : d->ska,
+               d->skb < 1 ? 1 : d->skb, d->fold_len, d->fold_sign, d->mirror_sign,
+               d->mirror_div < 1 ? 1 : d->mirror_div};
+    if (d->fold <= 2) {
+        PYPDE_REQUIRE(d->A1 && d->B1 && d->k1 >= 0, "output fold needs the second product (A1, B1, k1)");
+        const long long half = (d->fold_len + 1) / 2;
+        const long long dim = d->fold == 1 ? g.m : g.n;
+        PYPDE_REQUIRE(dim % half == 0, "output fold: the folded extent %lld is not a multiple of %lld",
+                      dim, half);
+    } else {
+        PYPDE_REQUIRE(d->fold_len % 2 == 0 && 2 * (long long)g.k == d->fold_len,
+                      "input fold: k (%d) must be half of an even fold_len (%d)", g.k, d->fold_len);
+    }
+    int variant = d->variant;
+    if (variant == 0) {
+        const long long tiles128 = (long long)((g.m + 127) / 128) * ((g.n + 127) / 128) * d->batch;
+        const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * d->batch;
+        variant = tiles128 >= 512 ? 3 : (tiles64 >= 512 ? 1 : 2);
+    }
+    PYPDE_REQUIRE(variant >= 1 && variant <= 3, "folded GEMMs support variants 1-3 (got %d)", variant);
+    const bool ta = d->transA, tb = d->transB;
+    // smaller wave tiles than the plain kernels: the output folds carry two
+    // accumulator sets (32x32 per wave at 128x128 over 16 waves)
+    switch (variant) {
+    case 1: launch_fold<64, 64, 4, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    case 2: launch_fold<32, 32, 2, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    default: launch_fold<128, 128, 4, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    }
+    return check_launch("pypde_dgemm_batched (folded)");
 }
 
 }  // namespace pypde

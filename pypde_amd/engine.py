@@ -111,6 +111,30 @@ def _bidx(s1=0, d1=1, m1=0, s2=0, d2=1, m2=0):
     return b
 
 
+def _mirror_signs(G, Lb, tol=1e-12):
+    """Signs s_b with G[b Lb + Lb-1-j, k] = s_b (-1)^k G[b Lb + j, k] for every
+    block b of Lb rows, or None when G is not mirror-symmetric this way."""
+    nb = G.shape[0] // Lb
+    par = (-1.0) ** np.arange(G.shape[1])
+    signs = []
+    for b in range(nb):
+        blk = G[b * Lb:(b + 1) * Lb]
+        scale = max(np.abs(blk).max(), 1e-300)
+        found = None
+        for sg in (1.0, -1.0):
+            if np.abs(blk[::-1] - sg * par * blk).max() <= tol * scale:
+                found = sg
+                break
+        if found is None:
+            return None
+        signs.append(found)
+    return signs
+
+
+def _sign_bits(signs):
+    return sum(1 << b for b, sg in enumerate(signs) if sg < 0)
+
+
 class _Gemm:
     def __init__(self, name, **kw):
         self.name = name
@@ -137,8 +161,17 @@ class _Gemm:
         d.gamma = kw.get("gamma", 0.0)
         d.gamma_div = kw.get("gamma_div", 1)
         d.zero_first = kw.get("zero_first", 0)
+        d.csn = kw.get("csn", 1)
+        d.fold = kw.get("fold", 0)
+        d.A1, d.B1 = kw.get("A1", None), kw.get("B1", None)
+        d.lda1, d.ldb1, d.k1 = kw.get("lda1", 0), kw.get("ldb1", 0), kw.get("k1", 0)
+        d.ska, d.skb = kw.get("ska", 1), kw.get("skb", 1)
+        d.fold_len, d.fold_sign = kw.get("fold_len", 0), kw.get("fold_sign", 0)
+        d.mirror_sign, d.mirror_div = kw.get("mirror_sign", 0.0), kw.get("mirror_div", 1)
         self.desc = d
-        self.flops = 2.0 * d.m * d.n * d.k * d.batch
+        self.alt = None   # an equivalent unfolded _Gemm; the tuner keeps the faster
+        # MFMA work actually issued (a fold halves it against the unfolded product)
+        self.flops = 2.0 * d.m * d.n * d.k * d.batch * (2 if d.fold in (1, 2) else 1)
 
     def __call__(self, stream):
         check(lib.pypde_dgemm_batched(ctypes.byref(self.desc), stream), self.name)
@@ -220,6 +253,25 @@ class RBCBatch:
                                np.vstack([ax.GT, ax.GdT])]))                   # [3, 2L0, M0]
         self.GGy = T(np.vstack([ay.G, ay.Gd]))                                # [2L1, M1]
         self.BFy = T(ay.BF)                                                   # [M1, L1]
+        # parity-folded halves of the transform operators (even lengths only)
+        self.fold_x = self.fold_y = self.fold_bf = None
+        if M0 % 2 == 0 and L0 % 2 == 0:
+            GG = [np.vstack([ax.G, ax.Gd]), np.vstack([ax.GT, ax.GdT])]
+            sg = [_mirror_signs(g, L0) for g in GG]
+            if sg[0] is not None and sg[0] == sg[1]:
+                tops = [np.vstack([g[:L0 // 2], g[L0:L0 + L0 // 2]]) for g in (GG[0], GG[0], GG[1])]
+                self.fold_x = dict(E=T(np.stack([t[:, 0::2] for t in tops])),
+                                   O=T(np.stack([t[:, 1::2] for t in tops])), sign=_sign_bits(sg[0]))
+        if M1 % 2 == 0 and L1 % 2 == 0:
+            GGy_h = np.vstack([ay.G, ay.Gd])
+            sg = _mirror_signs(GGy_h, L1)
+            if sg is not None:
+                top = np.vstack([GGy_h[:L1 // 2], GGy_h[L1:L1 + L1 // 2]])
+                self.fold_y = dict(E=T(top[:, 0::2]), O=T(top[:, 1::2]), sign=_sign_bits(sg))
+            sg = _mirror_signs(ay.BF.T, L1)   # BF[i, L1-1-j] = s (-1)^i BF[i, j]
+            if sg is not None:
+                h = L1 // 2
+                self.fold_bf = dict(B=T(np.stack([ay.BF[0::2, :h], ay.BF[1::2, :h]])), sign=sg[0])
         self.PBy = T(np.stack([ay.B, ay.BD]))                                 # [2, M1, N1]
         self.BSy = T(ay.BS)                                                   # [M1, M1]
         K = L0 + N0 + M0
@@ -351,20 +403,46 @@ class RBCBatch:
                        B=p(self.BSy), ldb=M1, tb=1, C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1,
                        iC=_bidx(s1=sZ)))
         # 1  x-pass: values and x-derivatives on the L0 grid
-        L.append(_Gemm("xpass", m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
-                       iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
-                       C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1)))
+        g = _Gemm("xpass", m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
+                  iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
+                  C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1))
+        if self.fold_x is not None:
+            fx, h = self.fold_x, M0 // 2
+            g = self._with_alt(g, _Gemm(
+                "xpass", m=L0, n=M1, k=h, batch=3 * B, A=p(fx["E"]), lda=h,
+                iA=_bidx(s1=L0 * h, m1=3), B=p(X), ldb=2 * M1, iB=_bidx(s1=sX),
+                C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1), fold=1, A1=p(fx["O"]),
+                B1=p(X) + 8 * M1, lda1=h, ldb1=2 * M1, k1=h, fold_len=L0, fold_sign=fx["sign"]))
+        L.append(g)
         # 2  y-pass: values and y-derivatives on the L1 grid
-        L.append(_Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
-                       iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
-                       C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1)))
+        g = _Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
+                  iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
+                  C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1))
+        if self.fold_y is not None:
+            fy, h = self.fold_y, M1 // 2
+            g = self._with_alt(g, _Gemm(
+                "ypass", m=2 * L0, n=L1, k=h, batch=3 * B, A=p(Y1), lda=M1, ska=2,
+                iA=_bidx(s1=2 * L0 * M1), B=p(fy["E"]), ldb=h, tb=1,
+                C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1), fold=2, A1=p(Y1) + 8,
+                B1=p(fy["O"]), lda1=M1, ldb1=h, k1=h, fold_len=L1, fold_sign=fy["sign"]))
+        L.append(g)
         # 3  convective products
         L.append(_Call("products", lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
                        ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
         # 4  Z[0] = -dt P BF_y^T
-        L.append(_Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
-                       iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
-                       iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3))
+        g = _Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
+                  iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
+                  iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3)
+        if self.fold_bf is not None:
+            fb, h = self.fold_bf, L1 // 2
+            # entries e = 2 f + q: output columns q::2 of field-instance f
+            g = self._with_alt(g, _Gemm(
+                "z_conv", m=L0, n=M1 // 2, k=h, batch=6 * B, A=p(Pr), lda=L1,
+                iA=_bidx(s1=L0 * L1, d1=2), B=p(fb["B"]), ldb=h, tb=1,
+                iB=_bidx(s1=(M1 // 2) * h, m1=2), C=p(Z), ldc=M1, csn=2,
+                iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt), alpha_div=6,
+                fold=3, fold_len=L1, mirror_sign=fb["sign"], mirror_div=1))
+        L.append(g)
         # 7  X = A_f Z_f + const_f
         L.append(_Gemm("rhs", m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
                        iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
@@ -449,6 +527,12 @@ class RBCBatch:
             e1.record(stream)
             records.append((f.name, e0, e1, f))
 
+    @staticmethod
+    def _with_alt(plain, folded):
+        """The folded launch, remembering the plain one as the alternative."""
+        folded.alt = plain
+        return folded
+
     def tune(self):
         """Autotune the kernel variant of every GEMM launch of the step (the
         state is restored afterwards, so in-place launches can be timed)."""
@@ -457,10 +541,23 @@ class RBCBatch:
             return
         saved = [t.clone() for t in (self.X, self.pres, self.P)]
         s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        for launch in self.launches:
-            if isinstance(launch, _Gemm):
-                autotune.tune_gemm_desc(launch.desc, lambda d: check(
-                    lib.pypde_dgemm_batched(ctypes.byref(d), s), launch.name), allow_alias=True)
+
+        def launch_fn(name):
+            return lambda d: check(lib.pypde_dgemm_batched(ctypes.byref(d), s), name)
+        for i, launch in enumerate(self.launches):
+            if not isinstance(launch, _Gemm):
+                continue
+            autotune.tune_gemm_desc(launch.desc, launch_fn(launch.name), allow_alias=True)
+            if launch.alt is not None:
+                alt = launch.alt
+                autotune.tune_gemm_desc(alt.desc, launch_fn(alt.name), allow_alias=True)
+                forms = (launch, alt)
+                pick = autotune.choose(
+                    ("form",) + autotune.gemm_key(launch.desc) + autotune.gemm_key(alt.desc),
+                    lambda c: launch_fn(forms[c].name)(forms[c].desc), (0, 1))
+                if pick == 1:
+                    alt.alt = launch
+                    self.launches[i] = alt
         torch.cuda.synchronize()
         for t, v in zip((self.X, self.pres, self.P), saved):
             t.copy_(v)

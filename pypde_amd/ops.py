@@ -142,6 +142,93 @@ def matmul_axis(mat, x, axis=0, out=None, alpha=1.0, beta=0.0, add=None):
     return out
 
 
+def _mirror_sign(M, tol=1e-12):
+    """s with M[n-1-j, k] = s (-1)^k M[j, k] for all j, k, else None."""
+    par = (-1.0) ** np.arange(M.shape[1])
+    scale = max(float(np.abs(M).max()), 1e-300)
+    for sg in (1.0, -1.0):
+        if np.abs(M[::-1] - sg * par * M).max() <= tol * scale:
+            return sg
+    return None
+
+
+class MirrorOperator:
+    """A dense operator M [n_out, n_in] applied along an axis (like matmul_axis)
+    with the parity-folded GEMM when M is mirror-symmetric: on the output side,
+    M[n_out-1-j, k] = s (-1)^k M[j, k] (backward transforms: half-size products
+    over even and odd k, both mirrored outputs written by one launch), or on the
+    input side with even n_in and n_out, M[k, n_in-1-j] = s (-1)^k M[k, j]
+    (forward transforms: inputs folded X[j] +- X[n_in-1-j] in the loader, even
+    and odd outputs as two batch entries).  Either halves the MFMA work; other
+    operators fall back to the plain GEMM."""
+
+    def __init__(self, M):
+        M = np.asarray(M.detach().cpu().numpy() if isinstance(M, torch.Tensor) else M, dtype=np.float64)
+        self.shape = M.shape
+        self.M = to_dev(M)
+        self.mode = 0
+        n_out, n_in = M.shape
+        so = _mirror_sign(M)
+        si = _mirror_sign(M.T) if (n_in % 2 == 0 and n_out % 2 == 0) else None
+        if si is not None:    # preferred: one accumulator set
+            h = n_in // 2
+            self.mode, self.sign = 2, si
+            self.P = to_dev(np.stack([M[0::2, :h], M[1::2, :h]]))
+        elif so is not None:
+            h = (n_out + 1) // 2
+            self.mode, self.sign = 1, so
+            self.E = to_dev(np.ascontiguousarray(M[:h, 0::2]))
+            self.O = to_dev(np.ascontiguousarray(M[:h, 1::2]))
+
+    def apply(self, x, axis=0, out=None):
+        _check_axis(x, axis)
+        n_out, n_in = self.shape
+        if _len(x, axis) != n_in:
+            raise AssertionError("Shape mismatch: operator has %d columns, input axis has %d"
+                                 % (n_in, _len(x, axis)))
+        if self.mode == 0 or x.stride(-1) != 1:
+            return matmul_axis(self.M, x, axis, out=out)
+        if out is None:
+            out = torch.empty(_out_shape(x, axis, n_out), dtype=DTYPE, device=x.device)
+        x3, o3 = _as3(x), _as3(out)
+        B = x3.shape[0]
+        sx, so = x3.stride(0) if B > 1 else 0, o3.stride(0) if B > 1 else 0
+        ke, ko = (n_in + 1) // 2, n_in // 2
+        if self.mode == 1 and axis == 0:
+            h = (n_out + 1) // 2
+            nc, ld = x3.shape[2], x3.stride(1)
+            d = _desc(0, 0, h, nc, ke, B, 1.0, 0.0, self.E.data_ptr(), ke, 0, x3.data_ptr(), 2 * ld,
+                      sx, None, 0, 0, o3.data_ptr(), o3.stride(1), so)
+            d.fold, d.A1, d.B1, d.lda1, d.ldb1, d.k1 = 1, self.O.data_ptr(), x3.data_ptr() + 8 * ld, ko, 2 * ld, ko
+            d.fold_len, d.fold_sign = n_out, (1 if self.sign < 0 else 0)
+        elif self.mode == 1:
+            h = (n_out + 1) // 2
+            nr, ld = x3.shape[1], x3.stride(1)
+            d = _desc(0, 1, nr, h, ke, B, 1.0, 0.0, x3.data_ptr(), ld, sx, self.E.data_ptr(), ke, 0,
+                      None, 0, 0, o3.data_ptr(), o3.stride(1), so)
+            d.fold, d.A1, d.B1, d.lda1, d.ldb1, d.k1 = 2, x3.data_ptr() + 8, self.O.data_ptr(), ld, ko, ko
+            d.ska, d.fold_len, d.fold_sign = 2, n_out, (1 if self.sign < 0 else 0)
+        elif axis == 0:
+            h, ho = n_in // 2, n_out // 2
+            nc, ld, ldo = x3.shape[2], x3.stride(1), o3.stride(1)
+            d = _desc(0, 0, ho, nc, h, 2 * B, 1.0, 0.0, self.P.data_ptr(), h, 0, x3.data_ptr(), ld, 0,
+                      None, 0, 0, o3.data_ptr(), 2 * ldo, 0)
+            d.iA, d.iB = _bi(ho * h, m1=2), _bi(sx, d1=2)
+            d.iC = _bi(so, d1=2, s2=ldo, m2=2)
+            d.fold, d.fold_len, d.mirror_sign, d.mirror_div = 4, n_in, self.sign, 1
+        else:
+            h, ho = n_in // 2, n_out // 2
+            nr, ld = x3.shape[1], x3.stride(1)
+            d = _desc(0, 1, nr, ho, h, 2 * B, 1.0, 0.0, x3.data_ptr(), ld, 0, self.P.data_ptr(), h, 0,
+                      None, 0, 0, o3.data_ptr(), o3.stride(1), 0)
+            d.iA, d.iB = _bi(sx, d1=2), _bi(ho * h, m1=2)
+            d.iC = _bi(so, d1=2, s2=1, m2=2)
+            d.csn = 2
+            d.fold, d.fold_len, d.mirror_sign, d.mirror_div = 3, n_in, self.sign, 1
+        _launch_desc(d)
+        return out
+
+
 def gemm(A, B, transA=False, transB=False, alpha=1.0, beta=0.0, D=None, out=None):
     """Plain (batched) row-major GEMM on 2-D or 3-D tensors."""
     A3 = A if A.dim() == 3 else A.unsqueeze(0)

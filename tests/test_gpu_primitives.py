@@ -234,3 +234,131 @@ def test_recurrence_chunking_lengths(amd, N, axis):
     for e in range(3):
         ref = o.FdmaAxis(A, axis)(b[e])
         assert rel(x[e], ref) < 1e-13, e
+
+
+def _sym_operator(rng, nb, Lb, K, signs):
+    """Rows in blocks of Lb, mirror-symmetric within each block:
+    G[b*Lb + Lb-1-j, k] = signs[b] (-1)^k G[b*Lb + j, k]."""
+    G = np.zeros((nb * Lb, K))
+    Lh = (Lb + 1) // 2
+    par = (-1.0) ** np.arange(K)
+    for b in range(nb):
+        top = rng.standard_normal((Lh, K))
+        if Lb % 2:  # middle row: only the components the symmetry allows
+            top[-1] *= (signs[b] * par == 1)
+        G[b * Lb:b * Lb + Lh] = top
+        for j in range(Lb - Lh):
+            G[b * Lb + Lb - 1 - j] = signs[b] * par * top[j]
+    return G
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("Lb,K", [(8, 10), (7, 9), (96, 62)])
+def test_gemm_output_fold(amd, variant, Lb, K):
+    """fold 1 (rows) and 2 (columns): C = G X and C = Y G^T for mirror-symmetric
+    G from half-size products over even / odd k."""
+    import ctypes
+    _, ops, torch = amd
+    from pypde_amd._lib import lib
+    rng = np.random.default_rng(Lb + K + variant)
+    nb, n, batch = 2, 37, 3
+    signs = [1.0, -1.0]
+    G = _sym_operator(rng, nb, Lb, K, signs)
+    Lh = (Lb + 1) // 2
+    top = np.concatenate([G[b * Lb:b * Lb + Lh] for b in range(nb)])
+    Ge, Go = ops.to_dev(np.ascontiguousarray(top[:, 0::2])), ops.to_dev(np.ascontiguousarray(top[:, 1::2]))
+    ke, ko = (K + 1) // 2, K // 2
+    X = rng.standard_normal((batch, K, n))
+    dX = ops.to_dev(X)
+    C = torch.empty((batch, nb * Lb, n), dtype=torch.float64, device=dX.device)
+    d = ops._desc(0, 0, nb * Lh, n, ke, batch, 1.0, 0.0, Ge.data_ptr(), ke, 0, dX.data_ptr(), 2 * n,
+                  dX.stride(0), None, 0, 0, C.data_ptr(), n, C.stride(0))
+    d.fold, d.A1, d.B1, d.lda1, d.ldb1, d.k1 = 1, Go.data_ptr(), dX.data_ptr() + 8 * n, ko, 2 * n, ko
+    d.fold_len, d.fold_sign, d.variant = Lb, 0b10, variant
+    assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+    assert rel(host(C), np.einsum("ik,bkn->bin", G, X)) < 1e-14
+    # columns: C = Y G^T
+    Y = rng.standard_normal((batch, n, K))
+    dY = ops.to_dev(Y)
+    C2 = torch.empty((batch, n, nb * Lb), dtype=torch.float64, device=dY.device)
+    d = ops._desc(0, 1, n, nb * Lh, ke, batch, 1.0, 0.0, dY.data_ptr(), K, dY.stride(0),
+                  Ge.data_ptr(), ke, 0, None, 0, 0, C2.data_ptr(), nb * Lb, C2.stride(0))
+    d.fold, d.A1, d.B1, d.lda1, d.ldb1, d.k1 = 2, dY.data_ptr() + 8, Go.data_ptr(), K, ko, ko
+    d.ska, d.fold_len, d.fold_sign, d.variant = 2, Lb, 0b10, variant
+    assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+    assert rel(host(C2), np.einsum("bnk,ik->bni", Y, G)) < 1e-14
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_gemm_input_fold(amd, variant):
+    """fold 3 / 4: X[k] + s_e X[Kf-1-k] folded into A (3) or B (4) with the sign
+    alternating over batch entries; outputs interleaved with column stride 2."""
+    import ctypes
+    _, ops, torch = amd
+    from pypde_amd._lib import lib
+    rng = np.random.default_rng(40 + variant)
+    Kf, m, n, batch = 24, 19, 13, 4
+    Kh = Kf // 2
+    # fold 3: C_e = sum_k (A[:, k] + s_e A[:, Kf-1-k]) B_e[k, :], entries e = 2q + p
+    A = rng.standard_normal((batch // 2, m, Kf))
+    B = rng.standard_normal((2, Kh, n))
+    dA, dB = ops.to_dev(A), ops.to_dev(B)
+    C = torch.zeros((batch // 2, m, 2 * n), dtype=torch.float64, device=dA.device)
+    d = ops._desc(0, 0, m, n, Kh, batch, 1.0, 0.0, dA.data_ptr(), Kf, 0, dB.data_ptr(), n, 0,
+                  None, 0, 0, C.data_ptr(), 2 * n, 0)
+    d.iA = ops._bi(m * Kf, d1=2)
+    d.iB = ops._bi(Kh * n, m1=2)
+    d.iC = ops._bi(m * 2 * n, d1=2, s2=1, m2=2)
+    d.csn = 2
+    d.fold, d.fold_len, d.mirror_sign, d.mirror_div, d.variant = 3, Kf, -1.0, 1, variant
+    assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+    ref = np.zeros((batch // 2, m, 2 * n))
+    for e in range(batch):
+        qq, pp = divmod(e, 2)
+        s = -1.0 if pp == 0 else 1.0
+        Af = A[qq][:, :Kh] + s * A[qq][:, ::-1][:, :Kh]
+        ref[qq][:, pp::2] = Af @ B[pp]
+    assert rel(host(C), ref) < 1e-14
+    # fold 4: C_e = A_e (B[k] + s_e B[Kf-1-k]) with B transposed ([n][Kf]); rows interleaved
+    At = rng.standard_normal((2, m, Kh))
+    Bt = rng.standard_normal((batch // 2, n, Kf))
+    dA, dB = ops.to_dev(At), ops.to_dev(Bt)
+    C = torch.zeros((batch // 2, 2 * m, n), dtype=torch.float64, device=dA.device)
+    d = ops._desc(0, 1, m, n, Kh, batch, 1.0, 0.0, dA.data_ptr(), Kh, 0, dB.data_ptr(), Kf, 0,
+                  None, 0, 0, C.data_ptr(), 2 * n, 0)
+    d.iA = ops._bi(m * Kh, m1=2)
+    d.iB = ops._bi(n * Kf, d1=2)
+    d.iC = ops._bi(2 * m * n, d1=2, s2=n, m2=2)
+    d.fold, d.fold_len, d.mirror_sign, d.mirror_div, d.variant = 4, Kf, 1.0, 1, variant
+    assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+    ref = np.zeros((batch // 2, 2 * m, n))
+    for e in range(batch):
+        qq, pp = divmod(e, 2)
+        s = 1.0 if pp == 0 else -1.0
+        Bf = Bt[qq][:, :Kh] + s * Bt[qq][:, ::-1][:, :Kh]
+        ref[qq][pp::2] = At[pp] @ Bf.T
+    assert rel(host(C), ref) < 1e-14
+
+
+@pytest.mark.parametrize("N", [23, 24, 64])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_mirror_operator_transforms(amd, N, axis):
+    """Folded transforms (ops.MirrorOperator) along both axes equal the dense
+    products: Chebyshev backward to the dealiased grid (output fold), forward
+    from it (input fold, even sizes), and a non-symmetric matrix (plain GEMM)."""
+    pypde_amd, ops, torch = amd
+    from pypde_amd.bases import cheb_backward_matrix, cheb_forward_matrix
+    rng = np.random.default_rng(N + 10 * axis)
+    L = int(1.5 * N)
+    Bk = cheb_backward_matrix(L, N)      # [L, N]
+    Fw = cheb_forward_matrix(L, N)       # [N, L]
+    R = rng.standard_normal((N + 1, L))
+    for M, want in ((Bk, 1), (Fw, 2 if (L % 2 == 0 and N % 2 == 0) else 0), (R, 0),
+                    (cheb_backward_matrix(N, N), 2 if N % 2 == 0 else 1)):
+        op = ops.MirrorOperator(M)
+        assert op.mode == want
+        n_in = M.shape[1]
+        x = rng.standard_normal((3, n_in, 7) if axis == 0 else (3, 7, n_in))
+        y = host(op.apply(ops.to_dev(x), axis))
+        ref = np.einsum("ik,bkn->bin", M, x) if axis == 0 else np.einsum("bnk,ik->bni", x, M)
+        assert rel(y, ref) < 1e-13, (M.shape, axis)
