@@ -1,0 +1,45 @@
+"""512x512 trajectory (rbc, Ra=1e8, Pr=1, dt=1e-3, Euler, seed 0, noise 1e-3,
+2000 steps) against the reference's own run (navier/rbc2d.py:396-434, fixture
+tests/golden/rbc2d_c3_512.npz written by tests/golden/make_golden.py c3).
+
+Tolerances:
+  Nu_vol, KE     1e-10 relative (KE from step 50 on: the velocity starts at
+                 roundoff level, so its first relative digits are noise);
+  Nu_plate       5e-9 absolute: the plate derivative of a 512-mode Chebyshev
+                 series amplifies coefficient roundoff by about N^2, so two
+                 FP64 evaluations of the same field (FFT-based DCT against the
+                 dense cosine matrix) already differ by 0.7-2e-9 there;
+  final T        the leading 32x32 coefficient block, 1e-9 relative."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_c3_trajectory_512():
+    from pypde_amd.navier import NavierStokes
+    ref = np.load(os.path.join(HERE, "golden", "rbc2d_c3_512.npz"))
+    N, nsteps = 512, 2000
+    ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None)
+    rng = np.random.default_rng(0)
+    ns.set_state(T=ns.T.forward(rng.uniform(-1e-3, 1e-3, size=(N, N))))
+    nu, nv, ke = np.empty(nsteps), np.empty(nsteps), np.empty(nsteps)
+    for n in range(nsteps):
+        ns.update()
+        nu[n], nv[n] = ns.eval_Nu()
+        ke[n] = ns.eval_KE()
+    e_nu = np.abs(nu - ref["c3_nu_plate"])
+    e_nv = np.abs(nv - ref["c3_nu_vol"]) / np.abs(ref["c3_nu_vol"])
+    e_ke = np.abs(ke - ref["c3_ke"]) / np.abs(ref["c3_ke"])
+    print("c3: max |dNu_plate| %.2e, max rel dNu_vol %.2e, max rel dKE (n>=50) %.2e"
+          % (e_nu.max(), e_nv[1:].max(), e_ke[50:].max()))
+    assert e_nu.max() < 5e-9
+    assert e_nv[1:].max() < 1e-10
+    assert e_ke[50:].max() < 1e-10
+    T = ns.T.vhat.detach().cpu().numpy()[:32, :32]
+    Tr = ref["c3_T_final_32"]
+    assert np.linalg.norm(T - Tr) / np.linalg.norm(Tr) < 1e-9
