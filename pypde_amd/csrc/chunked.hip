@@ -13,7 +13,7 @@
 // of the pivots precomputed at plan time (no division in the kernel).
 //
 // Parallel layout: a block owns 64 lines (one lane each) and P = ceil(n / CH)
-// wave64s (CH = 32); wave w holds positions [w*CH, (w+1)*CH) of its lane's
+// wave64s (CH = 16 or 32, below); wave w holds positions [w*CH, (w+1)*CH) of its lane's
 // line in registers.  Each sweep runs twice per chunk: once to obtain the chunk's
 // affine transfer (end value and carry coefficient), then, after the carries
 // are combined across the P chunks through LDS, once more from the exact
@@ -28,7 +28,11 @@
 
 namespace pypde {
 
-constexpr int CH = 32;    // positions per chunk (per lane)
+// Positions per chunk (per lane): 16 halves the serial chain of each wave and
+// the registers it holds, and is faster for the four-diagonal and per-line
+// Poisson forms up to n = 256 (16 waves); the pre-stencil form and longer lines
+// use 32.
+constexpr int kMaxWaves = 16;
 constexpr int TW = 16;    // row-tile width for axis-1 transposes
 constexpr int TP = TW + 1;
 
@@ -76,6 +80,7 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
                                        int base, int n,
                                        int P, int w, int lane, double* xch)
 {
+    constexpr int CH = PRE ? NX - 2 : NX;
     // table slots: 0 F, 1 LM, 2 R, 3 MU, 4 NU, 5 pre
     if (PRE) {
 #pragma unroll
@@ -173,7 +178,7 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
 // Occupancy hint: the pre-stencil (tri-diagonal projection) form runs best
 // capped at 128 VGPRs (4 waves per SIMD, two 8-wave blocks per CU); the other
 // forms spill under that cap and run best uncapped (measured at n = 254).
-template <int AXIS, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
 __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs a)
 {
     extern __shared__ double lds[];
@@ -292,7 +297,7 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
     }
 }
 
-template <int AXIS, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
 static int launch_rec_t(const RecArgs& a, hipStream_t s)
 {
     const int P = (a.n + CH - 1) / CH;
@@ -305,7 +310,7 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
     size_t tiles = AXIS == 1 ? (size_t)P * 64 * TP : 0;
     size_t xch = (size_t)P * 12 * 64;
     const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);
-    auto fn = recurrence_kernel<AXIS, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
+    auto fn = recurrence_kernel<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -318,19 +323,29 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
 
 // Block size P*64 threads; the launch bound is the smallest that fits so the
 // compiler may give each lane enough registers for its chunk.
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
+static int launch_rec_ch(const RecArgs& a, hipStream_t s)
+{
+    const int P = (a.n + CH - 1) / CH;
+    if (P <= 4) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
+    if (P <= 8) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
+    return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+}
+
+// Block size P*64 threads; the launch bound is the smallest that fits so the
+// compiler may give each lane enough registers for its chunk.
 template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
 static int launch_rec(const RecArgs& a, hipStream_t s)
 {
-    const int P = (a.n + CH - 1) / CH;
-    PYPDE_REQUIRE(P >= 1 && P <= 16, "recurrence: line length %d exceeds %d", a.n, 16 * CH);
+    PYPDE_REQUIRE(a.n >= 1 && a.n <= kMaxWaves * 32, "recurrence: line length %d exceeds %d", a.n,
+                  kMaxWaves * 32);
+    const bool short16 = !PRE && a.n <= kMaxWaves * 16;
     if (a.axis == 0) {
-        if (P <= 4) return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
-        if (P <= 8) return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
-        return launch_rec_t<0, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+        if (short16) return launch_rec_ch<0, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
+        return launch_rec_ch<0, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
     }
-    if (P <= 4) return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
-    if (P <= 8) return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
-    return launch_rec_t<1, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+    if (short16) return launch_rec_ch<1, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
+    return launch_rec_ch<1, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
 }
 
 }  // namespace pypde
