@@ -84,7 +84,8 @@ typedef struct pypde_gemm_desc {
     int gamma_div;
     int zero_first;
     long long csn; /* column stride of C, D and E (0 or 1: contiguous) */
-    /* parity folds of mirror-symmetric operators (halve the MFMA work), variants 1-3:
+    /* parity folds of mirror-symmetric operators (halve the MFMA work), variants 1-5
+     * (64x64, 32x32, 128x128, 128x64, 64x128 tiles):
      *   fold 1 / 2: output rows / columns of C come in blocks of fold_len mirrored
      *     pairs; m (fold 1) or n (fold 2) counts the first halves of the blocks;
      *     E = A B over k (first product), O = A1 B1 over k1 (second product) and
@@ -92,7 +93,8 @@ typedef struct pypde_gemm_desc {
      *     of fold_sign is set (b = block index);
      *   fold 3 / 4: the k-th element of A (3) or B (4) is X[k] + s_e X[fold_len-1-k]
      *     with k < fold_len / 2 = k and s_e = mirror_sign * (-1)^((e / mirror_div) % 2).
-     * ska / skb: element stride along k of A (not transposed) / B (transposed). */
+     * ska / skb: element stride along k of A (not transposed) / B (transposed);
+     * ska applies to every variant, skb to the folded ones. */
     int fold;
     const double* A1;
     const double* B1;
@@ -213,6 +215,17 @@ int pypde_axpby(long long n, double alpha, const double* x, double beta, double*
  * (pypde/field_operations.py:124-129,160-164) and rbc2d.py:342. */
 int pypde_rbc_products(const double* Y, const double* dtbc, double* P, int L0, int L1,
                        int batch, void* stream);
+
+/* The same products from y-folded blocks (the half-size y-pass output): per
+ * instance and field, Y holds six [L0, L1/2] blocks -- value even / odd
+ * y-coefficient parts, d/dy even / odd, d/dx even / odd -- over the first half of
+ * the y grid; the field at column l is E + O and at L1-1-l it is s (E - O), with
+ * s = sign_value for values and d/dx and sign_dy for d/dy.  P receives per field
+ * two [L0, L1/2] blocks, P(l) + P(L1-1-l) and P(l) - P(L1-1-l), the input of the
+ * half-size forward transform along y.  L1 must be even.  Same reference
+ * interfaces as pypde_rbc_products. */
+int pypde_rbc_products_yfold(const double* Y, const double* dtbc, double* P, int L0, int L1,
+                             double sign_value, double sign_dy, int batch, void* stream);
 
 /* Discrete adjoint of pypde_rbc_products: from the cotangents Pb
  * [batch, 3, L0, L1] of the three products and the forward blocks Y, write the

@@ -84,6 +84,7 @@ struct GemmArgs {
     int gamma_div;
     int zero_first;           // set element (0,0) of every C[e] to 0 (pressure gauge)
     long long csn;            // column stride of C, D and E (1: contiguous rows)
+    long long ska;            // element stride of A along k when A is not transposed
 };
 
 // Output update shared by every kernel family:
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
             a_rc(q, r, c);
             const int gm = m0 + r, gk = k0 + c;
             double v = 0.0;
-            if (gm < g.m && gk < g.k) v = TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + gk];
+            if (gm < g.m && gk < g.k) v = TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + (long long)gk * g.ska];
             ra[q] = v;
         }
 #pragma unroll
@@ -323,7 +324,7 @@ __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
         const int gnc = vb[j] ? gn : 0;
         pb[j] = TB ? B + (long long)gnc * g.ldb : B + gnc;
     }
-    const long long sa = TA ? g.lda : 1;  // stride of one k step
+    const long long sa = TA ? g.lda : g.ska;  // stride of one k step
     const long long sb = TB ? 1 : g.ldb;
 
     v4d acc[TM][TN];
@@ -459,7 +460,7 @@ __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
             int r, c;
             a_rc(q, r, c);
             const int gm = m0 + r, gk = k0 + c;
-            ra[q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + gk]) : 0.0;
+            ra[q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + (long long)gk * g.ska]) : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
@@ -825,7 +826,7 @@ int dgemm(int transA, int transB, int m, int n, int k, double alpha, const doubl
 {
     GemmArgs g{A, B, D, C, lda, ldb, ldd, ldc, strided(strideA), strided(strideB),
                strided(strideD), strided(strideC), m, n, k, alpha, beta, nullptr, 1,
-               nullptr, 0, strided(0), nullptr, 0.0, 1, 0, 1};
+               nullptr, 0, strided(0), nullptr, 0.0, 1, 0, 1, 1};
     return launch(transA, transB, g, batch, 0, stream);
 }
 
@@ -838,7 +839,7 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
                bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
                d->alpha_div < 1 ? 1 : d->alpha_div, d->E, d->lde, bidx(d->iE), d->gamma_vec,
                d->gamma, d->gamma_div < 1 ? 1 : d->gamma_div, d->zero_first,
-               d->csn < 1 ? 1 : d->csn};
+               d->csn < 1 ? 1 : d->csn, d->ska < 1 ? 1 : d->ska};
     if (d->fold == 0) return launch(d->transA, d->transB, g, d->batch, d->variant, stream);
 
     // parity-folded forms
@@ -867,14 +868,16 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
         const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * d->batch;
         variant = tiles128 >= 512 ? 3 : (tiles64 >= 512 ? 1 : 2);
     }
-    PYPDE_REQUIRE(variant >= 1 && variant <= 3, "folded GEMMs support variants 1-3 (got %d)", variant);
+    PYPDE_REQUIRE(variant >= 1 && variant <= 5, "folded GEMMs support variants 1-5 (got %d)", variant);
     const bool ta = d->transA, tb = d->transB;
     // smaller wave tiles than the plain kernels: the output folds carry two
     // accumulator sets (32x32 per wave at 128x128 over 16 waves)
     switch (variant) {
     case 1: launch_fold<64, 64, 4, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
     case 2: launch_fold<32, 32, 2, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
-    default: launch_fold<128, 128, 4, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    case 3: launch_fold<128, 128, 4, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    case 4: launch_fold<128, 64, 4, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    default: launch_fold<64, 128, 2, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
     }
     return check_launch("pypde_dgemm_batched (folded)");
 }

@@ -39,6 +39,58 @@ __global__ void rbc_products_kernel(const double* __restrict__ Y, const double* 
     }
 }
 
+// Convective products from y-folded fields.  Per instance and field f in
+// (U, V, T), Y holds six [L0, H] blocks (H = L1 / 2) over the first half of the
+// y grid: q = 0, 1 value (even / odd y-coefficient part), 2, 3 d/dy, 4, 5 d/dx.
+// The field at grid column l is E + O and at its mirror L1-1-l it is s (E - O),
+// s = sv for values and d/dx, sd for d/dy.  The products at both columns are
+// written folded for the forward transform along y: per field, block 0 holds
+// P(l) + P(L1-1-l) and block 1 P(l) - P(L1-1-l).
+__global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const double* __restrict__ dtbc,
+                                          double* __restrict__ P, int L0, int L1, double sv, double sd,
+                                          int B)
+{
+    const int H = L1 / 2;
+    const long long per = (long long)L0 * H;
+    const long long fs = 6LL * per;
+    const long long total = per * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long b = t / per;
+        const long long r = t - b * per;
+        const int i = (int)(r / H), l = (int)(r - (long long)i * H);
+        const double* Yb = Y + b * 3 * fs + r;
+        double v1[3], v2[3], y1[3], y2[3], x1[3], x2[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const double* q = Yb + f * fs;
+            const double vE = q[0], vO = q[per];
+            const double dE = q[2 * per], dO = q[3 * per];
+            const double xE = q[4 * per], xO = q[5 * per];
+            v1[f] = vE + vO;
+            v2[f] = sv * (vE - vO);
+            y1[f] = dE + dO;
+            y2[f] = sd * (dE - dO);
+            x1[f] = xE + xO;
+            x2[f] = sv * (xE - xO);
+        }
+        const double* tb = dtbc + (long long)i * L1;
+        const double pu1 = x1[0] * v1[0] + y1[0] * v1[1];
+        const double pv1 = x1[1] * v1[0] + y1[1] * v1[1];
+        const double pt1 = (x1[2] * v1[0] + y1[2] * v1[1]) + v1[1] * tb[l];
+        const double pu2 = x2[0] * v2[0] + y2[0] * v2[1];
+        const double pv2 = x2[1] * v2[0] + y2[1] * v2[1];
+        const double pt2 = (x2[2] * v2[0] + y2[2] * v2[1]) + v2[1] * tb[L1 - 1 - l];
+        double* Pb = P + b * 3 * 2 * per + r;
+        Pb[0] = pu1 + pu2;
+        Pb[per] = pu1 - pu2;
+        Pb[2 * per] = pv1 + pv2;
+        Pb[3 * per] = pv1 - pv2;
+        Pb[4 * per] = pt1 + pt2;
+        Pb[5 * per] = pt1 - pt2;
+    }
+}
+
 // Adjoint of rbc_products_kernel: given the cotangents Pb of the three
 // products, the cotangents of the value / d/dy / d/dx blocks of Y (the
 // d2/dxdy blocks and the temperature value block receive zero).
@@ -118,6 +170,19 @@ extern "C" int pypde_rbc_products(const double* Y, const double* dtbc, double* P
     hipLaunchKernelGGL(rbc_products_kernel, dim3(grid_for((long long)L0 * L1 * batch)), dim3(256), 0,
                        as_stream(stream), Y, dtbc, P, L0, L1, batch);
     return check_launch("pypde_rbc_products");
+}
+
+extern "C" int pypde_rbc_products_yfold(const double* Y, const double* dtbc, double* P, int L0,
+                                        int L1, double sign_value, double sign_dy, int batch,
+                                        void* stream)
+{
+    PYPDE_REQUIRE(Y && dtbc && P, "null pointer");
+    PYPDE_REQUIRE(L0 > 0 && L1 > 0 && L1 % 2 == 0 && batch >= 0,
+                  "invalid product geometry (L1 must be even)");
+    if (batch == 0) return OK;
+    hipLaunchKernelGGL(rbc_products_yfold_kernel, dim3(grid_for((long long)L0 * (L1 / 2) * batch)), dim3(256), 0,
+                       as_stream(stream), Y, dtbc, P, L0, L1, sign_value, sign_dy, batch);
+    return check_launch("pypde_rbc_products_yfold");
 }
 
 extern "C" int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const double* Pb,

@@ -272,6 +272,17 @@ class RBCBatch:
             if sg is not None:
                 h = L1 // 2
                 self.fold_bf = dict(B=T(np.stack([ay.BF[0::2, :h], ay.BF[1::2, :h]])), sign=sg[0])
+        # y-folded layout of the dealiased fields: the y-pass writes even / odd
+        # coefficient halves over the first half of the y grid and the products
+        # kernel reconstructs both mirrored columns (half-size plain GEMMs)
+        self.yfold = None
+        if self.fold_y is not None and self.fold_bf is not None:
+            sv = _mirror_signs(ay.G, L1)
+            sd = _mirror_signs(ay.Gd, L1)
+            if sv is not None and sd is not None:
+                H = L1 // 2
+                ops_y = np.stack([ay.G[:H, 0::2], ay.G[:H, 1::2], ay.Gd[:H, 0::2], ay.Gd[:H, 1::2]])
+                self.yfold = dict(B=T(ops_y), sv=sv[0], sd=sd[0])
         self.PBy = T(np.stack([ay.B, ay.BD]))                                 # [2, M1, N1]
         self.BSy = T(ay.BS)                                                   # [M1, M1]
         K = L0 + N0 + M0
@@ -329,8 +340,12 @@ class RBCBatch:
         self.pres = torch.zeros((B, N0, N1), **f64)
         self.P = torch.zeros((B, M0, M1), **f64)
         self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
-        self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
-        self.Pr = torch.zeros((B, 3, L0, L1), **f64)
+        if self.yfold is not None:
+            self.Y2 = torch.zeros((B, 3, 6, L0, L1 // 2), **f64)
+            self.Pr = torch.zeros((B, 3, 2, L0, L1 // 2), **f64)
+        else:
+            self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+            self.Pr = torch.zeros((B, 3, L0, L1), **f64)
         self.Z = torch.zeros((B, 3, K, M1), **f64)
         self.Wd = torch.zeros((B, 2, M0, N1), **f64)
         self.div = torch.zeros((B, N0, N1), **f64)
@@ -390,7 +405,7 @@ class RBCBatch:
         p = lambda t: t.data_ptr()
         X, Y1, Y2, Pr, Z = self.X, self.Y1, self.Y2, self.Pr, self.Z
         sX = M0 * M1
-        sZ = K * M1
+        sZ = self.sZ = K * M1
         L = []
         # 5  Z[1] = -dt pres (B | BD)_y^T   for U, V      (needs only the state)
         L.append(_Gemm("z_pres", m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
@@ -414,35 +429,10 @@ class RBCBatch:
                 C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1), fold=1, A1=p(fx["O"]),
                 B1=p(X) + 8 * M1, lda1=h, ldb1=2 * M1, k1=h, fold_len=L0, fold_sign=fx["sign"]))
         L.append(g)
-        # 2  y-pass: values and y-derivatives on the L1 grid
-        g = _Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
-                  iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
-                  C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1))
-        if self.fold_y is not None:
-            fy, h = self.fold_y, M1 // 2
-            g = self._with_alt(g, _Gemm(
-                "ypass", m=2 * L0, n=L1, k=h, batch=3 * B, A=p(Y1), lda=M1, ska=2,
-                iA=_bidx(s1=2 * L0 * M1), B=p(fy["E"]), ldb=h, tb=1,
-                C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1), fold=2, A1=p(Y1) + 8,
-                B1=p(fy["O"]), lda1=M1, ldb1=h, k1=h, fold_len=L1, fold_sign=fy["sign"]))
-        L.append(g)
-        # 3  convective products
-        L.append(_Call("products", lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
-                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
-        # 4  Z[0] = -dt P BF_y^T
-        g = _Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
-                  iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
-                  iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3)
-        if self.fold_bf is not None:
-            fb, h = self.fold_bf, L1 // 2
-            # entries e = 2 f + q: output columns q::2 of field-instance f
-            g = self._with_alt(g, _Gemm(
-                "z_conv", m=L0, n=M1 // 2, k=h, batch=6 * B, A=p(Pr), lda=L1,
-                iA=_bidx(s1=L0 * L1, d1=2), B=p(fb["B"]), ldb=h, tb=1,
-                iB=_bidx(s1=(M1 // 2) * h, m1=2), C=p(Z), ldc=M1, csn=2,
-                iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt), alpha_div=6,
-                fold=3, fold_len=L1, mirror_sign=fb["sign"], mirror_div=1))
-        L.append(g)
+        if self.yfold is not None:
+            self._yfold_launches(L)
+        else:
+            self._plain_y_launches(L)
         # 7  X = A_f Z_f + const_f
         L.append(_Gemm("rhs", m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
                        iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
@@ -526,6 +516,77 @@ class RBCBatch:
             f(s)
             e1.record(stream)
             records.append((f.name, e0, e1, f))
+
+    def _yfold_launches(self, L):
+        """y-pass, products and y forward transform in the y-folded layout."""
+        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
+        H, Mh = L1 // 2, M1 // 2
+        p = lambda t: t.data_ptr()
+        Y1, Y2, Pr, Z = self.Y1, self.Y2, self.Pr, self.Z
+        yf = self.yfold
+        sZ = self.sZ
+        blk = L0 * H
+        # 2a  value rows x {value, d/dy} x {even, odd}: entries e = 4 fi + q
+        L.append(_Gemm("ypass_v", m=L0, n=H, k=Mh, batch=12 * B, A=p(Y1), lda=M1, ska=2,
+                       iA=_bidx(s1=2 * L0 * M1, d1=4, s2=1, m2=2), B=p(yf["B"]), ldb=Mh, tb=1,
+                       iB=_bidx(s1=H * Mh, m1=4), C=p(Y2), ldc=H,
+                       iC=_bidx(s1=6 * blk, d1=4, s2=blk, m2=4)))
+        # 2b  d/dx rows x value x {even, odd}: entries e = 2 fi + q
+        L.append(_Gemm("ypass_x", m=L0, n=H, k=Mh, batch=6 * B, A=p(Y1) + 8 * L0 * M1, lda=M1,
+                       ska=2, iA=_bidx(s1=2 * L0 * M1, d1=2, s2=1, m2=2), B=p(yf["B"]), ldb=Mh,
+                       tb=1, iB=_bidx(s1=H * Mh, m1=2), C=p(Y2) + 8 * 4 * blk, ldc=H,
+                       iC=_bidx(s1=6 * blk, d1=2, s2=blk, m2=2)))
+        # 3  convective products at both mirrored columns, folded for the forward transform
+        L.append(_Call("products", lib.pypde_rbc_products_yfold, ctypes.c_void_p(p(Y2)),
+                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1,
+                       float(yf["sv"]), float(yf["sd"]), B))
+        # 4  Z[0] = -dt P BF_y^T: output columns q::2 from the sum (q = 0) or
+        #    difference (q = 1) block when BF is mirror-even (sign > 0), else swapped
+        fb = self.fold_bf
+        if fb["sign"] > 0:
+            a0, iA = p(Pr), _bidx(s1=2 * blk, d1=2, s2=blk, m2=2)
+        else:
+            a0, iA = p(Pr) + 8 * blk, _bidx(s1=2 * blk, d1=2, s2=-blk, m2=2)
+        L.append(_Gemm("z_conv", m=L0, n=Mh, k=H, batch=6 * B, A=a0, lda=H, iA=iA,
+                       B=p(fb["B"]), ldb=H, tb=1, iB=_bidx(s1=Mh * H, m1=2), C=p(Z), ldc=M1,
+                       csn=2, iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt),
+                       alpha_div=6))
+
+    def _plain_y_launches(self, L):
+        """y-pass, products and y forward transform in the natural layout."""
+        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
+        p = lambda t: t.data_ptr()
+        Y1, Y2, Pr, Z = self.Y1, self.Y2, self.Pr, self.Z
+        sZ = self.sZ
+        # 2  y-pass: values and y-derivatives on the L1 grid
+        g = _Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
+                  iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
+                  C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1))
+        if self.fold_y is not None:
+            fy, h = self.fold_y, M1 // 2
+            g = self._with_alt(g, _Gemm(
+                "ypass", m=2 * L0, n=L1, k=h, batch=3 * B, A=p(Y1), lda=M1, ska=2,
+                iA=_bidx(s1=2 * L0 * M1), B=p(fy["E"]), ldb=h, tb=1,
+                C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1), fold=2, A1=p(Y1) + 8,
+                B1=p(fy["O"]), lda1=M1, ldb1=h, k1=h, fold_len=L1, fold_sign=fy["sign"]))
+        L.append(g)
+        # 3  convective products
+        L.append(_Call("products", lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
+                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
+        # 4  Z[0] = -dt P BF_y^T
+        g = _Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
+                  iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
+                  iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3)
+        if self.fold_bf is not None:
+            fb, h = self.fold_bf, L1 // 2
+            # entries e = 2 f + q: output columns q::2 of field-instance f
+            g = self._with_alt(g, _Gemm(
+                "z_conv", m=L0, n=M1 // 2, k=h, batch=6 * B, A=p(Pr), lda=L1,
+                iA=_bidx(s1=L0 * L1, d1=2), B=p(fb["B"]), ldb=h, tb=1,
+                iB=_bidx(s1=(M1 // 2) * h, m1=2), C=p(Z), ldc=M1, csn=2,
+                iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt), alpha_div=6,
+                fold=3, fold_len=L1, mirror_sign=fb["sign"], mirror_div=1))
+        L.append(g)
 
     @staticmethod
     def _with_alt(plain, folded):

@@ -252,7 +252,7 @@ def _sym_operator(rng, nb, Lb, K, signs):
     return G
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("Lb,K", [(8, 10), (7, 9), (96, 62)])
 def test_gemm_output_fold(amd, variant, Lb, K):
     """fold 1 (rows) and 2 (columns): C = G X and C = Y G^T for mirror-symmetric
@@ -289,7 +289,7 @@ def test_gemm_output_fold(amd, variant, Lb, K):
     assert rel(host(C2), np.einsum("bnk,ik->bni", Y, G)) < 1e-14
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_gemm_input_fold(amd, variant):
     """fold 3 / 4: X[k] + s_e X[Kf-1-k] folded into A (3) or B (4) with the sign
     alternating over batch entries; outputs interleaved with column stride 2."""
