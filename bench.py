@@ -11,8 +11,9 @@ conductive-plus-noise initial state; W untimed steps, then K steps timed
 between device synchronisations, replayed from a HIP graph.
 
 Also reported on the same line:
-  roofline       the dominant kernel of the timed step (HIP events around each
-                 launch over an eager pass of the same K steps)
+  roofline       the dominant launch of the timed step (largest time per step in an
+                 eager pass of the same K steps with HIP events around each launch),
+                 timed as 20 back-to-back launches replayed from a HIP graph
   batched        64 instances of 256x256 (Ra log-uniform in [1e4, 1e7]) in one
                  batched step: aggregate instance-steps/s and the ratio to the plain path
   hetero         256 instances of mixed sizes (N per axis in {64, 96, 128, 192}) and
@@ -52,7 +53,37 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # --no-extra`, each divided by what the same counter reads for a device copy of
 # known size (`python3 bench.py --copy-calibration`); the profile summaries
 # under profiles/ give the commands and raw counter values.
-TRAFFIC_BYTES = {}
+TRAFFIC_BYTES = {
+    # profiles/hbm_traffic_pmc.md: recurrence_kernel<0, 16, false, true, false, true, true,
+    # true, 1024> (per-line Poisson solve, 256x256): 4.001 MB read + 1.225 MB written
+    "poisson_lines": 5.226e6,
+}
+
+
+def launch_graph_time(eng, launch, reps=20):
+    """Seconds per launch of one launch of the step, replayed back to back from a
+    HIP graph (as in the captured step); the engine state is restored."""
+    import ctypes
+    import torch
+    saved = [t.clone() for t in (eng.X, eng.pres, eng.P)]
+    launch(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for _ in range(reps):
+            launch(st)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.replay()
+    e1.record()
+    e1.synchronize()
+    for t, v in zip((eng.X, eng.pres, eng.P), saved):
+        t.copy_(v)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / reps
 
 
 def copy_calibration(mib=256, reps=5):
@@ -111,7 +142,7 @@ def time_plain(args):
         per[name][1] += 1
     dom = max(per, key=lambda k: per[k][0])
     tot, cnt, launch = per[dom]
-    avg = tot / cnt
+    avg = launch_graph_time(eng, launch)
     bound, amount = launch_cost(dom, launch, eng)
     if bound == "mfma":
         achieved, peak, unit = amount / avg / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
