@@ -222,10 +222,14 @@ int pypde_rbc_products(const double* Y, const double* dtbc, double* P, int L0, i
  * the y grid; the field at column l is E + O and at L1-1-l it is s (E - O), with
  * s = sign_value for values and d/dx and sign_dy for d/dy.  P receives per field
  * two [L0, L1/2] blocks, P(l) + P(L1-1-l) and P(l) - P(L1-1-l), the input of the
- * half-size forward transform along y.  L1 must be even.  Same reference
- * interfaces as pypde_rbc_products. */
-int pypde_rbc_products_yfold(const double* Y, const double* dtbc, double* P, int L0, int L1,
-                             double sign_value, double sign_dy, int batch, void* stream);
+ * half-size forward transform along y.  L1 must be even.  Yold (may be NULL): the
+ * previous RK3 stage in the same layout; ratio * (products of the current
+ * gradients with its velocities) is added, the c-term of the three-stage scheme
+ * (rbc2d.py:261-283, ratio = c / b).  Same reference interfaces as
+ * pypde_rbc_products. */
+int pypde_rbc_products_yfold(const double* Y, const double* Yold, const double* dtbc, double* P,
+                             int L0, int L1, double sign_value, double sign_dy, double ratio,
+                             int batch, void* stream);
 
 /* Discrete adjoint of pypde_rbc_products: from the cotangents Pb
  * [batch, 3, L0, L1] of the three products and the forward blocks Y, write the

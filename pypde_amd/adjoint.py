@@ -36,6 +36,8 @@ class StepAdjoint:
     """VJP of one fused Euler step of an RBCBatch (all instances at once)."""
 
     def __init__(self, eng: RBCBatch):
+        if eng.integrator != "eu" or eng.beta != 1.0:
+            raise NotImplementedError("the discrete adjoint covers the Euler step with beta = 1")
         self.e = eng
         e = eng
         B, N0, N1, M0, M1, L0, L1 = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1

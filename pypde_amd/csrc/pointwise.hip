@@ -45,10 +45,12 @@ __global__ void rbc_products_kernel(const double* __restrict__ Y, const double* 
 // The field at grid column l is E + O and at its mirror L1-1-l it is s (E - O),
 // s = sv for values and d/dx, sd for d/dy.  The products at both columns are
 // written folded for the forward transform along y: per field, block 0 holds
-// P(l) + P(L1-1-l) and block 1 P(l) - P(L1-1-l).
-__global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const double* __restrict__ dtbc,
-                                          double* __restrict__ P, int L0, int L1, double sv, double sd,
-                                          int B)
+// P(l) + P(L1-1-l) and block 1 P(l) - P(L1-1-l).  With Yold (the previous RK3
+// stage, same layout), ratio times the products of the current gradients with
+// the old velocities is added (the c-term of rbc2d.py:261-283).
+__global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const double* __restrict__ Yold,
+                                          const double* __restrict__ dtbc, double* __restrict__ P,
+                                          int L0, int L1, double sv, double sd, double ratio, int B)
 {
     const int H = L1 / 2;
     const long long per = (long long)L0 * H;
@@ -75,12 +77,24 @@ __global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const do
             x2[f] = sv * (xE - xO);
         }
         const double* tb = dtbc + (long long)i * L1;
-        const double pu1 = x1[0] * v1[0] + y1[0] * v1[1];
-        const double pv1 = x1[1] * v1[0] + y1[1] * v1[1];
-        const double pt1 = (x1[2] * v1[0] + y1[2] * v1[1]) + v1[1] * tb[l];
-        const double pu2 = x2[0] * v2[0] + y2[0] * v2[1];
-        const double pv2 = x2[1] * v2[0] + y2[1] * v2[1];
-        const double pt2 = (x2[2] * v2[0] + y2[2] * v2[1]) + v2[1] * tb[L1 - 1 - l];
+        const double tb1 = tb[l], tb2 = tb[L1 - 1 - l];
+        double pu1 = x1[0] * v1[0] + y1[0] * v1[1];
+        double pv1 = x1[1] * v1[0] + y1[1] * v1[1];
+        double pt1 = (x1[2] * v1[0] + y1[2] * v1[1]) + v1[1] * tb1;
+        double pu2 = x2[0] * v2[0] + y2[0] * v2[1];
+        double pv2 = x2[1] * v2[0] + y2[1] * v2[1];
+        double pt2 = (x2[2] * v2[0] + y2[2] * v2[1]) + v2[1] * tb2;
+        if (Yold) {
+            const double* Ob = Yold + b * 3 * fs + r;
+            const double ux1 = Ob[0] + Ob[per], ux2 = sv * (Ob[0] - Ob[per]);
+            const double uz1 = Ob[fs] + Ob[fs + per], uz2 = sv * (Ob[fs] - Ob[fs + per]);
+            pu1 = pu1 + ratio * (x1[0] * ux1 + y1[0] * uz1);
+            pv1 = pv1 + ratio * (x1[1] * ux1 + y1[1] * uz1);
+            pt1 = pt1 + ratio * ((x1[2] * ux1 + y1[2] * uz1) + uz1 * tb1);
+            pu2 = pu2 + ratio * (x2[0] * ux2 + y2[0] * uz2);
+            pv2 = pv2 + ratio * (x2[1] * ux2 + y2[1] * uz2);
+            pt2 = pt2 + ratio * ((x2[2] * ux2 + y2[2] * uz2) + uz2 * tb2);
+        }
         double* Pb = P + b * 3 * 2 * per + r;
         Pb[0] = pu1 + pu2;
         Pb[per] = pu1 - pu2;
@@ -172,16 +186,16 @@ extern "C" int pypde_rbc_products(const double* Y, const double* dtbc, double* P
     return check_launch("pypde_rbc_products");
 }
 
-extern "C" int pypde_rbc_products_yfold(const double* Y, const double* dtbc, double* P, int L0,
-                                        int L1, double sign_value, double sign_dy, int batch,
-                                        void* stream)
+extern "C" int pypde_rbc_products_yfold(const double* Y, const double* Yold, const double* dtbc,
+                                        double* P, int L0, int L1, double sign_value,
+                                        double sign_dy, double ratio, int batch, void* stream)
 {
     PYPDE_REQUIRE(Y && dtbc && P, "null pointer");
     PYPDE_REQUIRE(L0 > 0 && L1 > 0 && L1 % 2 == 0 && batch >= 0,
                   "invalid product geometry (L1 must be even)");
     if (batch == 0) return OK;
     hipLaunchKernelGGL(rbc_products_yfold_kernel, dim3(grid_for((long long)L0 * (L1 / 2) * batch)), dim3(256), 0,
-                       as_stream(stream), Y, dtbc, P, L0, L1, sign_value, sign_dy, batch);
+                       as_stream(stream), Y, Yold, dtbc, P, L0, L1, sign_value, sign_dy, ratio, batch);
     return check_launch("pypde_rbc_products_yfold");
 }
 

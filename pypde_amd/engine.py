@@ -95,6 +95,8 @@ class AxisOps:
         self.BD = B @ D1 / scale           # pressure gradient, then B
         self.BS = B @ S
         self.BST = B @ St
+        self.BD2S = B @ (D1 @ D1 @ S) / scale ** 2     # explicit diffusion (beta != 1)
+        self.BD2ST = B @ (D1 @ D1 @ St) / scale ** 2
         self.C = S                         # divergence pieces
         self.Cd = D1 @ S / scale
         self.Sn = Sn                       # P (ChebNeumann) -> Chebyshev
@@ -207,16 +209,30 @@ def _rec(base, M, entry_stride):
 
 # ---------------------------------------------------------------------------
 class RBCBatch:
-    """Euler IMEX RBC2D step for `B` instances of one shape.
+    """IMEX RBC2D step (Euler, or three-stage RK3) for `B` instances of one shape.
 
     params: list of dicts with keys ra, pr, dt (one per instance).
     case: "rbc" (adiabatic sidewalls, T in ChebNeumann x ChebDirichlet),
           "linear" or "zero" (T in ChebDirichlet^2).
+    integrator: "eu" or "rk3" (the stage coefficients a, b, c of
+          rbc2d_base.py:86-102); beta: implicit fraction of the diffusion, the
+          rest treated explicitly (rbc2d.py:269-283).
     """
 
-    def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0):
+    def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0,
+                 integrator="eu"):
         if case not in ("rbc", "linear", "zero"):
             raise ValueError("Specified case is not available: ", ["rbc", "linear", "zero"])
+        if integrator not in ("eu", "rk3"):
+            raise ValueError("Specified integrator is not available: ", ["eu", "rk3"])
+        self.integrator = integrator
+        if integrator == "rk3":
+            self.ka = np.array([8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0])
+            self.kb = np.array([8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0])
+            self.kc = np.array([0, -17.0 / 60.0, -5.0 / 12.0])
+        else:
+            self.ka, self.kb, self.kc = np.array([1.0]), np.array([1.0]), np.array([0])
+        self.nstage = len(self.ka)
         self.shape = tuple(shape)
         self.case, self.dealias, self.aspect, self.beta = case, dealias, aspect, float(beta)
         self.params = [dict(p) for p in params]
@@ -247,6 +263,16 @@ class RBCBatch:
         self.t_dt = T(dt)
         self.t_idt = T(1.0 / dt)
         self.t_nu = T(nu)
+        # per stage: -dt b (convection), -dt a (pressure gradient), dt a (buoyancy),
+        # 1 / (dt a) (pressure update), explicit diffusion dt a (1 - beta) coef
+        self.st_mdtb = [T(-dt * self.kb[s]) for s in range(self.nstage)]
+        self.st_mdta = [T(-dt * self.ka[s]) for s in range(self.nstage)]
+        self.st_dta = [T(dt * self.ka[s]) for s in range(self.nstage)]
+        self.st_idta = [T(1.0 / (dt * self.ka[s])) for s in range(self.nstage)]
+        coef = np.stack([nu, nu, kappa], axis=1)                               # [B, 3]
+        self.st_fdiff = [T((dt * self.ka[s] * (1 - self.beta))[:, None] * coef)
+                         for s in range(self.nstage)]
+        self.diffusion = self.beta != 1.0
 
         # ---- operator stacks (device)
         self.GGx = T(np.stack([np.vstack([ax.G, ax.Gd]), np.vstack([ax.G, ax.Gd]),
@@ -285,12 +311,18 @@ class RBCBatch:
                 self.yfold = dict(B=T(ops_y), sv=sv[0], sd=sd[0])
         self.PBy = T(np.stack([ay.B, ay.BD]))                                 # [2, M1, N1]
         self.BSy = T(ay.BS)                                                   # [M1, M1]
-        K = L0 + N0 + M0
+        K = L0 + N0 + M0 + (2 * M0 if self.beta != 1.0 else 0)
         self.K = K
         zero_blk = np.zeros((M0, N0))
-        self.Arhs = T(np.stack([np.hstack([ax.BF, ax.BD, ax.BS]),
-                                np.hstack([ax.BF, ax.B, ax.BS]),
-                                np.hstack([ax.BF, zero_blk, ax.BST])]))        # [3, M0, K]
+        blocks = [[ax.BF, ax.BD, ax.BS], [ax.BF, ax.B, ax.BS], [ax.BF, zero_blk, ax.BST]]
+        if self.beta != 1.0:
+            # explicit diffusion f (d2x + d2y) of the stage-start field, applied
+            # through B: (B D2 S)_x X (B S)_y^T + (B S)_x X (B D2 S)_y^T
+            blocks[0] += [ax.BD2S, ax.BS]
+            blocks[1] += [ax.BD2S, ax.BS]
+            blocks[2] += [ax.BD2ST, ax.BST]
+            self.DFy = T(np.stack([ay.BS, ay.BD2S]))                          # [2, M1, M1]
+        self.Arhs = T(np.stack([np.hstack(b) for b in blocks]))              # [3, M0, K]
         self.BSTx = T(ax.BST)                                                 # [M0, M0]
         self.Cy = T(np.stack([ay.C, ay.Cd]))                                  # [2, N1, M1]
         self.Cx = T(np.hstack([ax.Cd, ax.C]))                                 # [N0, 2M0]
@@ -306,34 +338,44 @@ class RBCBatch:
         self.Snx = T(ax.Sn)                                                   # [N0, M0]
         self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
 
-        # Helmholtz LU tables per instance and field: [B, 3, 4, M]
-        hx = np.zeros((B, 3, 4, M0))
-        hy = np.zeros((B, 3, 4, M1))
+        # Helmholtz LU tables per stage, instance and field: [B, 3, 4, M]
         bx_cd, bx_t, by_cd = Base(N0, "CD"), Base(N0, side), Base(N1, "CD")
-        for b in range(B):
-            lam_u = dt[b] * 1.0 * self.beta * nu[b]
-            lam_t = dt[b] * 1.0 * self.beta * kappa[b]
-            for f, (basis, lam) in enumerate(((bx_cd, lam_u), (bx_cd, lam_u), (bx_t, lam_t))):
-                hx[b, f] = self._lu(hholtz_matrices(basis, lam, self.scale[0])[2])
-            for f, lam in enumerate((lam_u, lam_u, lam_t)):
-                hy[b, f] = self._lu(hholtz_matrices(by_cd, lam, self.scale[1])[2])
-        self.hx, self.hy = T(hx), T(hy)
-        # normalised (reciprocal-pivot) form for the register-resident kernel
-        self.rhx = T(np.stack(rec_from_fdma(hx[:, :, 0, :M0 - 2], hx[:, :, 1], hx[:, :, 2, :M0 - 2],
-                                            hx[:, :, 3, :M0 - 4]), axis=2))
-        self.rhy = T(np.stack(rec_from_fdma(hy[:, :, 0, :M1 - 2], hy[:, :, 1], hy[:, :, 2, :M1 - 2],
-                                            hy[:, :, 3, :M1 - 4]), axis=2))
+        self.st_hx, self.st_hy, self.st_rhx, self.st_rhy = [], [], [], []
+        for st in range(self.nstage):
+            hx = np.zeros((B, 3, 4, M0))
+            hy = np.zeros((B, 3, 4, M1))
+            for b in range(B):
+                lam_u = dt[b] * self.ka[st] * self.beta * nu[b]
+                lam_t = dt[b] * self.ka[st] * self.beta * kappa[b]
+                for f, (basis, lam) in enumerate(((bx_cd, lam_u), (bx_cd, lam_u), (bx_t, lam_t))):
+                    hx[b, f] = self._lu(hholtz_matrices(basis, lam, self.scale[0])[2])
+                for f, lam in enumerate((lam_u, lam_u, lam_t)):
+                    hy[b, f] = self._lu(hholtz_matrices(by_cd, lam, self.scale[1])[2])
+            self.st_hx.append(T(hx))
+            self.st_hy.append(T(hy))
+            # normalised (reciprocal-pivot) form for the register-resident kernel
+            self.st_rhx.append(T(np.stack(rec_from_fdma(hx[:, :, 0, :M0 - 2], hx[:, :, 1],
+                                                        hx[:, :, 2, :M0 - 2], hx[:, :, 3, :M0 - 4]),
+                                          axis=2)))
+            self.st_rhy.append(T(np.stack(rec_from_fdma(hy[:, :, 0, :M1 - 2], hy[:, :, 1],
+                                                        hy[:, :, 2, :M1 - 2], hy[:, :, 3, :M1 - 4]),
+                                          axis=2)))
+        self.hx, self.hy, self.rhx, self.rhy = (self.st_hx[0], self.st_hy[0], self.st_rhx[0],
+                                                self.st_rhy[0])
 
         # boundary lifting constants (shape-only) and per-instance RHS constants
         self._setup_bc()
-        cst = np.zeros((B, 3, M0, M1))
         Bx_, By_ = ax.B, ay.B
         cv = Bx_ @ self.Tbc_cheby_h @ By_.T
         ct = Bx_ @ self.dTbcdz2_h @ By_.T
-        for b in range(B):
-            cst[b, 1] = dt[b] * 1.0 * cv
-            cst[b, 2] = dt[b] * 1.0 * kappa[b] * ct
-        self.const = T(cst)
+        self.st_const = []
+        for st in range(self.nstage):
+            cst = np.zeros((B, 3, M0, M1))
+            for b in range(B):
+                cst[b, 1] = dt[b] * self.ka[st] * cv
+                cst[b, 2] = dt[b] * self.ka[st] * kappa[b] * ct
+            self.st_const.append(T(cst))
+        self.const = self.st_const[0]
 
         # ---- state and workspaces
         self.X = torch.zeros((B, 3, M0, M1), **f64)
@@ -342,8 +384,13 @@ class RBCBatch:
         self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
         if self.yfold is not None:
             self.Y2 = torch.zeros((B, 3, 6, L0, L1 // 2), **f64)
+            # second buffer: the previous RK3 stage's velocities (c-term)
+            self.Y2old = torch.zeros_like(self.Y2) if self.nstage > 1 else None
             self.Pr = torch.zeros((B, 3, 2, L0, L1 // 2), **f64)
         else:
+            if self.nstage > 1 or self.beta != 1.0:
+                raise NotImplementedError("RK3 and beta != 1 need even dealiased lengths in y "
+                                          "(the y-folded layout)")
             self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
             self.Pr = torch.zeros((B, 3, L0, L1), **f64)
         self.Z = torch.zeros((B, 3, K, M1), **f64)
@@ -400,70 +447,88 @@ class RBCBatch:
 
     # -----------------------------------------------------------------------
     def _build_launches(self):
+        self._keep = []
+        L = []
+        for st in range(self.nstage):
+            self._stage_launches(L, st)
+        self.launches = L
+
+    def _stage_launches(self, L, st):
+        """The launches of stage `st` (one stage for Euler, three for RK3)."""
         B, N0, N1, M0, M1, L0, L1, K = (self.B, self.N0, self.N1, self.M0, self.M1, self.L0,
                                         self.L1, self.K)
         p = lambda t: t.data_ptr()
-        X, Y1, Y2, Pr, Z = self.X, self.Y1, self.Y2, self.Pr, self.Z
+        X, Y1, Pr, Z = self.X, self.Y1, self.Pr, self.Z
         sX = M0 * M1
         sZ = self.sZ = K * M1
-        L = []
-        # 5  Z[1] = -dt pres (B | BD)_y^T   for U, V      (needs only the state)
-        L.append(_Gemm("z_pres", m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
+        tag = "" if self.nstage == 1 else "[%d]" % st
+        # 5  Z[1] = -dt a pres (B | BD)_y^T   for U, V      (needs only the stage-start state)
+        L.append(_Gemm("z_pres" + tag, m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
                        iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
                        iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
-                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.t_mdt),
+                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.st_mdta[st]),
                        alpha_div=2))
         # 6  Z[2] = X BS_y^T
-        L.append(_Gemm("z_old", m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1, iA=_bidx(s1=sX),
-                       B=p(self.BSy), ldb=M1, tb=1, C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1,
-                       iC=_bidx(s1=sZ)))
+        L.append(_Gemm("z_old" + tag, m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1,
+                       iA=_bidx(s1=sX), B=p(self.BSy), ldb=M1, tb=1,
+                       C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1, iC=_bidx(s1=sZ)))
+        if self.diffusion:
+            # 6b  Z[3] = f X BS_y^T, Z[4] = f X BD2S_y^T (explicit diffusion, f per field)
+            L.append(_Gemm("z_diff" + tag, m=M0, n=M1, k=M1, batch=6 * B, A=p(X), lda=M1,
+                           iA=_bidx(s1=sX, d1=2), B=p(self.DFy), ldb=M1, tb=1,
+                           iB=_bidx(s1=M1 * M1, m1=2), C=p(Z) + 8 * (L0 + N0 + M0) * M1,
+                           ldc=M1, iC=_bidx(s1=sZ, d1=2, s2=M0 * M1, m2=2),
+                           alpha_vec=p(self.st_fdiff[st]), alpha_div=2))
         # 1  x-pass: values and x-derivatives on the L0 grid
-        g = _Gemm("xpass", m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
+        g = _Gemm("xpass" + tag, m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
                   iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
                   C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1))
         if self.fold_x is not None:
             fx, h = self.fold_x, M0 // 2
             g = self._with_alt(g, _Gemm(
-                "xpass", m=L0, n=M1, k=h, batch=3 * B, A=p(fx["E"]), lda=h,
+                "xpass" + tag, m=L0, n=M1, k=h, batch=3 * B, A=p(fx["E"]), lda=h,
                 iA=_bidx(s1=L0 * h, m1=3), B=p(X), ldb=2 * M1, iB=_bidx(s1=sX),
                 C=p(Y1), ldc=M1, iC=_bidx(s1=2 * L0 * M1), fold=1, A1=p(fx["O"]),
                 B1=p(X) + 8 * M1, lda1=h, ldb1=2 * M1, k1=h, fold_len=L0, fold_sign=fx["sign"]))
         L.append(g)
         if self.yfold is not None:
-            self._yfold_launches(L)
+            self._yfold_launches(L, st, tag)
         else:
-            self._plain_y_launches(L)
+            self._plain_y_launches(L, tag)
         # 7  X = A_f Z_f + const_f
-        L.append(_Gemm("rhs", m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
+        L.append(_Gemm("rhs" + tag, m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
                        iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
-                       D=p(self.const), ldd=M1, iD=_bidx(s1=sX), beta=1.0,
+                       D=p(self.st_const[st]), ldd=M1, iD=_bidx(s1=sX), beta=1.0,
                        C=p(X), ldc=M1, iC=_bidx(s1=sX)))
-        # 8  buoyancy  X_V += dt BST_x Z_T[2]
-        L.append(_Gemm("buoyancy", m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx), lda=M0,
+        # 8  buoyancy  X_V += dt a BST_x Z_T[2]
+        L.append(_Gemm("buoyancy" + tag, m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx), lda=M0,
                        B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1, iB=_bidx(s1=3 * sZ),
                        D=p(X) + 8 * sX, ldd=M1, iD=_bidx(s1=3 * sX), beta=1.0,
-                       C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX), alpha_vec=p(self.t_dt)))
+                       C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX),
+                       alpha_vec=p(self.st_dta[st])))
         # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
-        rx = _rec(p(self.rhx), M0, 4 * M0)
-        ry = _rec(p(self.rhy), M1, 4 * M1)
-        self._keep = [gx, gy, rx, ry]
-        L.append(_Call("helmholtz_x", lib.pypde_recurrence_solve, ctypes.byref(gx), ctypes.byref(rx)))
-        L.append(_Call("helmholtz_y", lib.pypde_recurrence_solve, ctypes.byref(gy), ctypes.byref(ry)))
+        rx = _rec(p(self.st_rhx[st]), M0, 4 * M0)
+        ry = _rec(p(self.st_rhy[st]), M1, 4 * M1)
+        self._keep += [gx, gy, rx, ry]
+        L.append(_Call("helmholtz_x" + tag, lib.pypde_recurrence_solve, ctypes.byref(gx),
+                       ctypes.byref(rx)))
+        L.append(_Call("helmholtz_y" + tag, lib.pypde_recurrence_solve, ctypes.byref(gy),
+                       ctypes.byref(ry)))
         # 11  Wd = [U* C_y^T ; V* Cd_y^T]
-        L.append(_Gemm("div_y", m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
+        L.append(_Gemm("div_y" + tag, m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
                        iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.Cy), ldb=M1, tb=1,
                        iB=_bidx(s1=N1 * M1, m1=2), C=p(self.Wd), ldc=N1, iC=_bidx(s1=M0 * N1)))
         # 12  div = [Cd | C]_x Wd
-        L.append(_Gemm("div_x", m=N0, n=N1, k=2 * M0, batch=B, A=p(self.Cx), lda=2 * M0,
+        L.append(_Gemm("div_x" + tag, m=N0, n=N1, k=2 * M0, batch=B, A=p(self.Cx), lda=2 * M0,
                        B=p(self.Wd), ldb=N1, iB=_bidx(s1=2 * M0 * N1), C=p(self.div), ldc=N1,
                        iC=_bidx(s1=N0 * N1)))
         # 13-14  Poisson right-hand side  B_x div H_y^T
-        L.append(_Gemm("poisson_bx", m=M0, n=N1, k=N0, batch=B, A=p(self.Bx), lda=N0,
+        L.append(_Gemm("poisson_bx" + tag, m=M0, n=N1, k=N0, batch=B, A=p(self.Bx), lda=N0,
                        B=p(self.div), ldb=N1, iB=_bidx(s1=N0 * N1), C=p(self.Q1), ldc=N1,
                        iC=_bidx(s1=M0 * N1)))
-        L.append(_Gemm("poisson_hy", m=M0, n=M1, k=N1, batch=B, A=p(self.Q1), lda=N1,
+        L.append(_Gemm("poisson_hy" + tag, m=M0, n=M1, k=N1, batch=B, A=p(self.Q1), lda=N1,
                        iA=_bidx(s1=M0 * N1), B=p(self.Hy), ldb=N1, tb=1, C=p(self.RP), ldc=M1,
                        iC=_bidx(s1=sX)))
         # 15  per-line eigen-split solves along x
@@ -472,31 +537,111 @@ class RBCBatch:
         rp.LM, rp.R, rp.MU, rp.NU = (self.prec[i].data_ptr() for i in range(4))
         rp.per_line, rp.entry_stride = 1, 0
         self._keep += [gp, rp]
-        L.append(_Call("poisson_lines", lib.pypde_recurrence_solve, ctypes.byref(gp), ctypes.byref(rp)))
+        L.append(_Call("poisson_lines" + tag, lib.pypde_recurrence_solve, ctypes.byref(gp),
+                       ctypes.byref(rp)))
         # 16  P = RP Q_y^T
         # 17  ... with the gauge P[0,0] = 0 applied in the GEMM epilogue
-        L.append(_Gemm("poisson_qy", m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
+        L.append(_Gemm("poisson_qy" + tag, m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Qy), ldb=M1, tb=1, C=p(self.P), ldc=M1,
                        iC=_bidx(s1=sX), zero_first=1))
         # 18  W = P [S; T S; T D S/s]_y^T
         NW = N1 + 2 * M1
-        L.append(_Gemm("corr_y", m=M0, n=NW, k=M1, batch=B, A=p(self.P), lda=M1,
+        L.append(_Gemm("corr_y" + tag, m=M0, n=NW, k=M1, batch=B, A=p(self.P), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Wy), ldb=M1, tb=1, C=p(self.W1), ldc=NW,
                        iC=_bidx(s1=M0 * NW)))
-        # 19-20  pres = (pres - nu div beta) + S_x W_0 / dt   (one GEMM, fused epilogue)
-        L.append(_Gemm("pres_p", m=N0, n=N1, k=M0, batch=B, A=p(self.Snx), lda=M0,
+        # 19-20  pres = (pres - nu div beta) + S_x W_0 / (dt a)   (one GEMM, fused epilogue)
+        L.append(_Gemm("pres_p" + tag, m=N0, n=N1, k=M0, batch=B, A=p(self.Snx), lda=M0,
                        B=p(self.W1), ldb=NW, iB=_bidx(s1=M0 * NW), D=p(self.pres), ldd=N1,
                        iD=_bidx(s1=N0 * N1), beta=1.0, C=p(self.pres), ldc=N1,
-                       iC=_bidx(s1=N0 * N1), alpha_vec=p(self.t_idt),
+                       iC=_bidx(s1=N0 * N1), alpha_vec=p(self.st_idta[st]),
                        E=p(self.div), lde=N1, iE=_bidx(s1=N0 * N1), gamma_vec=p(self.t_nu),
                        gamma=self.beta))
         # 21  velocity correction
-        L.append(_Gemm("correct", m=M0, n=M1, k=M0, batch=2 * B, A=p(self.Acorr), lda=M0,
+        L.append(_Gemm("correct" + tag, m=M0, n=M1, k=M0, batch=2 * B, A=p(self.Acorr), lda=M0,
                        iA=_bidx(s1=M0 * M0, m1=2), B=p(self.W1) + 8 * N1, ldb=NW,
                        iB=_bidx(s1=M0 * NW, d1=2, s2=M1, m2=2), D=p(X), ldd=M1,
                        iD=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), beta=1.0, alpha=-1.0,
                        C=p(X), ldc=M1, iC=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2)))
-        self.launches = L
+
+    def _yfold_launches(self, L, st, tag):
+        """y-pass, products and y forward transform in the y-folded layout.  RK3
+        stages alternate between two field buffers so that the c-term of stage
+        st reads the velocities of stage st-1."""
+        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
+        H, Mh = L1 // 2, M1 // 2
+        p = lambda t: t.data_ptr()
+        Y1, Pr, Z = self.Y1, self.Pr, self.Z
+        Y2 = self.Y2 if st % 2 == 0 else self.Y2old
+        old = None if st == 0 else (self.Y2old if st % 2 == 0 else self.Y2)
+        ratio = float(self.kc[st]) / float(self.kb[st])
+        yf = self.yfold
+        sZ = self.sZ
+        blk = L0 * H
+        # 2a  value rows x {value, d/dy} x {even, odd}: entries e = 4 fi + q
+        L.append(_Gemm("ypass_v" + tag, m=L0, n=H, k=Mh, batch=12 * B, A=p(Y1), lda=M1, ska=2,
+                       iA=_bidx(s1=2 * L0 * M1, d1=4, s2=1, m2=2), B=p(yf["B"]), ldb=Mh, tb=1,
+                       iB=_bidx(s1=H * Mh, m1=4), C=p(Y2), ldc=H,
+                       iC=_bidx(s1=6 * blk, d1=4, s2=blk, m2=4)))
+        # 2b  d/dx rows x value x {even, odd}: entries e = 2 fi + q
+        L.append(_Gemm("ypass_x" + tag, m=L0, n=H, k=Mh, batch=6 * B, A=p(Y1) + 8 * L0 * M1,
+                       lda=M1, ska=2, iA=_bidx(s1=2 * L0 * M1, d1=2, s2=1, m2=2), B=p(yf["B"]),
+                       ldb=Mh, tb=1, iB=_bidx(s1=H * Mh, m1=2), C=p(Y2) + 8 * 4 * blk, ldc=H,
+                       iC=_bidx(s1=6 * blk, d1=2, s2=blk, m2=2)))
+        # 3  convective products at both mirrored columns (+ (c/b) times the
+        #    products with the previous stage's velocities), folded for the
+        #    forward transform
+        L.append(_Call("products" + tag, lib.pypde_rbc_products_yfold, ctypes.c_void_p(p(Y2)),
+                       ctypes.c_void_p(p(old) if old is not None else None),
+                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1,
+                       float(yf["sv"]), float(yf["sd"]), ratio, B))
+        # 4  Z[0] = -dt b P BF_y^T: output columns q::2 from the sum (q = 0) or
+        #    difference (q = 1) block when BF is mirror-even (sign > 0), else swapped
+        fb = self.fold_bf
+        if fb["sign"] > 0:
+            a0, iA = p(Pr), _bidx(s1=2 * blk, d1=2, s2=blk, m2=2)
+        else:
+            a0, iA = p(Pr) + 8 * blk, _bidx(s1=2 * blk, d1=2, s2=-blk, m2=2)
+        L.append(_Gemm("z_conv" + tag, m=L0, n=Mh, k=H, batch=6 * B, A=a0, lda=H, iA=iA,
+                       B=p(fb["B"]), ldb=H, tb=1, iB=_bidx(s1=Mh * H, m1=2), C=p(Z), ldc=M1,
+                       csn=2, iC=_bidx(s1=sZ, d1=2, s2=1, m2=2),
+                       alpha_vec=p(self.st_mdtb[st]), alpha_div=6))
+
+    def _plain_y_launches(self, L, tag):
+        """y-pass, products and y forward transform in the natural layout (Euler)."""
+        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
+        p = lambda t: t.data_ptr()
+        Y1, Y2, Pr, Z = self.Y1, self.Y2, self.Pr, self.Z
+        sZ = self.sZ
+        # 2  y-pass: values and y-derivatives on the L1 grid
+        g = _Gemm("ypass" + tag, m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
+                  iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
+                  C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1))
+        if self.fold_y is not None:
+            fy, h = self.fold_y, M1 // 2
+            g = self._with_alt(g, _Gemm(
+                "ypass" + tag, m=2 * L0, n=L1, k=h, batch=3 * B, A=p(Y1), lda=M1, ska=2,
+                iA=_bidx(s1=2 * L0 * M1), B=p(fy["E"]), ldb=h, tb=1,
+                C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1), fold=2, A1=p(Y1) + 8,
+                B1=p(fy["O"]), lda1=M1, ldb1=h, k1=h, fold_len=L1, fold_sign=fy["sign"]))
+        L.append(g)
+        # 3  convective products
+        L.append(_Call("products" + tag, lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
+                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
+        # 4  Z[0] = -dt P BF_y^T
+        g = _Gemm("z_conv" + tag, m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
+                  iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
+                  iC=_bidx(s1=sZ), alpha_vec=p(self.st_mdtb[0]), alpha_div=3)
+        if self.fold_bf is not None:
+            fb, h = self.fold_bf, L1 // 2
+            # entries e = 2 f + q: output columns q::2 of field-instance f
+            g = self._with_alt(g, _Gemm(
+                "z_conv" + tag, m=L0, n=M1 // 2, k=h, batch=6 * B, A=p(Pr), lda=L1,
+                iA=_bidx(s1=L0 * L1, d1=2), B=p(fb["B"]), ldb=h, tb=1,
+                iB=_bidx(s1=(M1 // 2) * h, m1=2), C=p(Z), ldc=M1, csn=2,
+                iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.st_mdtb[0]), alpha_div=6,
+                fold=3, fold_len=L1, mirror_sign=fb["sign"], mirror_div=1))
+        L.append(g)
+
 
     # -----------------------------------------------------------------------
     def step_eager(self):
@@ -516,77 +661,6 @@ class RBCBatch:
             f(s)
             e1.record(stream)
             records.append((f.name, e0, e1, f))
-
-    def _yfold_launches(self, L):
-        """y-pass, products and y forward transform in the y-folded layout."""
-        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
-        H, Mh = L1 // 2, M1 // 2
-        p = lambda t: t.data_ptr()
-        Y1, Y2, Pr, Z = self.Y1, self.Y2, self.Pr, self.Z
-        yf = self.yfold
-        sZ = self.sZ
-        blk = L0 * H
-        # 2a  value rows x {value, d/dy} x {even, odd}: entries e = 4 fi + q
-        L.append(_Gemm("ypass_v", m=L0, n=H, k=Mh, batch=12 * B, A=p(Y1), lda=M1, ska=2,
-                       iA=_bidx(s1=2 * L0 * M1, d1=4, s2=1, m2=2), B=p(yf["B"]), ldb=Mh, tb=1,
-                       iB=_bidx(s1=H * Mh, m1=4), C=p(Y2), ldc=H,
-                       iC=_bidx(s1=6 * blk, d1=4, s2=blk, m2=4)))
-        # 2b  d/dx rows x value x {even, odd}: entries e = 2 fi + q
-        L.append(_Gemm("ypass_x", m=L0, n=H, k=Mh, batch=6 * B, A=p(Y1) + 8 * L0 * M1, lda=M1,
-                       ska=2, iA=_bidx(s1=2 * L0 * M1, d1=2, s2=1, m2=2), B=p(yf["B"]), ldb=Mh,
-                       tb=1, iB=_bidx(s1=H * Mh, m1=2), C=p(Y2) + 8 * 4 * blk, ldc=H,
-                       iC=_bidx(s1=6 * blk, d1=2, s2=blk, m2=2)))
-        # 3  convective products at both mirrored columns, folded for the forward transform
-        L.append(_Call("products", lib.pypde_rbc_products_yfold, ctypes.c_void_p(p(Y2)),
-                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1,
-                       float(yf["sv"]), float(yf["sd"]), B))
-        # 4  Z[0] = -dt P BF_y^T: output columns q::2 from the sum (q = 0) or
-        #    difference (q = 1) block when BF is mirror-even (sign > 0), else swapped
-        fb = self.fold_bf
-        if fb["sign"] > 0:
-            a0, iA = p(Pr), _bidx(s1=2 * blk, d1=2, s2=blk, m2=2)
-        else:
-            a0, iA = p(Pr) + 8 * blk, _bidx(s1=2 * blk, d1=2, s2=-blk, m2=2)
-        L.append(_Gemm("z_conv", m=L0, n=Mh, k=H, batch=6 * B, A=a0, lda=H, iA=iA,
-                       B=p(fb["B"]), ldb=H, tb=1, iB=_bidx(s1=Mh * H, m1=2), C=p(Z), ldc=M1,
-                       csn=2, iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt),
-                       alpha_div=6))
-
-    def _plain_y_launches(self, L):
-        """y-pass, products and y forward transform in the natural layout."""
-        B, L0, L1, M1 = self.B, self.L0, self.L1, self.M1
-        p = lambda t: t.data_ptr()
-        Y1, Y2, Pr, Z = self.Y1, self.Y2, self.Pr, self.Z
-        sZ = self.sZ
-        # 2  y-pass: values and y-derivatives on the L1 grid
-        g = _Gemm("ypass", m=2 * L0, n=2 * L1, k=M1, batch=3 * B, A=p(Y1), lda=M1,
-                  iA=_bidx(s1=2 * L0 * M1), B=p(self.GGy), ldb=M1, tb=1,
-                  C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1))
-        if self.fold_y is not None:
-            fy, h = self.fold_y, M1 // 2
-            g = self._with_alt(g, _Gemm(
-                "ypass", m=2 * L0, n=L1, k=h, batch=3 * B, A=p(Y1), lda=M1, ska=2,
-                iA=_bidx(s1=2 * L0 * M1), B=p(fy["E"]), ldb=h, tb=1,
-                C=p(Y2), ldc=2 * L1, iC=_bidx(s1=4 * L0 * L1), fold=2, A1=p(Y1) + 8,
-                B1=p(fy["O"]), lda1=M1, ldb1=h, k1=h, fold_len=L1, fold_sign=fy["sign"]))
-        L.append(g)
-        # 3  convective products
-        L.append(_Call("products", lib.pypde_rbc_products, ctypes.c_void_p(p(Y2)),
-                       ctypes.c_void_p(p(self.dTbcdz1)), ctypes.c_void_p(p(Pr)), L0, L1, B))
-        # 4  Z[0] = -dt P BF_y^T
-        g = _Gemm("z_conv", m=L0, n=M1, k=L1, batch=3 * B, A=p(Pr), lda=L1,
-                  iA=_bidx(s1=L0 * L1), B=p(self.BFy), ldb=L1, tb=1, C=p(Z), ldc=M1,
-                  iC=_bidx(s1=sZ), alpha_vec=p(self.t_mdt), alpha_div=3)
-        if self.fold_bf is not None:
-            fb, h = self.fold_bf, L1 // 2
-            # entries e = 2 f + q: output columns q::2 of field-instance f
-            g = self._with_alt(g, _Gemm(
-                "z_conv", m=L0, n=M1 // 2, k=h, batch=6 * B, A=p(Pr), lda=L1,
-                iA=_bidx(s1=L0 * L1, d1=2), B=p(fb["B"]), ldb=h, tb=1,
-                iB=_bidx(s1=(M1 // 2) * h, m1=2), C=p(Z), ldc=M1, csn=2,
-                iC=_bidx(s1=sZ, d1=2, s2=1, m2=2), alpha_vec=p(self.t_mdt), alpha_div=6,
-                fold=3, fold_len=L1, mirror_sign=fb["sign"], mirror_div=1))
-        L.append(g)
 
     @staticmethod
     def _with_alt(plain, folded):

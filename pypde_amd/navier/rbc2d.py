@@ -3,10 +3,11 @@ navier/rbc2d.py:28-434).
 
 Same constructor (`NavierStokes(case, shape=, ra=, pr=, dt=, tsave=, dealias=,
 integrator=, beta=, aspect=)`), fields (T, U, V, P, pres with .vhat / .v),
-`update()`, `iterate()`, `eval_Nu()`.  The Euler step with beta = 1 runs on
-the fused engine (pypde_amd.engine.RBCBatch, 19 launches per step, optionally
-replayed from a HIP graph); RK3 and beta != 1 run the reference's operation
-sequence on the drop-in GPU operators.
+`update()`, `iterate()`, `eval_Nu()`.  The step runs on the fused engine
+(pypde_amd.engine.RBCBatch, 19 launches per stage, optionally replayed from a
+HIP graph) for both integrators and any beta; only RK3 or beta != 1 with an odd
+dealiased y length falls back to the reference's operation sequence on the
+drop-in GPU operators.
 """
 import numpy as np
 import torch
@@ -47,7 +48,8 @@ class NavierStokes(NavierStokesBase, Integrator):
     # -----------------------------------------------------------------------
     @property
     def fused(self):
-        return self.integrator != "rk3" and self.beta == 1.0
+        """True when the step runs on the fused engine."""
+        return getattr(self, "engine", None) is not None
 
     def setup_solver(self):
         if self.integrator == "rk3":
@@ -57,10 +59,15 @@ class NavierStokes(NavierStokesBase, Integrator):
         keep = None
         if hasattr(self, "engine") or hasattr(self, "solver_U"):
             keep = self._get_state()
-        if self.fused:
-            self.engine = RBCBatch(self.shape, [dict(ra=self.ra, pr=self.pr, dt=self.dt)],
-                                   case=self.case, dealias=self.dealias, aspect=self.aspect,
-                                   beta=self.beta)
+        try:
+            engine = RBCBatch(self.shape, [dict(ra=self.ra, pr=self.pr, dt=self.dt)],
+                              case=self.case, dealias=self.dealias, aspect=self.aspect,
+                              beta=self.beta, integrator=self.integrator)
+        except NotImplementedError:
+            # RK3 / beta != 1 with odd dealiased lengths: the op-by-op path
+            engine = None
+        if engine is not None:
+            self.engine = engine
             e = self.engine
             self.Tbc, self.dTbcdz1, self.dTbcdz2, self.Tbc_cheby = (e.Tbc, e.dTbcdz1, e.dTbcdz2,
                                                                    e.Tbc_cheby)
