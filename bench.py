@@ -16,6 +16,7 @@ Also reported on the same line:
                  timed as 20 back-to-back launches replayed from a HIP graph
   batched        64 instances of 256x256 (Ra log-uniform in [1e4, 1e7]) in one
                  batched step: aggregate instance-steps/s and the ratio to the plain path
+  variants       the same 256x256 step with RK3 (three stages) and with beta = 0.5
   hetero         256 instances of mixed sizes (N per axis in {64, 96, 128, 192}) and
                  Ra, one graph launch per step, 500 steps: aggregate instance-steps/s and
                  the ratio to looping the plain single-instance step over them
@@ -25,6 +26,9 @@ Also reported on the same line:
                  to the initial temperature (checkpointed reverse sweep): 200 steps
                  at 256x256 and 2000 steps at 128x128, Ra=5e4, dt=5e-3, checkpoints
                  every 100 steps; reported as trajectory steps/s of the whole gradient
+  steady         time to a steady state by Newton-Krylov on the fused step (20x16 and
+                 64x64 at Ra=1e4, 128x128 at Ra=5e4; 300 steps from a roll, then
+                 max|F| <= 1e-10 with F over 0.5 time units)
   cpu_baseline   the oracle (CPU restatement of the reference) on the host, one
                  thread, on a bounded sample of the same 256x256 workload
 """
@@ -309,6 +313,43 @@ def microbench(nf=512, reps=10):
                 chain_hbm_frac=round(total_bytes / (total_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
 
 
+def time_variant(steps=100, **kw):
+    """Steps/s of the fused 256x256 step for another integrator / beta (graph replay)."""
+    import torch
+    from pypde_amd.navier import NavierStokes
+    N = 256
+    ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None, **kw)
+    assert ns.fused
+    ns.set_state(T=ns.T.forward(seeded_noise((N, N), seed=0)))
+    ns.update_many(5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ns.update_many(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return dict(kw, steps=steps, steps_per_s=round(steps / sec, 1),
+                finite=bool(np.isfinite(ns.eval_Nu()[0])))
+
+
+def time_steady(shape, ra, dt=0.01, tau=0.5, pre=300, tol=1e-10):
+    """Time to a steady state by GPU Newton-Krylov: set_temperature(0.2), pre
+    steps, then solve_steady_state(dt=tau) to max|F| <= tol."""
+    import torch
+    from pypde_amd.navier import NavierStokes
+    ns = NavierStokes(case="rbc", shape=shape, ra=ra, pr=1.0, dt=dt, tsave=None)
+    assert ns.fused
+    ns.set_temperature(amplitude=0.2)
+    ns.update_many(pre)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol = ns.solve_steady_state(dt=tau, tol=tol, maxiter=60, disp=False)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return dict(shape=list(shape), ra=ra, dt=dt, tau=tau, sec=round(sec, 3), success=bool(sol.success),
+                nit=int(sol.nit), nfev=int(sol.nfev), fnorm=float(sol.fnorm),
+                nu_plate=float(ns.eval_Nu()[0]))
+
+
 def time_adjoint(N, nsteps, ckpt=100, ra=5e4, dt=5e-3):
     """Whole gradient dJ/dT0 of J = mean_k Nu_plate(T_k) over nsteps: forward run
     with checkpoints, segment recomputation and the reverse VJP sweep."""
@@ -387,7 +428,10 @@ def main():
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
         extra["hetero"] = time_hetero()
+        extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5)]
         extra["microbench"] = microbench()
+        extra["steady"] = [time_steady((20, 16), 1e4), time_steady((64, 64), 1e4),
+                           time_steady((128, 128), 5e4)]
         extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000)}
     cpu = None
     if not args.no_cpu:

@@ -117,14 +117,92 @@ class NavierStokesBase:
                 X[vm].reshape(self.V.vhat.shape).clone())
 
     def steady_fun(self, X, NS=None, dt=None):
-        """(Phi_dt(X) - X)/dt for the flattened state [T, U, V]."""
+        """(Phi(X) - X)/dt for the flattened state [T, U, V]
+        (rbc2d_base.py:318-338): dt=None takes one step of NS.dt, otherwise
+        iterate() runs over dt time units from time zero.  The pressure is
+        whatever NS holds, and the call advances it."""
         NS = self if NS is None else NS
         X = ops.to_dev(X)
         T, U, V = NS.reshape(X)
         NS.set_state(T=T, U=U, V=V)
-        NS.update()
+        if dt is None:
+            dt = NS.dt
+            NS.update()
+        else:
+            NS.reset_time()
+            NS.iterate(dt, callback=False)
+            NS.reset_time()
         Y = NS.vectorify()
-        return (Y - X) / NS.dt
+        return (Y - X) / dt
+
+    def _iterate_count(self, maxtime):
+        """Number of steps iterate(maxtime) takes from time zero (solver.Integrator)."""
+        t, n, eps = 0.0, 0, 1e-3 * self.dt
+        while t + eps < maxtime:
+            t += self.dt
+            n += 1
+        return n
+
+    def solve_steady_state(self, X0=None, dt=None, maxiter=300, disp=True, tol=1e-8,
+                           jac_options={"inner_maxiter": 30}):
+        """Steady state by Jacobian-free Newton-Krylov on the GPU (the
+        capability of rbc2d_base.py:259-291).
+
+        The unknown is [T, U, V, pres]: F = (Phi(z) - z)/tau, Phi the time-step
+        map over tau = dt (iterate(dt) steps; dt=None: one step of self.dt)
+        started from exactly z, so F is a function of the state alone.  X0 is
+        the reference's [T, U, V] vector (default: the current state); the
+        current pressure starts the pressure part.  Converged when
+        max|F| <= tol.  jac_options: inner_maxiter (GMRES restart cycles per
+        Newton step), inner_m (Krylov basis size, 30), rdiff (finite-difference
+        step, sqrt of the unit roundoff).
+
+        Returns a result with x (the [T, U, V] vector), pres, success, status,
+        message, nit, nfev, fun (F at x, [T, U, V] part), fnorm (max|F| over
+        all of z) and history; the solver state is set to the solution."""
+        from ..steady import newton_krylov
+        opts = dict(inner_maxiter=30, inner_m=30, rdiff=None)
+        opts.update(jac_options or {})
+        unknown = set(opts) - {"inner_maxiter", "inner_m", "rdiff"}
+        if unknown:
+            raise ValueError("unsupported jac_options: %s" % sorted(unknown))
+        nsteps = 1 if dt is None else self._iterate_count(dt)
+        if nsteps < 1:
+            raise ValueError("dt must cover at least one time step")
+        tau = nsteps * self.dt
+        if disp:
+            print("\nSolve steady state ...\n")
+        X0 = self.vectorify() if X0 is None else ops.to_dev(X0).reshape(-1)
+        nx = X0.numel()
+        pshape = self.pres.vhat.shape
+        z0 = torch.cat([X0, ops.to_dev(self.pres.vhat).reshape(-1)])
+
+        def load(z):
+            T, U, V = self.reshape(z[:nx])
+            self.set_state(T=T, U=U, V=V, pres=z[nx:].reshape(pshape))
+
+        def advance():
+            if hasattr(self, "update_many"):
+                self.update_many(nsteps)
+            else:
+                for _ in range(nsteps):
+                    self.update()
+
+        def F(z):
+            load(z)
+            advance()
+            Y = torch.cat([self.vectorify(), ops.to_dev(self.pres.vhat).reshape(-1)])
+            return (Y - z) / tau
+
+        sol = newton_krylov(F, z0, fatol=tol, maxiter=maxiter, inner_m=opts["inner_m"],
+                            inner_maxiter=opts["inner_maxiter"], rdiff=opts["rdiff"],
+                            disp=disp)
+        z = sol.x
+        load(z)
+        sol.x = z[:nx].clone()
+        sol.pres = z[nx:].reshape(pshape).clone()
+        sol.fun = sol.fun[:nx].clone()
+        return sol
 
 
 def eval_Nu(T, field, Lz=1.0, Tbc=None):
