@@ -1,9 +1,15 @@
-"""Build the pypde_amd HIP library (gfx950) in-tree: pypde_amd/libpypde_hip.so.
+"""Build the pypde_amd libraries in-tree.
 
     python pypde_amd/build.py        (or __graft_entry__.build())
 
-Each .hip source is compiled to an object with hipcc, then linked into one
-shared library exporting the C-ABI declared in include/pypde_amd.h.
+pypde_amd/libpypde_hip.so (gfx950): each .hip source is compiled to an object
+with hipcc, then linked into one shared library exporting the C-ABI declared in
+include/pypde_amd.h.
+
+pypde_amd/libpypde_h5.so (host): csrc/h5state.c with gcc against the HDF5 C
+library (headers and libhdf5 under HDF5_DIR, default /opt/conda), the state
+files of include/pypde_amd_h5.h.  Skipped with a message when HDF5 is absent;
+the IO functions then raise.
 """
 import os
 import subprocess
@@ -16,6 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpypde_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+H5LIB = os.path.join(HERE, "libpypde_h5.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -41,6 +49,27 @@ def _compile(src):
     return obj
 
 
+def build_h5(verbose=True):
+    """Compile the HDF5 state-file library; returns its path or None without HDF5."""
+    inc = os.path.join(HDF5_DIR, "include")
+    lib = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print("HDF5 headers not found under %s: libpypde_h5.so not built" % HDF5_DIR)
+        return None
+    tmp = H5LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", "-I" + inc,
+           "-I" + os.path.join(REPO, "include"), os.path.join(CSRC, "h5state.c"),
+           "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gcc failed for h5state.c:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, H5LIB)
+    if verbose:
+        print("built", H5LIB)
+    return H5LIB
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -54,6 +83,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_h5(verbose)
     return LIB
 
 

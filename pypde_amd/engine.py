@@ -256,8 +256,12 @@ class RBCBatch:
         ra = np.array([p["ra"] for p in params], float)
         pr = np.array([p.get("pr", 1.0) for p in params], float)
         dt = np.array([p["dt"] for p in params], float)
-        nu = np.sqrt(pr / (ra / 1.0 ** 3.0))
-        kappa = np.sqrt(1 / pr / (ra / 1.0 ** 3.0))
+        # nu, kappa from Ra and Pr over a unit height (rbc2d_base.py:46-56)
+        # unless the instance gives them
+        nu = np.array([p["nu"] if "nu" in p else np.sqrt(b / (a / 1.0 ** 3.0))
+                       for p, a, b in zip(params, ra, pr)], float)
+        kappa = np.array([p["kappa"] if "kappa" in p else np.sqrt(1 / b / (a / 1.0 ** 3.0))
+                          for p, a, b in zip(params, ra, pr)], float)
         self.dt, self.nu, self.kappa = dt, nu, kappa
         self.t_mdt = T(-dt)
         self.t_dt = T(dt)

@@ -115,6 +115,82 @@ class FieldBase:
     def dy(self):
         return _dx(self.y)
 
+    # -- read / write (field.py:83-173): HDF5 through pypde_amd.h5io
+    @staticmethod
+    def _filename(filename, leading_str, add_time, t):
+        if filename is None:
+            filename = leading_str
+            if add_time:
+                filename = filename + "_{:07.2f}".format(float(t))
+            filename = filename + ".h5"
+        return filename
+
+    def write(self, filename="file_0.h5", dict=None, leading_str="flow", add_time=True,
+              grp_name=""):
+        """Datasets {grp}/v, {grp}/vhat, time, x, y and the entries of dict
+        (created, or overwritten in place when they exist)."""
+        from .h5io import H5File
+        filename = self._filename(filename, leading_str, add_time, self.t)
+        if grp_name and grp_name[-1] != "/":
+            grp_name = grp_name + "/"
+        print("Write {:s} ...".format(filename))
+        with H5File(filename, "a") as hf:
+            self.write_single_hdf5(hf, grp_name + "v", self.v)
+            self.write_single_hdf5(hf, grp_name + "vhat", self.vhat)
+            self.write_single_hdf5(hf, "time", self.t)
+            if self.x is not None:
+                self.write_single_hdf5(hf, "x", self.x)
+            if self.y is not None:
+                self.write_single_hdf5(hf, "y", self.y)
+            if dict is not None:
+                for key in dict:
+                    self.write_single_hdf5(hf, key, dict[key])
+
+    @staticmethod
+    def write_single_hdf5(hf, name, data):
+        hf.write(name, data)
+
+    @staticmethod
+    def read_single_hdf5(hf, name):
+        data = hf.read(name)
+        if data is not None:
+            return data
+        print("Cannot read {:} ...".format(name))
+        print("Following keys exist: ")
+        for key in hf.keys():
+            print_grp(key)
+        return 0.0
+
+    def read(self, filename="file_0.h5", dict=None, leading_str="flow", add_time=True,
+             grp_name=""):
+        """Fill v, vhat (in place, so views bound elsewhere see the values),
+        t and the entries of dict from the file; a missing dataset reads 0.0."""
+        from .h5io import H5File
+        filename = self._filename(filename, leading_str, add_time, self.t)
+        print("Read {:s} ...".format(filename))
+        if grp_name and grp_name[-1] != "/":
+            grp_name = grp_name + "/"
+        with H5File(filename, "r") as hf:
+            for attr in ("v", "vhat"):
+                dst = getattr(self, attr)
+                val = self.read_single_hdf5(hf, grp_name + attr)
+                if np.ndim(val) == 0:
+                    dst.fill_(float(val))
+                else:
+                    if tuple(val.shape) != tuple(dst.shape):
+                        raise ValueError("could not broadcast %s of shape %s into %s"
+                                         % (grp_name + attr, val.shape, tuple(dst.shape)))
+                    dst.copy_(torch.as_tensor(val))
+            t = self.read_single_hdf5(hf, "time")
+            self.t = float(t)
+            if dict is not None:
+                for key in dict:
+                    dict[key] = self.read_single_hdf5(hf, key)
+
+
+def print_grp(name):
+    print("Group:", name)
+
 
 def _unpad(a, n, axis):
     if a.dim() == 1:
@@ -216,3 +292,14 @@ class MultiField:
     def update_time(self, dt):
         for f in self.fields:
             f.t += dt
+
+    def read(self, filename=None, leading_str="", add_time=True, dict={}):
+        for f, n in zip(self.fields, self.names):
+            f.read(filename=filename, leading_str=leading_str, add_time=add_time, dict=dict,
+                   grp_name=n)
+
+    def write(self, filename=None, leading_str="", add_time=True, dict={}):
+        for f, n in zip(self.fields, self.names):
+            f.backward()
+            f.write(filename=filename, leading_str=leading_str, add_time=add_time, dict=dict,
+                    grp_name=n)

@@ -60,7 +60,8 @@ class NavierStokes(NavierStokesBase, Integrator):
         if hasattr(self, "engine") or hasattr(self, "solver_U"):
             keep = self._get_state()
         try:
-            engine = RBCBatch(self.shape, [dict(ra=self.ra, pr=self.pr, dt=self.dt)],
+            engine = RBCBatch(self.shape, [dict(ra=self.ra, pr=self.pr, dt=self.dt, nu=self.nu,
+                                                kappa=self.kappa)],
                               case=self.case, dealias=self.dealias, aspect=self.aspect,
                               beta=self.beta, integrator=self.integrator)
         except NotImplementedError:

@@ -98,6 +98,40 @@ class NavierStokesBase:
     def save(self):
         self.field.save()
 
+    # -- state files (rbc2d_base.py:205-257), HDF5 through pypde_amd.h5io
+    def write(self, filename=None, leading_str="", add_time=True):
+        """Every field under its group (temp, ux, uy, pres) with nu, kappa,
+        ra, pr at the root."""
+        meta = {"nu": self.nu, "kappa": self.kappa,
+                "ra": Ra(self.nu, self.kappa, L=self.y[-1] - self.y[0]),
+                "pr": Pr(self.nu, self.kappa)}
+        self.field.write(filename=filename, leading_str=leading_str, add_time=add_time, dict=meta)
+
+    def read(self, filename=None, leading_str="", add_time=True):
+        """Fields, time, ra and pr from a state file, then the solver is rebuilt."""
+        meta = {"ra": self.ra, "pr": self.pr}
+        self.field.read(filename=filename, leading_str=leading_str, add_time=add_time, dict=meta)
+        self.time = self.field.fields[0].t
+        # the reference's order: nu and kappa are recomputed before the file's
+        # ra and pr are applied, so they follow the previous ra and pr
+        self.set_nu_kappa()
+        self.CONFIG.update({k: float(v) for k, v in meta.items()})
+        self.__dict__.update(**self.CONFIG)
+        self.setup_solver()
+
+    def read_from_filename(self, filename):
+        pass
+
+    def write_from_Ra(self, folder=""):
+        if folder and folder[-1] != "/":
+            folder = folder + "/"
+        self.write(filename=folder + self.fname_from_Ra(self.ra))
+
+    def read_from_Ra(self, folder=""):
+        if folder and folder[-1] != "/":
+            folder = folder + "/"
+        self.read(filename=folder + self.fname_from_Ra(self.ra))
+
     # -- steady state (rbc2d_base.py:297-341)
     def flatten(self):
         return (self.T.vhat.flatten().clone(), self.U.vhat.flatten().clone(),

@@ -7,25 +7,25 @@ import re
 from conftest import REPO
 
 
-def declared_symbols():
-    hdrs = [os.path.join(REPO, "include", f) for f in os.listdir(os.path.join(REPO, "include"))
-            if f.endswith(".h")]
-    names = set()
-    for h in hdrs:
-        src = open(h).read()
-        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        for m in re.finditer(r"\b(pypde_[a-z0-9_]+)\s*\(", src):
-            names.add(m.group(1))
-    return sorted(names)
+def declared_symbols(header="pypde_amd.h"):
+    src = open(os.path.join(REPO, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted({m.group(1) for m in re.finditer(r"\b(pypde_[a-z0-9_]+)\s*\(", src)})
+
+
+def test_every_header_is_checked():
+    assert sorted(f for f in os.listdir(os.path.join(REPO, "include")) if f.endswith(".h")) == \
+        ["pypde_amd.h", "pypde_amd_h5.h"]
 
 
 def test_library_exports_every_declared_symbol():
-    from pypde_amd import _lib
-    lib = ctypes.CDLL(_lib.LIB_PATH)
-    names = declared_symbols()
-    assert len(names) >= 10
-    missing = [n for n in names if not hasattr(lib, n)]
-    assert not missing, missing
+    from pypde_amd import _lib, h5io
+    for header, path in (("pypde_amd.h", _lib.LIB_PATH), ("pypde_amd_h5.h", h5io.LIB_PATH)):
+        lib = ctypes.CDLL(path)
+        names = declared_symbols(header)
+        assert len(names) >= 7, header
+        missing = [n for n in names if not hasattr(lib, n)]
+        assert not missing, (header, missing)
 
 
 def test_version_and_errors_without_gpu():
