@@ -29,6 +29,7 @@ Also reported on the same line:
   steady         time to a steady state by Newton-Krylov on the fused step (20x16 and
                  64x64 at Ra=1e4, 128x128 at Ra=5e4; 300 steps from a roll, then
                  max|F| <= 1e-10 with F over 0.5 time units)
+  adjoint_descent  the reference's adjoint-descent update (navier/rbc2d_adj.py) at 256x256
   cpu_baseline   the oracle (CPU restatement of the reference) on the host, one
                  thread, on a bounded sample of the same 256x256 workload
 """
@@ -350,6 +351,24 @@ def time_steady(shape, ra, dt=0.01, tau=0.5, pre=300, tol=1e-10):
                 nu_plate=float(ns.eval_Nu()[0]))
 
 
+def time_adjoint_descent(N, steps=20, ra=1e4, dt=0.005):
+    """Adjoint-descent updates per second (navier.rbc2d_adj drop-in) at N x N."""
+    import torch
+    from pypde_amd.navier import NavierStokesAdjoint
+    nsa = NavierStokesAdjoint(case="rbc", shape=(N, N), ra=ra, pr=1.0, dt=dt, tsave=None)
+    nsa.set_temperature(amplitude=0.2)
+    for _ in range(3):
+        nsa.update()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nsa.update()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return dict(N=N, ra=ra, dt=dt, steps=steps, ms_per_update=round(sec / steps * 1e3, 3),
+                finite=bool(torch.isfinite(nsa.T.vhat).all()))
+
+
 def time_adjoint(N, nsteps, ckpt=100, ra=5e4, dt=5e-3):
     """Whole gradient dJ/dT0 of J = mean_k Nu_plate(T_k) over nsteps: forward run
     with checkpoints, segment recomputation and the reverse VJP sweep."""
@@ -432,6 +451,7 @@ def main():
         extra["microbench"] = microbench()
         extra["steady"] = [time_steady((20, 16), 1e4), time_steady((64, 64), 1e4),
                            time_steady((128, 128), 5e4)]
+        extra["adjoint_descent"] = time_adjoint_descent(256)
         extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000)}
     cpu = None
     if not args.no_cpu:

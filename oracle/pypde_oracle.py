@@ -916,6 +916,118 @@ class NavierStokes:
         return (Y - X) / self.dt
 
 
+class NavierStokesAdjoint:
+    """Oracle restatement of navier.rbc2d_adj.NavierStokesAdjoint
+    (rbc2d_adj.py:15-356): adjoint-descent iterations for the steady-state
+    residual.  Each stage takes the residual R = (step(X) - X)/dt of an inner
+    NavierStokes solver (whose pressure carries over between calls), smooths it
+    by Poisson solves (nabla^2 A = R / nu, / kappa), and moves X along
+    convective and residual terms of the adjoint fields, then projects the
+    velocity as the forward step does."""
+
+    def __init__(self, case="rbc", **kwargs):
+        self.NS = NavierStokes(case=case, **kwargs)  # rbc2d_adj.py:44
+        ns = self.NS
+        for k in ("shape", "dt", "dealias", "nu", "kappa", "scale", "deriv_field", "a", "b",
+                  "c", "nstage"):
+            setattr(self, k, getattr(ns, k))
+        N0, N1 = self.shape
+        side = "CN" if case == "rbc" else "CD"
+
+        def fld(kx, ky, dealias=1.5):
+            return Field([Basis(N0, kx, dealias), Basis(N1, ky, dealias)])
+
+        self.T, self.U, self.V = fld(side, "CD"), fld("CD", "CD"), fld("CD", "CD")
+        self.P = Field([Basis(N0, "CN"), Basis(N1, "CN")])
+        self.pres = Field([Basis(N0, "CH"), Basis(N1, "CH")])
+        self.TA, self.UA, self.VA = fld(side, "CD"), fld("CD", "CD"), fld("CD", "CD")
+        # plans (rbc2d_adj.py:131-144): the inner solver's pressure plan and
+        # non-singular Poisson plans on each field's own bases for the smoothing
+        self.solver_P = ns.solver_P
+        self.nabla_U = solverplan_poisson2d(self.U.xs, singular=False, scale=self.scale)
+        self.nabla_V = solverplan_poisson2d(self.V.xs, singular=False, scale=self.scale)
+        self.nabla_T = solverplan_poisson2d(self.T.xs, singular=False, scale=self.scale)
+        # conductive profile on the (dealiased) physical grid (rbc2d_adj.py:112-117)
+        tbc = galerkin_to_cheby(ns.Tbc.vhat, ns.Tbc)
+        df = self.deriv_field
+        self.temp_bc = df.dealias.backward(tbc) if self.dealias else df.backward(tbc)
+
+    def grad(self, field, deriv):
+        return grad(field, deriv, scale=self.scale)
+
+    def _phys(self, f):
+        return f.dealias.backward(f.vhat) if self.dealias else f.backward(f.vhat)
+
+    def _conv(self, field, u, deriv):
+        return conv_term(field, u, deriv, self.deriv_field, self.dealias, self.scale)
+
+    def _conv_adj(self, deriv, ux, uz, temp):
+        """rbc2d_adj.py:162-187: d(UA) ux + d(VA) uz + d(TA) temp + d(TA) temp_bc,
+        d = d/dx for deriv (1, 0) and d/dy for (0, 1), then the forward transform."""
+        conv = self._conv(self.UA, ux, deriv)
+        conv = conv + self._conv(self.VA, uz, deriv)
+        conv = conv + self._conv(self.TA, temp, deriv)
+        conv = conv + self._conv(self.TA, self.temp_bc, deriv)
+        df = self.deriv_field
+        return df.dealias.forward(conv) if self.dealias else df.forward(conv)
+
+    def _residual(self):
+        """rbc2d_adj.py:189-205."""
+        ns = self.NS
+        ns.U.vhat, ns.V.vhat, ns.T.vhat = self.U.vhat.copy(), self.V.vhat.copy(), self.T.vhat.copy()
+        ns.update()
+        ns.U.vhat = (ns.U.vhat - self.U.vhat) / self.dt
+        ns.V.vhat = (ns.V.vhat - self.V.vhat) / self.dt
+        ns.T.vhat = (ns.T.vhat - self.T.vhat) / self.dt
+        for plan, src, dst, coef in ((self.nabla_U, ns.U, self.UA, self.nu),
+                                     (self.nabla_V, ns.V, self.VA, self.nu),
+                                     (self.nabla_T, ns.T, self.TA, self.kappa)):
+            dst.vhat = plan.solve_lhs(plan.solve_rhs(galerkin_to_cheby(src.vhat, src))) / coef
+
+    def _velocity_rhs(self, stage, deriv, own_A, resid):
+        """rbc2d_adj.py:207-257 (U: deriv (1, 0); V: deriv (0, 1))."""
+        a, b, c = self.a[stage], self.b[stage], self.c[stage]
+        rhs = np.zeros(self.shape)
+        rhs -= a * self.grad(self.pres, deriv)
+        rhs += b * self.NS.conv_term(own_A, self.ux, self.uz)
+        rhs += b * self._conv_adj(deriv, self.ux, self.uz, self.temp)
+        if c != 0:
+            rhs += c * self.NS.conv_term(own_A, self.ux_old, self.uz_old)
+            rhs += c * self._conv_adj(deriv, self.ux_old, self.uz_old, self.temp_old)
+        rhs += a * galerkin_to_cheby(resid.vhat, own_A)
+        return rhs
+
+    def update(self):
+        """rbc2d_adj.py:321-356."""
+        self.ux_old, self.uz_old, self.temp_old = 0, 0, 0
+        for rk in range(self.nstage):
+            a, b, c = self.a[rk], self.b[rk], self.c[rk]
+            self.ux, self.uz, self.temp = self._phys(self.U), self._phys(self.V), self._phys(self.T)
+            self._residual()
+            ns = self.NS
+            rhs = self._velocity_rhs(rk, (1, 0), self.UA, ns.U)
+            self.U.vhat = self.U.vhat + self.dt * cheby_to_galerkin(rhs, self.U)
+            rhs = self._velocity_rhs(rk, (0, 1), self.VA, ns.V)
+            self.V.vhat = self.V.vhat + self.dt * cheby_to_galerkin(rhs, self.V)
+            # projection (rbc2d_adj.py:283-315)
+            div = ns.divergence_velocity(self.U, self.V)
+            self.P.vhat = self.solver_P.solve_lhs(self.solver_P.solve_rhs(div))
+            self.P.vhat[0, 0] = 0
+            self.pres.vhat = self.pres.vhat + galerkin_to_cheby(self.P.vhat, self.P) / (self.dt * a)
+            dpdx, dpdz = self.grad(self.P, (1, 0)), self.grad(self.P, (0, 1))
+            self.U.vhat = self.U.vhat - cheby_to_galerkin(dpdx * 1.0, self.U)
+            self.V.vhat = self.V.vhat - cheby_to_galerkin(dpdz * 1.0, self.V)
+            # temperature (rbc2d_adj.py:259-281)
+            rhs = np.zeros(self.shape)
+            rhs += b * ns.conv_term(self.TA, self.ux, self.uz)
+            if c != 0:
+                rhs += c * ns.conv_term(self.TA, self.ux_old, self.uz_old)
+            rhs += a * galerkin_to_cheby(ns.T.vhat, self.T)
+            rhs += a * galerkin_to_cheby(self.VA.vhat, self.VA)
+            self.T.vhat = self.T.vhat + self.dt * cheby_to_galerkin(rhs, self.T)
+            self.ux_old, self.uz_old, self.temp_old = self.ux, self.uz, self.temp
+
+
 def eval_Nu(T, field, Lz=1.0, Tbc=None):
     Tv = T.backward(T.vhat)
     if Tbc is not None:

@@ -67,3 +67,17 @@ def test_steady_residual_matches_reference(golden):
     ns.V.vhat = X[mt + mu:].reshape(ns.V.vhat.shape).copy()
     r = ns.steady_residual()
     assert rel(r, golden["steady_res"]) < 1e-13
+
+
+def test_oracle_adjoint_descent_step_matches_reference(golden):
+    """NavierStokesAdjoint.update (rbc2d_adj.py:321-356) from the golden input."""
+    nsa = o.NavierStokesAdjoint(case="rbc", shape=(20, 16), ra=5e4, pr=1.0, dt=0.005,
+                                tsave=None, dealias=True, integrator="eu", beta=1.0)
+    nsa.T.vhat = golden["adj_in_T"].copy()
+    nsa.U.vhat = golden["adj_in_U"].copy()
+    nsa.V.vhat = golden["adj_in_V"].copy()
+    nsa.update()
+    for f in ("T", "U", "V"):
+        assert rel(getattr(nsa, f).vhat, golden["adj_out_" + f]) < 1e-14, f
+    assert rel(nsa.pres.vhat, golden["adj_out_pres"]) < 1e-14
+    assert rel(nsa.NS.pres.vhat, golden["adj_ns_pres"]) < 1e-14
