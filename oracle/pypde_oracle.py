@@ -245,13 +245,14 @@ def _tosparse_csc(A, tol=1e-12):
 # Bases (chebyshev.py)
 # --------------------------------------------------------------------------
 class Basis:
-    """Chebyshev ("CH"), ChebDirichlet ("CD"), ChebNeumann ("CN") and the BC
-    bases DirichletC ("DC") / NeumannC ("NC"); chebyshev.py:22-451,538-597."""
+    """Chebyshev ("CH"), ChebDirichlet ("CD"), ChebNeumann ("CN"),
+    ChebDirichletNeumann ("CDN") and the BC bases DirichletC ("DC") / NeumannC
+    ("NC"); chebyshev.py:22-597."""
 
     def __init__(self, N, key, dealias=None):
         key = {"Chebyshev": "CH", "ChebDirichlet": "CD", "ChebNeumann": "CN",
-               "DirichletC": "DC", "NeumannC": "NC"}.get(key, key)
-        if key not in ("CH", "CD", "CN", "DC", "NC"):
+               "ChebDirichletNeumann": "CDN", "DirichletC": "DC", "NeumannC": "NC"}.get(key, key)
+        if key not in ("CH", "CD", "CN", "CDN", "DC", "NC"):
             raise ValueError("Key {:} not available.".format(key))
         self.key = self.id = key
         self.N = N
@@ -284,6 +285,12 @@ class Basis:
         elif self.key == "CN":
             for i in range(M):
                 S[i, i], S[i + 2, i] = 1, -((i / (i + 2)) ** 2)
+        elif self.key == "CDN":  # chebyshev.py:470-484
+            k = np.arange(N)
+            d1 = 4 * (k[:-1] + 1) / (2 * k[:-1] ** 2 + 6 * k[:-1] + 5)
+            d2 = -(2 * k[:-2] ** 2 + 2 * k[:-2] + 1) / (2 * k[:-2] ** 2 + 6 * k[:-2] + 5)
+            for i in range(M):
+                S[i, i], S[i + 1, i], S[i + 2, i] = 1.0, d1[i], d2[i]
         elif self.key == "DC":
             S[0, 0], S[1, 0] = 0.5, -0.5
             S[0, 1], S[1, 1] = 0.5, 0.5
@@ -346,6 +353,8 @@ class Basis:
         assert uhat.shape[0] == self.N
         if self.key in ("DC", "NC"):
             return np.linalg.solve(self.S[:2, :2], uhat[:2])
+        if self.key == "CDN":  # chebyshev.py:486-506: least squares
+            return np.linalg.lstsq(self.S, uhat, rcond=None)[0]
         l2, d, u2 = self.stencil_inv_diags()
         return tdma_solve(l2, d, u2, self.ST_sp @ uhat, 2)
 

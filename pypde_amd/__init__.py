@@ -5,8 +5,8 @@ field operations, solver plans, navier.rbc2d) with float64 device tensors and
 hand-written gfx950 HIP kernels behind the C-ABI in include/pypde_amd.h.
 """
 from ._lib import PypdeError
-from .bases import (Base, Chebyshev, ChebDirichlet, ChebNeumann, DirichletC, NeumannC,
-                    MetaBase, GalerkinChebyshev, gauss_lobatto)
+from .bases import (Base, Chebyshev, ChebDirichlet, ChebDirichletNeumann, ChebNeumann,
+                    DirichletC, NeumannC, MetaBase, GalerkinChebyshev, gauss_lobatto)
 from .field import Field, FieldBC, MultiField, SpectralSpace, SpectralSpaceBC
 from .field_operations import (avg_vol, avg_x, cheby_to_galerkin, conv_term, convective_term,
                                galerkin_to_cheby, grad, interpolate)
@@ -16,7 +16,7 @@ from .templates import (solverplan_hholtz1d, solverplan_hholtz2d_adi, solverplan
                         solverplan_poisson2d)
 
 __all__ = [
-    "PypdeError", "Base", "Chebyshev", "ChebDirichlet", "ChebNeumann", "DirichletC", "NeumannC",
+    "PypdeError", "Base", "Chebyshev", "ChebDirichlet", "ChebNeumann", "ChebDirichletNeumann", "DirichletC", "NeumannC",
     "MetaBase", "GalerkinChebyshev", "gauss_lobatto", "Field", "FieldBC", "MultiField",
     "SpectralSpace", "SpectralSpaceBC", "avg_vol", "avg_x", "cheby_to_galerkin", "conv_term",
     "convective_term", "galerkin_to_cheby", "grad", "interpolate", "PDMA", "Integrator",

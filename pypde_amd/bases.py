@@ -1,5 +1,5 @@
-"""Spectral bases on the GPU: Chebyshev, ChebDirichlet, ChebNeumann and the
-boundary bases DirichletC / NeumannC.
+"""Spectral bases on the GPU: Chebyshev, ChebDirichlet, ChebNeumann,
+ChebDirichletNeumann and the boundary bases DirichletC / NeumannC.
 
 Drop-in for pypde.bases (pypde/bases/spectralbase.py:12-45,
 pypde/bases/chebyshev.py:22-597): same constructor (`Base(N, key, dealias=)`),
@@ -360,6 +360,61 @@ class ChebNeumann(GalerkinChebyshev):
         return S
 
 
+class ChebDirichletNeumann(GalerkinChebyshev):
+    """u(-1) = u'(1) = 0: phi_k = T_k + d1_k T_{k+1} + d2_k T_{k+2}
+    (pypde/bases/chebyshev.py:454-530).
+
+    The stencil has a -1 diagonal, so to_chebyshev is a three-diagonal banded
+    apply, and from_chebyshev the least-squares map the reference computes per
+    call with lstsq(S, u): pinv(S), an M x N table applied as one GEMM."""
+
+    def __init__(self, N, dealias=None):
+        GalerkinChebyshev.__init__(self, N, dealias)
+        self.id = "CDN"
+        self.bc = None
+
+    def _stencil(self):
+        S = np.zeros((self.N, self.M))
+        k = np.arange(self.N, dtype=float)
+        den = 2 * k ** 2 + 6 * k + 5
+        d1 = 4 * (k + 1) / den
+        d2 = -(2 * k ** 2 + 2 * k + 1) / den
+        for i in range(self.M):
+            S[i, i], S[i + 1, i], S[i + 2, i] = 1.0, d1[i], d2[i]
+        return S
+
+    def _band(self):
+        """Diagonals of S (N x M) for the banded apply: vals[q, r] = S[r, r + off[q]]."""
+        def build():
+            S = self._stencil()
+            vals = np.zeros((3, self.N))
+            for q, off in enumerate((-2, -1, 0)):
+                for r in range(self.N):
+                    if 0 <= r + off < self.M:
+                        vals[q, r] = S[r, r + off]
+            return vals
+        return self._dev.get("band", build)
+
+    def to_chebyshev(self, vhat, axis=0):
+        vhat = ops.to_dev(vhat)
+        assert ops._len(vhat, axis) == self.M, "{} {}".format(ops._len(vhat, axis), self.M)
+        return ops.banded_apply((-2, -1, 0), self._band(), vhat, axis, len_out=self.N)
+
+    def from_chebyshev(self, uhat, axis=0, exact=False):
+        uhat = ops.to_dev(uhat)
+        assert ops._len(uhat, axis) == self.N, "Shape mismatch ({:3}) ({:3})".format(
+            ops._len(uhat, axis), self.N)
+        P = self._dev.get("pinv", lambda: np.linalg.pinv(self._stencil()))
+        return ops.matmul_axis(P, uhat, axis)
+
+    def derivative(self, vhat, deriv, out_cheby=True, axis=0, scale_pow=1.0):
+        u = self.to_chebyshev(vhat, axis)
+        duhat = self.family.derivative(u, deriv, axis=axis, scale_pow=scale_pow)
+        if out_cheby:
+            return duhat
+        return self.from_chebyshev(duhat, axis)
+
+
 class _BCBase(GalerkinChebyshev):
     """Two-function bases carrying boundary values (chebyshev.py:538-597)."""
 
@@ -419,6 +474,7 @@ _KEYS = {
     "CH": Chebyshev, "Chebyshev": Chebyshev,
     "CD": ChebDirichlet, "ChebDirichlet": ChebDirichlet,
     "CN": ChebNeumann, "ChebNeumann": ChebNeumann,
+    "CDN": ChebDirichletNeumann, "ChebDirichletNeumann": ChebDirichletNeumann,
     "DC": DirichletC, "DirichletC": DirichletC,
     "NC": NeumannC, "NeumannC": NeumannC,
 }

@@ -25,7 +25,10 @@ def to_dev(a):
     if isinstance(a, torch.Tensor):
         t = a.to(device=device(), dtype=DTYPE)
     else:
-        t = torch.as_tensor(np.asarray(a, dtype=np.float64), device=device())
+        arr = np.asarray(a, dtype=np.float64)
+        if not arr.flags.writeable:  # read-only arrays (np.load) cannot back a tensor
+            arr = arr.copy()
+        t = torch.as_tensor(arr, device=device())
     return t.contiguous()
 
 

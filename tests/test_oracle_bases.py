@@ -161,3 +161,26 @@ def test_conductive_state_has_unit_nusselt(case):
     nu0 = ns.eval_Nu()
     assert abs(nu0[0] - 1.0) < 1e-12 and abs(nu0[1] - 1.0) < 1e-12
     assert ns.eval_KE() == 0.0
+
+
+def test_dirichlet_neumann_basis():
+    """ChebDirichletNeumann (chebyshev.py:454-530): every basis function has
+    phi(-1) = 0 and phi'(1) = 0; from_chebyshev inverts to_chebyshev; a
+    function with those boundary values is expanded to spectral accuracy."""
+    N = 30
+    b = o.Basis(N, "CDN")
+    S = b.S
+    assert S.shape == (N, N - 2)
+    for k in range(N - 2):
+        assert abs(npcheb.chebval(-1.0, S[:, k])) < 1e-14
+        assert abs(npcheb.chebval(1.0, npcheb.chebder(S[:, k]))) < 1e-12
+    # the diagonals match the closed form T_k + d1 T_k+1 + d2 T_k+2
+    k = 5.0
+    den = 2 * k * k + 6 * k + 5
+    assert S[5, 5] == 1.0 and abs(S[6, 5] - 4 * (k + 1) / den) < 1e-16
+    assert abs(S[7, 5] + (2 * k * k + 2 * k + 1) / den) < 1e-16
+    v = rng.standard_normal(N - 2)
+    assert rel(b.from_chebyshev(b.to_chebyshev(v)), v) < 1e-13
+    f = np.sin(np.pi * (b.x + 1) / 4)  # f(-1) = 0, f'(1) = 0
+    vhat = b.forward_fft(f)
+    assert np.abs(b.backward_fft(vhat) - f).max() < 1e-13
