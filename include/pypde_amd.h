@@ -181,7 +181,14 @@ int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, con
  * Tables are zero-padded so that chain starts/ends are encoded (LM_i = 0 for
  * i < 2, MU_i = 0 for i >= n-2, NU_i = 0 for i >= n-4) and hold reciprocal
  * pivots, so the kernel does not divide.  Tables are per batch entry at
- * entry_stride (0 = shared) or, with per_line = 1, laid out [n][nlines].
+ * entry_stride (0 = shared) or per line: per_line = 1 lays them out
+ * [n][nlines] (contiguous slots LM | R | MU | NU), per_line = 2 line-major,
+ * [nlines][n] per table.
+ * variant: 1 = block form (64 lines per block, the line split into chunks over
+ * up to 16 waves; for many lines), 2 = lane-chunk form (one wave per line, a
+ * chunk of n/64 positions per lane, carries combined by wave scans; for few
+ * lines), 0 = chosen by the number of lines (lane-chunk up to 2048 lines).
+ * per_line = 2 implies the lane-chunk form, per_line = 1 the block form.
  * Carries: the four-diagonal Helmholtz sweeps (Plan_fdma.solve, plans.py:184-224
  * -> fdma.f90:40-98), the Poisson line solves (Plan_Poisson.solve_fortran,
  * plans.py:310-314 -> fdma.f90:146-195, factors formed once at plan time), the
@@ -198,6 +205,7 @@ typedef struct pypde_recurrence {
     const double* pre;
     long long entry_stride;
     int per_line;
+    int variant;
 } pypde_recurrence;
 
 int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurrence* r, void* stream);

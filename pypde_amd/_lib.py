@@ -68,7 +68,8 @@ class Recurrence(ctypes.Structure):
 
     _fields_ = [("F", ctypes.c_void_p), ("LM", ctypes.c_void_p), ("R", ctypes.c_void_p),
                 ("MU", ctypes.c_void_p), ("NU", ctypes.c_void_p), ("pre", ctypes.c_void_p),
-                ("entry_stride", ctypes.c_longlong), ("per_line", ctypes.c_int)]
+                ("entry_stride", ctypes.c_longlong), ("per_line", ctypes.c_int),
+                ("variant", ctypes.c_int)]
 
 
 if not os.path.exists(LIB_PATH):

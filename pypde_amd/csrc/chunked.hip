@@ -348,6 +348,194 @@ static int launch_rec(const RecArgs& a, hipStream_t s)
     return launch_rec_ch<1, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
 }
 
+// ---------------------------------------------------------------------------
+// Lane-chunk form, for launches with few lines (one instance: a few hundred
+// lines, which the block form above spreads over only a handful of CUs).
+// One wave per line; lane l holds positions [l*C, l*C + C) (C = 2, 4 or 8, so
+// n <= 64*C; C even keeps each lane's first position even, so the parity of
+// a position is the parity of its index in the chunk).  The chunk transfers of
+// the two sweeps are combined across the 64 lanes by Hillis-Steele scans over
+// shuffles: the forward chain's affine maps y -> A y + B upward, the backward
+// chain's 2-D maps (Z1, Z2) -> s + M (Z1, Z2) downward.  Tables come straight
+// from global memory (shared tables are a few KB and stay in cache; per-line
+// tables are read line-major, [nlines][n], so a wave reads its line's
+// positions contiguously).  No LDS and no block barrier: waves are independent.
+template <int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+struct LaneTab {
+    const double* t[6];   // F, LM, R, MU, NU, pre, already offset to this line / entry
+    __device__ __forceinline__ double operator()(int which, int pos) const { return t[which][pos]; }
+};
+
+template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+__global__ void __launch_bounds__(64) lane_chunk_kernel(RecArgs a)
+{
+    constexpr int NX = PRE ? C + 2 : C;
+    const int lane = threadIdx.x;
+    const long long gl = blockIdx.x;               // (batch entry, line)
+    const int b = (int)(gl / a.nlines);
+    const int line = (int)(gl - (long long)b * a.nlines);
+    const int n = a.n;
+    const int base = lane * C;
+    const double* __restrict__ src = a.src + (long long)b * a.src_bs;
+    double* __restrict__ dst = a.dst + (long long)b * a.dst_bs;
+
+    LaneTab<C, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR> T;
+    {
+        const double* tabs[6] = {a.F, a.LM, a.R, a.MU, a.NU, a.pre};
+        const long long off = (long long)b * a.tbs + (LINEMAJOR ? (long long)line * n : 0);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) T.t[t] = tabs[t] ? tabs[t] + off : nullptr;
+    }
+
+    double v[NX];
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+        const int pos = base + k;
+        double x = 0.0;
+        if (pos < a.n_in)
+            x = AXIS == 0 ? src[(long long)pos * a.src_ld + line] : src[(long long)line * a.src_ld + pos];
+        v[k] = x;
+    }
+
+    if (PRE) {
+#pragma unroll
+        for (int k = 0; k < C; ++k)
+            if (base + k < n) v[k] = v[k] + T(5, base + k) * v[k + 2];
+    }
+
+    if (FWD) {
+        // chunk transfer per parity: y_out = A y_in + B
+        double A[2] = {1.0, 1.0}, B[2] = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const int p = k & 1;
+            if (base + k < n) {
+                const double f = HASF ? T(0, base + k) : 1.0;
+                const double m = T(1, base + k);
+                B[p] = f * v[k] + m * B[p];
+                A[p] = m * A[p];
+            }
+        }
+        // inclusive scan upward: lane l composes its map after lanes < l
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const double Au = __shfl_up(A[p], d, 64);
+                const double Bu = __shfl_up(B[p], d, 64);
+                if (lane >= d) {
+                    B[p] = A[p] * Bu + B[p];
+                    A[p] = A[p] * Au;
+                }
+            }
+        }
+        double y[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double yin = __shfl_up(B[p], 1, 64);
+            y[p] = lane == 0 ? 0.0 : yin;
+        }
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const int p = k & 1;
+            if (base + k < n) {
+                const double f = HASF ? T(0, base + k) : 1.0;
+                y[p] = f * v[k] + T(1, base + k) * y[p];
+                v[k] = y[p];
+            }
+        }
+    }
+
+    // backward: the chunk's lowest two same-parity values as an affine map of
+    // the next chunk's lowest two, (Z1, Z2) -> s + M (Z1, Z2)
+    double s1[2] = {0, 0}, a1[2] = {1, 1}, b1[2] = {0, 0};
+    double s2[2] = {0, 0}, a2[2] = {0, 0}, b2[2] = {1, 1};
+#pragma unroll
+    for (int k = C - 1; k >= 0; --k) {
+        const int p = k & 1;
+        if (base + k < n) {
+            const double r = HASR ? T(2, base + k) : 1.0;
+            const double mu = T(3, base + k);
+            const double nu = HASNU ? T(4, base + k) : 0.0;
+            const double sv = r * v[k] + mu * s1[p] + nu * s2[p];
+            const double al = mu * a1[p] + nu * a2[p];
+            const double be = mu * b1[p] + nu * b2[p];
+            s2[p] = s1[p]; a2[p] = a1[p]; b2[p] = b1[p];
+            s1[p] = sv; a1[p] = al; b1[p] = be;
+        }
+    }
+    // inclusive scan downward: lane l's map applied after the maps of lanes > l
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double t1 = __shfl_down(s1[p], d, 64), t2 = __shfl_down(s2[p], d, 64);
+            const double ta1 = __shfl_down(a1[p], d, 64), tb1 = __shfl_down(b1[p], d, 64);
+            const double ta2 = __shfl_down(a2[p], d, 64), tb2 = __shfl_down(b2[p], d, 64);
+            if (lane + d < 64) {
+                const double n1 = s1[p] + a1[p] * t1 + b1[p] * t2;
+                const double n2 = s2[p] + a2[p] * t1 + b2[p] * t2;
+                const double na1 = a1[p] * ta1 + b1[p] * ta2, nb1 = a1[p] * tb1 + b1[p] * tb2;
+                const double na2 = a2[p] * ta1 + b2[p] * ta2, nb2 = a2[p] * tb1 + b2[p] * tb2;
+                s1[p] = n1; s2[p] = n2;
+                a1[p] = na1; b1[p] = nb1; a2[p] = na2; b2[p] = nb2;
+            }
+        }
+    }
+    double Z1[2], Z2[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const double z1 = __shfl_down(s1[p], 1, 64), z2 = __shfl_down(s2[p], 1, 64);
+        Z1[p] = lane == 63 ? 0.0 : z1;
+        Z2[p] = lane == 63 ? 0.0 : z2;
+    }
+#pragma unroll
+    for (int k = C - 1; k >= 0; --k) {
+        const int p = k & 1;
+        if (base + k < n) {
+            const double r = HASR ? T(2, base + k) : 1.0;
+            const double z = r * v[k] + T(3, base + k) * Z1[p] + (HASNU ? T(4, base + k) * Z2[p] : 0.0);
+            v[k] = z;
+            Z2[p] = Z1[p];
+            Z1[p] = z;
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const int pos = base + k;
+        if (pos < n) {
+            if (AXIS == 0) dst[(long long)pos * a.dst_ld + line] = v[k];
+            else dst[(long long)line * a.dst_ld + pos] = v[k];
+        }
+    }
+}
+
+template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+static int launch_lane_t(const RecArgs& a, hipStream_t s)
+{
+    const long long blocks = (long long)a.nlines * a.batch;
+    if (blocks == 0) return OK;
+    PYPDE_REQUIRE(blocks <= 0x7fffffffLL, "recurrence: %lld lines exceed the grid", blocks);
+    hipLaunchKernelGGL((lane_chunk_kernel<AXIS, C, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>),
+                       dim3((unsigned)blocks), dim3(64), 0, s, a);
+    return check_launch("pypde_recurrence_solve");
+}
+
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+static int launch_lane(const RecArgs& a, hipStream_t s)
+{
+    PYPDE_REQUIRE(a.n >= 1 && a.n <= 64 * 8, "recurrence: line length %d exceeds 512", a.n);
+    if (a.axis == 0) {
+        if (a.n <= 128) return launch_lane_t<0, 2, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+        if (a.n <= 256) return launch_lane_t<0, 4, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+        return launch_lane_t<0, 8, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+    }
+    if (a.n <= 128) return launch_lane_t<1, 2, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+    if (a.n <= 256) return launch_lane_t<1, 4, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+    return launch_lane_t<1, 8, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
+}
+
 }  // namespace pypde
 
 using namespace pypde;
@@ -363,12 +551,34 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
     PYPDE_REQUIRE(g->len_in == g->len_out + (pre ? 2 : 0),
                   "recurrence: len_in must be len_out%s", pre ? " + 2" : "");
     PYPDE_REQUIRE(!(g->src == g->dst && pre), "pre-stencil cannot run in place");
+    PYPDE_REQUIRE(r->per_line >= 0 && r->per_line <= 2, "per_line must be 0, 1 or 2");
+    PYPDE_REQUIRE(r->variant >= 0 && r->variant <= 2, "variant must be 0, 1 or 2");
     RecArgs a{g->src, g->dst, g->src_ld, g->dst_ld, g->src_bs, g->dst_bs, g->len_in, g->len_out,
               g->nlines, g->batch, g->axis, r->F, r->LM, r->R, r->MU, r->NU, r->pre,
               r->entry_stride, r->per_line};
     hipStream_t s = as_stream(stream);
     const bool fwd = r->LM != nullptr, hf = r->F != nullptr, hr = r->R != nullptr,
                hn = r->NU != nullptr;
+    // lane-chunk form: asked for, required by line-major tables, or (variant 0)
+    // chosen when the launch has few lines
+    const long long lines = (long long)g->nlines * g->batch;
+    const bool lane = r->variant == 2 || r->per_line == 2 ||
+                      (r->variant == 0 && r->per_line == 0 && lines <= 2048);
+    if (lane) {
+        PYPDE_REQUIRE(r->per_line != 1, "the lane-chunk form reads per-line tables line-major (per_line = 2)");
+        if (r->per_line == 2) {
+            PYPDE_REQUIRE(!pre && fwd && !hf && hr && hn,
+                          "per-line tables support the (LM, R, MU, NU) form only");
+            return launch_lane<false, true, false, true, true, true>(a, s);
+        }
+        if (!pre && fwd && !hf && hr && hn) return launch_lane<false, true, false, true, true, false>(a, s);
+        if (pre && fwd && hf && !hr && !hn) return launch_lane<true, true, true, false, false, false>(a, s);
+        if (!pre && !fwd && !hf && hr && !hn) return launch_lane<false, false, false, true, false, false>(a, s);
+        if (!pre && fwd && hf && hr && hn) return launch_lane<false, true, true, true, true, false>(a, s);
+        set_error("pypde_recurrence_solve: unsupported table combination (pre=%d F=%d LM=%d R=%d NU=%d)",
+                  (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
+        return ERR_UNSUP;
+    }
     // the combinations used by the plans
     if (r->per_line) {
         PYPDE_REQUIRE(!pre && fwd && !hf && hr && hn,

@@ -187,6 +187,31 @@ class _Call:
         check(self.fn(*self.args, stream), self.name)
 
 
+class _RecCall(_Call):
+    """A recurrence launch with its block-form (1) and lane-chunk (2) structs;
+    the engine's tuner picks one."""
+
+    def __init__(self, name, g, structs, variant):
+        self.g, self.structs = g, structs
+        _Call.__init__(self, name, lib.pypde_recurrence_solve, ctypes.byref(g),
+                       ctypes.byref(structs[variant]))
+        self.variant = variant
+
+    def set_variant(self, v):
+        self.variant = v
+        self.args = (ctypes.byref(self.g), ctypes.byref(self.structs[v]))
+
+    def key(self):
+        g, r = self.g, self.structs[1]
+        return ("rec", g.axis, g.len_in, g.len_out, g.nlines, g.batch, bool(r.per_line),
+                r.F is not None, r.pre is not None)
+
+
+def _rec_default(lines):
+    """Lane-chunk form for few lines, block form otherwise."""
+    return 2 if lines <= 2048 else 1
+
+
 def _lines(src, dst, ld, bs, n, nlines, batch, axis):
     g = Lines()
     g.src, g.dst = src, dst
@@ -198,12 +223,12 @@ def _lines(src, dst, ld, bs, n, nlines, batch, axis):
     return g
 
 
-def _rec(base, M, entry_stride):
+def _rec(base, M, entry_stride, variant=0):
     """Recurrence tables stacked [entry][LM | R | MU | NU][M] at `base`."""
     r = Recurrence()
     r.LM, r.R, r.MU, r.NU = base, base + 8 * M, base + 16 * M, base + 24 * M
     r.F = r.pre = None
-    r.entry_stride, r.per_line = entry_stride, 0
+    r.entry_stride, r.per_line, r.variant = entry_stride, 0, variant
     return r
 
 
@@ -338,6 +363,7 @@ class RBCBatch:
         self.ptab = T(tab)
         self.psing = torch.as_tensor(sing, device=dev)
         self.prec = T(np.stack(rec_from_poisson(tab, sing)))  # [LM | R | MU | NU][M0][M1]
+        self.precT = self.prec.transpose(1, 2).contiguous()   # line-major [4][M1][M0]
         self.Wy = T(np.vstack([ay.Sn, ay.TSn, ay.TDSn]))                      # [N1+2M1, M1]
         self.Snx = T(ax.Sn)                                                   # [N0, M0]
         self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
@@ -513,13 +539,10 @@ class RBCBatch:
         # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
-        rx = _rec(p(self.st_rhx[st]), M0, 4 * M0)
-        ry = _rec(p(self.st_rhy[st]), M1, 4 * M1)
-        self._keep += [gx, gy, rx, ry]
-        L.append(_Call("helmholtz_x" + tag, lib.pypde_recurrence_solve, ctypes.byref(gx),
-                       ctypes.byref(rx)))
-        L.append(_Call("helmholtz_y" + tag, lib.pypde_recurrence_solve, ctypes.byref(gy),
-                       ctypes.byref(ry)))
+        rx = {v: _rec(p(self.st_rhx[st]), M0, 4 * M0, v) for v in (1, 2)}
+        ry = {v: _rec(p(self.st_rhy[st]), M1, 4 * M1, v) for v in (1, 2)}
+        L.append(_RecCall("helmholtz_x" + tag, gx, rx, _rec_default(3 * B * M1)))
+        L.append(_RecCall("helmholtz_y" + tag, gy, ry, _rec_default(3 * B * M0)))
         # 11  Wd = [U* C_y^T ; V* Cd_y^T]
         L.append(_Gemm("div_y" + tag, m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
                        iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.Cy), ldb=M1, tb=1,
@@ -537,12 +560,13 @@ class RBCBatch:
                        iC=_bidx(s1=sX)))
         # 15  per-line eigen-split solves along x
         gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
-        rp = Recurrence()
-        rp.LM, rp.R, rp.MU, rp.NU = (self.prec[i].data_ptr() for i in range(4))
-        rp.per_line, rp.entry_stride = 1, 0
-        self._keep += [gp, rp]
-        L.append(_Call("poisson_lines" + tag, lib.pypde_recurrence_solve, ctypes.byref(gp),
-                       ctypes.byref(rp)))
+        rp = {}
+        for v, tabs, layout in ((1, self.prec, 1), (2, self.precT, 2)):
+            r = Recurrence()
+            r.LM, r.R, r.MU, r.NU = (tabs[i].data_ptr() for i in range(4))
+            r.per_line, r.entry_stride, r.variant = layout, 0, v
+            rp[v] = r
+        L.append(_RecCall("poisson_lines" + tag, gp, rp, _rec_default(B * M1)))
         # 16  P = RP Q_y^T
         # 17  ... with the gauge P[0,0] = 0 applied in the GEMM epilogue
         L.append(_Gemm("poisson_qy" + tag, m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
@@ -683,6 +707,14 @@ class RBCBatch:
 
         def launch_fn(name):
             return lambda d: check(lib.pypde_dgemm_batched(ctypes.byref(d), s), name)
+        for launch in self.launches:
+            if isinstance(launch, _RecCall):
+                def run(v, launch=launch):
+                    check(lib.pypde_recurrence_solve(ctypes.byref(launch.g),
+                                                     ctypes.byref(launch.structs[v]), s),
+                          launch.name)
+                launch.set_variant(autotune.choose(launch.key(), run, (1, 2),
+                                                   default=launch.variant))
         for i, launch in enumerate(self.launches):
             if not isinstance(launch, _Gemm):
                 continue
@@ -703,8 +735,11 @@ class RBCBatch:
         torch.cuda.synchronize()
 
     def variants(self):
-        """{launch name: GEMM kernel variant} chosen for this step."""
-        return {l.name: l.desc.variant for l in self.launches if isinstance(l, _Gemm)}
+        """{launch name: kernel variant} chosen for this step (GEMM variants;
+        recurrences: 1 block form, 2 lane-chunk form)."""
+        out = {l.name: l.desc.variant for l in self.launches if isinstance(l, _Gemm)}
+        out.update({l.name: l.variant for l in self.launches if isinstance(l, _RecCall)})
+        return out
 
     def capture(self, nsteps=1):
         """Capture `nsteps` consecutive steps into one HIP graph (the state is

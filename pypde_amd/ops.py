@@ -422,13 +422,15 @@ def axpby(alpha, x, beta, y):
 # register-resident recurrences (normalised tables, see include/pypde_amd.h)
 # ---------------------------------------------------------------------------
 class RecTables:
-    """Device copy of a normalised recurrence (F, LM, R, MU, NU, pre tables)."""
+    """Device copy of a normalised recurrence (F, LM, R, MU, NU, pre tables).
+    Per-line tables are kept in both layouts: [LM | R | MU | NU][n][nlines]
+    for the block form and line-major [4][nlines][n] for the lane-chunk form."""
 
     def __init__(self, F=None, LM=None, R=None, MU=None, NU=None, pre=None, per_line=False,
                  entry_stride=0):
         if per_line:
-            # per-line tables live in one [LM | R | MU | NU][n][nlines] block
             self.block = to_dev(np.stack([LM, R, MU, NU]))
+            self.block_lm = self.block.transpose(1, 2).contiguous()
             self.t = dict(F=None, LM=self.block[0], R=self.block[1], MU=self.block[2],
                           NU=self.block[3], pre=None)
         else:
@@ -438,19 +440,37 @@ class RecTables:
         self.entry_stride = int(entry_stride)
         self.has_pre = pre is not None
 
-    def struct(self):
+    def form(self):
+        return tuple(self.t[k] is not None for k in ("F", "LM", "R", "NU", "pre"))
+
+    def struct(self, variant=0, lines=0):
+        """The C struct for kernel variant 1 (block) or 2 (lane-chunk); 0 lets
+        the library choose, except for per-line tables, whose layout follows
+        the variant (lane-chunk up to 2048 lines)."""
         from ._lib import Recurrence
         r = Recurrence()
-        for k, v in self.t.items():
+        tabs = self.t
+        if self.per_line:
+            if variant == 0:
+                variant = 2 if lines <= 2048 else 1
+            if variant == 2:
+                tabs = dict(F=None, LM=self.block_lm[0], R=self.block_lm[1],
+                            MU=self.block_lm[2], NU=self.block_lm[3], pre=None)
+        for k, v in tabs.items():
             setattr(r, k, v.data_ptr() if v is not None else None)
         r.entry_stride = self.entry_stride
-        r.per_line = int(self.per_line)
+        r.per_line = (2 if variant == 2 else 1) if self.per_line else 0
+        r.variant = variant
         return r
 
 
-def recurrence_solve(tabs, x, axis=0, out=None, lpb=0):
+REC_CANDIDATES = (1, 2)
+
+
+def recurrence_solve(tabs, x, axis=0, out=None, lpb=0, variant=None):
     """Banded line solve in normalised form along axis (in place when out is x
-    and there is no pre-stencil)."""
+    and there is no pre-stencil).  variant None: the kernel form is autotuned
+    per shape (timed out of place on scratch copies)."""
     _check_axis(x, axis)
     n_in = _len(x, axis)
     n = n_in - 2 if tabs.has_pre else n_in
@@ -458,7 +478,23 @@ def recurrence_solve(tabs, x, axis=0, out=None, lpb=0):
         out = x if not tabs.has_pre else torch.empty(_out_shape(x, axis, n), dtype=DTYPE,
                                                      device=x.device)
     g = _geom(_as3(x), _as3(out), axis, n_in, n, lpb)
-    r = tabs.struct()
-    check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(r), stream_ptr()),
+    lines = g.nlines * g.batch
+    s = stream_ptr()
+    if variant is None:
+        from . import autotune
+        key = ("rec", axis, n_in, n, g.nlines, g.batch, tabs.form(), tabs.per_line)
+        if autotune.enabled() and not autotune._capturing() and key not in autotune.cache():
+            xs = _as3(x).clone()
+            os_ = torch.empty(_out_shape(_as3(x), axis, n), dtype=DTYPE, device=x.device)
+            gs = _geom(xs, os_, axis, n_in, n, lpb)
+
+            def run(v):
+                rs = tabs.struct(v, lines)
+                check(lib.pypde_recurrence_solve(ctypes.byref(gs), ctypes.byref(rs), s),
+                      "pypde_recurrence_solve")
+            autotune.choose(key, run, REC_CANDIDATES, default=0)
+        variant = autotune.cache().get(key, 0)
+    r = tabs.struct(variant, lines)
+    check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(r), s),
           "pypde_recurrence_solve")
     return out
