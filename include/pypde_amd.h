@@ -113,6 +113,14 @@ typedef struct pypde_gemm_desc {
  * alpha_vec).  Same reference interfaces as pypde_dgemm. */
 int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream);
 
+/* Grouped launch: `count` (1..4) independent plain descriptors with the same
+ * transposes and the same kernel variant in one kernel launch (the tile grids
+ * are concatenated); every member computes exactly what pypde_dgemm_batched
+ * computes for it.  For steps whose GEMMs are too small to fill the device
+ * alone.  Folded descriptors are rejected.  Same reference interfaces as
+ * pypde_dgemm. */
+int pypde_dgemm_grouped(const pypde_gemm_desc* descs, int count, void* stream);
+
 /* ---- line operators (one lane per line, LDS-staged panels) ------------ */
 /* Galerkin -> Chebyshev stencil u = S v, S(i,i)=1, S(i+2,i)=s[i]; len_out = len_in+2.
  * Replaces GalerkinChebyshev.to_chebyshev (pypde/bases/chebyshev.py:287-293). */

@@ -128,8 +128,33 @@ struct Epilogue {
     double beta_e_gamma;
 };
 
+// Several independent GEMMs of one kernel form in one launch (grouped
+// launch): the grid is the concatenation of the members' tile grids, block t
+// runs tile t - start[i] of member i.  Members share the kernel template, so
+// each computes exactly what its own launch would.
+constexpr int kMaxGroup = 4;
+struct GemmGroup {
+    GemmArgs a[kMaxGroup];
+    int tx[kMaxGroup], ty[kMaxGroup];
+    int start[kMaxGroup + 1];
+    int count;
+};
+
+__device__ __forceinline__ int group_tile(const GemmGroup& gg, int& bx, int& by, int& bz)
+{
+    const int t = blockIdx.x;
+    int i = 0;
+    while (i + 1 < gg.count && t >= gg.start[i + 1]) ++i;
+    const int local = t - gg.start[i];
+    bx = local % gg.tx[i];
+    const int r = local / gg.tx[i];
+    by = r % gg.ty[i];
+    bz = r / gg.ty[i];
+    return i;
+}
+
 template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
+__device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int BK = 16;
     constexpr int NT = 64 * WGM * WGN;  // threads
@@ -152,9 +177,8 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave / WGN, wn = wave % WGN;
-    const int bz = blockIdx.z;
-    const int m0 = blockIdx.y * BM;
-    const int n0 = blockIdx.x * BN;
+    const int m0 = by * BM;
+    const int n0 = bx * BN;
 
     const double* __restrict__ A = g.A + bofs(bz, g.iA);
     const double* __restrict__ B = g.B + bofs(bz, g.iB);
@@ -268,6 +292,20 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
             }
 }
 
+template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
+{
+    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_group(GemmGroup gg)
+{
+    int bx, by, bz;
+    const int i = group_tile(gg, bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(gg.a[i], bx, by, bz);
+}
+
 template <int BM, int BN, int WGM, int WGN, bool DBUF>
 static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
@@ -279,10 +317,34 @@ static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStrea
     else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, true, true>), grid, block, 0, s, g);
 }
 
+static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
+{
+    gg.start[0] = 0;
+    for (int i = 0; i < gg.count; ++i) {
+        gg.tx[i] = (gg.a[i].n + bn - 1) / bn;
+        gg.ty[i] = (gg.a[i].m + bm - 1) / bm;
+        const long long t = (gg.a[i].m > 0 && gg.a[i].n > 0) ? (long long)gg.tx[i] * gg.ty[i] * batch[i] : 0;
+        gg.start[i + 1] = gg.start[i] + (int)t;
+    }
+}
+
+template <int BM, int BN, int WGM, int WGN, bool DBUF>
+static void launch_gemm_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
+{
+    group_grid(gg, batch, BM, BN);
+    if (gg.start[gg.count] == 0) return;
+    dim3 grid(gg.start[gg.count]);
+    dim3 block(64 * WGM * WGN);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, false, false>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, false, true>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, true, false>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, true, true>), grid, block, 0, s, gg);
+}
+
 // ---------------------------------------------------------------------------
 // Split-K register-direct kernel.
 template <int TM, int TN, int KW, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
+__device__ __forceinline__ void dgemm_splitk_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int U = 4;  // MFMA steps per register block
     __shared__ double red[KW * TM * TN * 4 * 64];
@@ -292,9 +354,8 @@ __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
     const int wave = tid >> 6;
     const int r16 = lane & 15;
     const int q = lane >> 4;
-    const int bz = blockIdx.z;
-    const int m0 = blockIdx.y * (16 * TM);
-    const int n0 = blockIdx.x * (16 * TN);
+    const int m0 = by * (16 * TM);
+    const int n0 = bx * (16 * TN);
 
     const double* __restrict__ A = g.A + bofs(bz, g.iA);
     const double* __restrict__ B = g.B + bofs(bz, g.iB);
@@ -393,6 +454,20 @@ __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
     }
 }
 
+template <int TM, int TN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
+{
+    dgemm_splitk_body<TM, TN, KW, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int TM, int TN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_splitk_group(GemmGroup gg)
+{
+    int bx, by, bz;
+    const int i = group_tile(gg, bx, by, bz);
+    dgemm_splitk_body<TM, TN, KW, TA, TB>(gg.a[i], bx, by, bz);
+}
+
 template <int TM, int TN, int KW>
 static void launch_splitk(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
@@ -404,6 +479,19 @@ static void launch_splitk(bool ta, bool tb, const GemmArgs& g, int batch, hipStr
     else hipLaunchKernelGGL((dgemm_splitk_kernel<TM, TN, KW, true, true>), grid, block, 0, s, g);
 }
 
+template <int TM, int TN, int KW>
+static void launch_splitk_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
+{
+    group_grid(gg, batch, 16 * TM, 16 * TN);
+    if (gg.start[gg.count] == 0) return;
+    dim3 grid(gg.start[gg.count]);
+    dim3 block(64 * KW);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_splitk_group<TM, TN, KW, false, false>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_splitk_group<TM, TN, KW, false, true>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_splitk_group<TM, TN, KW, true, false>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_splitk_group<TM, TN, KW, true, true>), grid, block, 0, s, gg);
+}
+
 // ---------------------------------------------------------------------------
 // Split-K, LDS-staged kernel (latency): KW waves share one BM x BN output tile;
 // wave w owns a contiguous K range and stages its own 16-deep slabs of A and B
@@ -411,7 +499,7 @@ static void launch_splitk(bool ta, bool tb, const GemmArgs& g, int batch, hipStr
 // contiguous dimension of each operand), so the K loop has no block barriers.
 // The KW partial tiles are summed through LDS at the end.
 template <int BM, int BN, int KW, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
+__device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int BK = 16;
     constexpr int PA = BM + 1, PB = BN + 1;
@@ -426,9 +514,8 @@ __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int bz = blockIdx.z;
-    const int m0 = blockIdx.y * BM;
-    const int n0 = blockIdx.x * BN;
+    const int m0 = by * BM;
+    const int n0 = bx * BN;
     const double* __restrict__ A = g.A + bofs(bz, g.iA);
     const double* __restrict__ B = g.B + bofs(bz, g.iB);
     double* As = lds + wave * SLAB;
@@ -531,6 +618,20 @@ __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
     }
 }
 
+template <int BM, int BN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
+{
+    dgemm_wsplit_body<BM, BN, KW, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int BM, int BN, int KW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * KW) dgemm_wsplit_group(GemmGroup gg)
+{
+    int bx, by, bz;
+    const int i = group_tile(gg, bx, by, bz);
+    dgemm_wsplit_body<BM, BN, KW, TA, TB>(gg.a[i], bx, by, bz);
+}
+
 template <int BM, int BN, int KW>
 static void launch_wsplit(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
@@ -540,6 +641,19 @@ static void launch_wsplit(bool ta, bool tb, const GemmArgs& g, int batch, hipStr
     else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, true>), grid, block, 0, s, g);
     else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, false>), grid, block, 0, s, g);
     else hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, true>), grid, block, 0, s, g);
+}
+
+template <int BM, int BN, int KW>
+static void launch_wsplit_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
+{
+    group_grid(gg, batch, BM, BN);
+    if (gg.start[gg.count] == 0) return;
+    dim3 grid(gg.start[gg.count]);
+    dim3 block(64 * KW);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, false>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, true>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, false>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, true>), grid, block, 0, s, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -882,6 +996,66 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
     return check_launch("pypde_dgemm_batched (folded)");
 }
 
+static GemmArgs args_of(const pypde_gemm_desc* d)
+{
+    GemmArgs g{d->A, d->B, d->D, d->C, d->lda, d->ldb, d->ldd, d->ldc, bidx(d->iA), bidx(d->iB),
+               bidx(d->iD), bidx(d->iC), d->m, d->n, d->k, d->alpha, d->beta, d->alpha_vec,
+               d->alpha_div < 1 ? 1 : d->alpha_div, d->E, d->lde, bidx(d->iE), d->gamma_vec,
+               d->gamma, d->gamma_div < 1 ? 1 : d->gamma_div, d->zero_first,
+               d->csn < 1 ? 1 : d->csn, d->ska < 1 ? 1 : d->ska};
+    return g;
+}
+
+int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
+{
+    PYPDE_REQUIRE(d, "null GEMM descriptor array");
+    PYPDE_REQUIRE(count >= 1 && count <= kMaxGroup, "grouped GEMM: count %d not in 1..%d", count, kMaxGroup);
+    GemmGroup gg;
+    int batch[kMaxGroup];
+    gg.count = count;
+    const int ta = d[0].transA, tb = d[0].transB;
+    int variant = d[0].variant;
+    long long tiles = 0;
+    for (int i = 0; i < count; ++i) {
+        const pypde_gemm_desc& e = d[i];
+        PYPDE_REQUIRE(e.fold == 0, "grouped GEMM: member %d is folded (not supported)", i);
+        PYPDE_REQUIRE(e.transA == ta && e.transB == tb, "grouped GEMM: member %d has other transposes", i);
+        PYPDE_REQUIRE(e.variant == variant, "grouped GEMM: member %d has another variant", i);
+        PYPDE_REQUIRE(e.m >= 0 && e.n >= 0 && e.k >= 0 && e.batch >= 0, "negative GEMM size (member %d)", i);
+        PYPDE_REQUIRE(!e.alpha_vec || e.alpha_div >= 1, "alpha_div must be >= 1");
+        PYPDE_REQUIRE(!e.gamma_vec || e.gamma_div >= 1, "gamma_div must be >= 1");
+        if (e.m > 0 && e.n > 0 && e.batch > 0)
+            PYPDE_REQUIRE(e.A && e.B && e.C, "null GEMM operand (member %d)", i);
+        gg.a[i] = args_of(&e);
+        batch[i] = e.batch;
+        tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
+    }
+    PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
+    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 8 && variant <= 11) ||
+                      (variant >= 16 && variant < 24),
+                  "unknown GEMM variant %d", variant);
+    if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
+    switch (variant) {
+    case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
+    case 2: launch_gemm_group<32, 32, 2, 2, false>(ta, tb, gg, batch, stream); break;
+    case 3: launch_gemm_group<128, 128, 4, 2, false>(ta, tb, gg, batch, stream); break;
+    case 4: launch_gemm_group<128, 128, 4, 2, true>(ta, tb, gg, batch, stream); break;
+    case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
+    case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
+    case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
+    case 11: launch_wsplit_group<32, 64, 4>(ta, tb, gg, batch, stream); break;
+    case 16: launch_splitk_group<1, 1, 4>(ta, tb, gg, batch, stream); break;
+    case 17: launch_splitk_group<2, 1, 4>(ta, tb, gg, batch, stream); break;
+    case 18: launch_splitk_group<1, 2, 4>(ta, tb, gg, batch, stream); break;
+    case 19: launch_splitk_group<2, 2, 4>(ta, tb, gg, batch, stream); break;
+    case 20: launch_splitk_group<1, 1, 8>(ta, tb, gg, batch, stream); break;
+    case 21: launch_splitk_group<2, 1, 8>(ta, tb, gg, batch, stream); break;
+    case 22: launch_splitk_group<1, 2, 8>(ta, tb, gg, batch, stream); break;
+    default: launch_splitk_group<2, 2, 8>(ta, tb, gg, batch, stream); break;
+    }
+    return check_launch("pypde_dgemm_grouped");
+}
+
 }  // namespace pypde
 
 extern "C" int pypde_dgemm(int transA, int transB, int m, int n, int k, double alpha,
@@ -897,4 +1071,9 @@ extern "C" int pypde_dgemm(int transA, int transB, int m, int n, int k, double a
 extern "C" int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream)
 {
     return pypde::dgemm_batched(d, pypde::as_stream(stream));
+}
+
+extern "C" int pypde_dgemm_grouped(const pypde_gemm_desc* descs, int count, void* stream)
+{
+    return pypde::dgemm_grouped(descs, count, pypde::as_stream(stream));
 }

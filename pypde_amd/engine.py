@@ -179,6 +179,62 @@ class _Gemm:
         check(lib.pypde_dgemm_batched(ctypes.byref(self.desc), stream), self.name)
 
 
+class _GemmGroup:
+    """Independent plain GEMMs of the step in one launch (pypde_dgemm_grouped):
+    their tile grids are concatenated, so two latency-bound products share the
+    device instead of running one after the other.  The tuner picks the
+    group's kernel variant, or runs the members separately (split) when that
+    is faster."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.name = "+".join(m.name for m in self.members)
+        self.arr = (GemmDesc * len(self.members))()
+        self.split = False
+        self.flops = sum(m.flops for m in self.members)
+        self.set_variant(self.members[0].desc.variant)
+
+    @property
+    def variant(self):
+        return self.members[0].desc.variant
+
+    def set_variant(self, v):
+        for i, m in enumerate(self.members):
+            m.desc.variant = v
+            self.arr[i] = m.desc
+
+    def __call__(self, stream):
+        if self.split:
+            for m in self.members:
+                m(stream)
+            return
+        check(lib.pypde_dgemm_grouped(ctypes.byref(self.arr), len(self.members), stream),
+              self.name)
+
+
+# independent launches of one stage that may share a launch
+_GROUPS = (("z_pres", "z_old"), ("ypass_v", "ypass_x"), ("pres_p", "correct"))
+
+
+def _group_launches(L, tag):
+    by_name = {l.name: i for i, l in enumerate(L)}
+    for names in _GROUPS:
+        idx = [by_name.get(n + tag) for n in names]
+        if any(i is None for i in idx):
+            continue
+        ms = [L[i] for i in idx]
+        if not all(isinstance(m, _Gemm) and m.alt is None and m.desc.fold == 0 for m in ms):
+            continue
+        if len({(m.desc.transA, m.desc.transB) for m in ms}) != 1:
+            continue
+        if sorted(idx) != list(range(min(idx), min(idx) + len(idx))):
+            continue  # only neighbours in the launch order
+        L[min(idx)] = _GemmGroup(ms)
+        for i in sorted(idx)[1:][::-1]:
+            del L[i]
+        by_name = {l.name: i for i, l in enumerate(L)}
+
+
 class _Call:
     def __init__(self, name, fn, *args):
         self.name, self.fn, self.args = name, fn, args
@@ -242,10 +298,13 @@ class RBCBatch:
     integrator: "eu" or "rk3" (the stage coefficients a, b, c of
           rbc2d_base.py:86-102); beta: implicit fraction of the diffusion, the
           rest treated explicitly (rbc2d.py:269-283).
+    group_launches: independent GEMMs of a stage share one launch
+          (pypde_dgemm_grouped) where the tuner finds it faster.
     """
 
     def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0,
-                 integrator="eu"):
+                 integrator="eu", group_launches=True):
+        self.group_launches = bool(group_launches)
         if case not in ("rbc", "linear", "zero"):
             raise ValueError("Specified case is not available: ", ["rbc", "linear", "zero"])
         if integrator not in ("eu", "rk3"):
@@ -480,7 +539,11 @@ class RBCBatch:
         self._keep = []
         L = []
         for st in range(self.nstage):
-            self._stage_launches(L, st)
+            S = []
+            self._stage_launches(S, st)
+            if self.group_launches:
+                _group_launches(S, "" if self.nstage == 1 else "[%d]" % st)
+            L += S
         self.launches = L
 
     def _stage_launches(self, L, st):
@@ -715,6 +778,29 @@ class RBCBatch:
                           launch.name)
                 launch.set_variant(autotune.choose(launch.key(), run, (1, 2),
                                                    default=launch.variant))
+        for launch in self.launches:
+            if not isinstance(launch, _GemmGroup):
+                continue
+            for m in launch.members:
+                autotune.tune_gemm_desc(m.desc, launch_fn(m.name), allow_alias=True)
+            own = [m.desc.variant for m in launch.members]
+
+            def run_group(v, g=launch):
+                g.set_variant(v)
+                check(lib.pypde_dgemm_grouped(ctypes.byref(g.arr), len(g.members), s), g.name)
+            key = ("group",) + tuple(autotune.gemm_key(m.desc) for m in launch.members)
+            best = autotune.choose(key, run_group, autotune.GEMM_CANDIDATES,
+                                   default=own[0])
+            t_group = autotune.timings().get(key, {}).get(best)
+            t_split = sum(autotune.timings().get(autotune.gemm_key(m.desc), {}).get(v, 0.0)
+                          for m, v in zip(launch.members, own)) if t_group is not None else None
+            if t_group is not None and t_split and t_split < t_group:
+                for m, v in zip(launch.members, own):
+                    m.desc.variant = v
+                launch.split = True
+            else:
+                launch.set_variant(best)
+                launch.split = False
         for i, launch in enumerate(self.launches):
             if not isinstance(launch, _Gemm):
                 continue
@@ -738,6 +824,8 @@ class RBCBatch:
         """{launch name: kernel variant} chosen for this step (GEMM variants;
         recurrences: 1 block form, 2 lane-chunk form)."""
         out = {l.name: l.desc.variant for l in self.launches if isinstance(l, _Gemm)}
+        out.update({l.name: ("split" if l.split else l.variant) for l in self.launches
+                    if isinstance(l, _GemmGroup)})
         out.update({l.name: l.variant for l in self.launches if isinstance(l, _RecCall)})
         return out
 

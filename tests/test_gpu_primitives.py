@@ -362,3 +362,75 @@ def test_mirror_operator_transforms(amd, N, axis):
         y = host(op.apply(ops.to_dev(x), axis))
         ref = np.einsum("ik,bkn->bin", M, x) if axis == 0 else np.einsum("bnk,ik->bni", x, M)
         assert rel(y, ref) < 1e-13, (M.shape, axis)
+
+
+PLAIN_VARIANTS = (1, 2, 3, 4, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_grouped_matches_members(amd, ta, tb):
+    """pypde_dgemm_grouped: every member equals its own batched launch bit for
+    bit, for each plain variant; members differ in shape, batch and epilogue;
+    one member is empty."""
+    import ctypes
+    pypde_amd, ops, torch = amd
+    from pypde_amd._lib import GemmDesc, lib
+    rng = np.random.default_rng(40 + 2 * ta + tb)
+    shapes = [(37, 53, 29, 3), (130, 70, 91, 1), (0, 16, 8, 2), (64, 200, 254, 2)]
+    ops_, outs_single, outs_group, descs = [], [], [], []
+    for (m, n, k, b) in shapes:
+        A = ops.to_dev(rng.standard_normal((b, k, m) if ta else (b, m, k)))
+        B = ops.to_dev(rng.standard_normal((b, n, k) if tb else (b, k, n)))
+        D = ops.to_dev(rng.standard_normal((b, m, n)))
+        av = ops.to_dev(rng.uniform(0.5, 1.5, b))
+        ops_.append((A, B, D, av))
+        outs_single.append(torch.zeros((b, m, n), dtype=torch.float64, device="cuda"))
+        outs_group.append(torch.zeros((b, m, n), dtype=torch.float64, device="cuda"))
+    s = ops.stream_ptr()
+    for v in PLAIN_VARIANTS:
+        arr = (GemmDesc * len(shapes))()
+        for i, ((m, n, k, b), (A, B, D, av)) in enumerate(zip(shapes, ops_)):
+            def mk(C):
+                d = ops._desc(ta, tb, m, n, k, b, 0.75, -1.5, A.data_ptr(), m if ta else k,
+                              A[0].numel(), B.data_ptr(), k if tb else n, B[0].numel(),
+                              D.data_ptr(), n, m * n, C.data_ptr(), n, m * n,
+                              alpha_vec=av.data_ptr())
+                d.zero_first = 1 if i == 1 else 0
+                d.variant = v
+                return d
+            d1 = mk(outs_single[i])
+            ops.check(lib.pypde_dgemm_batched(ctypes.byref(d1), s), "single")
+            arr[i] = mk(outs_group[i])
+        ops.check(lib.pypde_dgemm_grouped(ctypes.byref(arr), len(shapes), s), "grouped")
+        torch.cuda.synchronize()
+        for i in range(len(shapes)):
+            assert torch.equal(outs_single[i], outs_group[i]), (v, i)
+            outs_group[i].zero_()
+    # the results themselves
+    (m, n, k, b), (A, B, D, av) = shapes[0], ops_[0]
+    An, Bn, Dn, avn = (host(t) for t in (A, B, D, av))
+    for e in range(b):
+        ref = 0.75 * avn[e] * ((An[e].T if ta else An[e]) @ (Bn[e].T if tb else Bn[e])) - 1.5 * Dn[e]
+        assert rel(host(outs_single[0][e]), ref) < 1e-14
+
+
+def test_gemm_grouped_rejects_mixed_members(amd):
+    import ctypes
+    pypde_amd, ops, torch = amd
+    from pypde_amd._lib import GemmDesc, PypdeError, lib
+    x = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
+    p = x.data_ptr()
+
+    def mk(ta=0, tb=0, v=2, fold=0):
+        d = ops._desc(ta, tb, 8, 8, 8, 1, 1.0, 0.0, p, 8, 0, p, 8, 0, None, 0, 0, p, 8, 0)
+        d.variant, d.fold = v, fold
+        return d
+    s = ops.stream_ptr()
+    for pair in ((mk(), mk(tb=1)), (mk(), mk(v=9)), (mk(), mk(fold=1))):
+        arr = (GemmDesc * 2)(*pair)
+        with pytest.raises(PypdeError):
+            ops.check(lib.pypde_dgemm_grouped(ctypes.byref(arr), 2, s), "grouped")
+    arr = (GemmDesc * 5)(*[mk() for _ in range(5)])
+    for count in (0, 5):
+        with pytest.raises(PypdeError):
+            ops.check(lib.pypde_dgemm_grouped(ctypes.byref(arr), count, s), "grouped")

@@ -64,6 +64,28 @@ def test_graph_replay_matches_eager(golden):
         assert np.array_equal(host(getattr(a, f).vhat), host(getattr(b, f).vhat)), f
 
 
+def test_grouped_launches_match_separate(golden):
+    """The step with independent GEMMs sharing launches equals the step with
+    one launch per GEMM (kernel variants may differ: 1e-13)."""
+    import torch
+    from pypde_amd.engine import RBCBatch
+    pre = "step_rbc_eu_in_"
+    engs = [RBCBatch((20, 16), [dict(ra=1e5, pr=1.0, dt=0.01)], case="rbc", group_launches=g)
+            for g in (True, False)]
+    assert any("+" in l.name for l in engs[0].launches)
+    assert not any("+" in l.name for l in engs[1].launches)
+    for e in engs:
+        e.X[0, 0].copy_(torch.as_tensor(golden[pre + "U"]))
+        e.X[0, 1].copy_(torch.as_tensor(golden[pre + "V"]))
+        e.X[0, 2].copy_(torch.as_tensor(golden[pre + "T"]))
+        e.pres[0].copy_(torch.as_tensor(golden[pre + "pres"]))
+        for _ in range(5):
+            e.step_eager()
+    for t in ("X", "pres"):
+        a, b = (host(getattr(e, t)) for e in engs)
+        assert rel(a, b) < 1e-13, t
+
+
 @pytest.mark.parametrize("case", ["linear", "rbc"])
 def test_c1_trajectory_1000_steps(golden, case):
     """Configuration c1: 64x64, Ra=1e5, Pr=1, dt=0.01, Euler, seeded noise 1e-3:
