@@ -213,7 +213,8 @@ class _GemmGroup:
 
 
 # independent launches of one stage that may share a launch
-_GROUPS = (("z_pres", "z_old"), ("ypass_v", "ypass_x"), ("pres_p", "correct"))
+_GROUPS = (("z_pres", "z_old"), ("ypass_v", "ypass_x"), ("div_x", "poisson_rhs"),
+           ("poisson_qy", "corr_y", "corr_y0"), ("pres_p", "correct"))
 
 
 def _group_launches(L, tag):
@@ -423,7 +424,19 @@ class RBCBatch:
         self.psing = torch.as_tensor(sing, device=dev)
         self.prec = T(np.stack(rec_from_poisson(tab, sing)))  # [LM | R | MU | NU][M0][M1]
         self.precT = self.prec.transpose(1, 2).contiguous()   # line-major [4][M1][M0]
-        self.Wy = T(np.vstack([ay.Sn, ay.TSn, ay.TDSn]))                      # [N1+2M1, M1]
+        Wy = np.vstack([ay.Sn, ay.TSn, ay.TDSn])
+        self.Wy = T(Wy)                                                       # [N1+2M1, M1]
+        # composites that take the Poisson chain off the critical path:
+        #   RP = B_x div H_y^T = (B [Cd | C])_x [X_f (H_y C_f)^T]_f
+        #   W1 = P W_y^T = RP (W_y Q_y)^T, row 0 with row 0 of Q_y zeroed (the
+        #        gauge P[0,0] = 0 of the solve)
+        self.HCy = T(np.stack([np.vstack([ay.C, pm["Hy"] @ ay.C]),
+                               np.vstack([ay.Cd, pm["Hy"] @ ay.Cd])]))        # [2, N1+M1, M1]
+        self.BCx = T(ax.B @ np.hstack([ax.Cd, ax.C]))                         # [M0, 2M0]
+        Qy0 = np.array(pm["Qy"], copy=True)
+        Qy0[0, :] = 0.0
+        self.WQy = T(Wy @ pm["Qy"])                                           # [N1+2M1, M1]
+        self.WQy0 = T(Wy @ Qy0)
         self.Snx = T(ax.Sn)                                                   # [N0, M0]
         self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
 
@@ -483,9 +496,8 @@ class RBCBatch:
             self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
             self.Pr = torch.zeros((B, 3, L0, L1), **f64)
         self.Z = torch.zeros((B, 3, K, M1), **f64)
-        self.Wd = torch.zeros((B, 2, M0, N1), **f64)
         self.div = torch.zeros((B, N0, N1), **f64)
-        self.Q1 = torch.zeros((B, M0, N1), **f64)
+        self.W2 = torch.zeros((B, 2, M0, N1 + M1), **f64)
         self.RP = torch.zeros((B, M0, M1), **f64)
         self.W1 = torch.zeros((B, M0, N1 + 2 * M1), **f64)
         self._build_launches()
@@ -606,21 +618,19 @@ class RBCBatch:
         ry = {v: _rec(p(self.st_rhy[st]), M1, 4 * M1, v) for v in (1, 2)}
         L.append(_RecCall("helmholtz_x" + tag, gx, rx, _rec_default(3 * B * M1)))
         L.append(_RecCall("helmholtz_y" + tag, gy, ry, _rec_default(3 * B * M0)))
-        # 11  Wd = [U* C_y^T ; V* Cd_y^T]
-        L.append(_Gemm("div_y" + tag, m=M0, n=N1, k=M1, batch=2 * B, A=p(X), lda=M1,
-                       iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.Cy), ldb=M1, tb=1,
-                       iB=_bidx(s1=N1 * M1, m1=2), C=p(self.Wd), ldc=N1, iC=_bidx(s1=M0 * N1)))
-        # 12  div = [Cd | C]_x Wd
+        # 11  W2_f = X*_f [C ; H_y C]_f^T  (f = U with C_y, V with Cd_y): the y factors
+        #     of the divergence and of the Poisson right-hand side in one product
+        NH = N1 + M1
+        L.append(_Gemm("div_y" + tag, m=M0, n=NH, k=M1, batch=2 * B, A=p(X), lda=M1,
+                       iA=_bidx(s1=3 * sX, d1=2, s2=sX, m2=2), B=p(self.HCy), ldb=M1, tb=1,
+                       iB=_bidx(s1=NH * M1, m1=2), C=p(self.W2), ldc=NH, iC=_bidx(s1=M0 * NH)))
+        # 12  div = [Cd | C]_x Wd  and  RP = (B [Cd | C])_x Wh  (neighbours: one launch)
         L.append(_Gemm("div_x" + tag, m=N0, n=N1, k=2 * M0, batch=B, A=p(self.Cx), lda=2 * M0,
-                       B=p(self.Wd), ldb=N1, iB=_bidx(s1=2 * M0 * N1), C=p(self.div), ldc=N1,
+                       B=p(self.W2), ldb=NH, iB=_bidx(s1=2 * M0 * NH), C=p(self.div), ldc=N1,
                        iC=_bidx(s1=N0 * N1)))
-        # 13-14  Poisson right-hand side  B_x div H_y^T
-        L.append(_Gemm("poisson_bx" + tag, m=M0, n=N1, k=N0, batch=B, A=p(self.Bx), lda=N0,
-                       B=p(self.div), ldb=N1, iB=_bidx(s1=N0 * N1), C=p(self.Q1), ldc=N1,
-                       iC=_bidx(s1=M0 * N1)))
-        L.append(_Gemm("poisson_hy" + tag, m=M0, n=M1, k=N1, batch=B, A=p(self.Q1), lda=N1,
-                       iA=_bidx(s1=M0 * N1), B=p(self.Hy), ldb=N1, tb=1, C=p(self.RP), ldc=M1,
-                       iC=_bidx(s1=sX)))
+        L.append(_Gemm("poisson_rhs" + tag, m=M0, n=M1, k=2 * M0, batch=B, A=p(self.BCx),
+                       lda=2 * M0, B=p(self.W2) + 8 * N1, ldb=NH, iB=_bidx(s1=2 * M0 * NH),
+                       C=p(self.RP), ldc=M1, iC=_bidx(s1=sX)))
         # 15  per-line eigen-split solves along x
         gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
         rp = {}
@@ -630,15 +640,17 @@ class RBCBatch:
             r.per_line, r.entry_stride, r.variant = layout, 0, v
             rp[v] = r
         L.append(_RecCall("poisson_lines" + tag, gp, rp, _rec_default(B * M1)))
-        # 16  P = RP Q_y^T
-        # 17  ... with the gauge P[0,0] = 0 applied in the GEMM epilogue
+        # 16  P = RP Q_y^T with the gauge P[0,0] = 0 (GEMM epilogue), and
+        # 17  W1 = P [S; T S; T D S/s]_y^T computed from RP directly (same launch)
+        NW = N1 + 2 * M1
         L.append(_Gemm("poisson_qy" + tag, m=M0, n=M1, k=M1, batch=B, A=p(self.RP), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.Qy), ldb=M1, tb=1, C=p(self.P), ldc=M1,
                        iC=_bidx(s1=sX), zero_first=1))
-        # 18  W = P [S; T S; T D S/s]_y^T
-        NW = N1 + 2 * M1
-        L.append(_Gemm("corr_y" + tag, m=M0, n=NW, k=M1, batch=B, A=p(self.P), lda=M1,
-                       iA=_bidx(s1=sX), B=p(self.Wy), ldb=M1, tb=1, C=p(self.W1), ldc=NW,
+        L.append(_Gemm("corr_y" + tag, m=M0 - 1, n=NW, k=M1, batch=B, A=p(self.RP) + 8 * M1,
+                       lda=M1, iA=_bidx(s1=sX), B=p(self.WQy), ldb=M1, tb=1,
+                       C=p(self.W1) + 8 * NW, ldc=NW, iC=_bidx(s1=M0 * NW)))
+        L.append(_Gemm("corr_y0" + tag, m=1, n=NW, k=M1, batch=B, A=p(self.RP), lda=M1,
+                       iA=_bidx(s1=sX), B=p(self.WQy0), ldb=M1, tb=1, C=p(self.W1), ldc=NW,
                        iC=_bidx(s1=M0 * NW)))
         # 19-20  pres = (pres - nu div beta) + S_x W_0 / (dt a)   (one GEMM, fused epilogue)
         L.append(_Gemm("pres_p" + tag, m=N0, n=N1, k=M0, batch=B, A=p(self.Snx), lda=M0,
