@@ -13,7 +13,9 @@ between device synchronisations, replayed from a HIP graph.
 Also reported on the same line:
   roofline       the dominant launch of the timed step (largest time per step in an
                  eager pass of the same K steps with HIP events around each launch),
-                 timed as 20 back-to-back launches replayed from a HIP graph
+                 timed as 20 back-to-back launches replayed from a HIP graph; for the
+                 K-split right-hand side, its GEMM kernel (partial products and the
+                 buoyancy product in one launch)
   batched        64 instances of 256x256 (Ra log-uniform in [1e4, 1e7]) in one
                  batched step: aggregate instance-steps/s and the ratio to the plain path
   variants       the same 256x256 step with RK3 (three stages) and with beta = 0.5
@@ -105,8 +107,8 @@ def copy_calibration(mib=256, reps=5):
 
 def launch_cost(name, launch, eng):
     """(bound, algorithmic amount per launch) of one launch of the fused step."""
-    from pypde_amd.engine import _Gemm
-    if isinstance(launch, _Gemm):
+    from pypde_amd.engine import _Gemm, _GemmGroup
+    if isinstance(launch, (_Gemm, _GemmGroup)):
         return "mfma", launch.flops
     B, N0, N1, M0, M1, L0, L1 = eng.B, eng.N0, eng.N1, eng.M0, eng.M1, eng.L0, eng.L1
     el = {
@@ -147,6 +149,13 @@ def time_plain(args):
         per[name][1] += 1
     dom = max(per, key=lambda k: per[k][0])
     tot, cnt, launch = per[dom]
+    from pypde_amd.engine import _SplitRhs
+    if isinstance(launch, _SplitRhs):
+        # a K-split product: its GEMM kernel (the partial products and the
+        # buoyancy product in one launch) is the roofline kernel; the summing
+        # kernel after it is timed in the kernel table with the launch
+        launch = launch.group
+        dom = launch.name
     avg = launch_graph_time(eng, launch)
     bound, amount = launch_cost(dom, launch, eng)
     if bound == "mfma":

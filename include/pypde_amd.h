@@ -121,6 +121,17 @@ int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream);
  * pypde_dgemm. */
 int pypde_dgemm_grouped(const pypde_gemm_desc* descs, int count, void* stream);
 
+/* Sum of K-split GEMM partials onto an addend (the second half of a split-K
+ * product; the first half is a batched GEMM whose batch runs over K slices):
+ *   C[e] = (D[e] + sum_{s<S} W[e*S + s]) + X[e / xm]   (X term only when e % xm == xr)
+ * for `entries` contiguous matrices of mn elements; D and X may be NULL.  The
+ * partials are summed in slice order, so the result does not depend on the
+ * launch order.  Serves the fused step's right-hand-side assembly (the RHS of
+ * update_U / update_V / update_T, navier/rbc2d.py:213-260, with the buoyancy
+ * term of update_V as the X addend of the V entries). */
+int pypde_splitk_reduce(const double* W, int S, const double* D, const double* X, int xm, int xr,
+                        double* C, long long mn, int entries, void* stream);
+
 /* ---- line operators (one lane per line, LDS-staged panels) ------------ */
 /* Galerkin -> Chebyshev stencil u = S v, S(i,i)=1, S(i+2,i)=s[i]; len_out = len_in+2.
  * Replaces GalerkinChebyshev.to_chebyshev (pypde/bases/chebyshev.py:287-293). */

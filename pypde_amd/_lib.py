@@ -103,6 +103,7 @@ _SIGS = {
     "pypde_axpby": ([_ll, _d, _vp, _d, _vp, _vp], _i),
     "pypde_dgemm_batched": ([ctypes.c_void_p, _vp], _i),
     "pypde_dgemm_grouped": ([ctypes.c_void_p, _i, _vp], _i),
+    "pypde_splitk_reduce": ([_vp, _i, _vp, _vp, _i, _i, _vp, ctypes.c_longlong, _i, _vp], _i),
     "pypde_rbc_products": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "pypde_rbc_products_yfold": ([_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp], _i),
     "pypde_scaled_sub": ([_ll, _i, _vp, _d, _vp, _vp, _vp], _i),

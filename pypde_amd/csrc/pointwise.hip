@@ -163,6 +163,26 @@ __global__ void set_entries_kernel(double* p, long long stride, int n, double va
     if (e < n) p[(long long)e * stride] = value;
 }
 
+// Sum of K-split partial products onto an addend, in a fixed order:
+//   C[e] = (D[e] + (W[e S] + W[e S + 1] + ... + W[e S + S-1])) + X[e / xm]  (X only for e % xm == xr)
+// over `entries` matrices of mn elements each (all contiguous).  The sum runs
+// in partial order whatever the launch order, so results are reproducible.
+__global__ void splitk_reduce_kernel(const double* __restrict__ W, int S, const double* __restrict__ D,
+                                     const double* __restrict__ X, int xm, int xr,
+                                     double* __restrict__ C, long long mn, long long total)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long e = t / mn, r = t - e * mn;
+        const double* w = W + e * S * mn + r;
+        double acc = w[0];
+        for (int k = 1; k < S; ++k) acc = acc + w[k * mn];
+        double v = D ? D[t] + acc : acc;
+        if (X && (int)(e % xm) == xr) v = v + X[(e / xm) * mn + r];
+        C[t] = v;
+    }
+}
+
 static unsigned grid_for(long long n)
 {
     long long b = (n + 255) / 256;
@@ -228,4 +248,18 @@ extern "C" int pypde_set_entries(double* p, long long stride, int n, double valu
     hipLaunchKernelGGL(set_entries_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), p,
                        stride, n, value);
     return check_launch("pypde_set_entries");
+}
+
+extern "C" int pypde_splitk_reduce(const double* W, int S, const double* D, const double* X,
+                                   int xm, int xr, double* C, long long mn, int entries,
+                                   void* stream)
+{
+    PYPDE_REQUIRE(W && C, "null pointer");
+    PYPDE_REQUIRE(S >= 1 && S <= 64, "splits must be 1..64 (got %d)", S);
+    PYPDE_REQUIRE(mn >= 0 && entries >= 0, "negative size");
+    PYPDE_REQUIRE(!X || (xm >= 1 && xr >= 0 && xr < xm), "extra addend needs 0 <= xr < xm");
+    if (mn == 0 || entries == 0) return OK;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for(mn * entries)), dim3(256), 0,
+                       as_stream(stream), W, S, D, X, xm < 1 ? 1 : xm, xr, C, mn, mn * entries);
+    return check_launch("pypde_splitk_reduce");
 }

@@ -212,9 +212,27 @@ class _GemmGroup:
               self.name)
 
 
+class _SplitRhs:
+    """Launches 7-8 with the right-hand-side product split along K: a grouped
+    launch of the partial products (batch over field x K slice) and the
+    buoyancy product, then pypde_splitk_reduce sums the slices in order onto
+    the constant term and adds the buoyancy to the V entries.  For one or a few
+    instances, where the unsplit product has too few output tiles to fill the
+    device."""
+
+    def __init__(self, name, group, reduce_args, S):
+        self.name, self.group, self.reduce_args, self.S = name, group, reduce_args, S
+        self.flops = group.flops
+
+    def __call__(self, stream):
+        self.group(stream)
+        check(lib.pypde_splitk_reduce(*self.reduce_args, stream), self.name)
+
+
 # independent launches of one stage that may share a launch
-_GROUPS = (("z_pres", "z_old"), ("ypass_v", "ypass_x"), ("div_x", "poisson_rhs"),
-           ("poisson_qy", "corr_y", "corr_y0"), ("pres_p", "correct"))
+_GROUPS = (("z_conv", "z_pres", "z_old", "z_diff"), ("z_conv", "z_pres", "z_old"),
+           ("z_pres", "z_old", "z_diff"), ("z_pres", "z_old"), ("ypass_v", "ypass_x"),
+           ("div_x", "poisson_rhs"), ("poisson_qy", "corr_y", "corr_y0"), ("pres_p", "correct"))
 
 
 def _group_launches(L, tag):
@@ -300,7 +318,8 @@ class RBCBatch:
           rbc2d_base.py:86-102); beta: implicit fraction of the diffusion, the
           rest treated explicitly (rbc2d.py:269-283).
     group_launches: independent GEMMs of a stage share one launch
-          (pypde_dgemm_grouped) where the tuner finds it faster.
+          (pypde_dgemm_grouped) where the tuner finds it faster, and for few
+          instances the right-hand-side product may be split along K.
     """
 
     def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0,
@@ -549,6 +568,15 @@ class RBCBatch:
     # -----------------------------------------------------------------------
     def _build_launches(self):
         self._keep = []
+        self._rhs_pairs = []
+        B, M0, M1, K = self.B, self.M0, self.M1, self.K
+        tiles = 3 * B * ((M0 + 31) // 32) * ((M1 + 31) // 32)
+        self._splits = [S for S in (2, 3, 4, 6, 8) if K % S == 0 and K // S >= 32] \
+            if (tiles <= 768 and self.group_launches) else []
+        if self._splits:
+            f64 = dict(dtype=ops.DTYPE, device=self.X.device)
+            self.rhs_work = torch.zeros(3 * B * max(self._splits) * M0 * M1, **f64)
+            self.rhs_workb = torch.zeros(B * M0 * M1, **f64)
         L = []
         for st in range(self.nstage):
             S = []
@@ -567,19 +595,22 @@ class RBCBatch:
         sX = M0 * M1
         sZ = self.sZ = K * M1
         tag = "" if self.nstage == 1 else "[%d]" % st
-        # 5  Z[1] = -dt a pres (B | BD)_y^T   for U, V      (needs only the stage-start state)
-        L.append(_Gemm("z_pres" + tag, m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
+        # 5-6 read only the stage-start state (X and pres are not written before
+        # launch 7), so they are issued next to launch 4 and may share its launch
+        Lz = []
+        # 5  Z[1] = -dt a pres (B | BD)_y^T   for U, V
+        Lz.append(_Gemm("z_pres" + tag, m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
                        iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
                        iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
                        iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.st_mdta[st]),
                        alpha_div=2))
         # 6  Z[2] = X BS_y^T
-        L.append(_Gemm("z_old" + tag, m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1,
+        Lz.append(_Gemm("z_old" + tag, m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1,
                        iA=_bidx(s1=sX), B=p(self.BSy), ldb=M1, tb=1,
                        C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1, iC=_bidx(s1=sZ)))
         if self.diffusion:
             # 6b  Z[3] = f X BS_y^T, Z[4] = f X BD2S_y^T (explicit diffusion, f per field)
-            L.append(_Gemm("z_diff" + tag, m=M0, n=M1, k=M1, batch=6 * B, A=p(X), lda=M1,
+            Lz.append(_Gemm("z_diff" + tag, m=M0, n=M1, k=M1, batch=6 * B, A=p(X), lda=M1,
                            iA=_bidx(s1=sX, d1=2), B=p(self.DFy), ldb=M1, tb=1,
                            iB=_bidx(s1=M1 * M1, m1=2), C=p(Z) + 8 * (L0 + N0 + M0) * M1,
                            ldc=M1, iC=_bidx(s1=sZ, d1=2, s2=M0 * M1, m2=2),
@@ -600,6 +631,7 @@ class RBCBatch:
             self._yfold_launches(L, st, tag)
         else:
             self._plain_y_launches(L, tag)
+        L += Lz
         # 7  X = A_f Z_f + const_f
         L.append(_Gemm("rhs" + tag, m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
                        iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
@@ -611,6 +643,21 @@ class RBCBatch:
                        D=p(X) + 8 * sX, ldd=M1, iD=_bidx(s1=3 * sX), beta=1.0,
                        C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX),
                        alpha_vec=p(self.st_dta[st])))
+        cands = {}
+        for S in self._splits:
+            kc = K // S
+            part = _Gemm("rhs_part" + tag, m=M0, n=M1, k=kc, batch=3 * B * S, A=p(self.Arhs),
+                         lda=K, iA=_bidx(s1=M0 * K, d1=S, m1=3, s2=kc, d2=1, m2=S), B=p(Z),
+                         ldb=M1, iB=_bidx(s1=sZ, d1=S, s2=kc * M1, d2=1, m2=S),
+                         C=p(self.rhs_work), ldc=M1, iC=_bidx(s1=sX))
+            buoy = _Gemm("buoyancy_part" + tag, m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx),
+                         lda=M0, B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1,
+                         iB=_bidx(s1=3 * sZ), C=p(self.rhs_workb), ldc=M1, iC=_bidx(s1=sX),
+                         alpha_vec=p(self.st_dta[st]))
+            red = (ctypes.c_void_p(p(self.rhs_work)), S, ctypes.c_void_p(p(self.st_const[st])),
+                   ctypes.c_void_p(p(self.rhs_workb)), 3, 1, ctypes.c_void_p(p(X)), sX, 3 * B)
+            cands[S] = _SplitRhs("rhs+buoyancy/%d" % S + tag, _GemmGroup([part, buoy]), red, S)
+        self._rhs_pairs.append((L[-2], L[-1], cands))
         # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
@@ -791,28 +838,8 @@ class RBCBatch:
                 launch.set_variant(autotune.choose(launch.key(), run, (1, 2),
                                                    default=launch.variant))
         for launch in self.launches:
-            if not isinstance(launch, _GemmGroup):
-                continue
-            for m in launch.members:
-                autotune.tune_gemm_desc(m.desc, launch_fn(m.name), allow_alias=True)
-            own = [m.desc.variant for m in launch.members]
-
-            def run_group(v, g=launch):
-                g.set_variant(v)
-                check(lib.pypde_dgemm_grouped(ctypes.byref(g.arr), len(g.members), s), g.name)
-            key = ("group",) + tuple(autotune.gemm_key(m.desc) for m in launch.members)
-            best = autotune.choose(key, run_group, autotune.GEMM_CANDIDATES,
-                                   default=own[0])
-            t_group = autotune.timings().get(key, {}).get(best)
-            t_split = sum(autotune.timings().get(autotune.gemm_key(m.desc), {}).get(v, 0.0)
-                          for m, v in zip(launch.members, own)) if t_group is not None else None
-            if t_group is not None and t_split and t_split < t_group:
-                for m, v in zip(launch.members, own):
-                    m.desc.variant = v
-                launch.split = True
-            else:
-                launch.set_variant(best)
-                launch.split = False
+            if isinstance(launch, _GemmGroup):
+                self._tune_group(launch, s, launch_fn)
         for i, launch in enumerate(self.launches):
             if not isinstance(launch, _Gemm):
                 continue
@@ -827,10 +854,56 @@ class RBCBatch:
                 if pick == 1:
                     alt.alt = launch
                     self.launches[i] = alt
+        self._tune_rhs_split(s, launch_fn)
         torch.cuda.synchronize()
         for t, v in zip((self.X, self.pres, self.P), saved):
             t.copy_(v)
         torch.cuda.synchronize()
+
+    def _tune_group(self, launch, s, launch_fn):
+        """Variant of a grouped launch, or separate launches when faster."""
+        from . import autotune
+        for m in launch.members:
+            autotune.tune_gemm_desc(m.desc, launch_fn(m.name), allow_alias=True)
+        own = [m.desc.variant for m in launch.members]
+
+        def run_group(v, g=launch):
+            g.set_variant(v)
+            check(lib.pypde_dgemm_grouped(ctypes.byref(g.arr), len(g.members), s), g.name)
+        key = ("group",) + tuple(autotune.gemm_key(m.desc) for m in launch.members)
+        best = autotune.choose(key, run_group, autotune.GEMM_CANDIDATES, default=own[0])
+        t_group = autotune.timings().get(key, {}).get(best)
+        t_split = sum(autotune.timings().get(autotune.gemm_key(m.desc), {}).get(v, 0.0)
+                      for m, v in zip(launch.members, own)) if t_group is not None else None
+        if t_group is not None and t_split and t_split < t_group:
+            for m, v in zip(launch.members, own):
+                m.desc.variant = v
+            launch.split = True
+        else:
+            launch.set_variant(best)
+            launch.split = False
+
+    def _tune_rhs_split(self, s, launch_fn):
+        """Launches 7-8 as they are (0) or split along K into S slices."""
+        from . import autotune
+        for rhs, buoy, cands in self._rhs_pairs:
+            if not cands:
+                continue
+            for c in cands.values():
+                self._tune_group(c.group, s, launch_fn)
+
+            def run(S, rhs=rhs, buoy=buoy, cands=cands):
+                if S == 0:
+                    rhs(s)
+                    buoy(s)
+                else:
+                    cands[S](s)
+            key = ("rhs_split", self.B, self.M0, self.M1, self.K, autotune.gemm_key(rhs.desc))
+            pick = autotune.choose(key, run, (0,) + tuple(cands), default=0)
+            if pick:
+                i = self.launches.index(rhs)
+                assert self.launches[i + 1] is buoy
+                self.launches[i:i + 2] = [cands[pick]]
 
     def variants(self):
         """{launch name: kernel variant} chosen for this step (GEMM variants;
