@@ -4,6 +4,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
     python bench.py --micro              # microbench only (one JSON line)
     python bench.py --copy-calibration   # 256 MiB device copies, for scaling PMC byte counters
+    python bench.py --tune-db PATH       # tune every workload afresh, write the decisions
 
 Workload (value): the plain Euler IMEX step of navier.rbc2d.NavierStokes at
 256x256, rbc case, Ra=1e8, Pr=1, dt=1e-3, 3/2 dealiasing, seeded
@@ -167,7 +168,7 @@ def time_plain(args):
     finite = bool(np.isfinite(nu[0]) and np.isfinite(nu[1]))
     return dict(sec=sec, steps_per_s=args.steps / sec, dominant=dom, avg_s=avg, bound=bound,
                 amount=amount, achieved=achieved, peak=peak, unit=unit, kernels=kernel_table,
-                nu=nu, finite=finite, eager_sec=sum(v[0] for v in per.values()))
+                nu=nu, finite=finite, eager_sec=sum(v[0] for v in per.values()), engine=eng)
 
 
 def time_batched(args, nb=64, steps=None):
@@ -435,7 +436,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--all-launches", action="store_true",
+                    help="after the plain step, replay every launch of it 21 times in step "
+                         "order (one block per launch, for per-launch PMC counts)")
+    ap.add_argument("--tune-db", metavar="PATH",
+                    help="tune every workload of this benchmark from scratch and write the "
+                         "kernel decisions to PATH (the database pypde_amd/tune_db.json)")
     args = ap.parse_args()
+    if args.tune_db:
+        os.environ["PYPDE_TUNE_DB"] = ""  # measure, do not load stored decisions
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (got --gpus %d); multi-GPU runs are "
               "not part of it." % args.gpus, file=sys.stderr)
@@ -450,7 +459,34 @@ def main():
     if args.micro:
         print(json.dumps({"microbench": microbench()}))
         return
+    if args.tune_db:
+        from pypde_amd import autotune
+        args.steps, args.warmup = 20, 2
+        time_plain(args)
+        time_batched(args)
+        time_hetero()
+        time_variant(steps=5, integrator="rk3")
+        time_variant(steps=5, beta=0.5)
+        microbench()
+        for shape, ra in (((20, 16), 1e4), ((64, 64), 1e4), ((128, 128), 5e4)):
+            time_steady(shape, ra, pre=20, tol=1e-3)
+        time_adjoint_descent(256, steps=2)
+        time_adjoint(256, 20, ckpt=10)
+        time_adjoint(128, 20, ckpt=10)
+        n = autotune.save_db(args.tune_db)
+        print(json.dumps({"tune_db": args.tune_db, "entries": n}))
+        return
     plain = time_plain(args)
+    if args.all_launches:
+        from pypde_amd.engine import _GemmGroup, _SplitRhs
+        eng = plain["engine"]
+        order = []
+        for launch in eng.launches:
+            l = launch.group if isinstance(launch, _SplitRhs) else launch
+            launch_graph_time(eng, l)
+            kernels = len(l.members) if isinstance(l, _GemmGroup) and l.split else 1
+            order.append([l.name, kernels])
+        print(json.dumps({"launch_order": order}))
     extra = {}
     if not args.no_extra:
         b = time_batched(args)

@@ -14,7 +14,15 @@ captured into a graph: captured launches use the cached choice or the size
 heuristic of the library (variant 0).
 
 PYPDE_AUTOTUNE=0 in the environment switches tuning off (heuristics only).
+
+Decisions measured once on the target hardware can be shipped as a database
+(pypde_amd/tune_db.json, or the file PYPDE_TUNE_DB names; PYPDE_TUNE_DB=""
+ignores it): entries for the current device architecture are loaded on first
+use and taken as cached choices, so a fresh process runs the same kernels as
+the run that wrote them.  `python bench.py --tune-db PATH` writes one.
 """
+import ast
+import json
 import os
 
 import torch
@@ -39,6 +47,8 @@ def set_enabled(flag):
 
 def cache():
     """{key: chosen variant} of this process."""
+    if torch.cuda.is_available():
+        _ensure_db()
     return dict(_cache)
 
 
@@ -70,10 +80,56 @@ def time_candidates(run, candidates, reps=5):
     return out
 
 
+_DB_DEFAULT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.json")
+_db_loaded = False
+
+
+def _arch():
+    try:
+        return torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName.split(":")[0]
+    except Exception:
+        return None
+
+
+def load_db(path=None):
+    """Load the tuning decisions stored for this device architecture (returns
+    the number of entries taken)."""
+    path = os.environ.get("PYPDE_TUNE_DB", _DB_DEFAULT) if path is None else path
+    if not path or not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        db = json.load(f)
+    if db.get("arch") != _arch():
+        return 0
+    n = 0
+    for k, v in db.get("entries", {}).items():
+        key = ast.literal_eval(k)
+        if key not in _cache:
+            _cache[key] = v
+            n += 1
+    return n
+
+
+def save_db(path):
+    """Write this process's decisions, for the current device architecture."""
+    entries = {repr(k): v for k, v in sorted(_cache.items(), key=lambda kv: repr(kv[0]))}
+    with open(path, "w") as f:
+        json.dump({"arch": _arch(), "entries": entries}, f, indent=0, sort_keys=True)
+    return len(entries)
+
+
+def _ensure_db():
+    global _db_loaded
+    if not _db_loaded:
+        _db_loaded = True
+        load_db()
+
+
 def choose(key, run, candidates, default=0):
     """Cached best candidate for `key`, timing the candidates on first use."""
+    _ensure_db()
     v = _cache.get(key)
-    if v is not None:
+    if v is not None and v in candidates:
         return v
     if not _enabled or _capturing():
         return default
@@ -96,6 +152,7 @@ def tune_gemm_desc(d, launch, allow_alias=False):
     since repeated launches change its result; it keeps the size heuristic."""
     aliased = d.C in (d.A, d.B) or (d.D is not None and d.D == d.C and d.beta != 0.0)
     if aliased and not allow_alias:
+        _ensure_db()
         d.variant = _cache.get(gemm_key(d), 0)
         return d.variant
 

@@ -128,6 +128,31 @@ struct Epilogue {
     double beta_e_gamma;
 };
 
+// Workgroups are dispatched to the device's 8 XCDs in turn (workgroup i to
+// XCD i mod 8), and each XCD has its own L2.  Renumbering them so that XCD x
+// runs a contiguous range of tiles keeps the operand panels those tiles share
+// in one L2 instead of fetching every panel into all eight.  The map is a
+// bijection on [0, total): results do not change, only where tiles run.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_remap(int pid, int total)
+{
+    const int q = total / kXcd, r = total - q * kXcd;
+    const int x = pid % kXcd, local = pid / kXcd;
+    return x * q + (x < r ? x : r) + local;
+}
+
+// Tile (bx, by, bz) of a 3-D grid after the XCD renumbering.
+__device__ __forceinline__ void grid_tile(int& bx, int& by, int& bz)
+{
+    const int tx = gridDim.x, ty = gridDim.y;
+    const int total = tx * ty * gridDim.z;
+    const int t = xcd_remap(blockIdx.x + tx * (blockIdx.y + ty * blockIdx.z), total);
+    bx = t % tx;
+    const int r = t / tx;
+    by = r % ty;
+    bz = r / ty;
+}
+
 // Several independent GEMMs of one kernel form in one launch (grouped
 // launch): the grid is the concatenation of the members' tile grids, block t
 // runs tile t - start[i] of member i.  Members share the kernel template, so
@@ -142,7 +167,7 @@ struct GemmGroup {
 
 __device__ __forceinline__ int group_tile(const GemmGroup& gg, int& bx, int& by, int& bz)
 {
-    const int t = blockIdx.x;
+    const int t = xcd_remap(blockIdx.x, gg.start[gg.count]);
     int i = 0;
     while (i + 1 < gg.count && t >= gg.start[i + 1]) ++i;
     const int local = t - gg.start[i];
@@ -295,7 +320,9 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
 template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 {
-    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(g, bx, by, bz);
 }
 
 template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
@@ -457,7 +484,9 @@ __device__ __forceinline__ void dgemm_splitk_body(const GemmArgs& g, int bx, int
 template <int TM, int TN, int KW, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * KW) dgemm_splitk_kernel(GemmArgs g)
 {
-    dgemm_splitk_body<TM, TN, KW, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    dgemm_splitk_body<TM, TN, KW, TA, TB>(g, bx, by, bz);
 }
 
 template <int TM, int TN, int KW, bool TA, bool TB>
@@ -621,7 +650,9 @@ __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int
 template <int BM, int BN, int KW, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
 {
-    dgemm_wsplit_body<BM, BN, KW, TA, TB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    dgemm_wsplit_body<BM, BN, KW, TA, TB>(g, bx, by, bz);
 }
 
 template <int BM, int BN, int KW, bool TA, bool TB>
@@ -711,9 +742,10 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, 
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave / WGN, wn = wave % WGN;
-    const int bz = blockIdx.z;
-    const int m0 = blockIdx.y * BM;
-    const int n0 = blockIdx.x * BN;
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    const int m0 = by * BM;
+    const int n0 = bx * BN;
     const long long oA = bofs(bz, g.iA), oB = bofs(bz, g.iB);
     const double se = (MODE >= 3) ? (((bz / f.mirror_div) & 1) ? -f.mirror_sign : f.mirror_sign) : 0.0;
     const int kf1 = f.fold_len - 1;
@@ -908,6 +940,8 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     if (g.m == 0 || g.n == 0 || batch == 0) return OK;
     PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
+    PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
+                  "GEMM grid too large");
     PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 8 && variant <= 11) ||
                       (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
@@ -963,6 +997,8 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
     PYPDE_REQUIRE(g.A && g.B && g.C, "null GEMM operand");
     PYPDE_REQUIRE(d->batch <= 65535, "GEMM batch %d exceeds 65535", d->batch);
     PYPDE_REQUIRE(d->fold_len >= 1, "fold_len must be >= 1");
+    PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * d->batch < (1LL << 31),
+                  "GEMM grid too large");
     FoldArgs f{d->A1, d->B1, d->lda1, d->ldb1, d->k1, d->ska < 1 ? 1 : d->ska,
                d->skb < 1 ? 1 : d->skb, d->fold_len, d->fold_sign, d->mirror_sign,
                d->mirror_div < 1 ? 1 : d->mirror_div};

@@ -870,12 +870,20 @@ class RBCBatch:
         def run_group(v, g=launch):
             g.set_variant(v)
             check(lib.pypde_dgemm_grouped(ctypes.byref(g.arr), len(g.members), s), g.name)
-        key = ("group",) + tuple(autotune.gemm_key(m.desc) for m in launch.members)
-        best = autotune.choose(key, run_group, autotune.GEMM_CANDIDATES, default=own[0])
-        t_group = autotune.timings().get(key, {}).get(best)
-        t_split = sum(autotune.timings().get(autotune.gemm_key(m.desc), {}).get(v, 0.0)
-                      for m, v in zip(launch.members, own)) if t_group is not None else None
-        if t_group is not None and t_split and t_split < t_group:
+        keys = tuple(autotune.gemm_key(m.desc) for m in launch.members)
+        best = autotune.choose(("group",) + keys, run_group, autotune.GEMM_CANDIDATES,
+                               default=own[0])
+
+        def run_mode(c, g=launch):
+            if c == 0:
+                g.set_variant(best)
+                check(lib.pypde_dgemm_grouped(ctypes.byref(g.arr), len(g.members), s), g.name)
+            else:
+                for m, v in zip(g.members, own):
+                    m.desc.variant = v
+                    m(s)
+        separate = autotune.choose(("group_mode",) + keys, run_mode, (0, 1), default=0)
+        if separate:
             for m, v in zip(launch.members, own):
                 m.desc.variant = v
             launch.split = True
