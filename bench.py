@@ -56,15 +56,27 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
     return rng.uniform(-amplitude, amplitude, size=shape)
 
 
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE passes over `python3 bench.py --steps 20 --warmup 2 --no-cpu
-# --no-extra`, each divided by what the same counter reads for a device copy of
-# known size (`python3 bench.py --copy-calibration`); the profile summaries
-# under profiles/ give the commands and raw counter values.
+# HBM bytes per launch of each launch of the 256x256 step (kernel choices of
+# pypde_amd/tune_db.json), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
+# over `python3 bench.py --steps 20 --warmup 2 --no-cpu --no-extra
+# --all-launches` (the mean over 20 graph-replayed launches of each), each
+# counter divided by what it reads for a device copy of known size
+# (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
+# commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    # profiles/hbm_traffic_pmc.md: recurrence_kernel<0, 16, false, true, false, true, true,
-    # true, 1024> (per-line Poisson solve, 256x256): 4.001 MB read + 1.225 MB written
-    "poisson_lines": 5.226e6,
+    "xpass": 12608711,
+    "ypass_v+ypass_x": 23605516,
+    "products": 14242766,
+    "z_conv+z_pres+z_old": 18003126,
+    "rhs": 26872888,
+    "buoyancy": 5788321,
+    "helmholtz_x": 18904168,
+    "helmholtz_y": 3515566,
+    "div_y": 11551513,
+    "div_x+poisson_rhs": 12008058,
+    "poisson_lines": 8500475,
+    "poisson_qy+corr_y+corr_y0": 15160800,
+    "pres_p+correct": 10755771,
 }
 
 
