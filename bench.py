@@ -445,7 +445,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--all-launches", action="store_true",

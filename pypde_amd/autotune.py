@@ -29,7 +29,7 @@ import torch
 
 ROWS_CANDIDATES = (1, 2, 4, 8)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
-GEMM_CANDIDATES = (1, 2, 3, 4, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
+GEMM_CANDIDATES = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
 _cache = {}

@@ -918,6 +918,7 @@ static BIdx strided(long long s)
 // Variant codes (pypde_gemm_desc.variant):
 //   0 heuristic, 1 LDS 64x64 (4 waves), 2 LDS 32x32 (4 waves),
 //   3 LDS 128x128 (8 waves), 4 LDS 128x128 (8 waves, double-buffered LDS),
+//   5 LDS 128x64 (8 waves), 6 LDS 64x128 (8 waves),
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
 //   4 waves, 11 32x64 over 4 waves,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
@@ -942,7 +943,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 8 && variant <= 11) ||
+    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 11) ||
                       (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
@@ -951,6 +952,8 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 2: launch_gemm<32, 32, 2, 2, false>(transA, transB, g, batch, stream); break;
     case 3: launch_gemm<128, 128, 4, 2, false>(transA, transB, g, batch, stream); break;
     case 4: launch_gemm<128, 128, 4, 2, true>(transA, transB, g, batch, stream); break;
+    case 5: launch_gemm<128, 64, 4, 2, false>(transA, transB, g, batch, stream); break;
+    case 6: launch_gemm<64, 128, 2, 4, false>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1067,7 +1070,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 4) || (variant >= 8 && variant <= 11) ||
+    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 11) ||
                       (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
@@ -1076,6 +1079,8 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 2: launch_gemm_group<32, 32, 2, 2, false>(ta, tb, gg, batch, stream); break;
     case 3: launch_gemm_group<128, 128, 4, 2, false>(ta, tb, gg, batch, stream); break;
     case 4: launch_gemm_group<128, 128, 4, 2, true>(ta, tb, gg, batch, stream); break;
+    case 5: launch_gemm_group<128, 64, 4, 2, false>(ta, tb, gg, batch, stream); break;
+    case 6: launch_gemm_group<64, 128, 2, 4, false>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;

@@ -38,7 +38,7 @@ def test_mfma_gemm_layout(amd):
                 assert rel(host(C), ref) < 1e-14, (m, n, k, ta, tb)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23])
 def test_gemm_every_variant(amd, variant):
     """Each kernel variant of pypde_dgemm_batched on ragged shapes (m, n not
     multiples of the tile, k not a multiple of the split), all transposes,
@@ -364,7 +364,7 @@ def test_mirror_operator_transforms(amd, N, axis):
         assert rel(y, ref) < 1e-13, (M.shape, axis)
 
 
-PLAIN_VARIANTS = (1, 2, 3, 4, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
+PLAIN_VARIANTS = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
