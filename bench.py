@@ -64,19 +64,19 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12608711,
-    "ypass_v+ypass_x": 23605516,
+    "xpass": 12610766,
+    "ypass_v+ypass_x": 23797006,
     "products": 14242766,
-    "z_conv+z_pres+z_old": 18003126,
-    "rhs": 26872888,
-    "buoyancy": 5788321,
-    "helmholtz_x": 18904168,
-    "helmholtz_y": 3515566,
-    "div_y": 11551513,
-    "div_x+poisson_rhs": 12008058,
+    "z_conv+z_pres+z_old": 17949636,
+    "rhs": 26877192,
+    "buoyancy": 5797532,
+    "helmholtz_x": 18904174,
+    "helmholtz_y": 3515569,
+    "div_y": 11556629,
+    "div_x+poisson_rhs": 12006002,
     "poisson_lines": 8500475,
-    "poisson_qy+corr_y+corr_y0": 15160800,
-    "pres_p+correct": 10755771,
+    "poisson_qy+corr_y+corr_y0": 15176151,
+    "pres_p+correct": 10751671,
 }
 
 
