@@ -31,7 +31,11 @@ typedef struct pypde_lines {
     int nlines;               /* number of lines per batch entry          */
     int batch;                /* number of batch entries                  */
     int axis;                 /* 0: columns, 1: rows                      */
-    int lines_per_block;      /* 0 = library default (autotuner hint)     */
+    int lines_per_block;      /* 0 = library default (autotuner hint); for
+                                 the banded / stencil applies 1..8 outputs
+                                 per thread (axis 0) or rows per block
+                                 (axis 1), 18 / 20 / 24 the register-window
+                                 form with 2 / 4 / 8 outputs per thread */
 } pypde_lines;
 
 /* ---- library ---------------------------------------------------------- */

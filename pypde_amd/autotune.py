@@ -27,7 +27,7 @@ import os
 
 import torch
 
-ROWS_CANDIDATES = (1, 2, 4, 8)
+ROWS_CANDIDATES = (1, 2, 4, 8, 18, 20, 24)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
 GEMM_CANDIDATES = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
 

@@ -46,6 +46,7 @@ struct BandSpec {
     int off[kMaxDiag];
     int nd;
     int stencil;         // 1: stencil form y_i = s_{i-2} x_{i-2} + x_i (vals = s)
+    int qmap[kMaxDiag];  // diagonal q at offset off[0] + d (-1: none), for the window forms
 };
 
 // Output element r of a line from its input window (ascending columns, zero
@@ -507,9 +508,129 @@ static int launch_lines(LineGeom g, const Op& op, int ntab, int hint, void* stre
 
 // rows per thread (axis 0) / per block (axis 1): the lines_per_block hint, 1..8
 // (0 = 1), chosen per shape by the host-side autotuner.
+// Register-window forms: a thread owns R consecutive outputs of its line and
+// loads the R + span inputs they share once (span = last - first offset <= 8),
+// where the forms above re-read them for every output.  The sum runs over the
+// same terms in the same order (ascending offsets; terms outside the input
+// and zero coefficients skipped; the stencil's s x_{i-2} + x_i), so results
+// are bit-identical.  Axis 0: a thread per column and R rows; axis 1: a thread
+// per R consecutive entries of a row.
+template <int R, bool STENCIL>
+__device__ __forceinline__ void window_rows(const BandSpec& bs, int r0, int len_in, int len_out,
+                                            const double (&w)[R + 8], double (&y)[R])
+{
+    const int o0 = STENCIL ? -2 : bs.off[0];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = r0 + i;
+        double acc = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int d = 0; d < kMaxDiag; ++d) {
+            double a;
+            if (STENCIL) {
+                if (d != 0 && d != 2) continue;
+                if (d == 0) {
+                    if (r < 2) continue;
+                    a = bs.vals[r - 2];
+                } else {
+                    a = 1.0;
+                }
+            } else {
+                const int q = bs.qmap[d];
+                if (q < 0 || r >= len_out) continue;
+                a = bs.vals[(long long)q * len_out + r];
+            }
+            const int k = r + o0 + d;
+            if (k < 0 || k >= len_in) continue;
+            if (!STENCIL && a == 0.0) continue;
+            const double tt = (STENCIL && d == 2) ? w[i + d] : a * w[i + d];
+            acc = first ? tt : acc + tt;
+            first = false;
+        }
+        y[i] = acc;
+    }
+}
+
+template <int R, bool STENCIL>
+__global__ void __launch_bounds__(256) banded_axis0_win_kernel(LineGeom g, BandSpec bs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r0 = blockIdx.y * R;
+    const int b = blockIdx.z;
+    if (c >= g.nlines) return;
+    const double* __restrict__ col = g.src + (long long)b * g.src_bs + c;
+    double* __restrict__ out = g.dst + (long long)b * g.dst_bs + c;
+    const int o0 = STENCIL ? -2 : bs.off[0];
+    const int span = STENCIL ? 2 : bs.off[bs.nd - 1] - o0;
+    double w[R + 8];
+#pragma unroll
+    for (int j = 0; j < R + 8; ++j) {
+        const int k = r0 + o0 + j;
+        w[j] = (j <= R - 1 + span && k >= 0 && k < g.len_in) ? col[(long long)k * g.src_ld] : 0.0;
+    }
+    double y[R];
+    window_rows<R, STENCIL>(bs, r0, g.len_in, g.len_out, w, y);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (r0 + i < g.len_out) out[(long long)(r0 + i) * g.dst_ld] = y[i];
+}
+
+template <int R, bool STENCIL>
+__global__ void __launch_bounds__(256) banded_axis1_win_kernel(LineGeom g, BandSpec bs)
+{
+    // threads over (row, chunk of R entries), chunks of a row adjacent
+    const int nch = (g.len_out + R - 1) / R;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nch * g.nlines) return;
+    const int c = (int)(t / nch);
+    const int r0 = (int)(t - (long long)c * nch) * R;
+    const int b = blockIdx.z;
+    const double* __restrict__ row = g.src + (long long)b * g.src_bs + (long long)c * g.src_ld;
+    double* __restrict__ out = g.dst + (long long)b * g.dst_bs + (long long)c * g.dst_ld;
+    const int o0 = STENCIL ? -2 : bs.off[0];
+    const int span = STENCIL ? 2 : bs.off[bs.nd - 1] - o0;
+    double w[R + 8];
+#pragma unroll
+    for (int j = 0; j < R + 8; ++j) {
+        const int k = r0 + o0 + j;
+        w[j] = (j <= R - 1 + span && k >= 0 && k < g.len_in) ? row[k] : 0.0;
+    }
+    double y[R];
+    window_rows<R, STENCIL>(bs, r0, g.len_in, g.len_out, w, y);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (r0 + i < g.len_out) out[r0 + i] = y[i];
+}
+
+template <int R>
+static void launch_window(const LineGeom& g, const BandSpec& bs, void* stream)
+{
+    if (g.axis == 0) {
+        dim3 grid((g.nlines + 255) / 256, (g.len_out + R - 1) / R, g.batch);
+        if (bs.stencil) hipLaunchKernelGGL((banded_axis0_win_kernel<R, true>), grid, dim3(256), 0, as_stream(stream), g, bs);
+        else hipLaunchKernelGGL((banded_axis0_win_kernel<R, false>), grid, dim3(256), 0, as_stream(stream), g, bs);
+    } else {
+        const long long threads = (long long)((g.len_out + R - 1) / R) * g.nlines;
+        dim3 grid((unsigned)((threads + 255) / 256), 1, g.batch);
+        if (bs.stencil) hipLaunchKernelGGL((banded_axis1_win_kernel<R, true>), grid, dim3(256), 0, as_stream(stream), g, bs);
+        else hipLaunchKernelGGL((banded_axis1_win_kernel<R, false>), grid, dim3(256), 0, as_stream(stream), g, bs);
+    }
+}
+
 static int launch_banded(LineGeom g, const BandSpec& bs, int hint, void* stream, const char* name)
 {
     if ((long long)g.len_out * g.nlines * g.batch == 0) return OK;
+    if (hint == 18 || hint == 20 || hint == 24) {
+        // register-window forms, R = hint - 16 outputs per thread
+        PYPDE_REQUIRE(g.batch <= 65535 && g.len_out <= 65535 * 256 && g.nlines <= 65535 * 256 &&
+                          (long long)g.len_out * g.nlines < (1LL << 38),
+                      "%s: grid too large", name);
+        if (hint == 18) launch_window<2>(g, bs, stream);
+        else if (hint == 20) launch_window<4>(g, bs, stream);
+        else launch_window<8>(g, bs, stream);
+        return check_launch(name);
+    }
     g.W = std::min(kMaxRowsPerThread, std::max(1, hint));
     const int kRowsPerThread = g.W, kRowsPerBlock1 = g.W;
     if (g.axis == 0) {
@@ -558,10 +679,12 @@ extern "C" int pypde_banded_apply(const pypde_lines* in, int ndiag, const int* o
     BandSpec bs{};
     bs.vals = vals;
     bs.nd = ndiag;
+    for (int d = 0; d < kMaxDiag; ++d) bs.qmap[d] = -1;
     for (int q = 0; q < ndiag; ++q) {
         PYPDE_REQUIRE(offsets_host[q] >= -4 && offsets_host[q] <= 4, "offset out of [-4,4]");
         if (q) PYPDE_REQUIRE(offsets_host[q] > offsets_host[q - 1], "offsets must ascend");
         bs.off[q] = offsets_host[q];
+        bs.qmap[offsets_host[q] - offsets_host[0]] = q;
     }
     return launch_banded(g, bs, in->lines_per_block, stream, "pypde_banded_apply");
 }

@@ -434,3 +434,38 @@ def test_gemm_grouped_rejects_mixed_members(amd):
     for count in (0, 5):
         with pytest.raises(PypdeError):
             ops.check(lib.pypde_dgemm_grouped(ctypes.byref(arr), count, s), "grouped")
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_banded_every_row_hint_bit_exact(amd, axis):
+    """Every row hint of the banded and stencil applies (1..8 outputs per
+    thread / rows per block, 18/20/24 the register-window forms) gives results
+    bit-identical to the oracle: B (offsets 0, 2, 4), B S, a band with negative
+    offsets and a gap, and the ChebDirichlet / ChebNeumann stencils."""
+    pypde_amd, ops, torch = amd
+    from oracle import pypde_oracle as o
+    rng = np.random.default_rng(50 + axis)
+    N, other = 41, 37
+    ob = o.Basis(N, "CD")
+    B, BS, _ = o.hholtz_matrices(ob, 0.02, 0.5)
+    G = np.zeros((N - 3, N))
+    for i in range(N - 3):
+        for off in (-4, -1, 0, 3):
+            if 0 <= i + off < N:
+                G[i, i + off] = rng.standard_normal()
+    for Mat in (B, BS, G):
+        nin = Mat.shape[1]
+        x = rng.standard_normal((nin, other) if axis == 0 else (other, nin))
+        ref = o.MatAxis(Mat, axis)(x)
+        p = pypde_amd.PlanRHS(Mat, ndim=2, axis=axis)
+        assert p.kind == "banded"
+        for hint in (1, 2, 4, 8, 18, 20, 24):
+            y = host(ops.banded_apply(p.offsets, p.vals, ops.to_dev(x), axis, p.M_out, lpb=hint))
+            assert np.array_equal(y, ref), (Mat.shape, hint)
+    for key in ("CD", "CN"):
+        b, obk = pypde_amd.Base(N, key), o.Basis(N, key)
+        v = rng.standard_normal((N - 2, other) if axis == 0 else (other, N - 2))
+        ref = o._along(obk.to_chebyshev, v, axis)
+        for hint in (1, 2, 4, 8, 18, 20, 24):
+            u = host(ops.stencil_apply(b.stencil_vector(), ops.to_dev(v), axis, lpb=hint))
+            assert np.array_equal(u, ref), (key, hint)
