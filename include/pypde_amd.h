@@ -35,7 +35,12 @@ typedef struct pypde_lines {
                                  the banded / stencil applies 1..8 outputs
                                  per thread (axis 0) or rows per block
                                  (axis 1), 18 / 20 / 24 the register-window
-                                 form with 2 / 4 / 8 outputs per thread */
+                                 form with 2 / 4 / 8 outputs per thread,
+                                 33..40 (axis 1 only) the coefficient-
+                                 resident form: 33..36 four rows in
+                                 flight and 4 / 8 / 16 / 32 rows per
+                                 block, 37..40 eight rows in flight and
+                                 8 / 16 / 32 / 64 rows per block */
 } pypde_lines;
 
 /* ---- library ---------------------------------------------------------- */

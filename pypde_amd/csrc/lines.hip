@@ -603,6 +603,66 @@ __global__ void __launch_bounds__(256) banded_axis1_win_kernel(LineGeom g, BandS
         if (r0 + i < g.len_out) out[r0 + i] = y[i];
 }
 
+// Axis 1, coefficient-resident form: a thread owns entry r of the output rows,
+// keeps the coefficients of output r (and which terms exist) in registers, and
+// runs over g.W rows, U at a time with their loads issued together.  Each
+// load of a wave is 64 consecutive entries of a row shifted by one band
+// offset, so all accesses coalesce and the shifted copies hit in cache.  The
+// sum is band_row's, term for term.
+template <int U, bool STENCIL>
+__global__ void __launch_bounds__(256) banded_axis1_res_kernel(LineGeom g, BandSpec bs)
+{
+    constexpr int ND = STENCIL ? 2 : kMaxDiag;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.z;
+    if (r >= g.len_out) return;
+    double a[ND];
+    int off[ND];
+    bool use[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        if (STENCIL) {
+            // term 0: s_{r-2} x_{r-2} (r >= 2); term 1: x_r (r < len_in)
+            off[q] = q == 0 ? -2 : 0;
+            use[q] = q == 0 ? r >= 2 : r < g.len_in;
+            a[q] = (q == 0 && use[q]) ? bs.vals[r - 2] : 1.0;
+        } else {
+            const int k = r + bs.off[q];
+            off[q] = q < bs.nd ? bs.off[q] : 0;
+            use[q] = q < bs.nd && k >= 0 && k < g.len_in;
+            a[q] = use[q] ? bs.vals[(long long)q * g.len_out + r] : 0.0;
+            use[q] = use[q] && a[q] != 0.0;
+        }
+    }
+    const int c0 = blockIdx.y * g.W;
+    const int c1 = min(c0 + g.W, g.nlines);
+    const double* __restrict__ src = g.src + (long long)b * g.src_bs + r;
+    double* __restrict__ dst = g.dst + (long long)b * g.dst_bs + r;
+    for (int c = c0; c < c1; c += U) {
+        double x[U][ND];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double* row = src + (long long)min(c + u, c1 - 1) * g.src_ld;
+#pragma unroll
+            for (int q = 0; q < ND; ++q) x[u][q] = use[q] ? row[off[q]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (c + u >= c1) break;
+            double acc = 0.0;
+            bool first = true;
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                if (!use[q]) continue;
+                const double tt = (STENCIL && q == 1) ? x[u][q] : a[q] * x[u][q];
+                acc = first ? tt : acc + tt;
+                first = false;
+            }
+            dst[(long long)(c + u) * g.dst_ld] = acc;
+        }
+    }
+}
+
 template <int R>
 static void launch_window(const LineGeom& g, const BandSpec& bs, void* stream)
 {
@@ -621,6 +681,25 @@ static void launch_window(const LineGeom& g, const BandSpec& bs, void* stream)
 static int launch_banded(LineGeom g, const BandSpec& bs, int hint, void* stream, const char* name)
 {
     if ((long long)g.len_out * g.nlines * g.batch == 0) return OK;
+    if (hint >= 33 && hint <= 40) {
+        // coefficient-resident form along rows: 33..36 four rows in flight and
+        // 4, 8, 16, 32 rows per block; 37..40 eight rows and 8, 16, 32, 64
+        PYPDE_REQUIRE(g.axis == 1, "%s: row hints 33..40 are for axis 1", name);
+        const bool u8 = hint >= 37;
+        g.W = (u8 ? 8 : 4) << ((hint - 33) & 3);
+        PYPDE_REQUIRE(g.batch <= 65535 && (g.nlines + g.W - 1) / g.W <= 65535,
+                      "%s: grid too large", name);
+        dim3 grid((g.len_out + 255) / 256, (g.nlines + g.W - 1) / g.W, g.batch);
+        hipStream_t st = as_stream(stream);
+        if (bs.stencil) {
+            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, true>), grid, dim3(256), 0, st, g, bs);
+            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, true>), grid, dim3(256), 0, st, g, bs);
+        } else {
+            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, false>), grid, dim3(256), 0, st, g, bs);
+            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, false>), grid, dim3(256), 0, st, g, bs);
+        }
+        return check_launch(name);
+    }
     if (hint == 18 || hint == 20 || hint == 24) {
         // register-window forms, R = hint - 16 outputs per thread
         PYPDE_REQUIRE(g.batch <= 65535 && g.len_out <= 65535 * 256 && g.nlines <= 65535 * 256 &&

@@ -439,13 +439,16 @@ def test_gemm_grouped_rejects_mixed_members(amd):
 @pytest.mark.parametrize("axis", [0, 1])
 def test_banded_every_row_hint_bit_exact(amd, axis):
     """Every row hint of the banded and stencil applies (1..8 outputs per
-    thread / rows per block, 18/20/24 the register-window forms) gives results
+    thread / rows per block, 18/20/24 the register-window forms, 33..40 the
+    coefficient-resident form along rows, with row counts that leave partial
+    groups of four and partial blocks) gives results
     bit-identical to the oracle: B (offsets 0, 2, 4), B S, a band with negative
     offsets and a gap, and the ChebDirichlet / ChebNeumann stencils."""
     pypde_amd, ops, torch = amd
     from oracle import pypde_oracle as o
     rng = np.random.default_rng(50 + axis)
     N, other = 41, 37
+    hints = (1, 2, 4, 8, 18, 20, 24) + (tuple(range(33, 41)) if axis == 1 else ())
     ob = o.Basis(N, "CD")
     B, BS, _ = o.hholtz_matrices(ob, 0.02, 0.5)
     G = np.zeros((N - 3, N))
@@ -459,13 +462,24 @@ def test_banded_every_row_hint_bit_exact(amd, axis):
         ref = o.MatAxis(Mat, axis)(x)
         p = pypde_amd.PlanRHS(Mat, ndim=2, axis=axis)
         assert p.kind == "banded"
-        for hint in (1, 2, 4, 8, 18, 20, 24):
+        for hint in hints:
             y = host(ops.banded_apply(p.offsets, p.vals, ops.to_dev(x), axis, p.M_out, lpb=hint))
             assert np.array_equal(y, ref), (Mat.shape, hint)
+        # a batch of three arrays
+        x3 = rng.standard_normal((3,) + x.shape)
+        ref3 = np.stack([o.MatAxis(Mat, axis)(a) for a in x3])
+        for hint in hints:
+            y = host(ops.banded_apply(p.offsets, p.vals, ops.to_dev(x3), axis, p.M_out,
+                                      lpb=hint))
+            assert np.array_equal(y, ref3), (Mat.shape, hint, "batch")
     for key in ("CD", "CN"):
         b, obk = pypde_amd.Base(N, key), o.Basis(N, key)
         v = rng.standard_normal((N - 2, other) if axis == 0 else (other, N - 2))
         ref = o._along(obk.to_chebyshev, v, axis)
-        for hint in (1, 2, 4, 8, 18, 20, 24):
+        for hint in hints:
             u = host(ops.stencil_apply(b.stencil_vector(), ops.to_dev(v), axis, lpb=hint))
             assert np.array_equal(u, ref), (key, hint)
+    if axis == 0:
+        from pypde_amd._lib import PypdeError
+        with pytest.raises(PypdeError):
+            ops.banded_apply(p.offsets, p.vals, ops.to_dev(x), 0, p.M_out, lpb=33)
