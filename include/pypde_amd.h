@@ -216,8 +216,10 @@ int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, con
  * variant: 1 = block form (64 lines per block, the line split into chunks over
  * up to 16 waves; for many lines), 2 = lane-chunk form (one wave per line, a
  * chunk of n/64 positions per lane, carries combined by wave scans; for few
- * lines), 0 = chosen by the number of lines (lane-chunk up to 2048 lines).
- * per_line = 2 implies the lane-chunk form, per_line = 1 the block form.
+ * lines), 3 = block form with 32 lines per block (half-waves hold neighbouring
+ * chunks; half the LDS, so two blocks share a CU), 0 = chosen by the number of
+ * lines (lane-chunk up to 2048 lines, else variant 1).
+ * per_line = 2 implies the lane-chunk form, per_line = 1 a block form.
  * Carries: the four-diagonal Helmholtz sweeps (Plan_fdma.solve, plans.py:184-224
  * -> fdma.f90:40-98), the Poisson line solves (Plan_Poisson.solve_fortran,
  * plans.py:310-314 -> fdma.f90:146-195, factors formed once at plan time), the

@@ -30,11 +30,12 @@ namespace pypde {
 
 // Positions per chunk (per lane): 16 halves the serial chain of each wave and
 // the registers it holds, and is faster for the four-diagonal and per-line
-// Poisson forms up to n = 256 (16 waves); the pre-stencil form and longer lines
-// use 32.
+// Poisson forms up to n = 256 (16 waves); the pre-stencil form in 64-line
+// blocks and longer lines use 32.
 constexpr int kMaxWaves = 16;
 constexpr int TW = 16;    // row-tile width for axis-1 transposes
 constexpr int TP = TW + 1;
+constexpr int TP32 = 33;  // pitch of the 32 x 32 tiles of the 32-line form
 
 struct RecArgs {
     const double* src;
@@ -75,7 +76,9 @@ struct Tab {
 
 // Sweeps over one chunk held in registers.  GUARD: skip positions >= n (only
 // the last chunk of a line needs it).
-template <bool GUARD, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int NX>
+// P chunks per line, w = this lane's chunk, lane = its line within the block;
+// the exchange area holds LS lines per slot.
+template <bool GUARD, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int LS, int NX>
 __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF, FWD, HASR, HASNU>& T,
                                        int base, int n,
                                        int P, int w, int lane, double* xch)
@@ -100,16 +103,16 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
             }
             if (PERLINE && (k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
-        xch[(w * 4 + 0) * 64 + lane] = ye[0];
-        xch[(w * 4 + 1) * 64 + lane] = ye[1];
-        xch[(w * 4 + 2) * 64 + lane] = pe[0];
-        xch[(w * 4 + 3) * 64 + lane] = pe[1];
+        xch[(w * 4 + 0) * LS + lane] = ye[0];
+        xch[(w * 4 + 1) * LS + lane] = ye[1];
+        xch[(w * 4 + 2) * LS + lane] = pe[0];
+        xch[(w * 4 + 3) * LS + lane] = pe[1];
         __syncthreads();
         double y[2] = {0.0, 0.0};
         for (int u = 0; u < w; ++u) {
 #pragma unroll
             for (int p = 0; p < 2; ++p)
-                y[p] = xch[(u * 4 + p) * 64 + lane] + xch[(u * 4 + 2 + p) * 64 + lane] * y[p];
+                y[p] = xch[(u * 4 + p) * LS + lane] + xch[(u * 4 + 2 + p) * LS + lane] * y[p];
         }
         __syncthreads();
 #pragma unroll
@@ -144,18 +147,18 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
     }
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        double* o = xch + ((w * 2 + p) * 6) * 64 + lane;
-        o[0] = s1[p]; o[64] = a1[p]; o[128] = b1[p];
-        o[192] = s2[p]; o[256] = a2[p]; o[320] = b2[p];
+        double* o = xch + ((w * 2 + p) * 6) * LS + lane;
+        o[0] = s1[p]; o[LS] = a1[p]; o[2 * LS] = b1[p];
+        o[3 * LS] = s2[p]; o[4 * LS] = a2[p]; o[5 * LS] = b2[p];
     }
     __syncthreads();
     double Z1[2] = {0.0, 0.0}, Z2[2] = {0.0, 0.0};
     for (int u = P - 1; u > w; --u) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const double* o = xch + ((u * 2 + p) * 6) * 64 + lane;
-            const double z1 = o[0] + o[64] * Z1[p] + o[128] * Z2[p];
-            const double z2 = o[192] + o[256] * Z1[p] + o[320] * Z2[p];
+            const double* o = xch + ((u * 2 + p) * 6) * LS + lane;
+            const double z1 = o[0] + o[LS] * Z1[p] + o[2 * LS] * Z2[p];
+            const double z2 = o[3 * LS] + o[4 * LS] * Z1[p] + o[5 * LS] * Z2[p];
             Z1[p] = z1;
             Z2[p] = z2;
         }
@@ -178,25 +181,41 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
 // Occupancy hint: the pre-stencil (tri-diagonal projection) form runs best
 // capped at 128 VGPRs (4 waves per SIMD, two 8-wave blocks per CU); the other
 // forms spill under that cap and run best uncapped (measured at n = 254).
-template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
-__global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs a)
+//
+// LPB = lines per block.  64: lane = line, wave = chunk (P waves).  32: the two
+// half-waves hold two neighbouring chunks of the same 32 lines (ceil(P / 2)
+// waves), which halves the block's LDS and threads so that two blocks share a
+// CU and one block's loads overlap another's sweeps; accesses along columns
+// are two 256-byte row segments per load, and along rows each wave transposes
+// 32 x 32 tiles.
+template <int CH, bool PRE, bool PERLINE, int LPB>
+constexpr int rec_min_waves()
+{
+    return (PRE || (LPB == 32 && CH == 16 && !PERLINE)) ? 4 : 1;
+}
+
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT,
+          int LPB>
+__global__ void __launch_bounds__(MAXT, (rec_min_waves<CH, PRE, PERLINE, LPB>())) recurrence_kernel(RecArgs a)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int P = blockDim.x >> 6;
-    const int bpe = (a.nlines + 63) / 64;
+    const int wv = threadIdx.x >> 6;
+    const int li = LPB == 64 ? lane : (lane & 31);            // line within the block
+    const int w = LPB == 64 ? wv : 2 * wv + (lane >> 5);       // chunk
+    const int P = LPB == 64 ? (int)(blockDim.x >> 6) : 2 * (int)(blockDim.x >> 6);
+    const int bpe = (a.nlines + LPB - 1) / LPB;
     const int b = blockIdx.x / bpe;
-    const int l0 = (blockIdx.x - b * bpe) * 64;
-    const int line = l0 + lane;
+    const int l0 = (blockIdx.x - b * bpe) * LPB;
+    const int line = l0 + li;
     const bool live = line < a.nlines;
     const int base = w * CH;
     const double* __restrict__ src = a.src + (long long)b * a.src_bs;
     double* __restrict__ dst = a.dst + (long long)b * a.dst_bs;
     const int n = a.n;
 
-    // LDS: [present tables, n each (shared mode)] [transpose tiles P*64*TP (axis 1), reused
-    // as the carry-exchange area P*12*64 during the sweeps]
+    // LDS: [present tables, n each (shared mode)] [transpose tiles (axis 1), reused
+    // as the carry-exchange area P*12*LPB during the sweeps]
     double* ltab = lds;
     const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
                      (a.NU != nullptr) + (a.pre != nullptr);
@@ -217,8 +236,6 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
     T.line = live ? line : 0;
     T.n = n;
     T.g = PERLINE ? nullptr : ltab;
-    // per-line tables: one contiguous [6][n][nlines] view is not assumed; the
-    // slots are separate arrays, so route through a small pointer table
     constexpr int NX = PRE ? CH + 2 : CH;
     double v[NX];
 
@@ -228,8 +245,8 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
             const int pos = base + k;
             v[k] = (live && pos < a.n_in) ? src[(long long)pos * a.src_ld + line] : 0.0;
         }
-    } else {
-        double* tile = tiles + w * 64 * TP;
+    } else if constexpr (LPB == 64) {
+        double* tile = tiles + wv * 64 * TP;
 #pragma unroll
         for (int t0 = 0; t0 < CH; t0 += TW) {
 #pragma unroll
@@ -245,6 +262,33 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
             for (int c = 0; c < TW; ++c) v[t0 + c] = tile[lane * TP + c];
             wave_lds_sync();
         }
+    } else {
+        // 32 lines x 32 positions per round; CH = 16: one round holds both
+        // half-waves' chunks, CH = 32: round h holds half-wave h's chunk
+        double* tile = tiles + wv * 32 * TP32;
+        const int wbase = wv * 2 * CH;
+#pragma unroll
+        for (int rr = 0; rr < CH / 16; ++rr) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int r = (lane >> 5) + 2 * q;
+                const int c = lane & 31;
+                const int pos = wbase + rr * 32 + c;
+                const int ln = l0 + r;
+                tile[r * TP32 + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
+            }
+            wave_lds_sync();
+            if (CH == 16) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v[k] = tile[li * TP32 + (lane >> 5) * 16 + k];
+            } else if ((lane >> 5) == rr) {
+#pragma unroll
+                for (int k = 0; k < CH && k < 32; ++k) v[k] = tile[li * TP32 + k];
+            }
+            wave_lds_sync();
+        }
+    }
+    if constexpr (AXIS == 1) {
         // the pre-stencil's two look-ahead positions: direct loads
 #pragma unroll
         for (int k = CH; k < NX; ++k) {
@@ -256,6 +300,9 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
     // reused for the carry exchange
     __syncthreads();
 
+    // unguarded sweeps when every chunk of the wave lies inside the line (a
+    // wave-uniform test: the sweeps contain block barriers)
+    const bool inside = LPB == 64 ? base + CH <= n : (2 * wv + 2) * CH <= n;
     if constexpr (PERLINE) {
         // per-line mode only ever uses LM, R, MU, NU (Poisson): address them
         // through a 4-slot layout [slot][n][nlines] starting at LM
@@ -264,11 +311,11 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
         TL.line = live ? line : 0;
         TL.n = n;
         TL.g = a.LM + (live ? line : 0);  // slot s of this lane at g + (s-1)*n*nlines
-        if (base + CH <= n) sweeps<false, PRE, FWD, HASF, HASR, HASNU, true>(v, TL, base, n, P, w, lane, xch);
-        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, true>(v, TL, base, n, P, w, lane, xch);
+        if (inside) sweeps<false, PRE, FWD, HASF, HASR, HASNU, true, LPB>(v, TL, base, n, P, w, li, xch);
+        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, true, LPB>(v, TL, base, n, P, w, li, xch);
     } else {
-        if (base + CH <= n) sweeps<false, PRE, FWD, HASF, HASR, HASNU, false>(v, T, base, n, P, w, lane, xch);
-        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, false>(v, T, base, n, P, w, lane, xch);
+        if (inside) sweeps<false, PRE, FWD, HASF, HASR, HASNU, false, LPB>(v, T, base, n, P, w, li, xch);
+        else sweeps<true, PRE, FWD, HASF, HASR, HASNU, false, LPB>(v, T, base, n, P, w, li, xch);
     }
 
     if constexpr (AXIS == 0) {
@@ -277,8 +324,8 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
             const int pos = base + k;
             if (live && pos < a.n) dst[(long long)pos * a.dst_ld + line] = v[k];
         }
-    } else {
-        double* tile = tiles + w * 64 * TP;
+    } else if constexpr (LPB == 64) {
+        double* tile = tiles + wv * 64 * TP;
 #pragma unroll
         for (int t0 = 0; t0 < CH; t0 += TW) {
 #pragma unroll
@@ -294,58 +341,95 @@ __global__ void __launch_bounds__(MAXT, (PRE ? 4 : 1)) recurrence_kernel(RecArgs
             }
             wave_lds_sync();
         }
+    } else {
+        double* tile = tiles + wv * 32 * TP32;
+        const int wbase = wv * 2 * CH;
+#pragma unroll
+        for (int rr = 0; rr < CH / 16; ++rr) {
+            if (CH == 16) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) tile[li * TP32 + (lane >> 5) * 16 + k] = v[k];
+            } else if ((lane >> 5) == rr) {
+#pragma unroll
+                for (int k = 0; k < CH && k < 32; ++k) tile[li * TP32 + k] = v[k];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int r = (lane >> 5) + 2 * q;
+                const int c = lane & 31;
+                const int pos = wbase + rr * 32 + c;
+                const int ln = l0 + r;
+                if (ln < a.nlines && pos < a.n) dst[(long long)ln * a.dst_ld + pos] = tile[r * TP32 + c];
+            }
+            wave_lds_sync();
+        }
     }
 }
 
-template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT>
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT,
+          int LPB>
 static int launch_rec_t(const RecArgs& a, hipStream_t s)
 {
     const int P = (a.n + CH - 1) / CH;
-    const int bpe = (a.nlines + 63) / 64;
+    const int waves = LPB == 64 ? P : (P + 1) / 2;
+    const int chunks = LPB == 64 ? P : 2 * waves;
+    const int bpe = (a.nlines + LPB - 1) / LPB;
     const long long blocks = (long long)bpe * a.batch;
     if (blocks == 0) return OK;
+    PYPDE_REQUIRE(waves * 64 <= MAXT, "recurrence: %d waves exceed the launch bound", waves);
+    PYPDE_REQUIRE(blocks <= 0x7fffffffLL, "recurrence: %lld blocks exceed the grid", blocks);
     const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
                      (a.NU != nullptr) + (a.pre != nullptr);
     size_t tabs = PERLINE ? 0 : (size_t)ntab * a.n;
-    size_t tiles = AXIS == 1 ? (size_t)P * 64 * TP : 0;
-    size_t xch = (size_t)P * 12 * 64;
+    size_t tiles = AXIS == 1 ? (size_t)waves * (LPB == 64 ? 64 * TP : 32 * TP32) : 0;
+    size_t xch = (size_t)chunks * 12 * LPB;
     const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);
-    auto fn = recurrence_kernel<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT>;
+    auto fn = recurrence_kernel<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT, LPB>;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
     PYPDE_REQUIRE(shmem <= 160 * 1024, "recurrence: LDS %zu too large", shmem);
-    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(P * 64), shmem, s, a);
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(waves * 64), shmem, s, a);
     return check_launch("pypde_recurrence_solve");
 }
 
-// Block size P*64 threads; the launch bound is the smallest that fits so the
-// compiler may give each lane enough registers for its chunk.
-template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
+// Block size (waves * 64 threads); the launch bound is the smallest that fits
+// so the compiler may give each lane enough registers for its chunk.
+template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int LPB>
 static int launch_rec_ch(const RecArgs& a, hipStream_t s)
 {
     const int P = (a.n + CH - 1) / CH;
-    if (P <= 4) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256>(a, s);
-    if (P <= 8) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512>(a, s);
-    return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024>(a, s);
+    const int waves = LPB == 64 ? P : (P + 1) / 2;
+    if (waves <= 4) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 256, LPB>(a, s);
+    if (waves <= 8) return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 512, LPB>(a, s);
+    return launch_rec_t<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, 1024, LPB>(a, s);
 }
 
-// Block size P*64 threads; the launch bound is the smallest that fits so the
-// compiler may give each lane enough registers for its chunk.
-template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
-static int launch_rec(const RecArgs& a, hipStream_t s)
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int LPB>
+static int launch_rec_lpb(const RecArgs& a, hipStream_t s)
 {
     PYPDE_REQUIRE(a.n >= 1 && a.n <= kMaxWaves * 32, "recurrence: line length %d exceeds %d", a.n,
                   kMaxWaves * 32);
-    const bool short16 = !PRE && a.n <= kMaxWaves * 16;
+    // 16 positions per chunk up to n = 256, except for the pre-stencil form in
+    // 64-line blocks (16 waves of it do not fit two blocks per CU)
+    const bool short16 = (!PRE || LPB == 32) && a.n <= kMaxWaves * 16;
     if (a.axis == 0) {
-        if (short16) return launch_rec_ch<0, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
-        return launch_rec_ch<0, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
+        if (short16) return launch_rec_ch<0, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
+        return launch_rec_ch<0, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
     }
-    if (short16) return launch_rec_ch<1, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
-    return launch_rec_ch<1, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE>(a, s);
+    if (short16) return launch_rec_ch<1, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
+    return launch_rec_ch<1, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
+}
+
+// Block form with 64 (half = false) or 32 (half = true) lines per block.
+template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
+static int launch_rec(const RecArgs& a, hipStream_t s, bool half)
+{
+    if (half) return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 32>(a, s);
+    return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 64>(a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -552,7 +636,7 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                   "recurrence: len_in must be len_out%s", pre ? " + 2" : "");
     PYPDE_REQUIRE(!(g->src == g->dst && pre), "pre-stencil cannot run in place");
     PYPDE_REQUIRE(r->per_line >= 0 && r->per_line <= 2, "per_line must be 0, 1 or 2");
-    PYPDE_REQUIRE(r->variant >= 0 && r->variant <= 2, "variant must be 0, 1 or 2");
+    PYPDE_REQUIRE(r->variant >= 0 && r->variant <= 3, "variant must be 0, 1, 2 or 3");
     RecArgs a{g->src, g->dst, g->src_ld, g->dst_ld, g->src_bs, g->dst_bs, g->len_in, g->len_out,
               g->nlines, g->batch, g->axis, r->F, r->LM, r->R, r->MU, r->NU, r->pre,
               r->entry_stride, r->per_line};
@@ -562,6 +646,8 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
     // lane-chunk form: asked for, required by line-major tables, or (variant 0)
     // chosen when the launch has few lines
     const long long lines = (long long)g->nlines * g->batch;
+    PYPDE_REQUIRE(!(r->variant == 3 && r->per_line == 2),
+                  "the block forms read per-line tables [n][nlines] (per_line = 1)");
     const bool lane = r->variant == 2 || r->per_line == 2 ||
                       (r->variant == 0 && r->per_line == 0 && lines <= 2048);
     if (lane) {
@@ -579,6 +665,8 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                   (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
         return ERR_UNSUP;
     }
+    // block form: 64 lines per block (variant 0 / 1) or 32 (variant 3)
+    const bool half = r->variant == 3;
     // the combinations used by the plans
     if (r->per_line) {
         PYPDE_REQUIRE(!pre && fwd && !hf && hr && hn,
@@ -587,12 +675,12 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                       r->MU == r->R + (long long)g->len_out * g->nlines &&
                       r->NU == r->MU + (long long)g->len_out * g->nlines,
                       "per-line tables must be contiguous slots LM | R | MU | NU of [n][nlines]");
-        return launch_rec<false, true, false, true, true, true>(a, s);
+        return launch_rec<false, true, false, true, true, true>(a, s, half);
     }
-    if (!pre && fwd && !hf && hr && hn) return launch_rec<false, true, false, true, true, false>(a, s);
-    if (pre && fwd && hf && !hr && !hn) return launch_rec<true, true, true, false, false, false>(a, s);
-    if (!pre && !fwd && !hf && hr && !hn) return launch_rec<false, false, false, true, false, false>(a, s);
-    if (!pre && fwd && hf && hr && hn) return launch_rec<false, true, true, true, true, false>(a, s);
+    if (!pre && fwd && !hf && hr && hn) return launch_rec<false, true, false, true, true, false>(a, s, half);
+    if (pre && fwd && hf && !hr && !hn) return launch_rec<true, true, true, false, false, false>(a, s, half);
+    if (!pre && !fwd && !hf && hr && !hn) return launch_rec<false, false, false, true, false, false>(a, s, half);
+    if (!pre && fwd && hf && hr && hn) return launch_rec<false, true, true, true, true, false>(a, s, half);
     set_error("pypde_recurrence_solve: unsupported table combination (pre=%d F=%d LM=%d R=%d NU=%d)",
               (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
     return ERR_UNSUP;

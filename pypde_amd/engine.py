@@ -262,9 +262,14 @@ class _Call:
         check(self.fn(*self.args, stream), self.name)
 
 
+# recurrence kernel forms: 1 block (64 lines per block), 2 lane-chunk,
+# 3 block with 32 lines per block
+REC_VARIANTS = (1, 2, 3)
+
+
 class _RecCall(_Call):
-    """A recurrence launch with its block-form (1) and lane-chunk (2) structs;
-    the engine's tuner picks one."""
+    """A recurrence launch with a struct per kernel form (REC_VARIANTS); the
+    engine's tuner picks one."""
 
     def __init__(self, name, g, structs, variant):
         self.g, self.structs = g, structs
@@ -661,8 +666,8 @@ class RBCBatch:
         # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
-        rx = {v: _rec(p(self.st_rhx[st]), M0, 4 * M0, v) for v in (1, 2)}
-        ry = {v: _rec(p(self.st_rhy[st]), M1, 4 * M1, v) for v in (1, 2)}
+        rx = {v: _rec(p(self.st_rhx[st]), M0, 4 * M0, v) for v in REC_VARIANTS}
+        ry = {v: _rec(p(self.st_rhy[st]), M1, 4 * M1, v) for v in REC_VARIANTS}
         L.append(_RecCall("helmholtz_x" + tag, gx, rx, _rec_default(3 * B * M1)))
         L.append(_RecCall("helmholtz_y" + tag, gy, ry, _rec_default(3 * B * M0)))
         # 11  W2_f = X*_f [C ; H_y C]_f^T  (f = U with C_y, V with Cd_y): the y factors
@@ -681,7 +686,7 @@ class RBCBatch:
         # 15  per-line eigen-split solves along x
         gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
         rp = {}
-        for v, tabs, layout in ((1, self.prec, 1), (2, self.precT, 2)):
+        for v, tabs, layout in ((1, self.prec, 1), (2, self.precT, 2), (3, self.prec, 1)):
             r = Recurrence()
             r.LM, r.R, r.MU, r.NU = (tabs[i].data_ptr() for i in range(4))
             r.per_line, r.entry_stride, r.variant = layout, 0, v
@@ -835,7 +840,7 @@ class RBCBatch:
                     check(lib.pypde_recurrence_solve(ctypes.byref(launch.g),
                                                      ctypes.byref(launch.structs[v]), s),
                           launch.name)
-                launch.set_variant(autotune.choose(launch.key(), run, (1, 2),
+                launch.set_variant(autotune.choose(launch.key(), run, REC_VARIANTS,
                                                    default=launch.variant))
         for launch in self.launches:
             if isinstance(launch, _GemmGroup):

@@ -311,7 +311,7 @@ def stencil_apply(s, x, axis=0, lpb=0):
     if lpb == 0:
         from . import autotune
         g.lines_per_block = autotune.choose(("stencil", axis, M, g.nlines, g.batch), run,
-                                            autotune.ROWS_CANDIDATES, default=1)
+                                            autotune.rows_candidates(axis), default=1)
     run(g.lines_per_block)
     return out
 
@@ -359,7 +359,7 @@ def banded_apply(offsets, vals, x, axis=0, len_out=None, lpb=0):
         from . import autotune
         g.lines_per_block = autotune.choose(
             ("banded", axis, n_in, n_out, g.nlines, g.batch, tuple(int(o) for o in offsets)),
-            run, autotune.ROWS_CANDIDATES, default=1)
+            run, autotune.rows_candidates(axis), default=1)
     run(g.lines_per_block)
     return out
 
@@ -444,9 +444,10 @@ class RecTables:
         return tuple(self.t[k] is not None for k in ("F", "LM", "R", "NU", "pre"))
 
     def struct(self, variant=0, lines=0):
-        """The C struct for kernel variant 1 (block) or 2 (lane-chunk); 0 lets
-        the library choose, except for per-line tables, whose layout follows
-        the variant (lane-chunk up to 2048 lines)."""
+        """The C struct for kernel variant 1 (block, 64 lines per block), 2
+        (lane-chunk) or 3 (block, 32 lines per block); 0 lets the library
+        choose, except for per-line tables, whose layout follows the variant
+        (lane-chunk up to 2048 lines)."""
         from ._lib import Recurrence
         r = Recurrence()
         tabs = self.t
@@ -464,7 +465,7 @@ class RecTables:
         return r
 
 
-REC_CANDIDATES = (1, 2)
+REC_CANDIDATES = (1, 2, 3)
 
 
 def recurrence_solve(tabs, x, axis=0, out=None, lpb=0, variant=None):

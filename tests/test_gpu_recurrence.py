@@ -1,5 +1,6 @@
-"""Both kernel forms of pypde_recurrence_solve (block form, variant 1;
-lane-chunk form, variant 2) against a sequential evaluation of the normalised
+"""The kernel forms of pypde_recurrence_solve (block form with 64 lines per
+block, variant 1; lane-chunk form, variant 2; block form with 32 lines per
+block, variant 3) against a sequential evaluation of the normalised
 recurrence, for every table combination the plans use, both axes, line lengths
 around the chunk sizes of either form, per-entry and per-line tables (both
 layouts), in place and out of place."""
@@ -77,13 +78,13 @@ def run(ops, lib, Recurrence, x, out, axis, n_in, n, dev, variant, per_line, ent
 
 @pytest.mark.parametrize("form", sorted(FORMS))
 @pytest.mark.parametrize("axis", [0, 1])
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_shared_and_per_entry_tables(form, axis, variant):
     import torch
     from pypde_amd import ops
     from pypde_amd._lib import Recurrence, lib
     names = FORMS[form]
-    rng = np.random.default_rng(sorted(FORMS).index(form) * 4 + axis * 2 + variant)
+    rng = np.random.default_rng(sorted(FORMS).index(form) * 8 + axis * 4 + variant)
     for n in (3, 5, 64, 127, 128, 129, 200, 256, 257, 400, 512):
         pre = "pre" in names
         n_in = n + 2 if pre else n
@@ -121,9 +122,9 @@ def test_shared_and_per_entry_tables(form, axis, variant):
                 assert rel(ge, reference(xe, t1, n)) < 1e-13, (form, axis, variant, n, "inplace")
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_per_line_tables(variant):
-    """The Poisson form with tables per line: [n][nlines] for the block form,
+    """The Poisson form with tables per line: [n][nlines] for the block forms,
     line-major [nlines][n] for the lane-chunk form (axis 0, lines = columns)."""
     import torch
     from pypde_amd import ops
@@ -132,7 +133,7 @@ def test_per_line_tables(variant):
     names = FORMS["fdma"]
     for n, nl in ((3, 5), (64, 64), (129, 70), (254, 254), (400, 33), (512, 9)):
         t = tables(rng, n, names, lead=(nl,))          # [nlines][n]
-        if variant == 1:
+        if variant in (1, 3):
             block = ops.to_dev(np.stack([t[k].T for k in names]))   # [4][n][nlines]
             layout = 1
         else:
@@ -156,6 +157,6 @@ def test_layout_and_variant_checks():
     x = torch.zeros((8, 4), dtype=torch.float64, device="cuda")
     t = torch.zeros((4, 8, 4), dtype=torch.float64, device="cuda")
     dev = {k: t[j] for j, k in enumerate(FORMS["fdma"])}
-    for per_line, variant in ((1, 2), (3, 0), (0, 5)):
+    for per_line, variant in ((1, 2), (3, 0), (0, 5), (2, 3), (0, 4)):
         with pytest.raises(PypdeError):
             run(ops, lib, Recurrence, x, x, 0, 8, 8, dev, variant, per_line)
