@@ -536,6 +536,7 @@ def main():
                      "kernel": plain["dominant"], "avg_us": round(plain["avg_s"] * 1e6, 3)},
         "cpu_baseline": cpu,
         "kernels_us": plain["kernels"],
+        "launch_variants": plain["engine"].variants(),
         "eager_ms_per_step": round(plain["eager_sec"] / args.steps * 1e3, 4),
         "nu_final": plain["nu"],
         "finite": plain["finite"],

@@ -81,7 +81,9 @@ typedef struct pypde_gemm_desc {
     int variant; /* kernel choice: 0 size heuristic; 1 LDS-tiled 64x64; 2 LDS-tiled
                     32x32; 3 LDS-tiled 128x128; 4 the same, double-buffered; 5 LDS-tiled
                     128x64; 6 LDS-tiled 64x128;
-                    8..11 split-K LDS-staged (32x32 over 4 / 8 waves, 64x32, 32x64); 16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)
+                    8..11 split-K LDS-staged (32x32 over 4 / 8 waves, 64x32, 32x64);
+                    12..15 the same with two slabs of loads in flight per wave;
+                    16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)
                     + 4 (KW == 8) for a (16 TM) x (16 TN) tile over KW waves */
     /* optional second addend and gauge (fused pressure update of rbc2d.py:384-394):
      *   C[e] = (beta D[e] - (gamma_vec[e / gamma_div] E[e]) gamma) + alpha_e op(A) op(B)

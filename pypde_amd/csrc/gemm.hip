@@ -526,8 +526,10 @@ static void launch_splitk_group(bool ta, bool tb, GemmGroup& gg, const int* batc
 // wave w owns a contiguous K range and stages its own 16-deep slabs of A and B
 // through a private LDS region with coalesced loads (consecutive lanes walk the
 // contiguous dimension of each operand), so the K loop has no block barriers.
+// DEPTH slabs of loads are in flight per wave (registers of DEPTH slabs; the
+// slab order and so the sums are the same for every DEPTH).
 // The KW partial tiles are summed through LDS at the end.
-template <int BM, int BN, int KW, bool TA, bool TB>
+template <int BM, int BN, int KW, bool TA, bool TB, int DEPTH>
 __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int BK = 16;
@@ -561,7 +563,7 @@ __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double ra[LA], rb[LB];
+    double ra[DEPTH][LA], rb[DEPTH][LB];
     auto a_rc = [&](int q, int& r, int& c) {
         if (!TA) { c = lane % BK; r = lane / BK + q * (64 / BK); }
         else     { r = lane % BM; c = lane / BM + q * (64 / BM); }
@@ -570,56 +572,64 @@ __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int
         if (!TB) { c = lane % BN; r = lane / BN + q * (64 / BN); }
         else     { r = lane % BK; c = lane / BK + q * (64 / BK); }
     };
-    auto load = [&](int k0) {
+    auto load = [&](int st, int k0) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
             a_rc(q, r, c);
             const int gm = m0 + r, gk = k0 + c;
-            ra[q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + (long long)gk * g.ska]) : 0.0;
+            ra[st][q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + (long long)gk * g.ska]) : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             int r, c;
             b_rc(q, r, c);
             const int gk = k0 + r, gn = n0 + c;
-            rb[q] = (gk < ke && gn < g.n) ? (TB ? B[(long long)gn * g.ldb + gk] : B[(long long)gk * g.ldb + gn]) : 0.0;
+            rb[st][q] = (gk < ke && gn < g.n) ? (TB ? B[(long long)gn * g.ldb + gk] : B[(long long)gk * g.ldb + gn]) : 0.0;
         }
     };
-    auto store = [&]() {
+    auto store = [&](int st) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
             a_rc(q, r, c);
-            As[c * PA + r] = ra[q];
+            As[c * PA + r] = ra[st][q];
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             int r, c;
             b_rc(q, r, c);
-            Bs[r * PB + c] = rb[q];
+            Bs[r * PB + c] = rb[st][q];
         }
     };
 
-    if (kb < ke) load(kb);
-    for (int k0 = kb; k0 < ke; k0 += BK) {
-        wave_lds_sync();  // previous slab fully read
-        store();
-        wave_lds_sync();
-        if (k0 + BK < ke) load(k0 + BK);
+    // slab kb + (i * DEPTH + st) * BK lives in register stage st
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            double af[TM], bf[TN];
+    for (int st = 0; st < DEPTH; ++st)
+        if (kb + st * BK < ke) load(st, kb + st * BK);
+    for (int k0 = kb; k0 < ke; k0 += DEPTH * BK) {
 #pragma unroll
-            for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + i * 16 + (lane & 15)];
+        for (int st = 0; st < DEPTH; ++st) {
+            const int ks = k0 + st * BK;
+            if (ks >= ke) break;  // uniform across the wave
+            wave_lds_sync();  // previous slab fully read
+            store(st);
+            wave_lds_sync();
+            if (ks + DEPTH * BK < ke) load(st, ks + DEPTH * BK);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + j * 16 + (lane & 15)];
+            for (int kk = 0; kk < BK; kk += 4) {
+                const int kr = kk + (lane >> 4);
+                double af[TM], bf[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i)
+                for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + i * 16 + (lane & 15)];
 #pragma unroll
-                for (int j = 0; j < TN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+            }
         }
     }
 
@@ -647,44 +657,44 @@ __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int
     }
 }
 
-template <int BM, int BN, int KW, bool TA, bool TB>
+template <int BM, int BN, int KW, bool TA, bool TB, int DEPTH>
 __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_kernel(GemmArgs g)
 {
     int bx, by, bz;
     grid_tile(bx, by, bz);
-    dgemm_wsplit_body<BM, BN, KW, TA, TB>(g, bx, by, bz);
+    dgemm_wsplit_body<BM, BN, KW, TA, TB, DEPTH>(g, bx, by, bz);
 }
 
-template <int BM, int BN, int KW, bool TA, bool TB>
+template <int BM, int BN, int KW, bool TA, bool TB, int DEPTH>
 __global__ void __launch_bounds__(64 * KW) dgemm_wsplit_group(GemmGroup gg)
 {
     int bx, by, bz;
     const int i = group_tile(gg, bx, by, bz);
-    dgemm_wsplit_body<BM, BN, KW, TA, TB>(gg.a[i], bx, by, bz);
+    dgemm_wsplit_body<BM, BN, KW, TA, TB, DEPTH>(gg.a[i], bx, by, bz);
 }
 
-template <int BM, int BN, int KW>
+template <int BM, int BN, int KW, int DEPTH = 1>
 static void launch_wsplit(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
     dim3 block(64 * KW);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, false>), grid, block, 0, s, g);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, true>), grid, block, 0, s, g);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, false>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, true>), grid, block, 0, s, g);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, false, DEPTH>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, false, true, DEPTH>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, false, DEPTH>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_wsplit_kernel<BM, BN, KW, true, true, DEPTH>), grid, block, 0, s, g);
 }
 
-template <int BM, int BN, int KW>
+template <int BM, int BN, int KW, int DEPTH = 1>
 static void launch_wsplit_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
 {
     group_grid(gg, batch, BM, BN);
     if (gg.start[gg.count] == 0) return;
     dim3 grid(gg.start[gg.count]);
     dim3 block(64 * KW);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, false>), grid, block, 0, s, gg);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, true>), grid, block, 0, s, gg);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, false>), grid, block, 0, s, gg);
-    else hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, true>), grid, block, 0, s, gg);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, false, DEPTH>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, false, true, DEPTH>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, false, DEPTH>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_wsplit_group<BM, BN, KW, true, true, DEPTH>), grid, block, 0, s, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -943,7 +953,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 11) ||
+    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 15) ||
                       (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
@@ -958,6 +968,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
     case 11: launch_wsplit<32, 64, 4>(transA, transB, g, batch, stream); break;
+    case 12: launch_wsplit<32, 32, 4, 2>(transA, transB, g, batch, stream); break;
+    case 13: launch_wsplit<32, 32, 8, 2>(transA, transB, g, batch, stream); break;
+    case 14: launch_wsplit<64, 32, 4, 2>(transA, transB, g, batch, stream); break;
+    case 15: launch_wsplit<32, 64, 4, 2>(transA, transB, g, batch, stream); break;
     case 16: launch_splitk<1, 1, 4>(transA, transB, g, batch, stream); break;
     case 17: launch_splitk<2, 1, 4>(transA, transB, g, batch, stream); break;
     case 18: launch_splitk<1, 2, 4>(transA, transB, g, batch, stream); break;
@@ -1070,7 +1084,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 11) ||
+    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 15) ||
                       (variant >= 16 && variant < 24),
                   "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
@@ -1085,6 +1099,10 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
     case 11: launch_wsplit_group<32, 64, 4>(ta, tb, gg, batch, stream); break;
+    case 12: launch_wsplit_group<32, 32, 4, 2>(ta, tb, gg, batch, stream); break;
+    case 13: launch_wsplit_group<32, 32, 8, 2>(ta, tb, gg, batch, stream); break;
+    case 14: launch_wsplit_group<64, 32, 4, 2>(ta, tb, gg, batch, stream); break;
+    case 15: launch_wsplit_group<32, 64, 4, 2>(ta, tb, gg, batch, stream); break;
     case 16: launch_splitk_group<1, 1, 4>(ta, tb, gg, batch, stream); break;
     case 17: launch_splitk_group<2, 1, 4>(ta, tb, gg, batch, stream); break;
     case 18: launch_splitk_group<1, 2, 4>(ta, tb, gg, batch, stream); break;

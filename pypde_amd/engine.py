@@ -920,7 +920,8 @@ class RBCBatch:
 
     def variants(self):
         """{launch name: kernel variant} chosen for this step (GEMM variants;
-        recurrences: 1 block form, 2 lane-chunk form)."""
+        grouped launches: their variant or "split"; recurrences: 1 block form,
+        2 lane-chunk form, 3 block form with 32 lines per block)."""
         out = {l.name: l.desc.variant for l in self.launches if isinstance(l, _Gemm)}
         out.update({l.name: ("split" if l.split else l.variant) for l in self.launches
                     if isinstance(l, _GemmGroup)})

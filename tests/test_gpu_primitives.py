@@ -38,7 +38,7 @@ def test_mfma_gemm_layout(amd):
                 assert rel(host(C), ref) < 1e-14, (m, n, k, ta, tb)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23])
 def test_gemm_every_variant(amd, variant):
     """Each kernel variant of pypde_dgemm_batched on ragged shapes (m, n not
     multiples of the tile, k not a multiple of the split), all transposes,
@@ -364,7 +364,34 @@ def test_mirror_operator_transforms(amd, N, axis):
         assert rel(y, ref) < 1e-13, (M.shape, axis)
 
 
-PLAIN_VARIANTS = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 16, 17, 18, 19, 20, 21, 22, 23)
+PLAIN_VARIANTS = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23)
+
+
+def test_gemm_prefetch_depth_is_bit_identical(amd):
+    """Variants 12..15 (two slabs of loads in flight per wave) run the same
+    slabs in the same order as 8..11, so the results agree bit for bit."""
+    import ctypes
+    _, ops, torch = amd
+    from pypde_amd._lib import lib
+    rng = np.random.default_rng(77)
+    for (m, n, k, batch) in [(37, 53, 29, 3), (254, 254, 894, 2), (64, 64, 700, 1), (5, 3, 1, 2),
+                             (100, 40, 65, 1)]:
+        for ta in (0, 1):
+            for tb in (0, 1):
+                A = rng.standard_normal((batch, k, m) if ta else (batch, m, k))
+                B = rng.standard_normal((batch, k, n) if not tb else (batch, n, k))
+                dA, dB = ops.to_dev(A), ops.to_dev(B)
+                outs = {}
+                for v in (8, 9, 10, 11, 12, 13, 14, 15):
+                    C = torch.empty((batch, m, n), dtype=torch.float64, device=dA.device)
+                    d = ops._desc(ta, tb, m, n, k, batch, 1.0, 0.0, dA.data_ptr(), dA.stride(1),
+                                  dA.stride(0), dB.data_ptr(), dB.stride(1), dB.stride(0), None, 0,
+                                  0, C.data_ptr(), n, m * n)
+                    d.variant = v
+                    assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) == 0
+                    outs[v] = host(C)
+                for v in (8, 9, 10, 11):
+                    assert np.array_equal(outs[v], outs[v + 4]), (m, n, k, ta, tb, v)
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
