@@ -64,19 +64,22 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12610766,
-    "ypass_v+ypass_x": 23797006,
-    "products": 14242766,
-    "z_conv+z_pres+z_old": 17949636,
+    "xpass": 12588237,
+    "ypass_v+ypass_x": 23879421,
+    "products": 14242768,
+    "z_conv+z_pres+z_old": 17990844,
+    "rhs_part+buoyancy_part": 21389193,
+    "splitk_reduce": 6950335,
+    "helmholtz_x": 18904172,
+    "helmholtz_y": 3502257,
+    "div_y": 11541273,
+    "div_x+poisson_rhs": 12059250,
+    "poisson_lines": 8500476,
+    "poisson_qy+corr_y+corr_y0": 15164894,
+    "pres_p+correct": 10773176,
+    # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
     "rhs": 26877192,
     "buoyancy": 5797532,
-    "helmholtz_x": 18904174,
-    "helmholtz_y": 3515569,
-    "div_y": 11556629,
-    "div_x+poisson_rhs": 12006002,
-    "poisson_lines": 8500475,
-    "poisson_qy+corr_y+corr_y0": 15176151,
-    "pres_p+correct": 10751671,
 }
 
 
@@ -494,10 +497,11 @@ def main():
         eng = plain["engine"]
         order = []
         for launch in eng.launches:
-            l = launch.group if isinstance(launch, _SplitRhs) else launch
-            launch_graph_time(eng, l)
-            kernels = len(l.members) if isinstance(l, _GemmGroup) and l.split else 1
-            order.append([l.name, kernels])
+            parts = [launch.group, launch.reduce] if isinstance(launch, _SplitRhs) else [launch]
+            for l in parts:
+                launch_graph_time(eng, l)
+                kernels = len(l.members) if isinstance(l, _GemmGroup) and l.split else 1
+                order.append([l.name, kernels])
         print(json.dumps({"launch_order": order}))
     extra = {}
     if not args.no_extra:

@@ -222,11 +222,12 @@ class _SplitRhs:
 
     def __init__(self, name, group, reduce_args, S):
         self.name, self.group, self.reduce_args, self.S = name, group, reduce_args, S
+        self.reduce = _Call("splitk_reduce", lib.pypde_splitk_reduce, *reduce_args)
         self.flops = group.flops
 
     def __call__(self, stream):
         self.group(stream)
-        check(lib.pypde_splitk_reduce(*self.reduce_args, stream), self.name)
+        self.reduce(stream)
 
 
 # independent launches of one stage that may share a launch
