@@ -32,7 +32,7 @@ ROWS_CANDIDATES = (1, 2, 4, 8, 18, 20, 24)
 # 4..32 rows per block; 37..40: eight rows in flight, 8..64 rows per block)
 ROWS_CANDIDATES_AXIS1 = ROWS_CANDIDATES + (33, 34, 35, 36, 37, 38, 39, 40)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
-GEMM_CANDIDATES = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23)
+GEMM_CANDIDATES = tuple(range(1, 27))
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
 _cache = {}

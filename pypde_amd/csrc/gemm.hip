@@ -178,7 +178,11 @@ __device__ __forceinline__ int group_tile(const GemmGroup& gg, int& bx, int& by,
     return i;
 }
 
-template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+// PIPE 0: one LDS buffer, the next slab's loads in flight during a slab; 1:
+// two LDS buffers (one barrier per slab); 2: one LDS buffer, two slabs of
+// loads in flight (registers for two).  Every form runs the same slabs in the
+// same order, so results are bit-identical.
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
 __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int BK = 16;
@@ -191,7 +195,8 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
     constexpr int TN = WN / 16;
     constexpr int LA = BM * BK / NT;  // A elements loaded per thread per slab
     constexpr int LB = BN * BK / NT;
-    constexpr int NBUF = DBUF ? 2 : 1;
+    constexpr int NBUF = PIPE == 1 ? 2 : 1;
+    constexpr int DEPTH = PIPE == 2 ? 2 : 1;
     static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
     static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "tile / thread mismatch");
     static_assert(NT % BK == 0 && NT % BM == 0 && NT % BN == 0, "load mapping");
@@ -214,7 +219,7 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double ra[LA], rb[LB];
+    double ra[DEPTH][LA], rb[DEPTH][LB];
 
     // (r, c) of the q-th element a thread moves: consecutive threads walk the
     // contiguous dimension of the operand in memory.
@@ -226,7 +231,7 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
         if (!TB) { c = tid % BN; r = tid / BN + q * (NT / BN); }
         else     { r = tid % BK; c = tid / BK + q * (NT / BK); }
     };
-    auto load_slab = [&](int k0) {
+    auto load_slab = [&](int st, int k0) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
@@ -234,7 +239,7 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
             const int gm = m0 + r, gk = k0 + c;
             double v = 0.0;
             if (gm < g.m && gk < g.k) v = TA ? A[(long long)gk * g.lda + gm] : A[(long long)gm * g.lda + (long long)gk * g.ska];
-            ra[q] = v;
+            ra[st][q] = v;
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
@@ -243,21 +248,21 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
             const int gk = k0 + r, gn = n0 + c;
             double v = 0.0;
             if (gk < g.k && gn < g.n) v = TB ? B[(long long)gn * g.ldb + gk] : B[(long long)gk * g.ldb + gn];
-            rb[q] = v;
+            rb[st][q] = v;
         }
     };
-    auto store_slab = [&](int buf) {
+    auto store_slab = [&](int buf, int st) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
             a_rc(q, r, c);
-            As[buf][c * PA + r] = ra[q];
+            As[buf][c * PA + r] = ra[st][q];
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             int r, c;
             b_rc(q, r, c);
-            Bs[buf][r * PB + c] = rb[q];
+            Bs[buf][r * PB + c] = rb[st][q];
         }
     };
     auto compute = [&](int buf) {
@@ -277,28 +282,36 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
         }
     };
 
-    if (DBUF) {
+    if (PIPE == 1) {
         // one barrier per slab: the next slab is stored into the other buffer
-        load_slab(0);
-        store_slab(0);
+        load_slab(0, 0);
+        store_slab(0, 0);
         __syncthreads();
         int cur = 0;
         for (int k0 = 0; k0 < g.k; k0 += BK) {
             const bool more = k0 + BK < g.k;
-            if (more) load_slab(k0 + BK);
+            if (more) load_slab(0, k0 + BK);
             compute(cur);
-            if (more) store_slab(cur ^ 1);
+            if (more) store_slab(cur ^ 1, 0);
             __syncthreads();
             cur ^= 1;
         }
     } else {
-        load_slab(0);
-        for (int k0 = 0; k0 < g.k; k0 += BK) {
-            __syncthreads();
-            store_slab(0);
-            __syncthreads();
-            if (k0 + BK < g.k) load_slab(k0 + BK);
-            compute(0);
+        // slab (i * DEPTH + st) * BK lives in register stage st
+#pragma unroll
+        for (int st = 0; st < DEPTH; ++st)
+            if (st * BK < g.k) load_slab(st, st * BK);
+        for (int k0 = 0; k0 < g.k; k0 += DEPTH * BK) {
+#pragma unroll
+            for (int st = 0; st < DEPTH; ++st) {
+                const int ks = k0 + st * BK;
+                if (ks >= g.k) break;  // uniform across the block
+                __syncthreads();
+                store_slab(0, st);
+                __syncthreads();
+                if (ks + DEPTH * BK < g.k) load_slab(st, ks + DEPTH * BK);
+                compute(0);
+            }
         }
     }
 
@@ -317,31 +330,31 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
             }
 }
 
-template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 {
     int bx, by, bz;
     grid_tile(bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(g, bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB>(g, bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, bool DBUF, bool TA, bool TB>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_group(GemmGroup gg)
 {
     int bx, by, bz;
     const int i = group_tile(gg, bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, DBUF, TA, TB>(gg.a[i], bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB>(gg.a[i], bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, bool DBUF>
+template <int BM, int BN, int WGM, int WGN, int PIPE>
 static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, false, false>), grid, block, 0, s, g);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, false, true>), grid, block, 0, s, g);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, true, false>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, DBUF, true, true>), grid, block, 0, s, g);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, false>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, true>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, false>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, true>), grid, block, 0, s, g);
 }
 
 static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
@@ -355,17 +368,17 @@ static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
     }
 }
 
-template <int BM, int BN, int WGM, int WGN, bool DBUF>
+template <int BM, int BN, int WGM, int WGN, int PIPE>
 static void launch_gemm_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
 {
     group_grid(gg, batch, BM, BN);
     if (gg.start[gg.count] == 0) return;
     dim3 grid(gg.start[gg.count]);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, false, false>), grid, block, 0, s, gg);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, false, true>), grid, block, 0, s, gg);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, true, false>), grid, block, 0, s, gg);
-    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, DBUF, true, true>), grid, block, 0, s, gg);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, false>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, true>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, false>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, true>), grid, block, 0, s, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -953,9 +966,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 15) ||
-                      (variant >= 16 && variant < 24),
-                  "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 26, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -964,6 +975,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 4: launch_gemm<128, 128, 4, 2, true>(transA, transB, g, batch, stream); break;
     case 5: launch_gemm<128, 64, 4, 2, false>(transA, transB, g, batch, stream); break;
     case 6: launch_gemm<64, 128, 2, 4, false>(transA, transB, g, batch, stream); break;
+    case 7: launch_gemm<128, 64, 4, 2, 2>(transA, transB, g, batch, stream); break;
+    case 24: launch_gemm<64, 128, 2, 4, 2>(transA, transB, g, batch, stream); break;
+    case 25: launch_gemm<64, 64, 2, 2, 2>(transA, transB, g, batch, stream); break;
+    case 26: launch_gemm<128, 128, 4, 2, 2>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1084,9 +1099,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE((variant >= 0 && variant <= 6) || (variant >= 8 && variant <= 15) ||
-                      (variant >= 16 && variant < 24),
-                  "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 26, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1095,6 +1108,10 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 4: launch_gemm_group<128, 128, 4, 2, true>(ta, tb, gg, batch, stream); break;
     case 5: launch_gemm_group<128, 64, 4, 2, false>(ta, tb, gg, batch, stream); break;
     case 6: launch_gemm_group<64, 128, 2, 4, false>(ta, tb, gg, batch, stream); break;
+    case 7: launch_gemm_group<128, 64, 4, 2, 2>(ta, tb, gg, batch, stream); break;
+    case 24: launch_gemm_group<64, 128, 2, 4, 2>(ta, tb, gg, batch, stream); break;
+    case 25: launch_gemm_group<64, 64, 2, 2, 2>(ta, tb, gg, batch, stream); break;
+    case 26: launch_gemm_group<128, 128, 4, 2, 2>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
