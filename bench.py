@@ -309,6 +309,11 @@ def microbench(nf=512, reps=10):
         out[name] = rec
 
     fb = 8 * 2 * N * N
+    # the measured roof: a device copy of the same 512 x 256 x 256 array
+    cp = torch.empty_like(c)
+    run("device_copy", lambda: torch.mul(c, 1.0, out=cp), fb)
+    copy = out.pop("device_copy")
+    del cp
     # MFMA work issued by a folded transform: half the dense count 2 N^3 per field
     fl = N * N * N
     bufs["u1"] = opB.apply(c, 0)
@@ -334,9 +339,14 @@ def microbench(nf=512, reps=10):
     run("K12_fdma_y", lambda: lhs_y.solve(x), 8 * 2 * M * M)
     total_us = sum(v["us"] for v in out.values())
     total_bytes = 12501216 * nf
+    for v in out.values():
+        v["copy_frac"] = round(v["GBps"] / copy["GBps"], 4)
+    chain_gbs = total_bytes / (total_us * 1e-6) / 1e9
     return dict(fields=nf, N=N, kernels=out, chain_us=round(total_us, 1),
-                chain_GBps=round(total_bytes / (total_us * 1e-6) / 1e9, 1),
-                chain_hbm_frac=round(total_bytes / (total_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
+                chain_GBps=round(chain_gbs, 1),
+                chain_hbm_frac=round(chain_gbs / HBM_PEAK_GBS, 4),
+                chain_copy_frac=round(chain_gbs / copy["GBps"], 4),
+                device_copy=dict(copy, kernel="torch.mul(x, 1.0, out=y), 512 x 256 x 256 float64"))
 
 
 def time_variant(steps=100, **kw):
@@ -438,7 +448,21 @@ def cpu_baseline(steps=6, warm=1):
         sec = time.perf_counter() - t0
     return dict(value=round(steps / sec, 3), unit="steps/s", cores=1, kind="port",
                 sample="%d Euler steps of the 256x256 rbc workload (Ra=1e8, dt=1e-3) after %d warm-up" %
-                       (steps, warm), sec=round(sec, 2))
+                       (steps, warm), sec=round(sec, 2), host=host_cpu())
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=model, logical_cpus=os.cpu_count())
 
 
 def main():
