@@ -12,7 +12,7 @@
 // with zero-padded tables that encode chain starts and ends, and reciprocals
 // of the pivots precomputed at plan time (no division in the kernel).
 //
-// Parallel layout: a block owns 64 lines (one lane each) and P = ceil(n / CH)
+// Block form: a block owns 64 lines (one lane each) and P = ceil(n / CH)
 // wave64s (CH = 16 or 32, below); wave w holds positions [w*CH, (w+1)*CH) of its lane's
 // line in registers.  Each sweep runs twice per chunk: once to obtain the chunk's
 // affine transfer (end value and carry coefficient), then, after the carries
@@ -20,7 +20,10 @@
 // incoming values.  Loads and stores are coalesced: along columns (axis 0)
 // neighbouring lanes read neighbouring columns; along rows (axis 1) each wave
 // transposes 64 x 16 tiles (128-byte row segments) through its own LDS tile,
-// synchronising only within the wave.
+// synchronising only within the wave.  The 32-line variant of the block form
+// gives the two half-waves of a wave neighbouring chunks of 32 lines (32 x 32
+// transpose tiles along rows), and the lane-chunk form further below gives
+// each line one wave.
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
