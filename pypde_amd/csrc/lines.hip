@@ -3,9 +3,12 @@
 // Two kernel families:
 //
 // * Banded applies (stencil S, pseudo-inverse B, B S, ...): no recurrence, so
-//   every output element is computed by its own thread from at most nine
-//   inputs of its line.  Neighbouring threads own neighbouring columns (axis
-//   0) or neighbouring entries of a row (axis 1), so all accesses coalesce.
+//   every output element is a sum of at most nine inputs of its line.  A
+//   thread computes one output, or a register window of R consecutive outputs
+//   of its line, or (along rows) one entry of many rows with that entry's
+//   coefficients held in registers.  Neighbouring threads own neighbouring
+//   columns (axis 0) or neighbouring entries of a row (axis 1), so accesses
+//   coalesce.
 //
 // * Recurrences (four-diagonal LU sweeps, the offset-2 tridiagonal solve, the
 //   Poisson per-line solves, two-/five-band solves, the derivative recurrence):
