@@ -387,21 +387,28 @@ def time_steady(shape, ra, dt=0.01, tau=0.5, pre=300, tol=1e-10):
 
 
 def time_adjoint_descent(N, steps=20, ra=1e4, dt=0.005):
-    """Adjoint-descent updates per second (navier.rbc2d_adj drop-in) at N x N."""
+    """Adjoint-descent updates per second (navier.rbc2d_adj drop-in) at N x N,
+    replayed from the update's HIP graph and, for comparison, op by op."""
     import torch
     from pypde_amd.navier import NavierStokesAdjoint
-    nsa = NavierStokesAdjoint(case="rbc", shape=(N, N), ra=ra, pr=1.0, dt=dt, tsave=None)
-    nsa.set_temperature(amplitude=0.2)
-    for _ in range(3):
-        nsa.update()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        nsa.update()
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    return dict(N=N, ra=ra, dt=dt, steps=steps, ms_per_update=round(sec / steps * 1e3, 3),
-                finite=bool(torch.isfinite(nsa.T.vhat).all()))
+    out = dict(N=N, ra=ra, dt=dt, steps=steps)
+    for use_graph in (True, False):
+        nsa = NavierStokesAdjoint(case="rbc", shape=(N, N), ra=ra, pr=1.0, dt=dt, tsave=None,
+                                  use_graph=use_graph)
+        nsa.set_temperature(amplitude=0.2)
+        for _ in range(3):
+            nsa.update()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            nsa.update()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        key = "ms_per_update" if use_graph else "ms_per_update_op_by_op"
+        out[key] = round(sec / steps * 1e3, 3)
+        if use_graph:
+            out["finite"] = bool(torch.isfinite(nsa.T.vhat).all())
+    return out
 
 
 def time_adjoint(N, nsteps, ckpt=100, ra=5e4, dt=5e-3):

@@ -236,7 +236,7 @@ class NavierStokes(NavierStokesBase, Integrator):
             div = self.grad(self.U, deriv=(1, 0)) + self.grad(self.V, deriv=(0, 1))
             rhs = self.solver_P.solve_rhs(div)
             P = self.solver_P.solve_lhs(rhs)
-            P[0, 0] = 0
+            P[0, 0].zero_()  # a device fill (capturable; no host copy)
             self.P.vhat = P
             self.pres.vhat = self.pres.vhat - 1.0 * self.nu * div * self.beta
             self.pres.vhat = self.pres.vhat + 1.0 / (dt * a) * galerkin_to_cheby(self.P.vhat, self.P)

@@ -31,10 +31,16 @@ from .rbc2d_base import NavierStokesBase
 class NavierStokesAdjoint(NavierStokesBase, Integrator):
     avail_cases = ["rbc", "linear", "zero"]
 
-    def __init__(self, case="rbc", **kwargs):
+    def __init__(self, case="rbc", use_graph=True, **kwargs):
         if case not in self.avail_cases:
             raise ValueError("Specified case is not available: ", self.avail_cases)
         self.case = case
+        # update() as a HIP graph: the first call runs op by op (and tunes every
+        # kernel choice), the second captures one update, later calls replay it
+        self.use_graph = use_graph
+        self._graph = None
+        self._calls = 0
+        self._capturing = False
         Integrator.__init__(self)
         NavierStokesBase.__init__(self, **kwargs)
         side = "CN" if case == "rbc" else "CD"
@@ -111,7 +117,11 @@ class NavierStokesAdjoint(NavierStokesBase, Integrator):
         (rbc2d_adj.py:189-205)."""
         NS = self.NS
         NS.set_state(U=self.U.vhat, V=self.V.vhat, T=self.T.vhat)
-        NS.update()
+        if self._capturing and NS.engine is not None:
+            NS._sync_in()
+            NS.engine.step_eager()  # a graph replay cannot be captured
+        else:
+            NS.update()
         rU = (NS.U.vhat - self.U.vhat) / self.dt
         rV = (NS.V.vhat - self.V.vhat) / self.dt
         rT = (NS.T.vhat - self.T.vhat) / self.dt
@@ -158,7 +168,7 @@ class NavierStokesAdjoint(NavierStokesBase, Integrator):
         P = self.solver_P.solve_lhs(rhs)
         if singular:
             P = P.clone()
-            P[0, 0] = 0
+            P[0, 0].zero_()  # a device fill (capturable; no host copy)
         self.P.vhat = P
 
     def update_pres(self, div, stage):
@@ -175,7 +185,44 @@ class NavierStokesAdjoint(NavierStokesBase, Integrator):
         return f.dealias.backward(f.vhat) if self.dealias else f.backward(f.vhat)
 
     def update(self):
-        """One adjoint-descent step (rbc2d_adj.py:321-356)."""
+        """One adjoint-descent step (rbc2d_adj.py:321-356), replayed from a HIP
+        graph after the first two calls when use_graph is set."""
+        self._calls += 1
+        if not self.use_graph or (self._graph is None and self._calls == 1):
+            self._update_ops()
+            return
+        if self._graph is None:
+            self._capture()
+        else:
+            # state the caller replaced since the capture goes into the captured buffers
+            for f, buf in zip((self.U, self.V, self.T, self.pres), self._bufs):
+                if f.vhat is not buf:
+                    buf.copy_(ops.to_dev(f.vhat).reshape(buf.shape))
+                    f.vhat = buf
+        self._graph.replay()
+
+    def _capture(self):
+        """Capture one update.  U, V, T and pres carry over between updates: the
+        captured update reads them from their current buffers and copies its
+        results back into the same buffers, which the fields keep."""
+        state = (self.U, self.V, self.T, self.pres)
+        bufs = tuple(f.vhat for f in state)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        self._capturing = True
+        try:
+            with torch.cuda.graph(g):
+                self._update_ops()
+                for f, buf in zip(state, bufs):
+                    if f.vhat is not buf:
+                        buf.copy_(f.vhat)
+        finally:
+            self._capturing = False
+            for f, buf in zip(state, bufs):
+                f.vhat = buf
+        self._graph, self._bufs = g, bufs
+
+    def _update_ops(self):
         self.ux_old, self.uz_old, self.temp_old = 0, 0, 0
         for rk in range(self.nstage):
             self.ux, self.uz, self.temp = self._phys(self.U), self._phys(self.V), self._phys(self.T)
