@@ -48,6 +48,7 @@ class StepAdjoint:
         self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
         self.Y2b = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
         self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
+        self.graph = None
 
     def forward_fields(self, X):
         """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients)."""
@@ -125,6 +126,24 @@ class StepAdjoint:
             g(e.GGx[f], y1b, transA=True, beta=1.0, D=Xin[:, f], out=Xin[:, f])
         return Xin, pin
 
+    def vjp_replay(self, X, pres, Xb, pb):
+        """vjp() replayed from a HIP graph (captured on the first call, after one
+        op-by-op call that tunes every kernel choice).  The same launches in the
+        same order, so the results are vjp()'s bit for bit.  The returned tensors
+        are the graph's output buffers: valid until the next call."""
+        if self.graph is None:
+            self.gin = [t.clone() for t in (X, pres, Xb, pb)]
+            self.vjp(*self.gin)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.gout = self.vjp(*self.gin)
+            self.graph = g
+        for buf, t in zip(self.gin, (X, pres, Xb, pb)):
+            buf.copy_(t)
+        self.graph.replay()
+        return self.gout
+
 
 class NusseltFunctional:
     """Plate Nusselt number as an affine functional of the temperature
@@ -166,11 +185,21 @@ class NusseltFunctional:
 class Trajectory:
     """Forward trajectory with checkpoints and its reverse (adjoint) sweep."""
 
-    def __init__(self, eng: RBCBatch, ckpt=100):
+    def __init__(self, eng: RBCBatch, ckpt=100, use_graph=True):
         self.e = eng
         self.adj = StepAdjoint(eng)
         self.nu = NusseltFunctional(eng)
         self.ckpt = ckpt
+        self.use_graph = use_graph
+
+    def _step(self):
+        """One forward step: the engine's graph replay, or its launches."""
+        if self.use_graph:
+            if self.e.graph is None or self.e.graph_steps != 1:
+                self.e.capture(1)
+            self.e.step(1)
+        else:
+            self.e.step_eager()
 
     def _state(self):
         return self.e.X.clone(), self.e.pres.clone()
@@ -185,7 +214,7 @@ class Trajectory:
         cks = {0: self._state()}
         acc = torch.zeros(self.e.B, dtype=ops.DTYPE, device=X0.device)
         for k in range(1, nsteps + 1):
-            self.e.step_eager()
+            self._step()
             acc += self.nu.value(self.e.X[:, 2])
             if k % self.ckpt == 0 and k < nsteps:
                 cks[k] = self._state()
@@ -207,13 +236,16 @@ class Trajectory:
             states = []
             for k in range(k0, k1):
                 states.append(self._state())
-                self.e.step_eager()
+                self._step()
             for k in range(k1, k0, -1):
                 # Nu(T_k) enters J for k >= 1
                 Xb[:, 2] += g
                 X, pres = states[k - 1 - k0]
-                Xb, pb = self.adj.vjp(X, pres, Xb, pb)
-        return J, Xb, pb
+                if self.use_graph:
+                    Xb, pb = self.adj.vjp_replay(X, pres, Xb, pb)
+                else:
+                    Xb, pb = self.adj.vjp(X, pres, Xb, pb)
+        return J, Xb.clone(), pb.clone()
 
 
 class TimeAveragedNusselt(torch.autograd.Function):

@@ -152,3 +152,31 @@ def test_residual_gradient_direction_vs_reference_adjoint(golden):
     cos = float(step @ gv / (np.linalg.norm(step) * np.linalg.norm(gv)))
     print("cosine(reference adjoint-descent step, discrete gradient) = %.4f" % cos)
     assert cos < 0.0
+
+
+def test_graph_replayed_sweep_is_bit_identical():
+    """Trajectory with graph-replayed forward steps and VJPs gives the op-by-op
+    sweep's J and gradients bit for bit (same launches in the same order), also
+    on a second call that reuses the captured graphs."""
+    import torch
+    from pypde_amd.adjoint import Trajectory
+    from pypde_amd.engine import RBCBatch
+    out = []
+    for use_graph in (True, False):
+        eng = RBCBatch((24, 20), [dict(ra=5e4, pr=1.0, dt=0.005),
+                                  dict(ra=2e4, pr=1.0, dt=0.005)], case="rbc")
+        rng = np.random.default_rng(12)
+        X0 = torch.zeros_like(eng.X)
+        X0[:, 2] = torch.as_tensor(rng.uniform(-0.05, 0.05, tuple(X0[:, 2].shape)),
+                                   device=eng.X.device)
+        pres0 = torch.zeros_like(eng.pres)
+        traj = Trajectory(eng, ckpt=7, use_graph=use_graph)
+        runs = []
+        for nsteps in (23, 15):
+            J, Xb, pb = traj.grad_nusselt(X0, pres0, nsteps)
+            runs.append([t.detach().cpu().numpy() for t in (J, Xb, pb)])
+        out.append(runs)
+        assert (traj.adj.graph is not None) == use_graph
+    for a, b in zip(out[0], out[1]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
