@@ -32,6 +32,21 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _fields_gemm(ta, tb, m, n, k, batch, A, lda, iA, B, ldb, iB, C, ldc, iC, alpha=1.0,
+                 alpha_vec=None, beta=0.0, D=None, iD=None):
+    """One batched GEMM over (instance, field) entries e = 3 b + f: operators
+    indexed per field through a modulus-3 batch index, per-instance scales
+    alpha_vec[e / 3]; operands are device pointers."""
+    d = ops._desc(ta, tb, m, n, k, batch, alpha, beta, A, lda, 0, B, ldb, 0, D,
+                  ldc if D is not None else 0, 0, C, ldc, 0)
+    d.iA, d.iB, d.iC = iA, iB, iC
+    if D is not None:
+        d.iD = iD
+    if alpha_vec is not None:
+        d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), 3
+    ops._launch_desc(d)
+
+
 class StepAdjoint:
     """VJP of one fused Euler step of an RBCBatch (all instances at once)."""
 
@@ -51,11 +66,17 @@ class StepAdjoint:
         self.graph = None
 
     def forward_fields(self, X):
-        """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients)."""
+        """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients), one
+        launch each over every instance and field."""
         e = self.e
-        for f in range(3):
-            ops.bgemm(e.GGx[f], X[:, f], out=self.Y1[:, f])
-            ops.bgemm(self.Y1[:, f], e.GGy, transB=True, out=self.Y2[:, f])
+        B3, M0, M1, L0, L1 = 3 * e.B, e.M0, e.M1, e.L0, e.L1
+        X = X.contiguous()
+        _fields_gemm(0, 0, 2 * L0, M1, M0, B3, e.GGx.data_ptr(), M0, ops._bi(2 * L0 * M0, m1=3),
+                     X.data_ptr(), M1, ops._bi(M0 * M1), self.Y1.data_ptr(), M1,
+                     ops._bi(2 * L0 * M1))
+        _fields_gemm(0, 1, 2 * L0, 2 * L1, M1, B3, self.Y1.data_ptr(), M1, ops._bi(2 * L0 * M1),
+                     e.GGy.data_ptr(), M1, ops._bi(0), self.Y2.data_ptr(), 2 * L1,
+                     ops._bi(4 * L0 * L1))
 
     def vjp(self, X, pres, Xb, pb):
         """Cotangents of the step input (X, pres) from those of its output."""
@@ -100,30 +121,35 @@ class StepAdjoint:
                        transpose=True, table_bs=4 * M0)
         # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt BST_x^T R^_V
         Zb = torch.empty((B, 3, K, M1), **f64)
-        for f in range(3):
-            g(e.Arhs[f], Rb[:, f], transA=True, out=Zb[:, f])
+        bi = ops._bi
+        _fields_gemm(1, 0, K, M1, M0, 3 * B, e.Arhs.data_ptr(), K, bi(M0 * K, m1=3),
+                     Rb.data_ptr(), M1, bi(M0 * M1), Zb.data_ptr(), M1, bi(K * M1))
         zt = Zb[:, 2, L0 + N0:]
         g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.t_dt, beta=1.0, D=zt, out=zt)
         # 6  X^ (input) = Z^_f[2] BS_y
         Xin = torch.empty((B, 3, M0, M1), **f64)
-        for f in range(3):
-            g(Zb[:, f, L0 + N0:], e.BSy, out=Xin[:, f])
+        _fields_gemm(0, 0, M0, M1, M1, 3 * B, Zb.data_ptr() + 8 * (L0 + N0) * M1, M1,
+                     bi(K * M1), e.BSy.data_ptr(), M1, bi(0), Xin.data_ptr(), M1, bi(M0 * M1))
         # 5  pres^ (input) = pres^ - dt (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
         pin = pb.clone()
         g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
         g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
         # 4  P^r_f = -dt Z^_f[0] BF_y
         Prb = torch.empty((B, 3, L0, L1), **f64)
-        for f in range(3):
-            g(Zb[:, f, :L0], e.BFy, alpha_vec=e.t_mdt, out=Prb[:, f])
+        _fields_gemm(0, 0, L0, L1, M1, 3 * B, Zb.data_ptr(), M1, bi(K * M1), e.BFy.data_ptr(),
+                     L1, bi(0), Prb.data_ptr(), L1, bi(L0 * L1), alpha_vec=e.t_mdt)
         # 3  products
         self.forward_fields(X)
         check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb), _vp(self.Y2b),
                                          L0, L1, B, ops.stream_ptr()), "products_vjp")
         # 2  Y1^ = Y2^ GGy ; 1 X^ += GG_f^T Y1^
-        for f in range(3):
-            y1b = g(self.Y2b[:, f], e.GGy)
-            g(e.GGx[f], y1b, transA=True, beta=1.0, D=Xin[:, f], out=Xin[:, f])
+        y1b = torch.empty((B, 3, 2 * L0, M1), **f64)
+        _fields_gemm(0, 0, 2 * L0, M1, 2 * L1, 3 * B, self.Y2b.data_ptr(), 2 * L1,
+                     bi(4 * L0 * L1), e.GGy.data_ptr(), M1, bi(0), y1b.data_ptr(), M1,
+                     bi(2 * L0 * M1))
+        _fields_gemm(1, 0, M0, M1, 2 * L0, 3 * B, e.GGx.data_ptr(), M0, bi(2 * L0 * M0, m1=3),
+                     y1b.data_ptr(), M1, bi(2 * L0 * M1), Xin.data_ptr(), M1, bi(M0 * M1),
+                     beta=1.0, D=Xin.data_ptr(), iD=bi(M0 * M1))
         return Xin, pin
 
     def vjp_replay(self, X, pres, Xb, pb):
