@@ -64,6 +64,53 @@ class StepAdjoint:
         self.Y2b = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
         self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
         self.graph = None
+        # transposed solves along x on reversed lines (four-diagonal recurrences)
+        self.hrt, prt, prt_lm = e.transposed_line_tables()
+        self.prt = (prt, prt_lm)
+
+    def _reversed_solve(self, x, n, nlines, batch, tabs, per_line):
+        """In place, along axis 0 of the contiguous x [batch, n, nlines]: the
+        transposed line solves as the four-diagonal recurrence over each line
+        read backwards (the first row at the last one, row stride -nlines).
+        tabs: per-entry tables [batch][LM | R | MU | NU][n], or per-line tables
+        ([n][nlines] slots, line-major slots).  The kernel form is tuned per
+        shape on a scratch array."""
+        from . import autotune
+        from ._lib import Lines, Recurrence
+
+        def geom(t):
+            g = Lines()
+            g.src = g.dst = t.data_ptr() + 8 * (n - 1) * nlines
+            g.src_ld = g.dst_ld = -nlines
+            g.src_bs = g.dst_bs = n * nlines
+            g.len_in = g.len_out = n
+            g.nlines, g.batch, g.axis, g.lines_per_block = nlines, batch, 0, 0
+            return g
+
+        def struct(v):
+            r = Recurrence()
+            if per_line:
+                t = tabs[1] if v == 2 else tabs[0]
+                r.LM, r.R, r.MU, r.NU = (t[i].data_ptr() for i in range(4))
+                r.per_line, r.entry_stride = (2 if v == 2 else 1), 0
+            else:
+                p = tabs.data_ptr()
+                r.LM, r.R, r.MU, r.NU = p, p + 8 * n, p + 16 * n, p + 24 * n
+                r.per_line, r.entry_stride = 0, 4 * n
+            r.F = r.pre = None
+            r.variant = v
+            return r
+
+        stream = ops.stream_ptr()
+
+        def run(g, v):
+            check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(struct(v)), stream),
+                  "transposed line solve")
+        key = ("rec_reversed", n, nlines, batch, bool(per_line))
+        if autotune.enabled() and not autotune._capturing() and key not in autotune.cache():
+            gs = geom(torch.zeros_like(x))
+            autotune.choose(key, lambda v: run(gs, v), (1, 2, 3), default=2)
+        run(geom(x), autotune.cache().get(key, 2))
 
     def forward_fields(self, X):
         """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients), one
@@ -99,7 +146,7 @@ class StepAdjoint:
         check(lib.pypde_set_entries(_vp(Pb), M0 * M1, B, 0.0, ops.stream_ptr()), "gauge_vjp")
         # 16  XP^ = P^ Qy ; 15 transposed Poisson line solves ; 14 Q1^ = RP^ Hy
         XPb = g(Pb, e.Qy)
-        ops.poisson_solve(e.ptab, e.psing, XPb, 0, transpose=True)
+        self._reversed_solve(XPb, M0, M1, B, self.prt, True)
         Q1b = g(XPb, e.Hy)
         # 13  div^ += Bx^T Q1^
         g(e.Bx, Q1b, transA=True, beta=1.0, D=divb, out=divb)
@@ -117,8 +164,7 @@ class StepAdjoint:
         R3 = Rb.reshape(3 * B, M0, M1)
         ops.fdma_solve((hy[:, :, 0], hy[:, :, 1], hy[:, :, 2], hy[:, :, 3]), R3, axis=1,
                        transpose=True, table_bs=4 * M1)
-        ops.fdma_solve((hx[:, :, 0], hx[:, :, 1], hx[:, :, 2], hx[:, :, 3]), R3, axis=0,
-                       transpose=True, table_bs=4 * M0)
+        self._reversed_solve(R3, M0, M1, 3 * B, self.hrt, False)
         # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt BST_x^T R^_V
         Zb = torch.empty((B, 3, K, M1), **f64)
         bi = ops._bi

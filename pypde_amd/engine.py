@@ -42,7 +42,8 @@ from . import ops
 from ._lib import BIdx, GemmDesc, Lines, check, lib
 from .bases import Base, cheb_backward_matrix, cheb_forward_matrix, pseudoinverse_d2
 from ._lib import Recurrence
-from .solver import fdma_lu, poisson_line_tables, rec_from_fdma, rec_from_poisson
+from .solver import (fdma_lu, poisson_line_tables, rec_from_fdma, rec_from_poisson,
+                     rec_poisson_reversed_transpose, rec_reversed_transpose)
 from .templates import hholtz_matrices, poisson_matrices
 
 
@@ -466,18 +467,14 @@ class RBCBatch:
         self.Acorr = T(np.stack([ax.TDSn, ax.TSn]))                           # [2, M0, M0]
 
         # Helmholtz LU tables per stage, instance and field: [B, 3, 4, M]
-        bx_cd, bx_t, by_cd = Base(N0, "CD"), Base(N0, side), Base(N1, "CD")
         self.st_hx, self.st_hy, self.st_rhx, self.st_rhy = [], [], [], []
         for st in range(self.nstage):
             hx = np.zeros((B, 3, 4, M0))
             hy = np.zeros((B, 3, 4, M1))
             for b in range(B):
-                lam_u = dt[b] * self.ka[st] * self.beta * nu[b]
-                lam_t = dt[b] * self.ka[st] * self.beta * kappa[b]
-                for f, (basis, lam) in enumerate(((bx_cd, lam_u), (bx_cd, lam_u), (bx_t, lam_t))):
-                    hx[b, f] = self._lu(hholtz_matrices(basis, lam, self.scale[0])[2])
-                for f, lam in enumerate((lam_u, lam_u, lam_t)):
-                    hy[b, f] = self._lu(hholtz_matrices(by_cd, lam, self.scale[1])[2])
+                for f in range(3):
+                    hx[b, f] = self._lu(self._helmholtz_matrix(st, b, f, 0))
+                    hy[b, f] = self._lu(self._helmholtz_matrix(st, b, f, 1))
             self.st_hx.append(T(hx))
             self.st_hy.append(T(hy))
             # normalised (reciprocal-pivot) form for the register-resident kernel
@@ -528,6 +525,36 @@ class RBCBatch:
         self._build_launches()
         self.tune()
         self.graph = None
+
+    def _helmholtz_matrix(self, st, b, f, axis):
+        """Dense Helmholtz operator B S - lam I S of stage st, instance b, field
+        f (U, V, T) along axis (hholtz.py:70-93): lam = dt a beta nu (U, V) or
+        dt a beta kappa (T); T's x basis follows the case."""
+        key = self.side if (axis == 0 and f == 2) else "CD"
+        n = self.N0 if axis == 0 else self.N1
+        coef = self.kappa[b] if f == 2 else self.nu[b]
+        lam = self.dt[b] * self.ka[st] * self.beta * coef
+        bases = self.__dict__.setdefault("_hbases", {})
+        if (n, key) not in bases:
+            bases[(n, key)] = Base(n, key)
+        return hholtz_matrices(bases[(n, key)], lam, self.scale[axis])[2]
+
+    def transposed_line_tables(self):
+        """Normalised tables of the transposed line solves on reversed lines
+        (stage 0), for the discrete adjoint: the x-direction Helmholtz solves per
+        (instance, field) entry, [B, 3, LM | R | MU | NU, M0], and the Poisson
+        line solves per line, [LM | R | MU | NU][M0][M1] and line-major
+        [4][M1][M0]."""
+        f64 = dict(dtype=ops.DTYPE, device=self.X.device)
+        hx = np.zeros((self.B, 3, 4, self.M0))
+        for b in range(self.B):
+            for f in range(3):
+                hx[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(0, b, f, 0)))
+        pm = poisson_matrices([Base(self.N0, "CN"), Base(self.N1, "CN")], singular=True,
+                              scale=self.scale)
+        pr = torch.as_tensor(np.stack(rec_poisson_reversed_transpose(pm["Ax"], pm["Cx"],
+                                                                     pm["wy"], True)), **f64)
+        return torch.as_tensor(hx, **f64), pr, pr.transpose(1, 2).contiguous()
 
     @staticmethod
     def _lu(A):

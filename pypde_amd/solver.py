@@ -53,6 +53,41 @@ def rec_from_poisson(tab, sing):
     return tuple(np.ascontiguousarray(t.T) for t in (LM, R, MU, NU))
 
 
+def _fdma_factor_diagonals(A):
+    """Diagonals -2, 0, 2, 4 of a dense four-diagonal matrix, LU-factored
+    (plans.py:226-234)."""
+    l, d, u1, u2 = (np.diag(A, k).copy() for k in (-2, 0, 2, 4))
+    fdma_lu(l, d, u1, u2)
+    return l, d, u1, u2
+
+
+def rec_reversed_transpose(A):
+    """Normalised tables of x = A^-T b solved on the reversed line.  With J
+    the reversal, J A^T J has A's band offsets -2..4, so (J A^T J)(J x) = J b is
+    a four-diagonal solve of the usual form over the line read backwards."""
+    A = _dense(A)
+    return rec_from_fdma(*_fdma_factor_diagonals(A[::-1, ::-1].T))
+
+
+def rec_poisson_reversed_transpose(A, C, alpha, singular):
+    """Per-line tables of x_j = (A + alpha_j C)^-T b_j on the reversed line,
+    the transpose of the line solves of poisson_line_tables.  A singular line
+    (|alpha_j| < 1e-10 with singular set) solves the trailing n-1 block and sets
+    x_0 = 0; its transpose maps b_0 to nothing and returns 0 there, so on the
+    reversed line it solves the leading n-1 block and returns 0 at position
+    n-1 (R = 0).  Laid out [n][nlines] per table."""
+    A, C = _dense(A), _dense(C)
+    Ar, Cr = A[::-1, ::-1].T, C[::-1, ::-1].T
+    alpha = np.asarray(alpha, dtype=float)
+    n, nl = A.shape[0], alpha.shape[0]
+    LM, R, MU, NU = (np.zeros((nl, n)) for _ in range(4))
+    for j in range(nl):
+        nn = n - 1 if (singular and abs(alpha[j]) < 1e-10) else n
+        M = Ar[:nn, :nn] + alpha[j] * Cr[:nn, :nn]
+        LM[j, :nn], R[j, :nn], MU[j, :nn], NU[j, :nn] = rec_from_fdma(*_fdma_factor_diagonals(M))
+    return tuple(np.ascontiguousarray(t.T) for t in (LM, R, MU, NU))
+
+
 def rec_from_tdma(s, a, b, w, den):
     """Galerkin projection (S^T S)^-1 S^T (tdma.f90:56-107) -> (pre, F, LM, MU)."""
     M = b.shape[0]
