@@ -108,8 +108,11 @@ class StepAdjoint:
                   "transposed line solve")
         key = ("rec_reversed", n, nlines, batch, bool(per_line))
         if autotune.enabled() and not autotune._capturing() and key not in autotune.cache():
-            gs = geom(torch.zeros_like(x))
+            scratch = torch.zeros_like(x)  # kept alive while the candidates run on it
+            gs = geom(scratch)
             autotune.choose(key, lambda v: run(gs, v), (1, 2, 3), default=2)
+            torch.cuda.synchronize()
+            del scratch
         run(geom(x), autotune.cache().get(key, 2))
 
     def forward_fields(self, X):

@@ -72,6 +72,22 @@ def _capturing():
         return False
 
 
+_delay_bufs = None
+
+
+def _queue_delay():
+    """Enqueue about a millisecond of GPU work (an FP64 matrix product) so that
+    the launches queued behind it while it runs execute back to back: the events
+    around them then time the GPU, not the host's launch rate, which for kernels
+    of a few microseconds is the larger of the two."""
+    global _delay_bufs
+    if _delay_bufs is None:
+        a = torch.ones((2560, 2560), dtype=torch.float64, device="cuda") / 2560.0
+        _delay_bufs = (a, torch.empty_like(a))
+    a, c = _delay_bufs
+    torch.matmul(a, a, out=c)
+
+
 def time_candidates(run, candidates, reps=5):
     """{candidate: us per launch}; run(c) enqueues one launch on the current stream."""
     out = {}
@@ -79,6 +95,7 @@ def time_candidates(run, candidates, reps=5):
         run(c)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
+        _queue_delay()
         e0.record()
         for _ in range(reps):
             run(c)
