@@ -65,7 +65,7 @@ class StepAdjoint:
         self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
         self.graph = None
         # transposed solves along x on reversed lines (four-diagonal recurrences)
-        self.hrt, prt, prt_lm = e.transposed_line_tables()
+        self.hrt, self.hrt_y, prt, prt_lm = e.transposed_line_tables()
         self.prt = (prt, prt_lm)
 
     def _reversed_solve(self, x, n, nlines, batch, tabs, per_line):
@@ -160,13 +160,13 @@ class StepAdjoint:
         g(Wdb[:, :M0], e.Cy[0], beta=1.0, D=Xb[:, 0], out=Rb[:, 0])
         g(Wdb[:, M0:], e.Cy[1], beta=1.0, D=Xb[:, 1], out=Rb[:, 1])
         Rb[:, 2].copy_(Xb[:, 2])
-        # 9-10  transposed Helmholtz ADI: R^ = Hx^-T X*^ Hy^-1
-        # tables [B, 3, (l | d | u1 | u2), M] with entry stride 4M; the views
-        # below only provide the base pointer of each slot
-        hx, hy = e.hx, e.hy
+        # 9-10  transposed Helmholtz ADI: R^ = Hx^-T X*^ Hy^-1, both as
+        # four-diagonal recurrences over reversed lines (tables of J H^T J)
         R3 = Rb.reshape(3 * B, M0, M1)
-        ops.fdma_solve((hy[:, :, 0], hy[:, :, 1], hy[:, :, 2], hy[:, :, 3]), R3, axis=1,
-                       transpose=True, table_bs=4 * M1)
+        # along y: on the transposed fields, so that the lines are columns
+        Rt = R3.transpose(1, 2).contiguous()
+        self._reversed_solve(Rt, M1, M0, 3 * B, self.hrt_y, False)
+        R3.copy_(Rt.transpose(1, 2))
         self._reversed_solve(R3, M0, M1, 3 * B, self.hrt, False)
         # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt BST_x^T R^_V
         Zb = torch.empty((B, 3, K, M1), **f64)

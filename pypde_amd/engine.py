@@ -541,20 +541,23 @@ class RBCBatch:
 
     def transposed_line_tables(self):
         """Normalised tables of the transposed line solves on reversed lines
-        (stage 0), for the discrete adjoint: the x-direction Helmholtz solves per
-        (instance, field) entry, [B, 3, LM | R | MU | NU, M0], and the Poisson
-        line solves per line, [LM | R | MU | NU][M0][M1] and line-major
+        (stage 0), for the discrete adjoint: the Helmholtz solves along x and
+        along y per (instance, field) entry, [B, 3, LM | R | MU | NU, M], and the
+        Poisson line solves per line, [LM | R | MU | NU][M0][M1] and line-major
         [4][M1][M0]."""
         f64 = dict(dtype=ops.DTYPE, device=self.X.device)
         hx = np.zeros((self.B, 3, 4, self.M0))
+        hy = np.zeros((self.B, 3, 4, self.M1))
         for b in range(self.B):
             for f in range(3):
                 hx[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(0, b, f, 0)))
+                hy[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(0, b, f, 1)))
         pm = poisson_matrices([Base(self.N0, "CN"), Base(self.N1, "CN")], singular=True,
                               scale=self.scale)
         pr = torch.as_tensor(np.stack(rec_poisson_reversed_transpose(pm["Ax"], pm["Cx"],
                                                                      pm["wy"], True)), **f64)
-        return torch.as_tensor(hx, **f64), pr, pr.transpose(1, 2).contiguous()
+        return (torch.as_tensor(hx, **f64), torch.as_tensor(hy, **f64), pr,
+                pr.transpose(1, 2).contiguous())
 
     @staticmethod
     def _lu(A):
