@@ -3,6 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
     python bench.py --micro              # microbench only (one JSON line)
+    python bench.py --adjoint            # adjoint gradient at 256x256 only (one JSON line)
     python bench.py --copy-calibration   # 256 MiB device copies, for scaling PMC byte counters
     python bench.py --tune-db PATH       # tune every workload afresh, write the decisions
 
@@ -481,6 +482,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="run the adjoint gradient at 256x256 (200 steps) only")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--all-launches", action="store_true",
                     help="after the plain step, replay every launch of it 21 times in step "
@@ -504,6 +507,9 @@ def main():
         return
     if args.micro:
         print(json.dumps({"microbench": microbench()}))
+        return
+    if args.adjoint:
+        print(json.dumps({"adjoint": time_adjoint(256, 200, ckpt=100)}))
         return
     if args.tune_db:
         from pypde_amd import autotune
