@@ -47,6 +47,8 @@ class NavierStokesAdjoint(NavierStokesBase, Integrator):
         N0, N1 = self.shape
         # inner solver for the residual (rbc2d_adj.py:44)
         self.NS = NavierStokes(case=case, **self.CONFIG)
+        # the captured update runs the inner step as the fused engine's launches
+        self.use_graph = self.use_graph and self.NS.engine is not None
 
         def fld(kx, ky):
             return Field([Base(N0, kx, dealias=3 / 2), Base(N1, ky, dealias=3 / 2)])
