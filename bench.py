@@ -550,7 +550,10 @@ def main():
         extra["steady"] = [time_steady((20, 16), 1e4), time_steady((64, 64), 1e4),
                            time_steady((128, 128), 5e4)]
         extra["adjoint_descent"] = time_adjoint_descent(256)
-        extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000)}
+        # checkpoints every 100 steps (the configured run), and every state kept
+        # in HBM (400 MB at 256x256 over 200 steps), which removes the recomputation
+        extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000),
+                            "256x256_all_states": time_adjoint(256, 200, ckpt=1)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(steps=args.cpu_steps)

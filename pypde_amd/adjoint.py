@@ -306,12 +306,14 @@ class Trajectory:
         for si in range(len(starts) - 1, -1, -1):
             k0 = starts[si]
             k1 = starts[si + 1] if si + 1 < len(starts) else nsteps
-            # recompute the segment's inputs
+            # recompute the segment's inputs X_k0 .. X_(k1-1) (the VJP of step k
+            # needs X_(k-1); X_k1 itself is not needed)
             self._set(*cks[k0])
             states = []
             for k in range(k0, k1):
                 states.append(self._state())
-                self._step()
+                if k + 1 < k1:
+                    self._step()
             for k in range(k1, k0, -1):
                 # Nu(T_k) enters J for k >= 1
                 Xb[:, 2] += g
