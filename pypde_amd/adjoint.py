@@ -33,10 +33,11 @@ def _vp(t):
 
 
 def _fields_gemm(ta, tb, m, n, k, batch, A, lda, iA, B, ldb, iB, C, ldc, iC, alpha=1.0,
-                 alpha_vec=None, beta=0.0, D=None, iD=None):
+                 alpha_vec=None, beta=0.0, D=None, iD=None, defer=False):
     """One batched GEMM over (instance, field) entries e = 3 b + f: operators
     indexed per field through a modulus-3 batch index, per-instance scales
-    alpha_vec[e / 3]; operands are device pointers."""
+    alpha_vec[e / 3]; operands are device pointers.  defer: return the
+    descriptor without launching."""
     d = ops._desc(ta, tb, m, n, k, batch, alpha, beta, A, lda, 0, B, ldb, 0, D,
                   ldc if D is not None else 0, 0, C, ldc, 0)
     d.iA, d.iB, d.iC = iA, iB, iC
@@ -44,6 +45,8 @@ def _fields_gemm(ta, tb, m, n, k, batch, A, lda, iA, B, ldb, iB, C, ldc, iC, alp
         d.iD = iD
     if alpha_vec is not None:
         d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), 3
+    if defer:
+        return d
     ops._launch_desc(d)
 
 
@@ -137,9 +140,12 @@ class StepAdjoint:
         # 21  velocity correction: W1b = -A_c0^T Ub, W2b = -A_c1^T Vb
         NW = N1 + 2 * M1
         Wb = torch.empty((B, M0, NW), **f64)
-        g(e.Snx, pb, transA=True, alpha_vec=e.t_idt, out=Wb[:, :, :N1])          # 20
-        g(e.Acorr[0], Xb[:, 0], transA=True, alpha=-1.0, out=Wb[:, :, N1:N1 + M1])
-        g(e.Acorr[1], Xb[:, 1], transA=True, alpha=-1.0, out=Wb[:, :, N1 + M1:])
+        ops.launch_grouped([
+            g(e.Snx, pb, transA=True, alpha_vec=e.t_idt, out=Wb[:, :, :N1], defer=True)[1],  # 20
+            g(e.Acorr[0], Xb[:, 0], transA=True, alpha=-1.0, out=Wb[:, :, N1:N1 + M1],
+              defer=True)[1],
+            g(e.Acorr[1], Xb[:, 1], transA=True, alpha=-1.0, out=Wb[:, :, N1 + M1:],
+              defer=True)[1]], "vjp_correction")
         divb = torch.zeros((B, N0, N1), **f64)
         check(lib.pypde_scaled_sub(N0 * N1, B, _vp(e.t_nu), e.beta, _vp(pb), _vp(divb),
                                    ops.stream_ptr()), "pres_div_vjp")
@@ -157,8 +163,10 @@ class StepAdjoint:
         Wdb = g(e.Cx, divb, transA=True)
         # 11  U*^ = Ub + Wd^_U C_y, V*^ = Vb + Wd^_V Cd_y, T*^ = Tb
         Rb = torch.empty((B, 3, M0, M1), **f64)
-        g(Wdb[:, :M0], e.Cy[0], beta=1.0, D=Xb[:, 0], out=Rb[:, 0])
-        g(Wdb[:, M0:], e.Cy[1], beta=1.0, D=Xb[:, 1], out=Rb[:, 1])
+        ops.launch_grouped([
+            g(Wdb[:, :M0], e.Cy[0], beta=1.0, D=Xb[:, 0], out=Rb[:, 0], defer=True)[1],
+            g(Wdb[:, M0:], e.Cy[1], beta=1.0, D=Xb[:, 1], out=Rb[:, 1], defer=True)[1]],
+            "vjp_div")
         Rb[:, 2].copy_(Xb[:, 2])
         # 9-10  transposed Helmholtz ADI: R^ = Hx^-T X*^ Hy^-1, both as
         # four-diagonal recurrences over reversed lines (tables of J H^T J)
@@ -175,18 +183,20 @@ class StepAdjoint:
                      Rb.data_ptr(), M1, bi(M0 * M1), Zb.data_ptr(), M1, bi(K * M1))
         zt = Zb[:, 2, L0 + N0:]
         g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.t_dt, beta=1.0, D=zt, out=zt)
-        # 6  X^ (input) = Z^_f[2] BS_y
+        # 6  X^ (input) = Z^_f[2] BS_y  and  4  P^r_f = -dt Z^_f[0] BF_y (one launch)
         Xin = torch.empty((B, 3, M0, M1), **f64)
-        _fields_gemm(0, 0, M0, M1, M1, 3 * B, Zb.data_ptr() + 8 * (L0 + N0) * M1, M1,
-                     bi(K * M1), e.BSy.data_ptr(), M1, bi(0), Xin.data_ptr(), M1, bi(M0 * M1))
+        Prb = torch.empty((B, 3, L0, L1), **f64)
+        ops.launch_grouped([
+            _fields_gemm(0, 0, M0, M1, M1, 3 * B, Zb.data_ptr() + 8 * (L0 + N0) * M1, M1,
+                         bi(K * M1), e.BSy.data_ptr(), M1, bi(0), Xin.data_ptr(), M1,
+                         bi(M0 * M1), defer=True),
+            _fields_gemm(0, 0, L0, L1, M1, 3 * B, Zb.data_ptr(), M1, bi(K * M1),
+                         e.BFy.data_ptr(), L1, bi(0), Prb.data_ptr(), L1, bi(L0 * L1),
+                         alpha_vec=e.t_mdt, defer=True)], "vjp_z")
         # 5  pres^ (input) = pres^ - dt (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
         pin = pb.clone()
         g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
         g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
-        # 4  P^r_f = -dt Z^_f[0] BF_y
-        Prb = torch.empty((B, 3, L0, L1), **f64)
-        _fields_gemm(0, 0, L0, L1, M1, 3 * B, Zb.data_ptr(), M1, bi(K * M1), e.BFy.data_ptr(),
-                     L1, bi(0), Prb.data_ptr(), L1, bi(L0 * L1), alpha_vec=e.t_mdt)
         # 3  products
         self.forward_fields(X)
         check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb), _vp(self.Y2b),

@@ -259,10 +259,11 @@ def gemm(A, B, transA=False, transB=False, alpha=1.0, beta=0.0, D=None, out=None
 
 
 def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0, D=None,
-          out=None):
+          out=None, defer=False):
     """Batched GEMM over a leading batch dimension with broadcasting of 2-D
     operands (one launch of pypde_dgemm_batched); alpha_vec [batch] scales
-    entry e by alpha * alpha_vec[e]."""
+    entry e by alpha * alpha_vec[e].  defer: return (out, descriptor) without
+    launching (for launch_grouped)."""
 
     def geo(T):
         if T.dim() == 2:
@@ -292,8 +293,28 @@ def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0,
               B3.data_ptr(), B3.stride(-2), sb if bb > 1 else 0,
               Dp, ldd, sD, o3.data_ptr(), o3.stride(-2), so,
               alpha_vec.data_ptr() if alpha_vec is not None else None)
+    if defer:
+        return out, d
     _launch_desc(d)
     return out
+
+
+def launch_grouped(descs, tag):
+    """Independent GEMM descriptors with the same transposes in one
+    pypde_dgemm_grouped launch (at most four), the kernel variant tuned per
+    group of shapes on first use."""
+    from . import autotune
+    from ._lib import GemmDesc
+    n = len(descs)
+    arr = (GemmDesc * n)(*descs)
+    s = stream_ptr()
+
+    def run(v):
+        for i in range(n):
+            arr[i].variant = v
+        check(lib.pypde_dgemm_grouped(ctypes.byref(arr), n, s), "pypde_dgemm_grouped")
+    key = ("ops_group", tag) + tuple(autotune.gemm_key(d) for d in descs)
+    run(autotune.choose(key, run, autotune.GEMM_CANDIDATES, default=0))
 
 
 # ---------------------------------------------------------------------------
