@@ -98,7 +98,8 @@ typedef struct pypde_gemm_desc {
     int zero_first;
     long long csn; /* column stride of C, D and E (0 or 1: contiguous) */
     /* parity folds of mirror-symmetric operators (halve the MFMA work), variants 1-5
-     * (64x64, 32x32, 128x128, 128x64, 64x128 tiles):
+     * (64x64, 32x32, 128x128, 128x64, 64x128 tiles); for the output folds also 6, 7
+     * (32x32 tiles, each product over its own waves with K split 2 / 4 ways):
      *   fold 1 / 2: output rows / columns of C come in blocks of fold_len mirrored
      *     pairs; m (fold 1) or n (fold 2) counts the first halves of the blocks;
      *     E = A B over k (first product), O = A1 B1 over k1 (second product) and

@@ -32,6 +32,8 @@ ROWS_CANDIDATES = (1, 2, 4, 8, 18, 20, 24)
 # 4..32 rows per block; 37..40: eight rows in flight, 8..64 rows per block)
 ROWS_CANDIDATES_AXIS1 = ROWS_CANDIDATES + (33, 34, 35, 36, 37, 38, 39, 40)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
+# output folds (modes 1, 2) also run split over waves
+OUTPUT_FOLD_CANDIDATES = FOLD_CANDIDATES + (6, 7)
 GEMM_CANDIDATES = tuple(range(1, 27))
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
@@ -184,5 +186,7 @@ def tune_gemm_desc(d, launch, allow_alias=False):
     def run(v):
         d.variant = v
         launch(d)
-    d.variant = choose(gemm_key(d), run, FOLD_CANDIDATES if d.fold else GEMM_CANDIDATES)
+    cands = (OUTPUT_FOLD_CANDIDATES if d.fold in (1, 2) else FOLD_CANDIDATES) if d.fold \
+        else GEMM_CANDIDATES
+    d.variant = choose(gemm_key(d), run, cands)
     return d.variant

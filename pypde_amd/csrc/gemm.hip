@@ -903,6 +903,174 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, 
             }
 }
 
+// Output folds (modes 1, 2) split over waves: waves [0, KW) run the first
+// product (even k) and waves [KW, 2 KW) the second (odd k), each wave a
+// contiguous K range of its product staged through its own LDS slabs (no
+// block barriers in the K loop), so a wave's serial chain is a 1 / (2 KW) share
+// of the single-block form's.  The partial tiles meet in LDS; E and O are each
+// summed in slice order and written through the fold epilogue.
+template <int BM, int BN, int KW, int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(128 * KW) dgemm_fold_wsplit_kernel(GemmArgs g, FoldArgs f)
+{
+    static_assert(MODE == 1 || MODE == 2, "output folds only");
+    constexpr int BK = 16;
+    constexpr int PA = BM + 1, PB = BN + 1;
+    constexpr int SLAB = BK * PA + BK * PB;
+    constexpr int TM = BM / 16, TN = BN / 16;
+    constexpr int LA = BM * BK / 64, LB = BN * BK / 64;
+    constexpr int NW = 2 * KW;
+    constexpr int E = BM * BN;
+    constexpr int AREA = (NW * SLAB > NW * E) ? NW * SLAB : NW * E;
+    static_assert(64 % BK == 0 && 64 % BM == 0 && 64 % BN == 0, "load mapping: BM, BN divide 64");
+    __shared__ double lds[AREA];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int prod = wave / KW, slice = wave - prod * KW;
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    const int m0 = by * BM;
+    const int n0 = bx * BN;
+    const long long oA = bofs(bz, g.iA), oB = bofs(bz, g.iB);
+    const double* __restrict__ A = (prod ? f.A1 : g.A) + oA;
+    const double* __restrict__ B = (prod ? f.B1 : g.B) + oB;
+    const long long lda = prod ? f.lda1 : g.lda, ldb = prod ? f.ldb1 : g.ldb;
+    const int K = prod ? f.k1 : g.k;
+    const int kc = ((K + KW - 1) / KW + BK - 1) / BK * BK;
+    const int kb = slice * kc;
+    const int ke = min(kb + kc, K);
+    double* As = lds + wave * SLAB;
+    double* Bs = As + BK * PA;
+
+    v4d acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[LA], rb[LB];
+    auto a_rc = [&](int q, int& r, int& c) {
+        if (!TA) { c = lane % BK; r = lane / BK + q * (64 / BK); }
+        else     { r = lane % BM; c = lane / BM + q * (64 / BM); }
+    };
+    auto b_rc = [&](int q, int& r, int& c) {
+        if (!TB) { c = lane % BN; r = lane / BN + q * (64 / BN); }
+        else     { r = lane % BK; c = lane / BK + q * (64 / BK); }
+    };
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            const int gm = m0 + r, gk = k0 + c;
+            ra[q] = (gm < g.m && gk < ke) ? (TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + (long long)gk * f.ska]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            const int gk = k0 + r, gn = n0 + c;
+            rb[q] = (gk < ke && gn < g.n) ? (TB ? B[(long long)gn * ldb + (long long)gk * f.skb] : B[(long long)gk * ldb + gn]) : 0.0;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int q = 0; q < LA; ++q) {
+            int r, c;
+            a_rc(q, r, c);
+            As[c * PA + r] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            int r, c;
+            b_rc(q, r, c);
+            Bs[r * PB + c] = rb[q];
+        }
+    };
+
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        wave_lds_sync();  // previous slab fully read
+        store();
+        wave_lds_sync();
+        if (k0 + BK < ke) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[kr * PA + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[kr * PB + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    __syncthreads();  // every wave is done with its slabs; the area becomes the reduction buffer
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                lds[wave * E + ((i * TN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+    __syncthreads();
+
+    const Epilogue ep(g, bz);
+    const int Lb = f.fold_len, Lh = (Lb + 1) / 2;
+    for (int e = tid; e < E; e += 64 * NW) {
+        double ev = 0.0, ov = 0.0;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) {
+            ev += lds[w * E + e];
+            ov += lds[(KW + w) * E + e];
+        }
+        const int l = e & 63;
+        const int r = (e >> 6) & 3;
+        const int t = e >> 8;
+        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        const int gn = n0 + (t % TN) * 16 + (l & 15);
+        if (gm >= g.m || gn >= g.n) continue;
+        const int x = (MODE == 1) ? gm : gn;
+        const int b = x / Lh, jj = x - b * Lh;
+        const int x1 = b * Lb + jj, x2 = b * Lb + Lb - 1 - jj;
+        const double sg = ((f.fold_sign >> b) & 1u) ? -1.0 : 1.0;
+        if (MODE == 1) {
+            ep.store(x1, gn, ev + ov);
+            if (x2 != x1) ep.store(x2, gn, sg * (ev - ov));
+        } else {
+            ep.store(gm, x1, ev + ov);
+            if (x2 != x1) ep.store(gm, x2, sg * (ev - ov));
+        }
+    }
+}
+
+template <int BM, int BN, int KW>
+static void launch_fold_wsplit(int mode, bool ta, bool tb, const GemmArgs& g, const FoldArgs& f,
+                               int batch, hipStream_t s)
+{
+    dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
+    dim3 block(128 * KW);
+#define PYPDE_FW(M, TA_, TB_) hipLaunchKernelGGL((dgemm_fold_wsplit_kernel<BM, BN, KW, M, TA_, TB_>), grid, block, 0, s, g, f)
+    if (mode == 1) {
+        if (!ta && !tb) PYPDE_FW(1, false, false);
+        else if (!ta && tb) PYPDE_FW(1, false, true);
+        else if (ta && !tb) PYPDE_FW(1, true, false);
+        else PYPDE_FW(1, true, true);
+    } else {
+        if (!ta && !tb) PYPDE_FW(2, false, false);
+        else if (!ta && tb) PYPDE_FW(2, false, true);
+        else if (ta && !tb) PYPDE_FW(2, true, false);
+        else PYPDE_FW(2, true, true);
+    }
+#undef PYPDE_FW
+}
+
 template <int BM, int BN, int WGM, int WGN, int MODE>
 static void launch_fold_m(bool ta, bool tb, const GemmArgs& g, const FoldArgs& f, int batch, hipStream_t s)
 {
@@ -1050,7 +1218,8 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
         const long long tiles64 = (long long)((g.m + 63) / 64) * ((g.n + 63) / 64) * d->batch;
         variant = tiles128 >= 512 ? 3 : (tiles64 >= 512 ? 1 : 2);
     }
-    PYPDE_REQUIRE(variant >= 1 && variant <= 5, "folded GEMMs support variants 1-5 (got %d)", variant);
+    PYPDE_REQUIRE(variant >= 1 && variant <= 7, "folded GEMMs support variants 1-7 (got %d)", variant);
+    PYPDE_REQUIRE(variant <= 5 || d->fold <= 2, "fold variants 6, 7 are for the output folds (1, 2)");
     const bool ta = d->transA, tb = d->transB;
     // smaller wave tiles than the plain kernels: the output folds carry two
     // accumulator sets (32x32 per wave at 128x128 over 16 waves)
@@ -1059,7 +1228,10 @@ int dgemm_batched(const pypde_gemm_desc* d, hipStream_t stream)
     case 2: launch_fold<32, 32, 2, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
     case 3: launch_fold<128, 128, 4, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
     case 4: launch_fold<128, 64, 4, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
-    default: launch_fold<64, 128, 2, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    case 5: launch_fold<64, 128, 2, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    // each product over its own waves, K split KW ways (32 x 32 tiles)
+    case 6: launch_fold_wsplit<32, 32, 2>(d->fold, ta, tb, g, f, d->batch, stream); break;
+    default: launch_fold_wsplit<32, 32, 4>(d->fold, ta, tb, g, f, d->batch, stream); break;
     }
     return check_launch("pypde_dgemm_batched (folded)");
 }

@@ -252,7 +252,7 @@ def _sym_operator(rng, nb, Lb, K, signs):
     return G
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("Lb,K", [(8, 10), (7, 9), (96, 62)])
 def test_gemm_output_fold(amd, variant, Lb, K):
     """fold 1 (rows) and 2 (columns): C = G X and C = Y G^T for mirror-symmetric
@@ -512,3 +512,19 @@ def test_banded_every_row_hint_bit_exact(amd, axis):
         from pypde_amd._lib import PypdeError
         with pytest.raises(PypdeError):
             ops.banded_apply(p.offsets, p.vals, ops.to_dev(x), 0, p.M_out, lpb=33)
+
+
+def test_split_fold_variants_reject_input_folds(amd):
+    """Fold variants 6 and 7 (each product over its own waves) serve the output
+    folds only; an input-fold descriptor asking for them is rejected before
+    any launch."""
+    import ctypes
+    _, ops, torch = amd
+    from pypde_amd._lib import lib
+    x = torch.zeros((64, 64), dtype=torch.float64, device="cuda")
+    p = x.data_ptr()
+    for fold, v in ((3, 6), (4, 7)):
+        d = ops._desc(0, 0, 8, 8, 4, 1, 1.0, 0.0, p, 8, 0, p, 8, 0, None, 0, 0,
+                      p + 8 * 32 * 64, 8, 0)
+        d.fold, d.fold_len, d.mirror_sign, d.mirror_div, d.variant = fold, 8, 1.0, 1, v
+        assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) != 0
