@@ -65,19 +65,19 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12610769,
-    "ypass_v+ypass_x": 23610658,
-    "products": 14242773,
-    "z_conv+z_pres+z_old": 17938633,
-    "rhs_part+buoyancy_part": 20555328,
-    "splitk_reduce": 8530993,
-    "helmholtz_x": 18904180,
-    "helmholtz_y": 3502259,
-    "div_y": 11544348,
-    "div_x+poisson_rhs": 12054134,
-    "poisson_lines": 8500479,
-    "poisson_qy+corr_y+corr_y0": 15116775,
-    "pres_p+correct": 10763964,
+    "xpass": 12620988,
+    "ypass_v+ypass_x": 23610614,
+    "products": 14242757,
+    "z_conv+z_pres+z_old": 17938606,
+    "rhs_part+buoyancy_part": 20555305,
+    "splitk_reduce": 8530986,
+    "helmholtz_x": 18904153,
+    "helmholtz_y": 3502253,
+    "div_y": 11542291,
+    "div_x+poisson_rhs": 12054128,
+    "poisson_lines": 8500470,
+    "poisson_qy+corr_y+corr_y0": 15116764,
+    "pres_p+correct": 10763956,
     # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
     "rhs": 26877192,
     "buoyancy": 5797532,
