@@ -350,11 +350,11 @@ def microbench(nf=512, reps=10):
                 device_copy=dict(copy, kernel="torch.mul(x, 1.0, out=y), 512 x 256 x 256 float64"))
 
 
-def time_variant(steps=100, **kw):
-    """Steps/s of the fused 256x256 step for another integrator / beta (graph replay)."""
+def time_variant(steps=100, N=256, **kw):
+    """Steps/s of the fused N x N step (Ra=1e8, dt=1e-3) for another integrator /
+    beta or size (graph replay)."""
     import torch
     from pypde_amd.navier import NavierStokes
-    N = 256
     ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None, **kw)
     assert ns.fused
     ns.set_state(T=ns.T.forward(seeded_noise((N, N), seed=0)))
@@ -364,7 +364,7 @@ def time_variant(steps=100, **kw):
     ns.update_many(steps)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
-    return dict(kw, steps=steps, steps_per_s=round(steps / sec, 1),
+    return dict(kw, N=N, steps=steps, steps_per_s=round(steps / sec, 1),
                 finite=bool(np.isfinite(ns.eval_Nu()[0])))
 
 
@@ -519,6 +519,7 @@ def main():
         time_hetero()
         time_variant(steps=5, integrator="rk3")
         time_variant(steps=5, beta=0.5)
+        time_variant(steps=5, N=512)
         microbench()
         for shape, ra in (((20, 16), 1e4), ((64, 64), 1e4), ((128, 128), 5e4)):
             time_steady(shape, ra, pre=20, tol=1e-3)
@@ -545,7 +546,8 @@ def main():
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
         extra["hetero"] = time_hetero()
-        extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5)]
+        extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5),
+                             time_variant(N=512)]
         extra["microbench"] = microbench()
         extra["steady"] = [time_steady((20, 16), 1e4), time_steady((64, 64), 1e4),
                            time_steady((128, 128), 5e4)]
