@@ -188,14 +188,15 @@ class NavierStokesBase:
         the reference's [T, U, V] vector (default: the current state); the
         current pressure starts the pressure part.  Converged when
         max|F| <= tol.  jac_options: inner_maxiter (GMRES restart cycles per
-        Newton step), inner_m (Krylov basis size, 30), rdiff (finite-difference
-        step, sqrt of the unit roundoff).
+        Newton step), inner_m (Krylov basis size, 100: the basis lives in
+        device memory, and 30 vectors stall between restarts on the harder
+        cases), rdiff (finite-difference step, sqrt of the unit roundoff).
 
         Returns a result with x (the [T, U, V] vector), pres, success, status,
         message, nit, nfev, fun (F at x, [T, U, V] part), fnorm (max|F| over
         all of z) and history; the solver state is set to the solution."""
         from ..steady import newton_krylov
-        opts = dict(inner_maxiter=30, inner_m=30, rdiff=None)
+        opts = dict(inner_maxiter=30, inner_m=100, rdiff=None)
         opts.update(jac_options or {})
         unknown = set(opts) - {"inner_maxiter", "inner_m", "rdiff"}
         if unknown:

@@ -91,7 +91,7 @@ def gmres(matvec, b, tol=1e-6, restart=30, maxiter=20, x0=None):
     return x, its, rel
 
 
-def newton_krylov(F, z0, fatol=1e-8, maxiter=300, inner_m=30, inner_maxiter=30,
+def newton_krylov(F, z0, fatol=1e-8, maxiter=300, inner_m=100, inner_maxiter=30,
                   rdiff=None, eta_max=0.9, disp=False):
     """Inexact Newton for F(z) = 0 (z, F(z) device tensors).
 
