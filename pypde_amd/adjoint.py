@@ -270,12 +270,22 @@ class NusseltFunctional:
 class Trajectory:
     """Forward trajectory with checkpoints and its reverse (adjoint) sweep."""
 
-    def __init__(self, eng: RBCBatch, ckpt=100, use_graph=True):
+    def __init__(self, eng: RBCBatch, ckpt=None, use_graph=True, budget=16 * 2**30):
+        """ckpt: steps between stored states; None keeps every state when the
+        trajectory fits in `budget` bytes of device memory (no recomputation),
+        else spaces the checkpoints to fit."""
         self.e = eng
         self.adj = StepAdjoint(eng)
         self.nu = NusseltFunctional(eng)
         self.ckpt = ckpt
+        self.budget = budget
         self.use_graph = use_graph
+
+    def _spacing(self, nsteps):
+        if self.ckpt is not None:
+            return self.ckpt
+        state = 8 * (self.e.X.numel() + self.e.pres.numel())
+        return max(1, -(-nsteps * state // self.budget))
 
     def _step(self):
         """One forward step: the engine's graph replay, or its launches."""
@@ -298,10 +308,11 @@ class Trajectory:
         self._set(X0, pres0)
         cks = {0: self._state()}
         acc = torch.zeros(self.e.B, dtype=ops.DTYPE, device=X0.device)
+        ckpt = self._spacing(nsteps)
         for k in range(1, nsteps + 1):
             self._step()
             acc += self.nu.value(self.e.X[:, 2])
-            if k % self.ckpt == 0 and k < nsteps:
+            if k % ckpt == 0 and k < nsteps:
                 cks[k] = self._state()
         return acc / nsteps, cks
 

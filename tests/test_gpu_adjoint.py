@@ -157,7 +157,8 @@ def test_residual_gradient_direction_vs_reference_adjoint(golden):
 def test_graph_replayed_sweep_is_bit_identical():
     """Trajectory with graph-replayed forward steps and VJPs gives the op-by-op
     sweep's J and gradients bit for bit (same launches in the same order), also
-    on a second call that reuses the captured graphs."""
+    on a second call that reuses the captured graphs, and with every state kept
+    instead of checkpoints."""
     import torch
     from pypde_amd.adjoint import Trajectory
     from pypde_amd.engine import RBCBatch
@@ -180,3 +181,16 @@ def test_graph_replayed_sweep_is_bit_identical():
     for a, b in zip(out[0], out[1]):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+    # every state kept (the default when the trajectory fits in device memory):
+    # no recomputation, the same gradient
+    eng = RBCBatch((24, 20), [dict(ra=5e4, pr=1.0, dt=0.005),
+                              dict(ra=2e4, pr=1.0, dt=0.005)], case="rbc")
+    rng = np.random.default_rng(12)
+    X0 = torch.zeros_like(eng.X)
+    X0[:, 2] = torch.as_tensor(rng.uniform(-0.05, 0.05, tuple(X0[:, 2].shape)),
+                               device=eng.X.device)
+    traj = Trajectory(eng)
+    assert traj._spacing(23) == 1
+    J, Xb, pb = traj.grad_nusselt(X0, torch.zeros_like(eng.pres), 23)
+    for x, y in zip([t.detach().cpu().numpy() for t in (J, Xb, pb)], out[0][0]):
+        assert np.array_equal(x, y)
