@@ -495,8 +495,8 @@ def main():
     if args.tune_db:
         os.environ["PYPDE_TUNE_DB"] = ""  # measure, do not load stored decisions
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (got --gpus %d); multi-GPU runs are "
-              "not part of it." % args.gpus, file=sys.stderr)
+        print("bench.py: pypde_amd runs one instance set on one GPU (got --gpus %d); "
+              "multi-GPU execution is not implemented." % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

@@ -1,7 +1,7 @@
 """Write the golden fixtures for the functions that no closed form pins.
 
-Run in the development container only (the reference tree is not present on
-GPU boxes):  python tests/golden/make_golden.py
+Run where the reference source tree is installed (machines that only run the
+tests do not need it):  python tests/golden/make_golden.py
 
 Imports preiter93/pypde from /root/reference to record input/output vectors of
   * navier.rbc2d.NavierStokes.update   (navier/rbc2d.py:396-434): single-step
