@@ -1110,8 +1110,10 @@ static BIdx strided(long long s)
 //   0 heuristic, 1 LDS 64x64 (4 waves), 2 LDS 32x32 (4 waves),
 //   3 LDS 128x128 (8 waves), 4 LDS 128x128 (8 waves, double-buffered LDS),
 //   5 LDS 128x64 (8 waves), 6 LDS 64x128 (8 waves),
+//   7, 24, 25, 26, 27 LDS 128x64, 64x128, 64x64, 128x128, 32x32 with two slabs
+//   of loads in flight, 28 LDS 32x32 double-buffered,
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
-//   4 waves, 11 32x64 over 4 waves,
+//   4 waves, 11 32x64 over 4 waves, 12..15 those four with two slabs in flight,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
 static int heuristic_variant(const GemmArgs& g, int batch)
 {
@@ -1134,7 +1136,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 26, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 28, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -1147,6 +1149,8 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 24: launch_gemm<64, 128, 2, 4, 2>(transA, transB, g, batch, stream); break;
     case 25: launch_gemm<64, 64, 2, 2, 2>(transA, transB, g, batch, stream); break;
     case 26: launch_gemm<128, 128, 4, 2, 2>(transA, transB, g, batch, stream); break;
+    case 27: launch_gemm<32, 32, 2, 2, 2>(transA, transB, g, batch, stream); break;
+    case 28: launch_gemm<32, 32, 2, 2, 1>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1271,7 +1275,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 26, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 28, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1284,6 +1288,8 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 24: launch_gemm_group<64, 128, 2, 4, 2>(ta, tb, gg, batch, stream); break;
     case 25: launch_gemm_group<64, 64, 2, 2, 2>(ta, tb, gg, batch, stream); break;
     case 26: launch_gemm_group<128, 128, 4, 2, 2>(ta, tb, gg, batch, stream); break;
+    case 27: launch_gemm_group<32, 32, 2, 2, 2>(ta, tb, gg, batch, stream); break;
+    case 28: launch_gemm_group<32, 32, 2, 2, 1>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
