@@ -82,7 +82,8 @@ typedef struct pypde_gemm_desc {
                     32x32; 3 LDS-tiled 128x128; 4 the same, double-buffered; 5 LDS-tiled
                     128x64; 6 LDS-tiled 64x128; 7, 24, 25, 26, 27 LDS-tiled 128x64,
                     64x128, 64x64, 128x128, 32x32 with two slabs of loads in flight;
-                    28 LDS-tiled 32x32, double-buffered;
+                    28 LDS-tiled 32x32, double-buffered; 29, 30 LDS-tiled 32x32 over
+                    2 waves / 1 wave; 31, 32 the same with two slabs in flight;
                     8..11 split-K LDS-staged (32x32 over 4 / 8 waves, 64x32, 32x64);
                     12..15 the same with two slabs of loads in flight per wave;
                     16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)

@@ -1111,7 +1111,9 @@ static BIdx strided(long long s)
 //   3 LDS 128x128 (8 waves), 4 LDS 128x128 (8 waves, double-buffered LDS),
 //   5 LDS 128x64 (8 waves), 6 LDS 64x128 (8 waves),
 //   7, 24, 25, 26, 27 LDS 128x64, 64x128, 64x64, 128x128, 32x32 with two slabs
-//   of loads in flight, 28 LDS 32x32 double-buffered,
+//   of loads in flight, 28 LDS 32x32 double-buffered, 29 and 30 LDS 32x32 over
+//   2 waves (16x32 each) and 1 wave (four 16x16 accumulators), 31 and 32 those
+//   two with two slabs of loads in flight,
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
 //   4 waves, 11 32x64 over 4 waves, 12..15 those four with two slabs in flight,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
@@ -1136,7 +1138,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 28, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 32, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -1151,6 +1153,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 26: launch_gemm<128, 128, 4, 2, 2>(transA, transB, g, batch, stream); break;
     case 27: launch_gemm<32, 32, 2, 2, 2>(transA, transB, g, batch, stream); break;
     case 28: launch_gemm<32, 32, 2, 2, 1>(transA, transB, g, batch, stream); break;
+    case 29: launch_gemm<32, 32, 2, 1, 0>(transA, transB, g, batch, stream); break;
+    case 30: launch_gemm<32, 32, 1, 1, 0>(transA, transB, g, batch, stream); break;
+    case 31: launch_gemm<32, 32, 2, 1, 2>(transA, transB, g, batch, stream); break;
+    case 32: launch_gemm<32, 32, 1, 1, 2>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1275,7 +1281,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 28, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 32, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1290,6 +1296,10 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 26: launch_gemm_group<128, 128, 4, 2, 2>(ta, tb, gg, batch, stream); break;
     case 27: launch_gemm_group<32, 32, 2, 2, 2>(ta, tb, gg, batch, stream); break;
     case 28: launch_gemm_group<32, 32, 2, 2, 1>(ta, tb, gg, batch, stream); break;
+    case 29: launch_gemm_group<32, 32, 2, 1, 0>(ta, tb, gg, batch, stream); break;
+    case 30: launch_gemm_group<32, 32, 1, 1, 0>(ta, tb, gg, batch, stream); break;
+    case 31: launch_gemm_group<32, 32, 2, 1, 2>(ta, tb, gg, batch, stream); break;
+    case 32: launch_gemm_group<32, 32, 1, 1, 2>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
