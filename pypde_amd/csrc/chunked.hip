@@ -458,7 +458,9 @@ __global__ void __launch_bounds__(64) lane_chunk_kernel(RecArgs a)
 {
     constexpr int NX = PRE ? C + 2 : C;
     const int lane = threadIdx.x;
-    const long long gl = blockIdx.x;               // (batch entry, line)
+    // (batch entry, line); along axis 0 neighbouring lines share every cache
+    // line they touch, so they run on one XCD
+    const long long gl = AXIS == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : blockIdx.x;
     const int b = (int)(gl / a.nlines);
     const int line = (int)(gl - (long long)b * a.nlines);
     const int n = a.n;

@@ -32,6 +32,21 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 constexpr int kWave = 64;
 
+// Workgroups are dispatched to the device's 8 XCDs in turn (workgroup i to
+// XCD i mod 8), and each XCD has its own L2.  Renumbering them so that XCD x
+// runs a contiguous range of work items keeps the cache lines that
+// neighbouring items share (GEMM operand panels, the 128-byte lines that
+// neighbouring columns of a row-major array share) in one L2 instead of
+// fetching them into all eight.  The map is a bijection on [0, total):
+// results do not change, only where the work runs.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_remap(int pid, int total)
+{
+    const int q = total / kXcd, r = total - q * kXcd;
+    const int x = pid % kXcd, local = pid / kXcd;
+    return x * q + (x < r ? x : r) + local;
+}
+
 }  // namespace pypde
 
 #define PYPDE_REQUIRE(cond, ...)                 \

@@ -128,18 +128,8 @@ struct Epilogue {
     double beta_e_gamma;
 };
 
-// Workgroups are dispatched to the device's 8 XCDs in turn (workgroup i to
-// XCD i mod 8), and each XCD has its own L2.  Renumbering them so that XCD x
-// runs a contiguous range of tiles keeps the operand panels those tiles share
-// in one L2 instead of fetching every panel into all eight.  The map is a
-// bijection on [0, total): results do not change, only where tiles run.
-constexpr int kXcd = 8;
-__device__ __forceinline__ int xcd_remap(int pid, int total)
-{
-    const int q = total / kXcd, r = total - q * kXcd;
-    const int x = pid % kXcd, local = pid / kXcd;
-    return x * q + (x < r ? x : r) + local;
-}
+// GEMM tiles go through xcd_remap (common.h): the tiles that one XCD runs are
+// a contiguous range, so the operand panels they share stay in one L2.
 
 // Tile (bx, by, bz) of a 3-D grid after the XCD renumbering.
 __device__ __forceinline__ void grid_tile(int& bx, int& by, int& bz)
