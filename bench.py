@@ -65,19 +65,19 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12620988,
-    "ypass_v+ypass_x": 23610614,
-    "products": 14242757,
-    "z_conv+z_pres+z_old": 17938606,
-    "rhs_part+buoyancy_part": 20555305,
-    "splitk_reduce": 8530986,
-    "helmholtz_x": 18904153,
-    "helmholtz_y": 3502253,
-    "div_y": 11542291,
-    "div_x+poisson_rhs": 12054128,
-    "poisson_lines": 8500470,
-    "poisson_qy+corr_y+corr_y0": 15116764,
-    "pres_p+correct": 10763956,
+    "xpass": 12619997,
+    "ypass_v+ypass_x": 23613760,
+    "products": 14242782,
+    "z_conv+z_pres+z_old": 17941723,
+    "rhs_part+buoyancy_part": 21318560,
+    "splitk_reduce": 6950349,
+    "helmholtz_x": 3658476,
+    "helmholtz_y": 3515574,
+    "div_y": 11537187,
+    "div_x+poisson_rhs": 12058232,
+    "poisson_lines": 3499873,
+    "poisson_qy+corr_y+corr_y0": 15149545,
+    "pres_p+correct": 10739395,
     # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
     "rhs": 26877192,
     "buoyancy": 5797532,

@@ -10,7 +10,8 @@
 // Two kernel families:
 //
 //  * LDS-tiled (throughput): block tile BM x BN (128x128 over 8 wave64s in a
-//    4x2 arrangement, or 64x64 / 32x32 over 4 waves in 2x2), BK = 16; each
+//    4x2 arrangement, or 64x64 / 32x32 over 4 waves in 2x2), BK = 16 (32 in
+//    variants 33 and 34); each
 //    wave owns a (BM/WGM)x(BN/WGN) sub-tile made of 16x16 MFMA tiles.  Tiles
 //    of A and B are staged through LDS (k-major, one double of padding per
 //    row), the next K-slab is prefetched into registers while the current one
@@ -172,10 +173,9 @@ __device__ __forceinline__ int group_tile(const GemmGroup& gg, int& bx, int& by,
 // two LDS buffers (one barrier per slab); 2: one LDS buffer, two slabs of
 // loads in flight (registers for two).  Every form runs the same slabs in the
 // same order, so results are bit-identical.
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
 __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int by, int bz)
 {
-    constexpr int BK = 16;
     constexpr int NT = 64 * WGM * WGN;  // threads
     constexpr int PA = BM + 1;
     constexpr int PB = BN + 1;
@@ -320,31 +320,31 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
             }
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 {
     int bx, by, bz;
     grid_tile(bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB>(g, bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK>(g, bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_group(GemmGroup gg)
 {
     int bx, by, bz;
     const int i = group_tile(gg, bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB>(gg.a[i], bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK>(gg.a[i], bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE>
+template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16>
 static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, false>), grid, block, 0, s, g);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, true>), grid, block, 0, s, g);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, false>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, true>), grid, block, 0, s, g);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, false, BK>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, true, BK>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, false, BK>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, true, BK>), grid, block, 0, s, g);
 }
 
 static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
@@ -358,17 +358,17 @@ static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
     }
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE>
+template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16>
 static void launch_gemm_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
 {
     group_grid(gg, batch, BM, BN);
     if (gg.start[gg.count] == 0) return;
     dim3 grid(gg.start[gg.count]);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, false>), grid, block, 0, s, gg);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, true>), grid, block, 0, s, gg);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, false>), grid, block, 0, s, gg);
-    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, true>), grid, block, 0, s, gg);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, false, BK>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, true, BK>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, false, BK>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, true, BK>), grid, block, 0, s, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,7 +1103,8 @@ static BIdx strided(long long s)
 //   7, 24, 25, 26, 27 LDS 128x64, 64x128, 64x64, 128x128, 32x32 with two slabs
 //   of loads in flight, 28 LDS 32x32 double-buffered, 29 and 30 LDS 32x32 over
 //   2 waves (16x32 each) and 1 wave (four 16x16 accumulators), 31 and 32 those
-//   two with two slabs of loads in flight,
+//   two with two slabs of loads in flight, 33 and 34 variants 2 and 27 with
+//   32-deep K slabs instead of 16,
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
 //   4 waves, 11 32x64 over 4 waves, 12..15 those four with two slabs in flight,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
@@ -1128,7 +1129,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 32, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 34, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -1147,6 +1148,8 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 30: launch_gemm<32, 32, 1, 1, 0>(transA, transB, g, batch, stream); break;
     case 31: launch_gemm<32, 32, 2, 1, 2>(transA, transB, g, batch, stream); break;
     case 32: launch_gemm<32, 32, 1, 1, 2>(transA, transB, g, batch, stream); break;
+    case 33: launch_gemm<32, 32, 2, 2, 0, 32>(transA, transB, g, batch, stream); break;
+    case 34: launch_gemm<32, 32, 2, 2, 2, 32>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1271,7 +1274,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 32, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 34, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1290,6 +1293,8 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 30: launch_gemm_group<32, 32, 1, 1, 0>(ta, tb, gg, batch, stream); break;
     case 31: launch_gemm_group<32, 32, 2, 1, 2>(ta, tb, gg, batch, stream); break;
     case 32: launch_gemm_group<32, 32, 1, 1, 2>(ta, tb, gg, batch, stream); break;
+    case 33: launch_gemm_group<32, 32, 2, 2, 0, 32>(ta, tb, gg, batch, stream); break;
+    case 34: launch_gemm_group<32, 32, 2, 2, 2, 32>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
