@@ -65,19 +65,19 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12619997,
-    "ypass_v+ypass_x": 23613760,
-    "products": 14242782,
-    "z_conv+z_pres+z_old": 17941723,
-    "rhs_part+buoyancy_part": 21318560,
-    "splitk_reduce": 6950349,
-    "helmholtz_x": 3658476,
-    "helmholtz_y": 3515574,
-    "div_y": 11537187,
-    "div_x+poisson_rhs": 12058232,
-    "poisson_lines": 3499873,
-    "poisson_qy+corr_y+corr_y0": 15149545,
-    "pres_p+correct": 10739395,
+    "xpass": 12610723,
+    "ypass_v+ypass_x": 23624890,
+    "products": 14242736,
+    "z_conv+z_pres+z_old": 17958023,
+    "rhs_part+buoyancy_part": 21320559,
+    "splitk_reduce": 6950328,
+    "helmholtz_x": 3658456,
+    "helmholtz_y": 3515555,
+    "div_y": 11543303,
+    "div_x+poisson_rhs": 12059240,
+    "poisson_lines": 3499865,
+    "poisson_qy+corr_y+corr_y0": 15136206,
+    "pres_p+correct": 10773162,
     # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
     "rhs": 26877192,
     "buoyancy": 5797532,

@@ -34,7 +34,7 @@ ROWS_CANDIDATES_AXIS1 = ROWS_CANDIDATES + (33, 34, 35, 36, 37, 38, 39, 40)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
 # output folds (modes 1, 2) also run split over waves
 OUTPUT_FOLD_CANDIDATES = FOLD_CANDIDATES + (6, 7)
-GEMM_CANDIDATES = tuple(range(1, 35))
+GEMM_CANDIDATES = tuple(range(1, 39))
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
 _cache = {}
@@ -90,20 +90,26 @@ def _queue_delay():
     torch.matmul(a, a, out=c)
 
 
-def time_candidates(run, candidates, reps=5):
-    """{candidate: us per launch}; run(c) enqueues one launch on the current stream."""
-    out = {}
-    for c in candidates:
-        run(c)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        _queue_delay()
-        e0.record()
-        for _ in range(reps):
-            run(c)
-        e1.record()
-        e1.synchronize()
-        out[c] = e0.elapsed_time(e1) * 1e3 / reps
+def time_candidates(run, candidates, reps=5, rounds=3):
+    """{candidate: us per launch}; run(c) enqueues one launch on the current stream.
+    Each candidate's time is the best of `rounds` rounds that each time every
+    candidate once, so that a slow moment of the device (clock ramp, another
+    queue) does not decide between candidates a few percent apart."""
+    cands = list(candidates)
+    for c in cands:
+        run(c)  # first launch of a kernel loads its code object
+    out = {c: float("inf") for c in cands}
+    for _ in range(rounds):
+        for c in cands:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            _queue_delay()
+            e0.record()
+            for _ in range(reps):
+                run(c)
+            e1.record()
+            e1.synchronize()
+            out[c] = min(out[c], e0.elapsed_time(e1) * 1e3 / reps)
     return out
 
 

@@ -10,8 +10,8 @@
 // Two kernel families:
 //
 //  * LDS-tiled (throughput): block tile BM x BN (128x128 over 8 wave64s in a
-//    4x2 arrangement, or 64x64 / 32x32 over 4 waves in 2x2), BK = 16 (32 in
-//    variants 33 and 34); each
+//    4x2 arrangement, or 64x64 / 32x32 over 4 waves in 2x2), BK = 16 (32 or
+//    64 in variants 33..38); each
 //    wave owns a (BM/WGM)x(BN/WGN) sub-tile made of 16x16 MFMA tiles.  Tiles
 //    of A and B are staged through LDS (k-major, one double of padding per
 //    row), the next K-slab is prefetched into registers while the current one
@@ -1104,7 +1104,8 @@ static BIdx strided(long long s)
 //   of loads in flight, 28 LDS 32x32 double-buffered, 29 and 30 LDS 32x32 over
 //   2 waves (16x32 each) and 1 wave (four 16x16 accumulators), 31 and 32 those
 //   two with two slabs of loads in flight, 33 and 34 variants 2 and 27 with
-//   32-deep K slabs instead of 16,
+//   32-deep K slabs instead of 16, 35 and 36 the same with 64-deep slabs,
+//   37 and 38 variants 1 and 5 with 32-deep slabs,
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
 //   4 waves, 11 32x64 over 4 waves, 12..15 those four with two slabs in flight,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
@@ -1129,7 +1130,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 34, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 38, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -1150,6 +1151,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 32: launch_gemm<32, 32, 1, 1, 2>(transA, transB, g, batch, stream); break;
     case 33: launch_gemm<32, 32, 2, 2, 0, 32>(transA, transB, g, batch, stream); break;
     case 34: launch_gemm<32, 32, 2, 2, 2, 32>(transA, transB, g, batch, stream); break;
+    case 35: launch_gemm<32, 32, 2, 2, 0, 64>(transA, transB, g, batch, stream); break;
+    case 36: launch_gemm<32, 32, 2, 2, 2, 64>(transA, transB, g, batch, stream); break;
+    case 37: launch_gemm<64, 64, 2, 2, 0, 32>(transA, transB, g, batch, stream); break;
+    case 38: launch_gemm<128, 64, 4, 2, 0, 32>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1274,7 +1279,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 34, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 38, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1295,6 +1300,10 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 32: launch_gemm_group<32, 32, 1, 1, 2>(ta, tb, gg, batch, stream); break;
     case 33: launch_gemm_group<32, 32, 2, 2, 0, 32>(ta, tb, gg, batch, stream); break;
     case 34: launch_gemm_group<32, 32, 2, 2, 2, 32>(ta, tb, gg, batch, stream); break;
+    case 35: launch_gemm_group<32, 32, 2, 2, 0, 64>(ta, tb, gg, batch, stream); break;
+    case 36: launch_gemm_group<32, 32, 2, 2, 2, 64>(ta, tb, gg, batch, stream); break;
+    case 37: launch_gemm_group<64, 64, 2, 2, 0, 32>(ta, tb, gg, batch, stream); break;
+    case 38: launch_gemm_group<128, 64, 4, 2, 0, 32>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;
