@@ -65,19 +65,19 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12610723,
-    "ypass_v+ypass_x": 23624890,
-    "products": 14242736,
-    "z_conv+z_pres+z_old": 17958023,
-    "rhs_part+buoyancy_part": 21320559,
-    "splitk_reduce": 6950328,
-    "helmholtz_x": 3658456,
-    "helmholtz_y": 3515555,
-    "div_y": 11543303,
-    "div_x+poisson_rhs": 12059240,
-    "poisson_lines": 3499865,
-    "poisson_qy+corr_y+corr_y0": 15136206,
-    "pres_p+correct": 10773162,
+    "xpass": 12623041,
+    "ypass_v+ypass_x": 23616773,
+    "products": 14242757,
+    "z_conv+z_pres+z_old": 17944500,
+    "rhs_part+buoyancy_part": 21316480,
+    "splitk_reduce": 6950337,
+    "helmholtz_x": 3658467,
+    "helmholtz_y": 3515566,
+    "div_y": 11538193,
+    "div_x+poisson_rhs": 12058217,
+    "poisson_lines": 3499868,
+    "poisson_qy+corr_y+corr_y0": 15151572,
+    "pres_p+correct": 10772144,
     # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
     "rhs": 26877192,
     "buoyancy": 5797532,

@@ -86,6 +86,7 @@ typedef struct pypde_gemm_desc {
                     2 waves / 1 wave; 31, 32 the same with two slabs in flight;
                     33, 34 variants 2 and 27 with 32-deep K slabs; 35, 36 the same
                     with 64-deep slabs; 37, 38 variants 1 and 5 with 32-deep slabs;
+                    39..42 variants 2, 27, 33, 34 with LDS rows padded by 2 doubles;
                     8..11 split-K LDS-staged (32x32 over 4 / 8 waves, 64x32, 32x64);
                     12..15 the same with two slabs of loads in flight per wave;
                     16..23 split-K register-direct, 16 + (TM-1) + 2 (TN-1)

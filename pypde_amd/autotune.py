@@ -34,7 +34,7 @@ ROWS_CANDIDATES_AXIS1 = ROWS_CANDIDATES + (33, 34, 35, 36, 37, 38, 39, 40)
 FOLD_CANDIDATES = (1, 2, 3, 4, 5)
 # output folds (modes 1, 2) also run split over waves
 OUTPUT_FOLD_CANDIDATES = FOLD_CANDIDATES + (6, 7)
-GEMM_CANDIDATES = tuple(range(1, 39))
+GEMM_CANDIDATES = tuple(range(1, 43))
 
 _enabled = os.environ.get("PYPDE_AUTOTUNE", "1") != "0"
 _cache = {}

@@ -173,12 +173,12 @@ __device__ __forceinline__ int group_tile(const GemmGroup& gg, int& bx, int& by,
 // two LDS buffers (one barrier per slab); 2: one LDS buffer, two slabs of
 // loads in flight (registers for two).  Every form runs the same slabs in the
 // same order, so results are bit-identical.
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK, int PAD>
 __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int by, int bz)
 {
     constexpr int NT = 64 * WGM * WGN;  // threads
-    constexpr int PA = BM + 1;
-    constexpr int PB = BN + 1;
+    constexpr int PA = BM + PAD;  // LDS row pitch (doubles)
+    constexpr int PB = BN + PAD;
     constexpr int WM = BM / WGM;  // wave tile rows
     constexpr int WN = BN / WGN;
     constexpr int TM = WM / 16;   // MFMA tiles per wave (rows)
@@ -320,31 +320,31 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
             }
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK, int PAD>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_kernel(GemmArgs g)
 {
     int bx, by, bz;
     grid_tile(bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK>(g, bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK, PAD>(g, bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK>
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK, int PAD>
 __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_group(GemmGroup gg)
 {
     int bx, by, bz;
     const int i = group_tile(gg, bx, by, bz);
-    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK>(gg.a[i], bx, by, bz);
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK, PAD>(gg.a[i], bx, by, bz);
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16>
+template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16, int PAD = 1>
 static void launch_gemm(bool ta, bool tb, const GemmArgs& g, int batch, hipStream_t s)
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, false, BK>), grid, block, 0, s, g);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, true, BK>), grid, block, 0, s, g);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, false, BK>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, true, BK>), grid, block, 0, s, g);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, false, BK, PAD>), grid, block, 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, false, true, BK, PAD>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, false, BK, PAD>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, WGM, WGN, PIPE, true, true, BK, PAD>), grid, block, 0, s, g);
 }
 
 static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
@@ -358,17 +358,17 @@ static void group_grid(GemmGroup& gg, const int* batch, int bm, int bn)
     }
 }
 
-template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16>
+template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16, int PAD = 1>
 static void launch_gemm_group(bool ta, bool tb, GemmGroup& gg, const int* batch, hipStream_t s)
 {
     group_grid(gg, batch, BM, BN);
     if (gg.start[gg.count] == 0) return;
     dim3 grid(gg.start[gg.count]);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, false, BK>), grid, block, 0, s, gg);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, true, BK>), grid, block, 0, s, gg);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, false, BK>), grid, block, 0, s, gg);
-    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, true, BK>), grid, block, 0, s, gg);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, false, BK, PAD>), grid, block, 0, s, gg);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, false, true, BK, PAD>), grid, block, 0, s, gg);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, false, BK, PAD>), grid, block, 0, s, gg);
+    else hipLaunchKernelGGL((dgemm_mfma_group<BM, BN, WGM, WGN, PIPE, true, true, BK, PAD>), grid, block, 0, s, gg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1105,7 +1105,11 @@ static BIdx strided(long long s)
 //   2 waves (16x32 each) and 1 wave (four 16x16 accumulators), 31 and 32 those
 //   two with two slabs of loads in flight, 33 and 34 variants 2 and 27 with
 //   32-deep K slabs instead of 16, 35 and 36 the same with 64-deep slabs,
-//   37 and 38 variants 1 and 5 with 32-deep slabs,
+//   37 and 38 variants 1 and 5 with 32-deep slabs, 39..42 variants 2, 27,
+//   33, 34 with LDS rows padded by two doubles instead of one (consecutive k
+//   of one tile row then lie two 8-byte banks apart and the next row fills
+//   the banks between, so a slab store by 32 lanes, 16 k x 2 rows, touches 32
+//   distinct banks),
 //   8 split-K LDS 32x32 over 4 waves, 9 the same over 8 waves, 10 64x32 over
 //   4 waves, 11 32x64 over 4 waves, 12..15 those four with two slabs in flight,
 //   16 + c split-K with TM = 1 + (c & 1), TN = 1 + ((c >> 1) & 1), KW = (c & 4) ? 8 : 4.
@@ -1130,7 +1134,7 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     PYPDE_REQUIRE(batch <= 65535, "GEMM batch %d exceeds 65535", batch);
     PYPDE_REQUIRE((long long)((g.m + 15) / 16) * ((g.n + 15) / 16) * batch < (1LL << 31),
                   "GEMM grid too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 38, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 42, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(g, batch);
     switch (variant) {
     case 1: launch_gemm<64, 64, 2, 2, false>(transA, transB, g, batch, stream); break;
@@ -1155,6 +1159,10 @@ static int launch(int transA, int transB, GemmArgs& g, int batch, int variant, h
     case 36: launch_gemm<32, 32, 2, 2, 2, 64>(transA, transB, g, batch, stream); break;
     case 37: launch_gemm<64, 64, 2, 2, 0, 32>(transA, transB, g, batch, stream); break;
     case 38: launch_gemm<128, 64, 4, 2, 0, 32>(transA, transB, g, batch, stream); break;
+    case 39: launch_gemm<32, 32, 2, 2, 0, 16, 2>(transA, transB, g, batch, stream); break;
+    case 40: launch_gemm<32, 32, 2, 2, 2, 16, 2>(transA, transB, g, batch, stream); break;
+    case 41: launch_gemm<32, 32, 2, 2, 0, 32, 2>(transA, transB, g, batch, stream); break;
+    case 42: launch_gemm<32, 32, 2, 2, 2, 32, 2>(transA, transB, g, batch, stream); break;
     case 8: launch_wsplit<32, 32, 4>(transA, transB, g, batch, stream); break;
     case 9: launch_wsplit<32, 32, 8>(transA, transB, g, batch, stream); break;
     case 10: launch_wsplit<64, 32, 4>(transA, transB, g, batch, stream); break;
@@ -1279,7 +1287,7 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
         tiles += (long long)((e.m + 15) / 16) * ((e.n + 15) / 16) * e.batch;
     }
     PYPDE_REQUIRE(tiles < (1LL << 31), "grouped GEMM too large");
-    PYPDE_REQUIRE(variant >= 0 && variant <= 38, "unknown GEMM variant %d", variant);
+    PYPDE_REQUIRE(variant >= 0 && variant <= 42, "unknown GEMM variant %d", variant);
     if (variant == 0) variant = heuristic_variant(gg.a[0], batch[0]);
     switch (variant) {
     case 1: launch_gemm_group<64, 64, 2, 2, false>(ta, tb, gg, batch, stream); break;
@@ -1304,6 +1312,10 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     case 36: launch_gemm_group<32, 32, 2, 2, 2, 64>(ta, tb, gg, batch, stream); break;
     case 37: launch_gemm_group<64, 64, 2, 2, 0, 32>(ta, tb, gg, batch, stream); break;
     case 38: launch_gemm_group<128, 64, 4, 2, 0, 32>(ta, tb, gg, batch, stream); break;
+    case 39: launch_gemm_group<32, 32, 2, 2, 0, 16, 2>(ta, tb, gg, batch, stream); break;
+    case 40: launch_gemm_group<32, 32, 2, 2, 2, 16, 2>(ta, tb, gg, batch, stream); break;
+    case 41: launch_gemm_group<32, 32, 2, 2, 0, 32, 2>(ta, tb, gg, batch, stream); break;
+    case 42: launch_gemm_group<32, 32, 2, 2, 2, 32, 2>(ta, tb, gg, batch, stream); break;
     case 8: launch_wsplit_group<32, 32, 4>(ta, tb, gg, batch, stream); break;
     case 9: launch_wsplit_group<32, 32, 8>(ta, tb, gg, batch, stream); break;
     case 10: launch_wsplit_group<64, 32, 4>(ta, tb, gg, batch, stream); break;

@@ -38,7 +38,7 @@ def test_mfma_gemm_layout(amd):
                 assert rel(host(C), ref) < 1e-14, (m, n, k, ta, tb)
 
 
-@pytest.mark.parametrize("variant", range(1, 39))
+@pytest.mark.parametrize("variant", range(1, 43))
 def test_gemm_every_variant(amd, variant):
     """Each kernel variant of pypde_dgemm_batched on ragged shapes (m, n not
     multiples of the tile, k not a multiple of the split), all transposes,
@@ -364,7 +364,7 @@ def test_mirror_operator_transforms(amd, N, axis):
         assert rel(y, ref) < 1e-13, (M.shape, axis)
 
 
-PLAIN_VARIANTS = tuple(range(1, 39))
+PLAIN_VARIANTS = tuple(range(1, 43))
 
 
 def test_gemm_prefetch_depth_is_bit_identical(amd):
@@ -374,7 +374,8 @@ def test_gemm_prefetch_depth_is_bit_identical(amd):
     double-buffered forms 4 and 28 against 3 and 2.  The 32x32 tile over fewer
     waves (29..32) runs the same MFMA chain per output, so it matches 2 too, and
     so do 33..36 (32- and 64-deep K slabs: the same chain, more zero padding);
-    37 and 38 match 1 and 5 the same way."""
+    37 and 38 match 1 and 5 the same way; 39..42 (another LDS row pitch)
+    match 2, 27, 33 and 34."""
     import ctypes
     _, ops, torch = amd
     from pypde_amd._lib import lib
@@ -388,7 +389,7 @@ def test_gemm_prefetch_depth_is_bit_identical(amd):
                 dA, dB = ops.to_dev(A), ops.to_dev(B)
                 outs = {}
                 for v in (8, 9, 10, 11, 12, 13, 14, 15, 5, 7, 6, 24, 1, 25, 3, 26, 4, 2, 27, 28, 29, 30,
-                          31, 32, 33, 34, 35, 36, 37, 38):
+                          31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42):
                     C = torch.empty((batch, m, n), dtype=torch.float64, device=dA.device)
                     d = ops._desc(ta, tb, m, n, k, batch, 1.0, 0.0, dA.data_ptr(), dA.stride(1),
                                   dA.stride(0), dB.data_ptr(), dB.stride(1), dB.stride(0), None, 0,
@@ -399,7 +400,7 @@ def test_gemm_prefetch_depth_is_bit_identical(amd):
                 for v, w in ((8, 12), (9, 13), (10, 14), (11, 15), (5, 7), (6, 24), (1, 25),
                              (3, 26), (3, 4), (2, 27), (2, 28), (2, 29), (2, 30), (2, 31),
                              (2, 32), (2, 33), (2, 34), (2, 35), (2, 36), (1, 37),
-                             (5, 38)):
+                             (5, 38), (2, 39), (27, 40), (33, 41), (34, 42)):
                     assert np.array_equal(outs[v], outs[w]), (m, n, k, ta, tb, v, w)
 
 
