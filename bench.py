@@ -10,7 +10,8 @@
 Workload (value): the plain Euler IMEX step of navier.rbc2d.NavierStokes at
 256x256, rbc case, Ra=1e8, Pr=1, dt=1e-3, 3/2 dealiasing, seeded
 conductive-plus-noise initial state; W untimed steps, then K steps timed
-between device synchronisations, replayed from a HIP graph.
+between device synchronisations, replayed from HIP graphs of ten steps
+(NavierStokes.update_many).
 
 Also reported on the same line:
   roofline       the dominant launch of the timed step (largest time per step in an

@@ -525,6 +525,8 @@ class RBCBatch:
         self._build_launches()
         self.tune()
         self.graph = None
+        self.graph_steps = 0
+        self.graphs = {}  # captured graphs by their number of steps
 
     def _helmholtz_matrix(self, st, b, f, axis):
         """Dense Helmholtz operator B S - lam I S of stage st, instance b, field
@@ -974,15 +976,20 @@ class RBCBatch:
                 self.step_eager()
         self.graph = g
         self.graph_steps = nsteps
+        self.graphs[nsteps] = g
 
     def step(self, n=1):
-        """Advance n steps (graph replay when a graph of matching length exists)."""
-        if self.graph is not None and n % self.graph_steps == 0:
-            for _ in range(n // self.graph_steps):
-                self.graph.replay()
-        else:
-            for _ in range(n):
-                self.step_eager()
+        """Advance n steps: replays of the longest captured graphs that fit (a
+        graph of several steps costs one launch instead of one per step), the
+        rest as launches.  Replays run exactly the captured launches, so every
+        split of n gives the same state bit for bit."""
+        for length in sorted(self.graphs, reverse=True):
+            g = self.graphs[length]
+            for _ in range(n // length):
+                g.replay()
+            n %= length
+        for _ in range(n):
+            self.step_eager()
 
     def gemm_flops(self):
         return sum(l.flops for l in self.launches if isinstance(l, _Gemm))

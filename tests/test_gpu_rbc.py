@@ -64,6 +64,24 @@ def test_graph_replay_matches_eager(golden):
         assert np.array_equal(host(getattr(a, f).vhat), host(getattr(b, f).vhat)), f
 
 
+def test_multi_step_graph_matches_single_steps(golden):
+    """update_many replays a graph of several steps plus single steps; the
+    state equals that of the same number of single updates bit for bit."""
+    a, b = make("rbc_eu", use_graph=True), make("rbc_eu", use_graph=True)
+    assert a.GRAPH_BLOCK > 1
+    pre = "step_rbc_eu_in_"
+    for ns in (a, b):
+        ns.set_state(T=golden[pre + "T"], U=golden[pre + "U"], V=golden[pre + "V"],
+                     pres=golden[pre + "pres"])
+    n = 2 * a.GRAPH_BLOCK + 3
+    a.update_many(n)
+    assert a.GRAPH_BLOCK in a.engine.graphs
+    for _ in range(n):
+        b.update()
+    for f in ("T", "U", "V", "pres", "P"):
+        assert np.array_equal(host(getattr(a, f).vhat), host(getattr(b, f).vhat)), f
+
+
 def test_grouped_launches_match_separate(golden):
     """The step with independent GEMMs sharing launches equals the step with
     one launch per GEMM (kernel variants may differ: 1e-13)."""
