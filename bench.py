@@ -147,6 +147,9 @@ def time_plain(args):
     ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None,
                       dealias=True, integrator="eu", beta=1.0, aspect=1.0)
     ns.set_state(T=ns.T.forward(seeded_noise((N, N), seed=0)))
+    # every graph the timed region replays is captured (and replayed once, the
+    # state restored) here, whatever W is
+    ns.prepare_graphs()
     ns.update_many(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -215,6 +218,20 @@ def time_batched(args, nb=64, steps=None):
                 finite=finite)
 
 
+def c4_specs(n=256, seed=7):
+    """The mixed-size batch: N per axis uniform from {64, 96, 128, 192}, Ra
+    log-uniform in [1e4, 1e7], Pr = 1, dt = clip(0.01 sqrt(1e5/Ra), 1e-3, 1e-2);
+    instance i is seeded with noise 1e-3 from seed 1000 + i."""
+    from pypde_amd.batch import default_dt
+    rng = np.random.default_rng(seed)
+    sizes = np.array([64, 96, 128, 192])
+    nx = rng.choice(sizes, size=n)
+    ny = rng.choice(sizes, size=n)
+    ras = 10.0 ** rng.uniform(4, 7, size=n)
+    return [dict(shape=(int(a), int(b)), ra=float(r), pr=1.0, dt=default_dt(r))
+            for a, b, r in zip(nx, ny, ras)]
+
+
 def time_hetero(steps=500, warm=5, n=256, seed=7):
     """256 independent instances, N drawn per axis from {64, 96, 128, 192},
     Ra log-uniform in [1e4, 1e7], Pr = 1, dt = clip(0.01 sqrt(1e5/Ra), 1e-3, 1e-2),
@@ -222,15 +239,9 @@ def time_hetero(steps=500, warm=5, n=256, seed=7):
     step for all size classes.  Compared with looping the plain single-instance
     graph-replayed step over the same instances."""
     import torch
-    from pypde_amd.batch import HeteroBatch, default_dt
+    from pypde_amd.batch import HeteroBatch
     from pypde_amd.engine import RBCBatch
-    rng = np.random.default_rng(seed)
-    sizes = np.array([64, 96, 128, 192])
-    nx = rng.choice(sizes, size=n)
-    ny = rng.choice(sizes, size=n)
-    ras = 10.0 ** rng.uniform(4, 7, size=n)
-    specs = [dict(shape=(int(a), int(b)), ra=float(r), pr=1.0, dt=default_dt(r))
-             for a, b, r in zip(nx, ny, ras)]
+    specs = c4_specs(n, seed)
     t0 = time.perf_counter()
     hb = HeteroBatch(specs, case="rbc")
     setup = time.perf_counter() - t0
@@ -264,32 +275,96 @@ def time_hetero(steps=500, warm=5, n=256, seed=7):
                 finite=finite, setup_sec=round(setup, 1))
 
 
+def c2_input(nf=512, N=256, seed=1):
+    """The microbench's coefficient fields: c[b, k, l] = g exp(-0.05 (k + l)) with
+    g ~ N(0, 1) drawn by numpy's default_rng(seed), shape (nf, N, N)."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(N, dtype=np.float64)
+    decay = np.exp(-0.05 * (k[:, None] + k[None, :]))
+    c = rng.standard_normal((nf, N, N))
+    c *= decay
+    return c
+
+
+class C2Chain:
+    """The microbench's kernel chain over a batch of N x N coefficient fields
+    (device tensor c [nf, N, N]): K1-K2 backward Chebyshev transforms along x
+    and y, K3-K4 forward transforms, K5-K6 ChebDirichlet from_chebyshev, K7-K8
+    to_chebyshev, K9-K10 the Helmholtz right-hand side B (lambda = 0.01) along x
+    and y, K11-K12 the four-diagonal solves along x then y (in place).
+    stages: (name, fn, algorithmic bytes per field, MFMA flops per field or
+    None); fn() runs one kernel on the buffers of the previous stage."""
+
+    def __init__(self, c):
+        import torch
+        import pypde_amd
+        from pypde_amd import ops
+        from pypde_amd.templates import hholtz_matrices
+        from pypde_amd.solver import Plan_fdma, PlanRHS
+        nf, N, _ = c.shape
+        M = N - 2
+        self.c, self.nf, self.N = c, nf, N
+        ch = pypde_amd.Base(N, "CH")
+        cd = pypde_amd.Base(N, "CD")
+        # parity-folded transforms (the cosine matrices are mirror-symmetric)
+        opB = ops.MirrorOperator(ch.backward_matrix())
+        opF = ops.MirrorOperator(ch.forward_matrix())
+        assert opB.mode and opF.mode
+        s = cd.stencil_vector()
+        Bm, _, Ax = hholtz_matrices(cd, 0.01, 1.0)
+        rhs_x, rhs_y = PlanRHS(Bm, 2, 0), PlanRHS(Bm, 2, 1)
+        lhs_x, lhs_y = Plan_fdma(Ax, 2, 0), Plan_fdma(Ax, 2, 1)
+        b = self.bufs = {}
+        b["u1"] = opB.apply(c, 0)
+        b["u2"] = opB.apply(b["u1"], 1)
+        b["c1"] = opF.apply(b["u2"], 0)
+        b["c2"] = opF.apply(b["c1"], 1)
+        fb = 8 * 2 * N * N
+        fl = N * N * N  # MFMA work of a folded transform: half the dense 2 N^3
+        self.stages = [
+            ("K1_backward_x", lambda: opB.apply(c, 0, out=b["u1"]), fb, fl),
+            ("K2_backward_y", lambda: opB.apply(b["u1"], 1, out=b["u2"]), fb, fl),
+            ("K3_forward_x", lambda: opF.apply(b["u2"], 0, out=b["c1"]), fb, fl),
+            ("K4_forward_y", lambda: opF.apply(b["c1"], 1, out=b["c2"]), fb, fl),
+            ("K5_from_cheb_x", lambda: b.__setitem__("g1", cd.from_chebyshev(b["c2"], 0)),
+             8 * (N * N + M * N), None),
+            ("K6_from_cheb_y", lambda: b.__setitem__("g2", cd.from_chebyshev(b["g1"], 1)),
+             8 * (M * N + M * M), None),
+            ("K7_to_cheb_x", lambda: b.__setitem__("t1", ops.stencil_apply(s, b["g2"], 0)),
+             8 * (M * M + N * M), None),
+            ("K8_to_cheb_y", lambda: b.__setitem__("t2", ops.stencil_apply(s, b["t1"], 1)),
+             8 * (N * M + N * N), None),
+            ("K9_rhs_Bx", lambda: b.__setitem__("r1", rhs_x.solve(b["c2"])), 8 * (N * N + M * N),
+             None),
+            ("K10_rhs_By", lambda: b.__setitem__("r2", rhs_y.solve(b["r1"])), 8 * (M * N + M * M),
+             None),
+            ("K11_fdma_x", lambda: lhs_x.solve(b["x"]), 8 * 2 * M * M, None),
+            ("K12_fdma_y", lambda: lhs_y.solve(b["x"]), 8 * 2 * M * M, None),
+        ]
+        self.bytes_per_field = sum(st[2] for st in self.stages)
+
+    def prepare(self, name):
+        """Set-up that is not part of a kernel: the in-place solves start from
+        the right-hand side of K10."""
+        if name == "K11_fdma_x":
+            self.bufs["x"] = self.bufs["r2"].clone()
+
+    def run_once(self):
+        """Every kernel once, in chain order (the parity run)."""
+        for name, fn, _, _ in self.stages:
+            self.prepare(name)
+            fn()
+
+
 def microbench(nf=512, reps=10):
-    """Kernel chain of 512 fields at 256x256 (per-field algorithmic bytes in the
-    docstring of each entry); returns per-kernel time and HBM fraction."""
+    """Kernel chain of 512 fields at 256x256 (C2Chain; per-field algorithmic bytes
+    in its stage list); returns per-kernel time and HBM fraction."""
     import torch
-    import pypde_amd
     from pypde_amd import ops
-    from pypde_amd.templates import hholtz_matrices
-    from pypde_amd.solver import Plan_fdma, PlanRHS
-    N, M = 256, 254
-    dev = ops.device()
-    g = torch.Generator(device="cpu").manual_seed(1)
-    k = torch.arange(N, dtype=torch.float64)
-    decay = torch.exp(-0.05 * (k[:, None] + k[None, :]))
-    c = (torch.randn((nf, N, N), generator=g, dtype=torch.float64) * decay).to(dev)
-    ch = pypde_amd.Base(N, "CH")
-    cd = pypde_amd.Base(N, "CD")
-    Bk, Fw = ch.backward_matrix(), ch.forward_matrix()
-    # parity-folded transforms (the cosine matrices are mirror-symmetric)
-    opB, opF = ops.MirrorOperator(Bk), ops.MirrorOperator(Fw)
-    assert opB.mode and opF.mode
-    s = cd.stencil_vector()
-    Bm, _, Ax = hholtz_matrices(cd, 0.01, 1.0)
-    rhs_x, rhs_y = PlanRHS(Bm, 2, 0), PlanRHS(Bm, 2, 1)
-    lhs_x, lhs_y = Plan_fdma(Ax, 2, 0), Plan_fdma(Ax, 2, 1)
+    N = 256
+    c = torch.as_tensor(c2_input(nf, N, seed=1)).to(ops.device())
+    chain = C2Chain(c)
     out = {}
-    bufs = {}
 
     def run(name, fn, bytes_per_field, flops=None):
         fn()  # warm
@@ -310,37 +385,16 @@ def microbench(nf=512, reps=10):
             rec["mfma_frac"] = round(flops * nf / t / 1e12 / FP64_PEAK_TFLOPS, 4)
         out[name] = rec
 
-    fb = 8 * 2 * N * N
-    # the measured roof: a device copy of the same 512 x 256 x 256 array
+    # the measured roof: a device copy of the same nf x 256 x 256 array
     cp = torch.empty_like(c)
-    run("device_copy", lambda: torch.mul(c, 1.0, out=cp), fb)
+    run("device_copy", lambda: torch.mul(c, 1.0, out=cp), 8 * 2 * N * N)
     copy = out.pop("device_copy")
     del cp
-    # MFMA work issued by a folded transform: half the dense count 2 N^3 per field
-    fl = N * N * N
-    bufs["u1"] = opB.apply(c, 0)
-    run("K1_backward_x", lambda: opB.apply(c, 0, out=bufs["u1"]), fb, fl)
-    bufs["u2"] = opB.apply(bufs["u1"], 1)
-    run("K2_backward_y", lambda: opB.apply(bufs["u1"], 1, out=bufs["u2"]), fb, fl)
-    bufs["c1"] = opF.apply(bufs["u2"], 0)
-    run("K3_forward_x", lambda: opF.apply(bufs["u2"], 0, out=bufs["c1"]), fb, fl)
-    bufs["c2"] = opF.apply(bufs["c1"], 1)
-    run("K4_forward_y", lambda: opF.apply(bufs["c1"], 1, out=bufs["c2"]), fb, fl)
-    run("K5_from_cheb_x", lambda: bufs.__setitem__("g1", cd.from_chebyshev(bufs["c2"], 0)),
-        8 * (N * N + M * N))
-    run("K6_from_cheb_y", lambda: bufs.__setitem__("g2", cd.from_chebyshev(bufs["g1"], 1)),
-        8 * (M * N + M * M))
-    run("K7_to_cheb_x", lambda: bufs.__setitem__("t1", ops.stencil_apply(s, bufs["g2"], 0)),
-        8 * (M * M + N * M))
-    run("K8_to_cheb_y", lambda: bufs.__setitem__("t2", ops.stencil_apply(s, bufs["t1"], 1)),
-        8 * (N * M + N * N))
-    run("K9_rhs_Bx", lambda: bufs.__setitem__("r1", rhs_x.solve(bufs["c2"])), 8 * (N * N + M * N))
-    run("K10_rhs_By", lambda: bufs.__setitem__("r2", rhs_y.solve(bufs["r1"])), 8 * (M * N + M * M))
-    x = bufs["r2"].clone()
-    run("K11_fdma_x", lambda: lhs_x.solve(x), 8 * 2 * M * M)
-    run("K12_fdma_y", lambda: lhs_y.solve(x), 8 * 2 * M * M)
+    for name, fn, nbytes, flops in chain.stages:
+        chain.prepare(name)
+        run(name, fn, nbytes, flops)
     total_us = sum(v["us"] for v in out.values())
-    total_bytes = 12501216 * nf
+    total_bytes = chain.bytes_per_field * nf
     for v in out.values():
         v["copy_frac"] = round(v["GBps"] / copy["GBps"], 4)
     chain_gbs = total_bytes / (total_us * 1e-6) / 1e9
@@ -359,6 +413,7 @@ def time_variant(steps=100, N=256, **kw):
     ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None, **kw)
     assert ns.fused
     ns.set_state(T=ns.T.forward(seeded_noise((N, N), seed=0)))
+    ns.prepare_graphs()
     ns.update_many(5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -71,13 +71,14 @@ class StepAdjoint:
         self.hrt, self.hrt_y, prt, prt_lm = e.transposed_line_tables()
         self.prt = (prt, prt_lm)
 
-    def _reversed_solve(self, x, n, nlines, batch, tabs, per_line):
+    @staticmethod
+    def _reversed_solve(x, n, nlines, batch, tabs, per_line, variant=None):
         """In place, along axis 0 of the contiguous x [batch, n, nlines]: the
         transposed line solves as the four-diagonal recurrence over each line
         read backwards (the first row at the last one, row stride -nlines).
         tabs: per-entry tables [batch][LM | R | MU | NU][n], or per-line tables
         ([n][nlines] slots, line-major slots).  The kernel form is tuned per
-        shape on a scratch array."""
+        shape on a scratch array; variant (1, 2 or 3) forces a form."""
         from . import autotune
         from ._lib import Lines, Recurrence
 
@@ -109,6 +110,9 @@ class StepAdjoint:
         def run(g, v):
             check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(struct(v)), stream),
                   "transposed line solve")
+        if variant is not None:
+            run(geom(x), variant)
+            return
         key = ("rec_reversed", n, nlines, batch, bool(per_line))
         if autotune.enabled() and not autotune._capturing() and key not in autotune.cache():
             scratch = torch.zeros_like(x)  # kept alive while the candidates run on it
@@ -290,7 +294,7 @@ class Trajectory:
     def _step(self):
         """One forward step: the engine's graph replay, or its launches."""
         if self.use_graph:
-            if self.e.graph is None or self.e.graph_steps != 1:
+            if 1 not in self.e.graphs:
                 self.e.capture(1)
             self.e.step(1)
         else:
@@ -387,6 +391,18 @@ class RBCStep(torch.autograd.Function):
         pb = torch.zeros_like(pres) if pb is None else pb.contiguous()
         Xin, pin = ctx.adj.vjp(X, pres, Xb, pb)
         return Xin, pin, None
+
+
+def steady_residual(eng: RBCBatch, X, pres):
+    """J_res = 0.5 ||(Phi(X) - X)/dt||^2 per instance over X = [U, V, T], pres
+    held fixed (rbc2d_base.py:321-341 restated); the engine state is left at
+    Phi(X)."""
+    eng.X.copy_(X)
+    eng.pres.copy_(pres)
+    eng.step_eager()
+    dt = torch.as_tensor(eng.dt, dtype=ops.DTYPE, device=X.device).reshape(-1, 1, 1, 1)
+    r = (eng.X - X) / dt
+    return 0.5 * (r * r).sum(dim=(1, 2, 3))
 
 
 def steady_residual_grad(eng: RBCBatch, X, pres, adj=None):

@@ -178,13 +178,25 @@ def gemm_key(d):
             d.D is not None and d.beta != 0.0, d.fold, d.csn, d.ska)
 
 
+def aliased(d):
+    """True when the descriptor's output is also one of its inputs."""
+    return d.C in (d.A, d.B) or (d.D is not None and d.D == d.C and d.beta != 0.0) or \
+        (d.E is not None and d.E == d.C) or (d.A1 is not None and d.C == d.A1) or \
+        (d.B1 is not None and d.C == d.B1)
+
+
+def cached(key, default=0):
+    """The stored decision for key (database or this process), else default."""
+    _ensure_db()
+    return _cache.get(key, default)
+
+
 def tune_gemm_desc(d, launch, allow_alias=False):
     """Fill d.variant; launch(d) enqueues the GEMM with the descriptor's variant.
     A descriptor whose output is also an input (C aliases A, B or D) is not
     timed unless the caller restores the output afterwards (allow_alias),
     since repeated launches change its result; it keeps the size heuristic."""
-    aliased = d.C in (d.A, d.B) or (d.D is not None and d.D == d.C and d.beta != 0.0)
-    if aliased and not allow_alias:
+    if aliased(d) and not allow_alias:
         _ensure_db()
         d.variant = _cache.get(gemm_key(d), 0)
         return d.variant

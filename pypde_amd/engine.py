@@ -974,6 +974,13 @@ class RBCBatch:
         with torch.cuda.graph(g):
             for _ in range(nsteps):
                 self.step_eager()
+        # one replay now, so that no later replay is the graph's first, then
+        # the state is put back
+        g.replay()
+        torch.cuda.synchronize()
+        for t, v in zip((self.X, self.pres, self.P), saved):
+            t.copy_(v)
+        torch.cuda.synchronize()
         self.graph = g
         self.graph_steps = nsteps
         self.graphs[nsteps] = g

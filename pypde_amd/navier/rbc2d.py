@@ -182,6 +182,19 @@ class NavierStokes(NavierStokesBase, Integrator):
     # 5 us of the 150 us 256x256 step, so blocks of 10 steps take 3% off
     GRAPH_BLOCK = 10
 
+    def prepare_graphs(self):
+        """Capture the one-step graph and the GRAPH_BLOCK-step graph now (the
+        state is left unchanged), so that no later update() or update_many()
+        call captures: a caller that times steps calls this first."""
+        if self.engine is None or not self.use_graph:
+            return
+        self._sync_in()
+        if not self._graph_ready:
+            self.engine.capture(1)
+            self._graph_ready = True
+        if self.GRAPH_BLOCK not in self.engine.graphs:
+            self.engine.capture(self.GRAPH_BLOCK)
+
     def update_many(self, n):
         """n steps at once, no host work between: replays of a graph of
         GRAPH_BLOCK steps, then single-step replays for the rest."""

@@ -314,6 +314,11 @@ def launch_grouped(descs, tag):
             arr[i].variant = v
         check(lib.pypde_dgemm_grouped(ctypes.byref(arr), n, s), "pypde_dgemm_grouped")
     key = ("ops_group", tag) + tuple(autotune.gemm_key(d) for d in descs)
+    # a member whose output is also an input would be accumulated once per
+    # timed launch: such a group is not timed (stored choice, else variant 0)
+    if any(autotune.aliased(d) for d in descs):
+        run(autotune.cached(key, default=0))
+        return
     run(autotune.choose(key, run, autotune.GEMM_CANDIDATES, default=0))
 
 

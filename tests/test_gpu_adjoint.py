@@ -194,3 +194,80 @@ def test_graph_replayed_sweep_is_bit_identical():
     J, Xb, pb = traj.grad_nusselt(X0, torch.zeros_like(eng.pres), 23)
     for x, y in zip([t.detach().cpu().numpy() for t in (J, Xb, pb)], out[0][0]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("n", [22, 126, 254, 510])
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_reversed_transposed_solves_match_dense(n, variant):
+    """The adjoint's transposed line solves (negative row stride, the line read
+    from its last row) in each kernel form, with per-entry Helmholtz tables and
+    with per-line Poisson tables, against dense A^T solves on the host, at the
+    line lengths of 24^2, 128^2, 256^2 and 512^2 steps."""
+    import torch
+    from pypde_amd import ops
+    from pypde_amd.adjoint import StepAdjoint
+    from pypde_amd.bases import Base
+    from pypde_amd.solver import rec_poisson_reversed_transpose, rec_reversed_transpose
+    from pypde_amd.templates import hholtz_matrices, poisson_matrices
+    dev = ops.device()
+    rng = np.random.default_rng(n + 7 * variant)
+    # per-entry tables: three (instance, field) entries with their own matrices
+    mats = [np.asarray(hholtz_matrices(Base(n + 2, key), lam, 0.5)[2])
+            for key, lam in (("CD", 1e-3), ("CN", 2e-2), ("CD", 0.3))]
+    tabs = torch.as_tensor(np.stack([np.stack(rec_reversed_transpose(A)) for A in mats]),
+                           dtype=torch.float64, device=dev)
+    nlines = 77
+    b = rng.standard_normal((3, n, nlines))
+    x = torch.as_tensor(b, device=dev).contiguous()
+    StepAdjoint._reversed_solve(x, n, nlines, 3, tabs, False, variant=variant)
+    xh = x.cpu().numpy()
+    for e, A in enumerate(mats):
+        ref = np.linalg.solve(A.T, b[e])
+        err = np.abs(xh[e] - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, ("entry", n, variant, e, err)
+    # per-line tables: the Poisson eigen-split lines, one singular line
+    pm = poisson_matrices([Base(n + 2, "CN"), Base(n + 2, "CN")], singular=True,
+                          scale=(0.5, 0.5))
+    A, C, w = (np.asarray(pm[k]) for k in ("Ax", "Cx", "wy"))
+    pt = torch.as_tensor(np.stack(rec_poisson_reversed_transpose(A, C, w, True)), device=dev)
+    tabs2 = (pt, pt.transpose(1, 2).contiguous())
+    nl = w.shape[0]
+    b = rng.standard_normal((1, n, nl))
+    x = torch.as_tensor(b, device=dev).contiguous()
+    StepAdjoint._reversed_solve(x, n, nl, 1, tabs2, True, variant=variant)
+    xh = x.cpu().numpy()[0]
+    worst = 0.0
+    for j in range(nl):
+        M = A + w[j] * C
+        if abs(w[j]) < 1e-10:
+            ref = np.zeros(n)
+            ref[1:] = np.linalg.solve(M[1:, 1:].T, b[0, 1:, j])
+        else:
+            ref = np.linalg.solve(M.T, b[0, :, j])
+        worst = max(worst, np.abs(xh[:, j] - ref).max() / np.abs(ref).max())
+    assert worst < 1e-11, ("per line", n, variant, worst)
+
+
+def test_step_vjp_128_matches_central_differences():
+    """One step at 128x128 (the c5 size): the step is quadratic in its state,
+    so the central difference is exact up to rounding; the VJP against it along
+    3 random directions, 1e-9 relative."""
+    import torch
+    from pypde_amd.adjoint import StepAdjoint
+    eng, X, pres = make(shape=(128, 128), ra=5e4, dt=0.005)
+    adj = StepAdjoint(eng)
+    rng = np.random.default_rng(21)
+    dev = X.device
+    wX = torch.as_tensor(rng.standard_normal(tuple(X.shape)), device=dev)
+    wp = torch.as_tensor(rng.standard_normal(tuple(pres.shape)), device=dev)
+    gX, gp = adj.vjp(X, pres, wX, wp)
+    for trial in range(3):
+        dX = torch.as_tensor(rng.standard_normal(tuple(X.shape)), device=dev)
+        dp = torch.as_tensor(rng.standard_normal(tuple(pres.shape)), device=dev)
+        eps = 1e-3
+        Xa, pa = run_step(eng, X + eps * dX, pres + eps * dp)
+        Xm, pm = run_step(eng, X - eps * dX, pres - eps * dp)
+        fd = ((wX * (Xa - Xm)).sum() + (wp * (pa - pm)).sum()).item() / (2 * eps)
+        an = ((gX * dX).sum() + (gp * dp).sum()).item()
+        print("128x128 step VJP, trial %d: fd %.15e vjp %.15e" % (trial, fd, an))
+        assert abs(fd - an) <= 1e-9 * abs(fd), (trial, fd, an)
