@@ -136,6 +136,8 @@ def launch_cost(name, launch, eng):
         "helmholtz_y": 2 * 3 * M0 * M1 * B,
         # in/out plus the per-line factor tables, counted once per launch
         "poisson_lines": 2 * M0 * M1 * B + 4 * M0 * M1,
+        # read X, pres and the constant terms, write R
+        "rhs_banded": (3 * M0 * M1 + N0 * N1 + 3 * M0 * M1 + 3 * M0 * M1) * B,
     }.get(name, 0)
     return "hbm", 8.0 * el
 

@@ -288,6 +288,52 @@ int pypde_rbc_products_yfold(const double* Y, const double* Yold, const double* 
 int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const double* Pb, double* Yb,
                            int L0, int L1, int batch, void* stream);
 
+/* Banded part of the Helmholtz right-hand side of the fused step, per instance
+ * b and field f in (U, V, T), all on Galerkin coefficients [M0, M1]:
+ *   R_f = cst_f + (B S)_x X_f (B S)_y^T                          old value
+ *         + mdta_b (B D/s)_x pres B_y^T            (f = U)        pressure gradient
+ *         + mdta_b B_x pres (B D/s)_y^T            (f = V)
+ *         + dta_b (B S_T)_x X_T (B S)_y^T          (f = V)        buoyancy
+ *         + fdiff_{b,f} [(B D2 S/s^2)_x X_f (B S)_y^T + (B S)_x X_f (B D2 S/s^2)_y^T]
+ * (S_T, the temperature's x stencil, replaces S in the x operators of f = T).
+ * tx holds the x operators as diagonals, diag[d][i] = A(i, i + off_d) with
+ * zeros where the column falls outside, M0 entries each, in this order:
+ *   B D/s (offsets 1,3,5), B (0,2,4), B S (-2,0,2,4), B S_T (-2,0,2,4),
+ *   B D2 S/s^2 (0,2), B D2 S_T/s^2 (0,2)                        18 diagonals;
+ * ty the y operators, M1 entries each: B (0,2,4), B D/s (1,3,5), B S (-2,0,2,4),
+ *   B D2 S/s^2 (0,2)                                              12 diagonals.
+ * cst (may be NULL) and R are [batch, 3, M0, M1]; X [batch, 3, M0, M1]; pres
+ * [batch, N0, N1]; mdta, dta per instance; fdiff [batch, 3] or NULL (beta = 1).
+ * Replaces the sparse PlanRHS applies of the U, V and T updates (the solve_rhs /
+ * solve_old plans of templates/hholtz.py:70-93 applied in navier/rbc2d.py:213-283,
+ * through pypde/solver/plans.py:54-74 and matrix.py:41-56). */
+typedef struct pypde_rbc_rhs {
+    const double* X;
+    const double* pres;
+    const double* cst;
+    double* R;
+    const double* mdta;
+    const double* dta;
+    const double* fdiff;
+    const double* tx;
+    const double* ty;
+    int M0, M1, N0, N1, batch;
+} pypde_rbc_rhs;
+
+int pypde_rbc_rhs_banded(const pypde_rbc_rhs* args, void* stream);
+
+/* Discrete adjoint of the products with the c-term of the three-stage scheme:
+ * the forward products are P_f = grad_f . (u + ratio u_old) (plus the lifted
+ * boundary term with uz + ratio uz_old for T), the gradients of the current
+ * stage times its velocities plus ratio = c / b times the previous stage's
+ * velocities (rbc2d.py:261-283).  Y and Yold are the natural-layout blocks
+ * [batch, 3, 2*L0, 2*L1] of the current and previous stage; Yb receives the
+ * cotangents of Y's blocks and Yob those of Yold's (velocity values only).
+ * No reference counterpart: adjoint of the three-stage step. */
+int pypde_rbc_products_vjp_c(const double* Y, const double* Yold, const double* dtbc,
+                             const double* Pb, double* Yb, double* Yob, double ratio, int L0,
+                             int L1, int batch, void* stream);
+
 /* y[b] -= (coef[b] * x[b]) * c2 over per elements of each batch entry:
  * the pressure update pres -= nu*div*beta of rbc2d.py:390-394, per instance. */
 int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,
@@ -295,6 +341,61 @@ int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,
 
 /* p[e*stride] = value for e < n: the gauge P[0,0] = 0 of rbc2d.py:384-385. */
 int pypde_set_entries(double* p, long long stride, int n, double value, void* stream);
+
+/* ---- step plans: a whole time step behind one call ------------------- */
+/* A step plan is the ordered list of the launches above that make up one fused
+ * time step, recorded once (descriptors copied into host memory) and replayed
+ * by one call.  Pointer arguments are recorded relative to `regions`: address
+ * ranges declared at creation (state arrays, workspace).  A run may move each
+ * region to another base -- a pointer inside region r is shifted by the move
+ * of r -- so one plan steps any buffers of the declared sizes; pointers outside
+ * every region (the plan-owned operator tables) are used as recorded.  Running
+ * a plan allocates nothing; runs are asynchronous on the given stream, and
+ * runs that overlap on the same buffers race.  Plans are immutable after
+ * recording and may be run from several host threads.
+ * No reference counterpart as an object: a plan carries the operation
+ * sequence of NavierStokes.update (navier/rbc2d.py:396-434), which the
+ * reference executes as Python calls. */
+#define PYPDE_PLAN_MAX_REGIONS 64
+typedef struct pypde_plan pypde_plan;
+
+int pypde_plan_create(int nregions, const void* const* bases, const long long* bytes,
+                      pypde_plan** out);
+int pypde_plan_destroy(pypde_plan* p);
+/* Recording: the arguments of pypde_dgemm_batched, pypde_dgemm_grouped,
+ * pypde_recurrence_solve, pypde_rbc_products, pypde_rbc_products_yfold,
+ * pypde_splitk_reduce and pypde_rbc_rhs_banded, without the stream.  Arguments
+ * are checked when run. */
+int pypde_plan_add_gemm(pypde_plan* p, const pypde_gemm_desc* d);
+int pypde_plan_add_grouped(pypde_plan* p, const pypde_gemm_desc* descs, int count);
+int pypde_plan_add_recurrence(pypde_plan* p, const pypde_lines* g, const pypde_recurrence* r);
+int pypde_plan_add_products(pypde_plan* p, const double* Y, const double* dtbc, double* P,
+                            int L0, int L1, int batch);
+int pypde_plan_add_products_yfold(pypde_plan* p, const double* Y, const double* Yold,
+                                  const double* dtbc, double* P, int L0, int L1,
+                                  double sign_value, double sign_dy, double ratio, int batch);
+int pypde_plan_add_splitk_reduce(pypde_plan* p, const double* W, int S, const double* D,
+                                 const double* X, int xm, int xr, double* C, long long mn,
+                                 int entries);
+int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* args);
+/* Number of recorded launches (-1 for a null plan); bytes of a region (-1 if none). */
+int pypde_plan_size(const pypde_plan* p);
+long long pypde_plan_region_bytes(const pypde_plan* p, int region);
+/* Run the recorded launches n_steps times; bases[r] (NULL entry or NULL array:
+ * as declared) is region r's base for this run. */
+int pypde_plan_run(const pypde_plan* p, const void* const* bases, int n_steps, void* stream);
+
+/* n_steps IMEX time steps of B RBC instances of one shape (one instance: the
+ * drop-in NavierStokes.update; B instances with per-instance Ra, Pr, dt: the
+ * batched form), by a step plan whose regions are, in order,
+ *   0  X     [B, 3, M0, M1]  Galerkin coefficients of U, V, T
+ *   1  pres  [B, N0, N1]     Chebyshev coefficients of the pressure
+ *   2  P     [B, M0, M1]     pressure-correction potential
+ *   3  workspace of pypde_plan_region_bytes(plan, 3) bytes (any contents).
+ * In place on X, pres and P.  Replaces NavierStokes.update / iterate
+ * (navier/rbc2d.py:396-434, pypde/solver/integrator.py:14-41). */
+int pypde_rbc_step(const pypde_plan* plan, double* X, double* pres, double* P, void* workspace,
+                   int n_steps, void* stream);
 
 #ifdef __cplusplus
 }

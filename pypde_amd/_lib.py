@@ -63,6 +63,16 @@ class GemmDesc(ctypes.Structure):
                 ("mirror_sign", ctypes.c_double), ("mirror_div", ctypes.c_int)]
 
 
+class RbcRhs(ctypes.Structure):
+    """Mirror of `pypde_rbc_rhs`."""
+
+    _fields_ = [("X", ctypes.c_void_p), ("pres", ctypes.c_void_p), ("cst", ctypes.c_void_p),
+                ("R", ctypes.c_void_p), ("mdta", ctypes.c_void_p), ("dta", ctypes.c_void_p),
+                ("fdiff", ctypes.c_void_p), ("tx", ctypes.c_void_p), ("ty", ctypes.c_void_p),
+                ("M0", ctypes.c_int), ("M1", ctypes.c_int), ("N0", ctypes.c_int),
+                ("N1", ctypes.c_int), ("batch", ctypes.c_int)]
+
+
 class Recurrence(ctypes.Structure):
     """Mirror of `pypde_recurrence`."""
 
@@ -110,6 +120,22 @@ _SIGS = {
     "pypde_set_entries": ([_vp, _ll, _i, _d, _vp], _i),
     "pypde_recurrence_solve": ([_lp, ctypes.c_void_p, _vp], _i),
     "pypde_rbc_products_vjp": ([_vp, _vp, _vp, _vp, _i, _i, _i, _vp], _i),
+    "pypde_rbc_products_vjp_c": ([_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _i, _i, _vp], _i),
+    "pypde_plan_create": ([_i, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], _i),
+    "pypde_plan_destroy": ([_vp], _i),
+    "pypde_plan_add_gemm": ([_vp, ctypes.c_void_p], _i),
+    "pypde_plan_add_grouped": ([_vp, ctypes.c_void_p, _i], _i),
+    "pypde_plan_add_recurrence": ([_vp, _lp, ctypes.c_void_p], _i),
+    "pypde_plan_add_products": ([_vp, _vp, _vp, _vp, _i, _i, _i], _i),
+    "pypde_plan_add_products_yfold": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i], _i),
+    "pypde_plan_add_splitk_reduce": ([_vp, _vp, _i, _vp, _vp, _i, _i, _vp, ctypes.c_longlong,
+                                      _i], _i),
+    "pypde_plan_add_rhs_banded": ([_vp, ctypes.c_void_p], _i),
+    "pypde_rbc_rhs_banded": ([ctypes.c_void_p, _vp], _i),
+    "pypde_plan_size": ([_vp], _i),
+    "pypde_plan_region_bytes": ([_vp, _i], ctypes.c_longlong),
+    "pypde_plan_run": ([_vp, ctypes.c_void_p, _i, _vp], _i),
+    "pypde_rbc_step": ([_vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
 }
 
 for _name, (_args, _res) in _SIGS.items():

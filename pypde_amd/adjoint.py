@@ -33,29 +33,32 @@ def _vp(t):
 
 
 def _fields_gemm(ta, tb, m, n, k, batch, A, lda, iA, B, ldb, iB, C, ldc, iC, alpha=1.0,
-                 alpha_vec=None, beta=0.0, D=None, iD=None, defer=False):
+                 alpha_vec=None, beta=0.0, D=None, iD=None, defer=False, alpha_div=3):
     """One batched GEMM over (instance, field) entries e = 3 b + f: operators
     indexed per field through a modulus-3 batch index, per-instance scales
-    alpha_vec[e / 3]; operands are device pointers.  defer: return the
-    descriptor without launching."""
+    alpha_vec[e / alpha_div] (per instance: 3, per entry: 1); operands are
+    device pointers.  defer: return the descriptor without launching."""
     d = ops._desc(ta, tb, m, n, k, batch, alpha, beta, A, lda, 0, B, ldb, 0, D,
                   ldc if D is not None else 0, 0, C, ldc, 0)
     d.iA, d.iB, d.iC = iA, iB, iC
     if D is not None:
         d.iD = iD
     if alpha_vec is not None:
-        d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), 3
+        d.alpha_vec, d.alpha_div = alpha_vec.data_ptr(), alpha_div
     if defer:
         return d
     ops._launch_desc(d)
 
 
 class StepAdjoint:
-    """VJP of one fused Euler step of an RBCBatch (all instances at once)."""
+    """VJP of one fused step of an RBCBatch (all instances at once): Euler or
+    the three-stage scheme, any implicit fraction beta.  A stage maps (X_s,
+    pres_s, X_(s-1)) to (X_(s+1), pres_(s+1)) -- the previous stage's state
+    enters through the velocities of the c-term (rbc2d.py:261-283) -- and the
+    step's VJP runs the stage VJPs in reverse, adding each c-term cotangent to
+    the stage before."""
 
     def __init__(self, eng: RBCBatch):
-        if eng.integrator != "eu" or eng.beta != 1.0:
-            raise NotImplementedError("the discrete adjoint covers the Euler step with beta = 1")
         self.e = eng
         e = eng
         B, N0, N1, M0, M1, L0, L1 = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1
@@ -65,10 +68,21 @@ class StepAdjoint:
         self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
         self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
         self.Y2b = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+        if e.nstage > 1:
+            # the previous stage's fields and their cotangents (c-term)
+            self.Y2p = torch.zeros_like(self.Y2)
+            self.Y2pb = torch.zeros_like(self.Y2)
         self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
         self.graph = None
-        # transposed solves along x on reversed lines (four-diagonal recurrences)
-        self.hrt, self.hrt_y, prt, prt_lm = e.transposed_line_tables()
+        # transposed solves on reversed lines (four-diagonal recurrences): the
+        # Helmholtz tables per stage (their lambda is dt a_s beta nu), the
+        # Poisson lines once
+        self.hrt_st, self.hrt_y_st = [], []
+        for st in range(e.nstage):
+            hx, hy, prt, prt_lm = e.transposed_line_tables(st)
+            self.hrt_st.append(hx)
+            self.hrt_y_st.append(hy)
+        self.hrt, self.hrt_y = self.hrt_st[0], self.hrt_y_st[0]
         self.prt = (prt, prt_lm)
 
     @staticmethod
@@ -122,30 +136,75 @@ class StepAdjoint:
             del scratch
         run(geom(x), autotune.cache().get(key, 2))
 
-    def forward_fields(self, X):
-        """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients), one
-        launch each over every instance and field."""
+    def forward_fields(self, X, Y2=None):
+        """Y1 = GG_f X_f, Y2 = Y1 GGy^T (the dealiased values / gradients, into
+        Y2 or the adjoint's own buffer), one launch each over every instance
+        and field."""
         e = self.e
+        Y2 = self.Y2 if Y2 is None else Y2
         B3, M0, M1, L0, L1 = 3 * e.B, e.M0, e.M1, e.L0, e.L1
         X = X.contiguous()
         _fields_gemm(0, 0, 2 * L0, M1, M0, B3, e.GGx.data_ptr(), M0, ops._bi(2 * L0 * M0, m1=3),
                      X.data_ptr(), M1, ops._bi(M0 * M1), self.Y1.data_ptr(), M1,
                      ops._bi(2 * L0 * M1))
         _fields_gemm(0, 1, 2 * L0, 2 * L1, M1, B3, self.Y1.data_ptr(), M1, ops._bi(2 * L0 * M1),
-                     e.GGy.data_ptr(), M1, ops._bi(0), self.Y2.data_ptr(), 2 * L1,
+                     e.GGy.data_ptr(), M1, ops._bi(0), Y2.data_ptr(), 2 * L1,
                      ops._bi(4 * L0 * L1))
 
+    def fields_vjp(self, Y2b, out, accumulate):
+        """out (+)= GG_f^T (Y2b GGy): the cotangent of X from that of its fields."""
+        e = self.e
+        B3, M0, M1, L0, L1 = 3 * e.B, e.M0, e.M1, e.L0, e.L1
+        bi = ops._bi
+        y1b = torch.empty((e.B, 3, 2 * L0, M1), dtype=ops.DTYPE, device=out.device)
+        _fields_gemm(0, 0, 2 * L0, M1, 2 * L1, B3, Y2b.data_ptr(), 2 * L1, bi(4 * L0 * L1),
+                     e.GGy.data_ptr(), M1, bi(0), y1b.data_ptr(), M1, bi(2 * L0 * M1))
+        _fields_gemm(1, 0, M0, M1, 2 * L0, B3, e.GGx.data_ptr(), M0, bi(2 * L0 * M0, m1=3),
+                     y1b.data_ptr(), M1, bi(2 * L0 * M1), out.data_ptr(), M1, bi(M0 * M1),
+                     beta=1.0 if accumulate else 0.0, D=out.data_ptr() if accumulate else None,
+                     iD=bi(M0 * M1))
+
     def vjp(self, X, pres, Xb, pb):
-        """Cotangents of the step input (X, pres) from those of its output."""
+        """Cotangents of the step input (X, pres) from those of its output.  For
+        the three-stage scheme the stage inputs are recomputed from (X, pres)
+        by the engine's stage launches (on its own buffers)."""
+        e = self.e
+        if e.nstage == 1:
+            Xin, pin, _ = self._vjp_stage(0, X, pres, None, Xb, pb)
+            return Xin, pin
+        states = [(X, pres)]
+        saved = [t.clone() for t in (e.X, e.pres, e.P)]
+        e.X.copy_(X)
+        e.pres.copy_(pres)
+        for st in range(e.nstage - 1):
+            e.run_stage(st)
+            states.append((e.X.clone(), e.pres.clone()))
+        for t, v in zip((e.X, e.pres, e.P), saved):
+            t.copy_(v)
+        carry = None   # c-term cotangent of the stage input, from the stage after
+        for st in range(e.nstage - 1, -1, -1):
+            Xs, ps = states[st]
+            Xprev = states[st - 1][0] if st > 0 else None
+            Xb, pb, cb = self._vjp_stage(st, Xs, ps, Xprev, Xb, pb)
+            if carry is not None:
+                Xb = Xb + carry
+            carry = cb
+        return Xb, pb
+
+    def _vjp_stage(self, st, X, pres, Xprev, Xb, pb):
+        """VJP of stage st: cotangents of its input (X, pres) and, for a stage
+        with a c-term, of the previous stage's state Xprev (else None)."""
         e = self.e
         B, N0, N1, M0, M1, L0, L1, K = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1, e.K
         f64 = dict(dtype=ops.DTYPE, device=X.device)
         g = ops.bgemm
+        cterm = Xprev is not None and float(e.kc[st]) != 0.0
         # 21  velocity correction: W1b = -A_c0^T Ub, W2b = -A_c1^T Vb
         NW = N1 + 2 * M1
         Wb = torch.empty((B, M0, NW), **f64)
         ops.launch_grouped([
-            g(e.Snx, pb, transA=True, alpha_vec=e.t_idt, out=Wb[:, :, :N1], defer=True)[1],  # 20
+            g(e.Snx, pb, transA=True, alpha_vec=e.st_idta[st], out=Wb[:, :, :N1],
+              defer=True)[1],  # 20
             g(e.Acorr[0], Xb[:, 0], transA=True, alpha=-1.0, out=Wb[:, :, N1:N1 + M1],
               defer=True)[1],
             g(e.Acorr[1], Xb[:, 1], transA=True, alpha=-1.0, out=Wb[:, :, N1 + M1:],
@@ -177,17 +236,17 @@ class StepAdjoint:
         R3 = Rb.reshape(3 * B, M0, M1)
         # along y: on the transposed fields, so that the lines are columns
         Rt = R3.transpose(1, 2).contiguous()
-        self._reversed_solve(Rt, M1, M0, 3 * B, self.hrt_y, False)
+        self._reversed_solve(Rt, M1, M0, 3 * B, self.hrt_y_st[st], False)
         R3.copy_(Rt.transpose(1, 2))
-        self._reversed_solve(R3, M0, M1, 3 * B, self.hrt, False)
-        # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt BST_x^T R^_V
+        self._reversed_solve(R3, M0, M1, 3 * B, self.hrt_st[st], False)
+        # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt a BST_x^T R^_V
         Zb = torch.empty((B, 3, K, M1), **f64)
         bi = ops._bi
         _fields_gemm(1, 0, K, M1, M0, 3 * B, e.Arhs.data_ptr(), K, bi(M0 * K, m1=3),
                      Rb.data_ptr(), M1, bi(M0 * M1), Zb.data_ptr(), M1, bi(K * M1))
-        zt = Zb[:, 2, L0 + N0:]
-        g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.t_dt, beta=1.0, D=zt, out=zt)
-        # 6  X^ (input) = Z^_f[2] BS_y  and  4  P^r_f = -dt Z^_f[0] BF_y (one launch)
+        zt = Zb[:, 2, L0 + N0:L0 + N0 + M0]
+        g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.st_dta[st], beta=1.0, D=zt, out=zt)
+        # 6  X^ (input) = Z^_f[2] BS_y  and  4  P^r_f = -dt b Z^_f[0] BF_y (one launch)
         Xin = torch.empty((B, 3, M0, M1), **f64)
         Prb = torch.empty((B, 3, L0, L1), **f64)
         ops.launch_grouped([
@@ -196,24 +255,38 @@ class StepAdjoint:
                          bi(M0 * M1), defer=True),
             _fields_gemm(0, 0, L0, L1, M1, 3 * B, Zb.data_ptr(), M1, bi(K * M1),
                          e.BFy.data_ptr(), L1, bi(0), Prb.data_ptr(), L1, bi(L0 * L1),
-                         alpha_vec=e.t_mdt, defer=True)], "vjp_z")
-        # 5  pres^ (input) = pres^ - dt (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
+                         alpha_vec=e.st_mdtb[st], defer=True)], "vjp_z")
+        if e.diffusion:
+            # 6b  explicit diffusion: Z[3] = f X BS_y^T, Z[4] = f X BD2S_y^T, so
+            #     X^ += f (Z^[3] BS_y + Z^[4] BD2S_y), f per (instance, field)
+            for blk in range(2):
+                _fields_gemm(0, 0, M0, M1, M1, 3 * B,
+                             Zb.data_ptr() + 8 * (L0 + N0 + (1 + blk) * M0) * M1, M1,
+                             bi(K * M1), e.DFy[blk].data_ptr(), M1, bi(0), Xin.data_ptr(), M1,
+                             bi(M0 * M1), alpha_vec=e.st_fdiff[st], alpha_div=1, beta=1.0,
+                             D=Xin.data_ptr(), iD=bi(M0 * M1))
+        # 5  pres^ (input) = pres^ - dt a (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
         pin = pb.clone()
-        g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
-        g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.t_mdt, beta=1.0, D=pin, out=pin)
-        # 3  products
+        g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.st_mdta[st], beta=1.0, D=pin, out=pin)
+        g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.st_mdta[st], beta=1.0, D=pin, out=pin)
+        # 3  products (with the c-term: the previous stage's velocities too)
         self.forward_fields(X)
-        check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb), _vp(self.Y2b),
-                                         L0, L1, B, ops.stream_ptr()), "products_vjp")
+        Xprev_b = None
+        if cterm:
+            self.forward_fields(Xprev, self.Y2p)
+            ratio = float(e.kc[st]) / float(e.kb[st])
+            check(lib.pypde_rbc_products_vjp_c(_vp(self.Y2), _vp(self.Y2p), _vp(e.dTbcdz1),
+                                               _vp(Prb), _vp(self.Y2b), _vp(self.Y2pb), ratio,
+                                               L0, L1, B, ops.stream_ptr()), "products_vjp_c")
+            Xprev_b = torch.empty((B, 3, M0, M1), **f64)
+            self.fields_vjp(self.Y2pb, Xprev_b, accumulate=False)
+        else:
+            check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb),
+                                             _vp(self.Y2b), L0, L1, B, ops.stream_ptr()),
+                  "products_vjp")
         # 2  Y1^ = Y2^ GGy ; 1 X^ += GG_f^T Y1^
-        y1b = torch.empty((B, 3, 2 * L0, M1), **f64)
-        _fields_gemm(0, 0, 2 * L0, M1, 2 * L1, 3 * B, self.Y2b.data_ptr(), 2 * L1,
-                     bi(4 * L0 * L1), e.GGy.data_ptr(), M1, bi(0), y1b.data_ptr(), M1,
-                     bi(2 * L0 * M1))
-        _fields_gemm(1, 0, M0, M1, 2 * L0, 3 * B, e.GGx.data_ptr(), M0, bi(2 * L0 * M0, m1=3),
-                     y1b.data_ptr(), M1, bi(2 * L0 * M1), Xin.data_ptr(), M1, bi(M0 * M1),
-                     beta=1.0, D=Xin.data_ptr(), iD=bi(M0 * M1))
-        return Xin, pin
+        self.fields_vjp(self.Y2b, Xin, accumulate=True)
+        return Xin, pin, Xprev_b
 
     def vjp_replay(self, X, pres, Xb, pb):
         """vjp() replayed from a HIP graph (captured on the first call, after one

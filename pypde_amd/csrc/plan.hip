@@ -1,0 +1,332 @@
+// Step plans: the launches of one fused time step as a replayable list.
+//
+// A plan records the library's own launches (batched and grouped GEMMs,
+// recurrences, the convective-product and split-K reduce kernels) in order,
+// with their descriptors copied into host memory at recording time.  Every
+// pointer argument is classified against the plan's regions -- address ranges
+// declared at creation (the state arrays, the workspace) -- so that a run can
+// move each region to another base: a pointer inside region r is shifted by
+// (new base of r - declared base of r); pointers outside every region
+// (operator tables) are used as recorded.  Running a plan allocates nothing.
+#include "common.h"
+#include "../../include/pypde_amd.h"
+
+#include <new>
+#include <vector>
+
+namespace pypde {
+
+enum PlanOpKind { OP_GEMM, OP_GROUPED, OP_RECURRENCE, OP_PRODUCTS, OP_PRODUCTS_YFOLD, OP_REDUCE,
+                  OP_RHS_BANDED };
+
+struct PlanOp {
+    PlanOpKind kind;
+    std::vector<pypde_gemm_desc> descs;  // OP_GEMM (one) / OP_GROUPED (count)
+    pypde_lines lines;                   // OP_RECURRENCE
+    pypde_recurrence rec;
+    pypde_rbc_rhs rhs;                   // OP_RHS_BANDED
+    // products: Y, Yold (yfold only), dtbc, P; reduce: W, D, X, C
+    const double* p0;
+    const double* p1;
+    const double* p2;
+    double* p3;
+    int L0, L1, batch;
+    double sign_value, sign_dy, ratio;
+    int S, xm, xr, entries;
+    long long mn;
+};
+
+}  // namespace pypde
+
+struct pypde_plan {
+    std::vector<uintptr_t> base;
+    std::vector<long long> bytes;
+    std::vector<pypde::PlanOp> ops;
+};
+
+namespace pypde {
+
+// Shift p by the move of the region that contains it (none: unchanged).
+template <typename T>
+static inline T* relocate(const pypde_plan* pl, const std::vector<intptr_t>& delta, T* p)
+{
+    if (!p) return p;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    for (size_t r = 0; r < pl->base.size(); ++r) {
+        if (a >= pl->base[r] && a < pl->base[r] + (uintptr_t)pl->bytes[r])
+            return reinterpret_cast<T*>(a + delta[r]);
+    }
+    return p;
+}
+
+static void relocate_desc(const pypde_plan* pl, const std::vector<intptr_t>& dl, pypde_gemm_desc& d)
+{
+    d.A = relocate(pl, dl, d.A);
+    d.B = relocate(pl, dl, d.B);
+    d.D = relocate(pl, dl, d.D);
+    d.C = relocate(pl, dl, d.C);
+    d.alpha_vec = relocate(pl, dl, d.alpha_vec);
+    d.E = relocate(pl, dl, d.E);
+    d.gamma_vec = relocate(pl, dl, d.gamma_vec);
+    d.A1 = relocate(pl, dl, d.A1);
+    d.B1 = relocate(pl, dl, d.B1);
+}
+
+static int run_op(const pypde_plan* pl, const std::vector<intptr_t>& dl, bool moved,
+                  const PlanOp& op, void* stream)
+{
+    switch (op.kind) {
+    case OP_GEMM:
+    case OP_GROUPED: {
+        if (!moved) {
+            return op.kind == OP_GEMM ? pypde_dgemm_batched(op.descs.data(), stream)
+                                      : pypde_dgemm_grouped(op.descs.data(), (int)op.descs.size(),
+                                                            stream);
+        }
+        pypde_gemm_desc d[4];
+        const int n = (int)op.descs.size();
+        for (int i = 0; i < n; ++i) {
+            d[i] = op.descs[i];
+            relocate_desc(pl, dl, d[i]);
+        }
+        return op.kind == OP_GEMM ? pypde_dgemm_batched(d, stream)
+                                  : pypde_dgemm_grouped(d, n, stream);
+    }
+    case OP_RECURRENCE: {
+        if (!moved) return pypde_recurrence_solve(&op.lines, &op.rec, stream);
+        pypde_lines g = op.lines;
+        pypde_recurrence r = op.rec;
+        g.src = relocate(pl, dl, g.src);
+        g.dst = relocate(pl, dl, g.dst);
+        r.F = relocate(pl, dl, r.F);
+        r.LM = relocate(pl, dl, r.LM);
+        r.R = relocate(pl, dl, r.R);
+        r.MU = relocate(pl, dl, r.MU);
+        r.NU = relocate(pl, dl, r.NU);
+        r.pre = relocate(pl, dl, r.pre);
+        return pypde_recurrence_solve(&g, &r, stream);
+    }
+    case OP_PRODUCTS:
+        return pypde_rbc_products(relocate(pl, dl, op.p0), relocate(pl, dl, op.p2),
+                                  relocate(pl, dl, op.p3), op.L0, op.L1, op.batch, stream);
+    case OP_PRODUCTS_YFOLD:
+        return pypde_rbc_products_yfold(relocate(pl, dl, op.p0), relocate(pl, dl, op.p1),
+                                        relocate(pl, dl, op.p2), relocate(pl, dl, op.p3), op.L0,
+                                        op.L1, op.sign_value, op.sign_dy, op.ratio, op.batch,
+                                        stream);
+    case OP_RHS_BANDED: {
+        if (!moved) return pypde_rbc_rhs_banded(&op.rhs, stream);
+        pypde_rbc_rhs a = op.rhs;
+        a.X = relocate(pl, dl, a.X);
+        a.pres = relocate(pl, dl, a.pres);
+        a.cst = relocate(pl, dl, a.cst);
+        a.R = relocate(pl, dl, a.R);
+        a.mdta = relocate(pl, dl, a.mdta);
+        a.dta = relocate(pl, dl, a.dta);
+        a.fdiff = relocate(pl, dl, a.fdiff);
+        a.tx = relocate(pl, dl, a.tx);
+        a.ty = relocate(pl, dl, a.ty);
+        return pypde_rbc_rhs_banded(&a, stream);
+    }
+    case OP_REDUCE:
+        return pypde_splitk_reduce(relocate(pl, dl, op.p0), op.S, relocate(pl, dl, op.p1),
+                                   relocate(pl, dl, op.p2), op.xm, op.xr, relocate(pl, dl, op.p3),
+                                   op.mn, op.entries, stream);
+    }
+    set_error("step plan: unknown operation");
+    return ERR_ARG;
+}
+
+static int add_op(pypde_plan* p, PlanOp&& op)
+{
+    try {
+        p->ops.push_back(std::move(op));
+    } catch (const std::bad_alloc&) {
+        set_error("step plan: out of host memory");
+        return ERR_ARG;
+    }
+    return OK;
+}
+
+}  // namespace pypde
+
+using namespace pypde;
+
+extern "C" int pypde_plan_create(int nregions, const void* const* bases, const long long* bytes,
+                                 pypde_plan** out)
+{
+    PYPDE_REQUIRE(out, "null output handle");
+    *out = nullptr;
+    PYPDE_REQUIRE(nregions >= 0 && nregions <= PYPDE_PLAN_MAX_REGIONS,
+                  "regions must be 0..%d (got %d)", PYPDE_PLAN_MAX_REGIONS, nregions);
+    PYPDE_REQUIRE(nregions == 0 || (bases && bytes), "null region arrays");
+    for (int r = 0; r < nregions; ++r) {
+        PYPDE_REQUIRE(bases[r] && bytes[r] > 0, "region %d: null base or empty range", r);
+        for (int q = 0; q < r; ++q) {
+            const uintptr_t a0 = (uintptr_t)bases[q], a1 = a0 + (uintptr_t)bytes[q];
+            const uintptr_t b0 = (uintptr_t)bases[r], b1 = b0 + (uintptr_t)bytes[r];
+            PYPDE_REQUIRE(b1 <= a0 || a1 <= b0, "regions %d and %d overlap", q, r);
+        }
+    }
+    pypde_plan* p = new (std::nothrow) pypde_plan();
+    PYPDE_REQUIRE(p, "step plan: out of host memory");
+    for (int r = 0; r < nregions; ++r) {
+        p->base.push_back((uintptr_t)bases[r]);
+        p->bytes.push_back(bytes[r]);
+    }
+    *out = p;
+    return OK;
+}
+
+extern "C" int pypde_plan_destroy(pypde_plan* p)
+{
+    delete p;
+    return OK;
+}
+
+extern "C" int pypde_plan_add_gemm(pypde_plan* p, const pypde_gemm_desc* d)
+{
+    PYPDE_REQUIRE(p && d, "null argument");
+    PlanOp op{};
+    op.kind = OP_GEMM;
+    op.descs.assign(d, d + 1);
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_grouped(pypde_plan* p, const pypde_gemm_desc* descs, int count)
+{
+    PYPDE_REQUIRE(p && descs, "null argument");
+    PYPDE_REQUIRE(count >= 1 && count <= 4, "grouped launch of 1..4 descriptors (got %d)", count);
+    PlanOp op{};
+    op.kind = OP_GROUPED;
+    op.descs.assign(descs, descs + count);
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_recurrence(pypde_plan* p, const pypde_lines* g,
+                                         const pypde_recurrence* r)
+{
+    PYPDE_REQUIRE(p && g && r, "null argument");
+    PlanOp op{};
+    op.kind = OP_RECURRENCE;
+    op.lines = *g;
+    op.rec = *r;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_products(pypde_plan* p, const double* Y, const double* dtbc, double* P,
+                                       int L0, int L1, int batch)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_PRODUCTS;
+    op.p0 = Y;
+    op.p2 = dtbc;
+    op.p3 = P;
+    op.L0 = L0;
+    op.L1 = L1;
+    op.batch = batch;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_products_yfold(pypde_plan* p, const double* Y, const double* Yold,
+                                             const double* dtbc, double* P, int L0, int L1,
+                                             double sign_value, double sign_dy, double ratio,
+                                             int batch)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_PRODUCTS_YFOLD;
+    op.p0 = Y;
+    op.p1 = Yold;
+    op.p2 = dtbc;
+    op.p3 = P;
+    op.L0 = L0;
+    op.L1 = L1;
+    op.sign_value = sign_value;
+    op.sign_dy = sign_dy;
+    op.ratio = ratio;
+    op.batch = batch;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_splitk_reduce(pypde_plan* p, const double* W, int S, const double* D,
+                                            const double* X, int xm, int xr, double* C,
+                                            long long mn, int entries)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_REDUCE;
+    op.p0 = W;
+    op.p1 = D;
+    op.p2 = X;
+    op.p3 = C;
+    op.S = S;
+    op.xm = xm;
+    op.xr = xr;
+    op.mn = mn;
+    op.entries = entries;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* args)
+{
+    PYPDE_REQUIRE(p && args, "null argument");
+    PlanOp op{};
+    op.kind = OP_RHS_BANDED;
+    op.rhs = *args;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_size(const pypde_plan* p)
+{
+    return p ? (int)p->ops.size() : -1;
+}
+
+extern "C" long long pypde_plan_region_bytes(const pypde_plan* p, int region)
+{
+    if (!p || region < 0 || region >= (int)p->base.size()) return -1;
+    return p->bytes[region];
+}
+
+extern "C" int pypde_plan_run(const pypde_plan* p, const void* const* bases, int n_steps,
+                              void* stream)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PYPDE_REQUIRE(n_steps >= 0, "negative step count");
+    const size_t nr = p->base.size();
+    std::vector<intptr_t> delta(nr, 0);
+    bool moved = false;
+    if (bases) {
+        for (size_t r = 0; r < nr; ++r) {
+            if (!bases[r]) continue;
+            delta[r] = (intptr_t)((uintptr_t)bases[r] - p->base[r]);
+            moved = moved || delta[r] != 0;
+        }
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        for (size_t i = 0; i < p->ops.size(); ++i) {
+            const int rc = run_op(p, delta, moved, p->ops[i], stream);
+            if (rc != OK) {
+                // keep the failing launch's message, name its place in the plan
+                char msg[768];
+                snprintf(msg, sizeof(msg), "%s", last_error());
+                set_error("step plan, step %d, launch %zu: %s", s, i, msg);
+                return rc;
+            }
+        }
+    }
+    return OK;
+}
+
+extern "C" int pypde_rbc_step(const pypde_plan* p, double* X, double* pres, double* P,
+                              void* workspace, int n_steps, void* stream)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PYPDE_REQUIRE(p->base.size() == 4,
+                  "an RBC step plan has the regions X, pres, P and workspace (got %zu regions)",
+                  p->base.size());
+    PYPDE_REQUIRE(X && pres && P && workspace, "null state or workspace pointer");
+    const void* bases[4] = {X, pres, P, workspace};
+    return pypde_plan_run(p, bases, n_steps, stream);
+}

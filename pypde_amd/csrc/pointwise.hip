@@ -145,6 +145,66 @@ __global__ void rbc_products_vjp_kernel(const double* __restrict__ Y, const doub
     }
 }
 
+// Adjoint of the products with the three-stage scheme's c-term,
+//   P_f = gx_f (ux + r ux_o) + gy_f (uz + r uz_o)   (+ (uz + r uz_o) dTbc/dz for T),
+// the current gradients times the current velocities plus r = c / b times the
+// previous stage's velocities (Yold).  Writes the cotangents of the current
+// blocks into Yb (as rbc_products_vjp_kernel, velocities now u + r u_o in the
+// gradient terms) and those of the previous stage's value blocks into Yob (U
+// and V values r * the velocity cotangent; every other block of Yob zero).
+__global__ void rbc_products_vjp_c_kernel(const double* __restrict__ Y, const double* __restrict__ Yold,
+                                          const double* __restrict__ dtbc,
+                                          const double* __restrict__ Pb, double* __restrict__ Yb,
+                                          double* __restrict__ Yob, double ratio, int L0, int L1,
+                                          int B)
+{
+    const long long per = (long long)L0 * L1;
+    const long long ld = 2LL * L1;
+    const long long fs = 4LL * per;
+    const long long total = per * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long b = t / per;
+        const long long r = t - b * per;
+        const int i = (int)(r / L1), j = (int)(r - (long long)i * L1);
+        const double* Yi = Y + b * 3 * fs;
+        const double* Yo_in = Yold + b * 3 * fs;
+        double* Yo = Yb + b * 3 * fs;
+        double* Yp = Yob + b * 3 * fs;
+        const long long v = (long long)i * ld + j;
+        const long long dy = v + L1;
+        const long long dx = (long long)(i + L0) * ld + j;
+        const long long dxy = dx + L1;
+        const double ux = Yi[v] + ratio * Yo_in[v];
+        const double uz = Yi[fs + v] + ratio * Yo_in[fs + v];
+        const double* Pbb = Pb + b * 3 * per;
+        const double pu = Pbb[r], pv = Pbb[per + r], pt = Pbb[2 * per + r];
+        Yo[dx] = pu * ux;
+        Yo[dy] = pu * uz;
+        Yo[fs + dx] = pv * ux;
+        Yo[fs + dy] = pv * uz;
+        Yo[2 * fs + dx] = pt * ux;
+        Yo[2 * fs + dy] = pt * uz;
+        const double gux = pu * Yi[dx] + pv * Yi[fs + dx] + pt * Yi[2 * fs + dx];
+        const double guz = pu * Yi[dy] + pv * Yi[fs + dy] + pt * (Yi[2 * fs + dy] + dtbc[r]);
+        Yo[v] = gux;
+        Yo[fs + v] = guz;
+        Yo[2 * fs + v] = 0.0;
+        Yo[dxy] = 0.0;
+        Yo[fs + dxy] = 0.0;
+        Yo[2 * fs + dxy] = 0.0;
+        Yp[v] = ratio * gux;
+        Yp[fs + v] = ratio * guz;
+        Yp[2 * fs + v] = 0.0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            Yp[f * fs + dy] = 0.0;
+            Yp[f * fs + dx] = 0.0;
+            Yp[f * fs + dxy] = 0.0;
+        }
+    }
+}
+
 // y[b] = y[b] - (coef[b] * x[b]) * c2   (pres -= nu*div*beta, rbc2d.py:391)
 __global__ void scaled_sub_kernel(long long per, int B, const double* __restrict__ coef, double c2,
                                   const double* __restrict__ x, double* __restrict__ y)
@@ -228,6 +288,19 @@ extern "C" int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const
     hipLaunchKernelGGL(rbc_products_vjp_kernel, dim3(grid_for((long long)L0 * L1 * batch)), dim3(256),
                        0, as_stream(stream), Y, dtbc, Pb, Yb, L0, L1, batch);
     return check_launch("pypde_rbc_products_vjp");
+}
+
+extern "C" int pypde_rbc_products_vjp_c(const double* Y, const double* Yold, const double* dtbc,
+                                        const double* Pb, double* Yb, double* Yob, double ratio,
+                                        int L0, int L1, int batch, void* stream)
+{
+    PYPDE_REQUIRE(Y && Yold && dtbc && Pb && Yb && Yob, "null pointer");
+    PYPDE_REQUIRE(L0 > 0 && L1 > 0 && batch >= 0, "invalid product geometry");
+    if (batch == 0) return OK;
+    hipLaunchKernelGGL(rbc_products_vjp_c_kernel, dim3(grid_for((long long)L0 * L1 * batch)),
+                       dim3(256), 0, as_stream(stream), Y, Yold, dtbc, Pb, Yb, Yob, ratio, L0, L1,
+                       batch);
+    return check_launch("pypde_rbc_products_vjp_c");
 }
 
 extern "C" int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,

@@ -39,7 +39,7 @@ import numpy as np
 import torch
 
 from . import ops
-from ._lib import BIdx, GemmDesc, Lines, check, lib
+from ._lib import BIdx, GemmDesc, Lines, RbcRhs, check, lib
 from .bases import Base, cheb_backward_matrix, cheb_forward_matrix, pseudoinverse_d2
 from ._lib import Recurrence
 from .solver import (fdma_lu, poisson_line_tables, rec_from_fdma, rec_from_poisson,
@@ -108,6 +108,26 @@ class AxisOps:
 # ---------------------------------------------------------------------------
 # launch records
 # ---------------------------------------------------------------------------
+def _diags(A, offsets, tol=1e-9):
+    """Diagonals of a banded operator: out[d, i] = A[i, i + offsets[d]] (0 where
+    the column is outside A).  Entries off these diagonals must be rounding
+    level (relative tol): the host products that form B D2 S leave up to about
+    2e-10 of max|A| there at N = 512, which the banded form drops."""
+    A = np.asarray(A)
+    m, n = A.shape
+    out = np.zeros((len(offsets), m))
+    mask = np.zeros_like(A, dtype=bool)
+    for d, off in enumerate(offsets):
+        for i in range(m):
+            if 0 <= i + off < n:
+                out[d, i] = A[i, i + off]
+                mask[i, i + off] = True
+    rest = np.abs(np.where(mask, 0.0, A)).max() if A.size else 0.0
+    if rest > tol * max(np.abs(A).max(), 1e-300):
+        raise ValueError("operator is not banded on offsets %s (off-band %.3e)" % (offsets, rest))
+    return out
+
+
 def _bidx(s1=0, d1=1, m1=0, s2=0, d2=1, m2=0):
     b = BIdx()
     b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s1, d1, m1, s2, d2, m2
@@ -214,12 +234,11 @@ class _GemmGroup:
 
 
 class _SplitRhs:
-    """Launches 7-8 with the right-hand-side product split along K: a grouped
-    launch of the partial products (batch over field x K slice) and the
-    buoyancy product, then pypde_splitk_reduce sums the slices in order onto
-    the constant term and adds the buoyancy to the V entries.  For one or a few
-    instances, where the unsplit product has too few output tiles to fill the
-    device."""
+    """Launch 7 with the right-hand-side product split along K: one launch of
+    the partial products (batch over field x K slice), then
+    pypde_splitk_reduce sums the slices in order onto the banded part.  For
+    one or a few instances, where the unsplit product has too few output tiles
+    to fill the device."""
 
     def __init__(self, name, group, reduce_args, S):
         self.name, self.group, self.reduce_args, self.S = name, group, reduce_args, S
@@ -232,9 +251,8 @@ class _SplitRhs:
 
 
 # independent launches of one stage that may share a launch
-_GROUPS = (("z_conv", "z_pres", "z_old", "z_diff"), ("z_conv", "z_pres", "z_old"),
-           ("z_pres", "z_old", "z_diff"), ("z_pres", "z_old"), ("ypass_v", "ypass_x"),
-           ("div_x", "poisson_rhs"), ("poisson_qy", "corr_y", "corr_y0"), ("pres_p", "correct"))
+_GROUPS = (("ypass_v", "ypass_x"), ("div_x", "poisson_rhs"), ("poisson_qy", "corr_y", "corr_y0"),
+           ("pres_p", "correct"))
 
 
 def _group_launches(L, tag):
@@ -262,6 +280,54 @@ class _Call:
 
     def __call__(self, stream):
         check(self.fn(*self.args, stream), self.name)
+
+
+class Plan:
+    """A step plan of the C-ABI (pypde_plan_create ... pypde_plan_destroy):
+    launches recorded with pointers relative to the given regions."""
+
+    # the recording entry point of each single-kernel launch of the step
+    _ADD = {"pypde_rbc_products": "pypde_plan_add_products",
+            "pypde_rbc_products_yfold": "pypde_plan_add_products_yfold",
+            "pypde_splitk_reduce": "pypde_plan_add_splitk_reduce",
+            "pypde_rbc_rhs_banded": "pypde_plan_add_rhs_banded"}
+
+    def __init__(self, nregions, bases, sizes):
+        h = ctypes.c_void_p()
+        check(lib.pypde_plan_create(nregions, bases, sizes, ctypes.byref(h)), "pypde_plan_create")
+        self.handle = h
+
+    def add(self, launch):
+        """Record one launch of the step (a _Gemm, _GemmGroup, _RecCall or _Call)."""
+        h = self.handle
+        if isinstance(launch, _Gemm):
+            check(lib.pypde_plan_add_gemm(h, ctypes.byref(launch.desc)), launch.name)
+        elif isinstance(launch, _GemmGroup):
+            if launch.split:
+                for m in launch.members:
+                    check(lib.pypde_plan_add_gemm(h, ctypes.byref(m.desc)), m.name)
+            else:
+                check(lib.pypde_plan_add_grouped(h, ctypes.byref(launch.arr), len(launch.members)),
+                      launch.name)
+        elif isinstance(launch, _RecCall):
+            check(lib.pypde_plan_add_recurrence(h, *launch.args), launch.name)
+        elif isinstance(launch, _Call) and launch.fn.__name__ in Plan._ADD:
+            add = getattr(lib, Plan._ADD[launch.fn.__name__])
+            check(add(h, *launch.args), launch.name)
+        else:
+            raise TypeError("launch %r cannot be recorded in a step plan" % (launch,))
+
+    def size(self):
+        return lib.pypde_plan_size(self.handle)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and lib is not None:
+            try:
+                lib.pypde_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
 
 
 # recurrence kernel forms: 1 block (64 lines per block), 2 lane-chunk,
@@ -437,8 +503,18 @@ class RBCBatch:
             blocks[1] += [ax.BD2S, ax.BS]
             blocks[2] += [ax.BD2ST, ax.BST]
             self.DFy = T(np.stack([ay.BS, ay.BD2S]))                          # [2, M1, M1]
+        # the dense right-hand-side operator of the discrete adjoint (its
+        # transpose maps the cotangent of X to those of the stacked parts)
         self.Arhs = T(np.stack([np.hstack(b) for b in blocks]))              # [3, M0, K]
         self.BSTx = T(ax.BST)                                                 # [M0, M0]
+        # the forward step applies the banded blocks as diagonals
+        # (pypde_rbc_rhs_banded) and only the forward transform BF densely
+        self.BFx = T(ax.BF)                                                   # [M0, L0]
+        self.tx = T(np.vstack([_diags(ax.BD, (1, 3, 5)), _diags(ax.B, (0, 2, 4)),
+                               _diags(ax.BS, (-2, 0, 2, 4)), _diags(ax.BST, (-2, 0, 2, 4)),
+                               _diags(ax.BD2S, (0, 2)), _diags(ax.BD2ST, (0, 2))]))   # [18, M0]
+        self.ty = T(np.vstack([_diags(ay.B, (0, 2, 4)), _diags(ay.BD, (1, 3, 5)),
+                               _diags(ay.BS, (-2, 0, 2, 4)), _diags(ay.BD2S, (0, 2))]))  # [12, M1]
         self.Cy = T(np.stack([ay.C, ay.Cd]))                                  # [2, N1, M1]
         self.Cx = T(np.hstack([ax.Cd, ax.C]))                                 # [N0, 2M0]
         pm = poisson_matrices([Base(N0, "CN"), Base(N1, "CN")], singular=True, scale=self.scale)
@@ -505,28 +581,45 @@ class RBCBatch:
         self.X = torch.zeros((B, 3, M0, M1), **f64)
         self.pres = torch.zeros((B, N0, N1), **f64)
         self.P = torch.zeros((B, M0, M1), **f64)
-        self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
+        if self.yfold is None and (self.nstage > 1 or self.beta != 1.0):
+            raise NotImplementedError("RK3 and beta != 1 need even dealiased lengths in y "
+                                      "(the y-folded layout)")
+        # candidate K splits of the right-hand-side product (few instances only)
+        tiles = 3 * B * ((M0 + 31) // 32) * ((M1 + 31) // 32)
+        self._splits = [S for S in (2, 3, 4, 6, 8) if L0 % S == 0 and L0 // S >= 32] \
+            if (tiles <= 768 and self.group_launches) else []
+        # every workspace of the step is a view into one buffer (the step plan's
+        # workspace region); the K-split partials come last, so that a plan
+        # that keeps fewer slices needs a shorter workspace
+        wsd = [("Y1", (B, 3, 2 * L0, M1))]
         if self.yfold is not None:
-            self.Y2 = torch.zeros((B, 3, 6, L0, L1 // 2), **f64)
-            # second buffer: the previous RK3 stage's velocities (c-term)
-            self.Y2old = torch.zeros_like(self.Y2) if self.nstage > 1 else None
-            self.Pr = torch.zeros((B, 3, 2, L0, L1 // 2), **f64)
+            wsd.append(("Y2", (B, 3, 6, L0, L1 // 2)))
+            if self.nstage > 1:  # the previous RK3 stage's velocities (c-term)
+                wsd.append(("Y2old", (B, 3, 6, L0, L1 // 2)))
+            wsd.append(("Pr", (B, 3, 2, L0, L1 // 2)))
         else:
-            if self.nstage > 1 or self.beta != 1.0:
-                raise NotImplementedError("RK3 and beta != 1 need even dealiased lengths in y "
-                                          "(the y-folded layout)")
-            self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
-            self.Pr = torch.zeros((B, 3, L0, L1), **f64)
-        self.Z = torch.zeros((B, 3, K, M1), **f64)
-        self.div = torch.zeros((B, N0, N1), **f64)
-        self.W2 = torch.zeros((B, 2, M0, N1 + M1), **f64)
-        self.RP = torch.zeros((B, M0, M1), **f64)
-        self.W1 = torch.zeros((B, M0, N1 + 2 * M1), **f64)
+            wsd += [("Y2", (B, 3, 2 * L0, 2 * L1)), ("Pr", (B, 3, L0, L1))]
+        wsd += [("Z", (B, 3, L0, M1)), ("Rb", (B, 3, M0, M1)), ("div", (B, N0, N1)),
+                ("W2", (B, 2, M0, N1 + M1)), ("RP", (B, M0, M1)), ("W1", (B, M0, N1 + 2 * M1))]
+        if self._splits:
+            wsd += [("rhs_work", (3 * B * max(self._splits), M0, M1))]
+        offs, total = {}, 0
+        for name, shape in wsd:
+            offs[name] = total
+            total += -(-int(np.prod(shape)) // 32) * 32   # 256-byte aligned views
+        self.ws = torch.zeros(total, **f64)
+        self.ws_offsets = offs
+        for name, shape in wsd:
+            n = int(np.prod(shape))
+            setattr(self, name, self.ws[offs[name]:offs[name] + n].view(shape))
+        if self.nstage == 1:
+            self.Y2old = None
         self._build_launches()
         self.tune()
         self.graph = None
         self.graph_steps = 0
         self.graphs = {}  # captured graphs by their number of steps
+        self.plan = None
 
     def _helmholtz_matrix(self, st, b, f, axis):
         """Dense Helmholtz operator B S - lam I S of stage st, instance b, field
@@ -541,9 +634,9 @@ class RBCBatch:
             bases[(n, key)] = Base(n, key)
         return hholtz_matrices(bases[(n, key)], lam, self.scale[axis])[2]
 
-    def transposed_line_tables(self):
+    def transposed_line_tables(self, st=0):
         """Normalised tables of the transposed line solves on reversed lines
-        (stage 0), for the discrete adjoint: the Helmholtz solves along x and
+        (stage st), for the discrete adjoint: the Helmholtz solves along x and
         along y per (instance, field) entry, [B, 3, LM | R | MU | NU, M], and the
         Poisson line solves per line, [LM | R | MU | NU][M0][M1] and line-major
         [4][M1][M0]."""
@@ -552,8 +645,8 @@ class RBCBatch:
         hy = np.zeros((self.B, 3, 4, self.M1))
         for b in range(self.B):
             for f in range(3):
-                hx[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(0, b, f, 0)))
-                hy[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(0, b, f, 1)))
+                hx[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(st, b, f, 0)))
+                hy[b, f] = np.stack(rec_reversed_transpose(self._helmholtz_matrix(st, b, f, 1)))
         pm = poisson_matrices([Base(self.N0, "CN"), Base(self.N1, "CN")], singular=True,
                               scale=self.scale)
         pr = torch.as_tensor(np.stack(rec_poisson_reversed_transpose(pm["Ax"], pm["Cx"],
@@ -607,14 +700,6 @@ class RBCBatch:
     def _build_launches(self):
         self._keep = []
         self._rhs_pairs = []
-        B, M0, M1, K = self.B, self.M0, self.M1, self.K
-        tiles = 3 * B * ((M0 + 31) // 32) * ((M1 + 31) // 32)
-        self._splits = [S for S in (2, 3, 4, 6, 8) if K % S == 0 and K // S >= 32] \
-            if (tiles <= 768 and self.group_launches) else []
-        if self._splits:
-            f64 = dict(dtype=ops.DTYPE, device=self.X.device)
-            self.rhs_work = torch.zeros(3 * B * max(self._splits) * M0 * M1, **f64)
-            self.rhs_workb = torch.zeros(B * M0 * M1, **f64)
         L = []
         for st in range(self.nstage):
             S = []
@@ -631,28 +716,8 @@ class RBCBatch:
         p = lambda t: t.data_ptr()
         X, Y1, Pr, Z = self.X, self.Y1, self.Pr, self.Z
         sX = M0 * M1
-        sZ = self.sZ = K * M1
+        sZ = self.sZ = L0 * M1
         tag = "" if self.nstage == 1 else "[%d]" % st
-        # 5-6 read only the stage-start state (X and pres are not written before
-        # launch 7), so they are issued next to launch 4 and may share its launch
-        Lz = []
-        # 5  Z[1] = -dt a pres (B | BD)_y^T   for U, V
-        Lz.append(_Gemm("z_pres" + tag, m=N0, n=M1, k=N1, batch=2 * B, A=p(self.pres), lda=N1,
-                       iA=_bidx(s1=N0 * N1, d1=2), B=p(self.PBy), ldb=N1, tb=1,
-                       iB=_bidx(s1=M1 * N1, m1=2), C=p(Z) + 8 * L0 * M1, ldc=M1,
-                       iC=_bidx(s1=3 * sZ, d1=2, s2=sZ, m2=2), alpha_vec=p(self.st_mdta[st]),
-                       alpha_div=2))
-        # 6  Z[2] = X BS_y^T
-        Lz.append(_Gemm("z_old" + tag, m=M0, n=M1, k=M1, batch=3 * B, A=p(X), lda=M1,
-                       iA=_bidx(s1=sX), B=p(self.BSy), ldb=M1, tb=1,
-                       C=p(Z) + 8 * (L0 + N0) * M1, ldc=M1, iC=_bidx(s1=sZ)))
-        if self.diffusion:
-            # 6b  Z[3] = f X BS_y^T, Z[4] = f X BD2S_y^T (explicit diffusion, f per field)
-            Lz.append(_Gemm("z_diff" + tag, m=M0, n=M1, k=M1, batch=6 * B, A=p(X), lda=M1,
-                           iA=_bidx(s1=sX, d1=2), B=p(self.DFy), ldb=M1, tb=1,
-                           iB=_bidx(s1=M1 * M1, m1=2), C=p(Z) + 8 * (L0 + N0 + M0) * M1,
-                           ldc=M1, iC=_bidx(s1=sZ, d1=2, s2=M0 * M1, m2=2),
-                           alpha_vec=p(self.st_fdiff[st]), alpha_div=2))
         # 1  x-pass: values and x-derivatives on the L0 grid
         g = _Gemm("xpass" + tag, m=2 * L0, n=M1, k=M0, batch=3 * B, A=p(self.GGx), lda=M0,
                   iA=_bidx(s1=2 * L0 * M0, m1=3), B=p(X), ldb=M1, iB=_bidx(s1=sX),
@@ -669,33 +734,32 @@ class RBCBatch:
             self._yfold_launches(L, st, tag)
         else:
             self._plain_y_launches(L, tag)
-        L += Lz
-        # 7  X = A_f Z_f + const_f
-        L.append(_Gemm("rhs" + tag, m=M0, n=M1, k=K, batch=3 * B, A=p(self.Arhs), lda=K,
-                       iA=_bidx(s1=M0 * K, m1=3), B=p(Z), ldb=M1, iB=_bidx(s1=sZ),
-                       D=p(self.st_const[st]), ldd=M1, iD=_bidx(s1=sX), beta=1.0,
-                       C=p(X), ldc=M1, iC=_bidx(s1=sX)))
-        # 8  buoyancy  X_V += dt a BST_x Z_T[2]
-        L.append(_Gemm("buoyancy" + tag, m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx), lda=M0,
-                       B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1, iB=_bidx(s1=3 * sZ),
-                       D=p(X) + 8 * sX, ldd=M1, iD=_bidx(s1=3 * sX), beta=1.0,
-                       C=p(X) + 8 * sX, ldc=M1, iC=_bidx(s1=3 * sX),
-                       alpha_vec=p(self.st_dta[st])))
+        # 5-6  banded part R_f of the right-hand side from the stage-start state:
+        #      constant, pressure gradient, old value, buoyancy, explicit diffusion
+        a = RbcRhs()
+        a.X, a.pres, a.cst, a.R = p(X), p(self.pres), p(self.st_const[st]), p(self.Rb)
+        a.mdta, a.dta = p(self.st_mdta[st]), p(self.st_dta[st])
+        a.fdiff = p(self.st_fdiff[st]) if self.diffusion else None
+        a.tx, a.ty = p(self.tx), p(self.ty)
+        a.M0, a.M1, a.N0, a.N1, a.batch = M0, M1, N0, N1, B
+        self._keep.append(a)
+        L.append(_Call("rhs_banded" + tag, lib.pypde_rbc_rhs_banded, ctypes.byref(a)))
+        # 7  X_f = BF_x Z_f + R_f   (the forward transform along x of the
+        #    convective term, onto the banded part)
+        L.append(_Gemm("rhs" + tag, m=M0, n=M1, k=L0, batch=3 * B, A=p(self.BFx), lda=L0,
+                       B=p(Z), ldb=M1, iB=_bidx(s1=sZ), D=p(self.Rb), ldd=M1, iD=_bidx(s1=sX),
+                       beta=1.0, C=p(X), ldc=M1, iC=_bidx(s1=sX)))
         cands = {}
         for S in self._splits:
-            kc = K // S
-            part = _Gemm("rhs_part" + tag, m=M0, n=M1, k=kc, batch=3 * B * S, A=p(self.Arhs),
-                         lda=K, iA=_bidx(s1=M0 * K, d1=S, m1=3, s2=kc, d2=1, m2=S), B=p(Z),
-                         ldb=M1, iB=_bidx(s1=sZ, d1=S, s2=kc * M1, d2=1, m2=S),
+            kc = L0 // S
+            part = _Gemm("rhs_part" + tag, m=M0, n=M1, k=kc, batch=3 * B * S, A=p(self.BFx),
+                         lda=L0, iA=_bidx(s1=kc, m1=S), B=p(Z), ldb=M1,
+                         iB=_bidx(s1=sZ, d1=S, s2=kc * M1, d2=1, m2=S),
                          C=p(self.rhs_work), ldc=M1, iC=_bidx(s1=sX))
-            buoy = _Gemm("buoyancy_part" + tag, m=M0, n=M1, k=M0, batch=B, A=p(self.BSTx),
-                         lda=M0, B=p(Z) + 8 * (2 * sZ + (L0 + N0) * M1), ldb=M1,
-                         iB=_bidx(s1=3 * sZ), C=p(self.rhs_workb), ldc=M1, iC=_bidx(s1=sX),
-                         alpha_vec=p(self.st_dta[st]))
-            red = (ctypes.c_void_p(p(self.rhs_work)), S, ctypes.c_void_p(p(self.st_const[st])),
-                   ctypes.c_void_p(p(self.rhs_workb)), 3, 1, ctypes.c_void_p(p(X)), sX, 3 * B)
-            cands[S] = _SplitRhs("rhs+buoyancy/%d" % S + tag, _GemmGroup([part, buoy]), red, S)
-        self._rhs_pairs.append((L[-2], L[-1], cands))
+            red = (ctypes.c_void_p(p(self.rhs_work)), S, ctypes.c_void_p(p(self.Rb)), None, 1, 0,
+                   ctypes.c_void_p(p(X)), sX, 3 * B)
+            cands[S] = _SplitRhs("rhs/%d" % S + tag, _GemmGroup([part]), red, S)
+        self._rhs_pairs.append((L[-1], cands))
         # 9-10  Helmholtz ADI sweeps (register-resident recurrences)
         gx = _lines(p(X), p(X), M1, sX, M0, M1, 3 * B, 0)
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
@@ -832,7 +896,59 @@ class RBCBatch:
 
 
     # -----------------------------------------------------------------------
+    def workspace_bytes(self):
+        """Bytes of the workspace the step's launches touch: the K-split
+        partials beyond the chosen number of slices are left out."""
+        used = 8 * self.ws.numel()
+        if self._splits:
+            used = 8 * self.ws_offsets["rhs_work"]
+            for l in self.launches:
+                if isinstance(l, _SplitRhs):
+                    used += 8 * 3 * self.B * l.S * self.M0 * self.M1
+        return used
+
+    def build_plan(self):
+        """Record the step's launches (as tuned) into a step plan of the C-ABI
+        (pypde_plan_*, regions X, pres, P and the workspace); pypde_rbc_step then
+        runs one step in one call."""
+        if self.plan is not None:
+            return self.plan
+        regions = [self.X, self.pres, self.P]
+        bases = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in regions], self.ws.data_ptr())
+        sizes = (ctypes.c_longlong * 4)(*[8 * t.numel() for t in regions], self.workspace_bytes())
+        self.plan = Plan(4, bases, sizes)
+        for l in self.launches:
+            parts = [l.group, l.reduce] if isinstance(l, _SplitRhs) else [l]
+            for part in parts:
+                self.plan.add(part)
+        return self.plan
+
     def step_eager(self):
+        """One step: the recorded step plan in one call (pypde_rbc_step)."""
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        plan = self.plan or self.build_plan()
+        check(lib.pypde_rbc_step(plan.handle, ctypes.c_void_p(self.X.data_ptr()),
+                                 ctypes.c_void_p(self.pres.data_ptr()),
+                                 ctypes.c_void_p(self.P.data_ptr()),
+                                 ctypes.c_void_p(self.ws.data_ptr()), 1, s), "pypde_rbc_step")
+
+    def stage_of(self, launch):
+        """The stage a launch of the step belongs to (names carry "[st]" when
+        the scheme has several stages)."""
+        if self.nstage == 1:
+            return 0
+        return int(launch.name.rsplit("[", 1)[1].rstrip("]"))
+
+    def run_stage(self, st):
+        """The launches of stage st alone, on the engine's buffers (the stages
+        before it must have run on them)."""
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for f in self.launches:
+            if self.stage_of(f) == st:
+                f(s)
+
+    def step_launches(self):
+        """One step as separate launches from Python (the plan's content)."""
         s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         for f in self.launches:
             f(s)
@@ -932,24 +1048,23 @@ class RBCBatch:
     def _tune_rhs_split(self, s, launch_fn):
         """Launches 7-8 as they are (0) or split along K into S slices."""
         from . import autotune
-        for rhs, buoy, cands in self._rhs_pairs:
+        for rhs, cands in self._rhs_pairs:
             if not cands:
                 continue
             for c in cands.values():
                 self._tune_group(c.group, s, launch_fn)
 
-            def run(S, rhs=rhs, buoy=buoy, cands=cands):
+            def run(S, rhs=rhs, cands=cands):
                 if S == 0:
                     rhs(s)
-                    buoy(s)
                 else:
                     cands[S](s)
-            key = ("rhs_split", self.B, self.M0, self.M1, self.K, autotune.gemm_key(rhs.desc))
+            key = ("rhs_split_banded", self.B, self.M0, self.M1, self.L0,
+                   autotune.gemm_key(rhs.desc))
             pick = autotune.choose(key, run, (0,) + tuple(cands), default=0)
             if pick:
                 i = self.launches.index(rhs)
-                assert self.launches[i + 1] is buoy
-                self.launches[i:i + 2] = [cands[pick]]
+                self.launches[i] = cands[pick]
 
     def variants(self):
         """{launch name: kernel variant} chosen for this step (GEMM variants;

@@ -45,3 +45,60 @@ def test_version_and_errors_without_gpu():
 def test_every_declared_symbol_has_a_python_signature():
     from pypde_amd import _lib
     assert set(declared_symbols()) <= set(_lib.exported_symbols())
+
+
+def test_step_plan_host_logic_without_gpu():
+    """Step plans are host objects: creation checks its regions, recording
+    copies descriptors, and the sizes read back (no kernel is launched)."""
+    from pypde_amd import _lib
+    lib = _lib.lib
+    h = ctypes.c_void_p()
+    bases = (ctypes.c_void_p * 2)(4096, 8192)
+    sizes = (ctypes.c_longlong * 2)(4096, 100)
+    assert lib.pypde_plan_create(2, bases, sizes, ctypes.byref(h)) == 0
+    assert lib.pypde_plan_size(h) == 0
+    assert lib.pypde_plan_region_bytes(h, 1) == 100
+    assert lib.pypde_plan_region_bytes(h, 2) == -1
+    d = _lib.GemmDesc()
+    assert lib.pypde_plan_add_gemm(h, ctypes.byref(d)) == 0
+    assert lib.pypde_plan_add_grouped(h, ctypes.byref(d), 5) != 0
+    assert b"1..4" in lib.pypde_last_error()
+    assert lib.pypde_plan_size(h) == 1
+    # zero steps run nothing; an RBC step needs the four regions
+    assert lib.pypde_plan_run(h, None, 0, None) == 0
+    assert lib.pypde_rbc_step(h, 1, 1, 1, 1, 1, None) != 0
+    assert b"regions" in lib.pypde_last_error()
+    assert lib.pypde_plan_destroy(h) == 0
+    # overlapping regions are rejected
+    sizes = (ctypes.c_longlong * 2)(4097, 100)
+    h2 = ctypes.c_void_p()
+    assert lib.pypde_plan_create(2, bases, sizes, ctypes.byref(h2)) != 0
+    assert b"overlap" in lib.pypde_last_error()
+    assert not h2.value
+
+
+def test_engine_launch_records_without_gpu():
+    """Every kind of launch the fused step issues can be recorded into a step
+    plan (host only: recording copies the arguments, nothing runs)."""
+    from pypde_amd import _lib
+    from pypde_amd.engine import Plan, _Call, _Gemm, _GemmGroup, _RecCall, _lines, _rec
+    base = 1 << 20
+    bases = (ctypes.c_void_p * 4)(base, 2 * base, 3 * base, 4 * base)
+    sizes = (ctypes.c_longlong * 4)(base, base, base, base)
+    plan = Plan(4, bases, sizes)
+    lib = _lib.lib
+    g = _Gemm("g", m=4, n=4, k=4, batch=1, A=base, lda=4, B=base, ldb=4, C=base, ldc=4)
+    plan.add(g)
+    plan.add(_GemmGroup([g, _Gemm("h", m=4, n=4, k=4, batch=1, A=base, lda=4, B=base, ldb=4,
+                                  C=2 * base, ldc=4)]))
+    gl = _lines(base, base, 4, 16, 4, 4, 1, 0)
+    plan.add(_RecCall("r", gl, {v: _rec(3 * base, 4, 16, v) for v in (1, 2, 3)}, 2))
+    vp = ctypes.c_void_p
+    plan.add(_Call("p", lib.pypde_rbc_products, vp(base), vp(base), vp(base), 6, 6, 1))
+    plan.add(_Call("py", lib.pypde_rbc_products_yfold, vp(base), None, vp(base), vp(base), 6, 6,
+                   1.0, -1.0, 0.0, 1))
+    plan.add(_Call("s", lib.pypde_splitk_reduce, vp(base), 2, vp(base), None, 1, 0, vp(base),
+                   16, 1))
+    a = _lib.RbcRhs()
+    plan.add(_Call("b", lib.pypde_rbc_rhs_banded, ctypes.byref(a)))
+    assert plan.size() == 7

@@ -16,11 +16,11 @@ from conftest import rel
 pytestmark = pytest.mark.gpu
 
 
-def make(shape=(24, 20), ra=5e4, dt=0.005, B=1, case="rbc"):
+def make(shape=(24, 20), ra=5e4, dt=0.005, B=1, case="rbc", integrator="eu", beta=1.0):
     import torch
     from pypde_amd.engine import RBCBatch
     params = [dict(ra=ra * (1 + 0.5 * b), pr=1.0, dt=dt) for b in range(B)]
-    eng = RBCBatch(shape, params, case=case)
+    eng = RBCBatch(shape, params, case=case, integrator=integrator, beta=beta)
     rng = np.random.default_rng(3)
     X = torch.as_tensor(rng.standard_normal(tuple(eng.X.shape)) * 1e-2, device=eng.X.device)
     pres = torch.as_tensor(rng.standard_normal(tuple(eng.pres.shape)) * 1e-3, device=eng.X.device)
@@ -40,6 +40,24 @@ def test_step_vjp_matches_central_differences(case):
     from pypde_amd.adjoint import StepAdjoint
     eng, X, pres = make(case=case, B=2)
     adj = StepAdjoint(eng)
+    _check_vjp_fd(eng, adj, X, pres, eps=1e-3, tol=1e-9)
+
+
+@pytest.mark.parametrize("integrator,beta", [("rk3", 1.0), ("eu", 0.5), ("rk3", 0.5)])
+def test_step_vjp_rk3_and_explicit_diffusion(integrator, beta):
+    """The three-stage step (a composition of three quadratic stages, the
+    c-term coupling each stage to the one before) and the step with an
+    explicit share of the diffusion: VJP against central differences with
+    eps = 1e-4 (truncation of order eps^2 times the cubic and higher terms),
+    1e-8 relative."""
+    from pypde_amd.adjoint import StepAdjoint
+    eng, X, pres = make(case="rbc", B=2, integrator=integrator, beta=beta)
+    adj = StepAdjoint(eng)
+    _check_vjp_fd(eng, adj, X, pres, eps=1e-4, tol=1e-8)
+
+
+def _check_vjp_fd(eng, adj, X, pres, eps, tol):
+    import torch
     rng = np.random.default_rng(11)
     dev = X.device
     wX = torch.as_tensor(rng.standard_normal(tuple(X.shape)), device=dev)
@@ -48,12 +66,12 @@ def test_step_vjp_matches_central_differences(case):
     for trial in range(3):
         dX = torch.as_tensor(rng.standard_normal(tuple(X.shape)), device=dev)
         dp = torch.as_tensor(rng.standard_normal(tuple(pres.shape)), device=dev)
-        eps = 1e-3
         Xa, pa = run_step(eng, X + eps * dX, pres + eps * dp)
         Xm, pm = run_step(eng, X - eps * dX, pres - eps * dp)
         fd = ((wX * (Xa - Xm)).sum() + (wp * (pa - pm)).sum()).item() / (2 * eps)
         an = ((gX * dX).sum() + (gp * dp).sum()).item()
-        assert abs(fd - an) <= 1e-9 * max(abs(fd), 1e-300), (trial, fd, an)
+        print("trial %d: fd %.15e vjp %.15e rel %.2e" % (trial, fd, an, abs(fd - an) / abs(fd)))
+        assert abs(fd - an) <= tol * max(abs(fd), 1e-300), (trial, fd, an)
 
 
 def test_autograd_step_matches_vjp():
@@ -271,3 +289,42 @@ def test_step_vjp_128_matches_central_differences():
         an = ((gX * dX).sum() + (gp * dp).sum()).item()
         print("128x128 step VJP, trial %d: fd %.15e vjp %.15e" % (trial, fd, an))
         assert abs(fd - an) <= 1e-9 * abs(fd), (trial, fd, an)
+
+
+@pytest.mark.parametrize("integrator,beta", [("rk3", 1.0), ("rk3", 0.5)])
+def test_time_averaged_nusselt_gradient_rk3(integrator, beta):
+    """J = mean_k Nu_plate(T_k) over 40 three-stage steps (checkpoints every
+    10) at 24x20: the checkpointed reverse sweep against central differences
+    (eps = 1e-5) along 3 unit directions, 1e-6 relative to each directional
+    derivative."""
+    import torch
+    from pypde_amd.adjoint import TimeAveragedNusselt, Trajectory
+    from pypde_amd.engine import RBCBatch
+    eng = RBCBatch((24, 20), [dict(ra=5e4, pr=1.0, dt=0.005)], case="rbc",
+                   integrator=integrator, beta=beta)
+    dev = eng.X.device
+    rng = np.random.default_rng(15)
+    X0 = torch.zeros_like(eng.X)
+    X0[:, 2] = torch.as_tensor(rng.uniform(-0.05, 0.05, tuple(X0[:, 2].shape)), device=dev)
+    pres0 = torch.zeros_like(eng.pres)
+    traj = Trajectory(eng, ckpt=10)
+    nsteps = 40
+    T0 = X0[:, 2].clone().requires_grad_(True)
+    J = TimeAveragedNusselt.apply(T0, traj, X0, pres0, nsteps)
+    J.sum().backward()
+    grad = T0.grad.detach()
+    for trial in range(3):
+        d = torch.as_tensor(rng.standard_normal(tuple(T0.shape)), device=dev)
+        d = d / torch.linalg.norm(d)
+        an = (grad * d).sum().item()
+        eps = 1e-5
+        Xp = X0.clone()
+        Xp[:, 2] += eps * d
+        Xm = X0.clone()
+        Xm[:, 2] -= eps * d
+        Jp, _ = traj.run(Xp, pres0, nsteps)
+        Jm, _ = traj.run(Xm, pres0, nsteps)
+        fd = ((Jp - Jm) / (2 * eps)).item()
+        print("%s beta %.1f direction %d: adjoint %.12e fd %.12e rel %.2e"
+              % (integrator, beta, trial, an, fd, abs(fd - an) / abs(an)))
+        assert abs(fd - an) <= 1e-6 * abs(an)
