@@ -342,6 +342,53 @@ int pypde_scaled_sub(long long per, int batch, const double* coef, double c2,
 /* p[e*stride] = value for e < n: the gauge P[0,0] = 0 of rbc2d.py:384-385. */
 int pypde_set_entries(double* p, long long stride, int n, double value, void* stream);
 
+/* ---- multi launches: one launch for many members (heterogeneous batches) -- */
+/* A heterogeneous batch runs each operator of the step once per size class.  A
+ * multi launch serves every class in one launch: the members' arguments are
+ * packed on the host into a table (pypde_*_multi_pack, into table_bytes of
+ * host memory, at least pypde_multi_table_bytes(kind)), the caller copies the
+ * table to device memory once, and each launch passes the device table with
+ * the info the pack filled.  Every member computes exactly what its own
+ * launch of the same kernel form computes.  No reference counterpart: the
+ * reference steps one NavierStokes object at a time (navier/rbc2d.py:396-434). */
+#define PYPDE_MULTI_GEMM 1
+#define PYPDE_MULTI_RECURRENCE 2
+#define PYPDE_MULTI_PRODUCTS 3
+#define PYPDE_MULTI_RHS 4
+#define PYPDE_MULTI_MAX 64 /* members per table */
+typedef struct pypde_multi_info {
+    int kind;       /* PYPDE_MULTI_* */
+    int count;      /* members */
+    int variant;    /* kernel form */
+    int flags;      /* GEMM: bit 0 transA, bit 1 transB; recurrence: the table form */
+    long long total;/* work items of the launch (tiles, lines, blocks) */
+} pypde_multi_info;
+
+long long pypde_multi_table_bytes(int kind);
+/* GEMM members: plain descriptors with the same transposes, run with one
+ * LDS-tiled variant (1..7, 24..42). */
+int pypde_gemm_multi_pack(const pypde_gemm_desc* descs, int count, int variant, void* table,
+                          long long table_bytes, pypde_multi_info* info);
+int pypde_dgemm_multi(const void* table, const pypde_multi_info* info, void* stream);
+/* Lane-chunk recurrences (pypde_recurrence_solve, variant 2), the (LM, R, MU, NU)
+ * form, one axis and one table layout (per_line 0 or 2) for all members; each
+ * member has its own line geometry and tables. */
+int pypde_recurrence_multi_pack(const pypde_lines* g, const pypde_recurrence* r, int count,
+                                void* table, long long table_bytes, pypde_multi_info* info);
+int pypde_recurrence_multi(const void* table, const pypde_multi_info* info, void* stream);
+/* y-folded convective products (pypde_rbc_products_yfold), member i with the
+ * arguments at index i of each array. */
+int pypde_products_multi_pack(const double* const* Y, const double* const* Yold,
+                              const double* const* dtbc, double* const* P, const int* L0,
+                              const int* L1, const double* sign_value, const double* sign_dy,
+                              const double* ratio, const int* batch, int count, void* table,
+                              long long table_bytes, pypde_multi_info* info);
+int pypde_products_multi(const void* table, const pypde_multi_info* info, void* stream);
+/* Banded right-hand sides (pypde_rbc_rhs_banded), one argument struct per member. */
+int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* table, long long table_bytes,
+                         pypde_multi_info* info);
+int pypde_rhs_multi(const void* table, const pypde_multi_info* info, void* stream);
+
 /* ---- step plans: a whole time step behind one call ------------------- */
 /* A step plan is the ordered list of the launches above that make up one fused
  * time step, recorded once (descriptors copied into host memory) and replayed
@@ -396,6 +443,7 @@ int pypde_plan_run(const pypde_plan* p, const void* const* bases, int n_steps, v
  * (navier/rbc2d.py:396-434, pypde/solver/integrator.py:14-41). */
 int pypde_rbc_step(const pypde_plan* plan, double* X, double* pres, double* P, void* workspace,
                    int n_steps, void* stream);
+
 
 #ifdef __cplusplus
 }

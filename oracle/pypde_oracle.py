@@ -23,8 +23,10 @@ Pinning (what checks each part; see tests/test_oracle_*.py):
   Helmholtz ADI     dense Kronecker (Ax (x) Ay) solve
   Poisson           dense Kronecker (Ax (x) Cy + Cx (x) Ay) solve, (0,0) gauge
   NS update         golden trajectories written by tests/golden/make_golden.py
+  rounding floor    dense_transforms(): the same step with dense transforms
   Nusselt           conductive state gives Nu = 1 exactly
 """
+import contextlib
 import ctypes
 import os
 
@@ -384,6 +386,31 @@ class Basis:
 
     def I(self, discardrow=0):
         return np.eye(self.N)[discardrow:, :]
+
+
+@contextlib.contextmanager
+def dense_transforms():
+    """Within the block, Basis evaluates its Chebyshev transforms as dense
+    products with the transform matrices (each the FFT-based transform applied
+    to the identity) instead of scipy's DCT-I: the same mathematics with
+    different rounding.  The distance between a trajectory computed this way
+    and the ordinary one is the rounding floor of that trajectory, the scale
+    on which two exact FP64 implementations of the reference differ."""
+    fwd, bwd = Basis._cheb_forward, Basis._cheb_backward
+    cache = {}
+
+    def matrix(fn, basis, n):
+        key = (fn.__name__, basis.N, n)
+        if key not in cache:
+            cache[key] = fn(basis, np.eye(n))
+        return cache[key]
+
+    Basis._cheb_forward = lambda self, f: matrix(fwd, self, f.shape[0]) @ f
+    Basis._cheb_backward = lambda self, c: matrix(bwd, self, c.shape[0]) @ c
+    try:
+        yield
+    finally:
+        Basis._cheb_forward, Basis._cheb_backward = fwd, bwd
 
 
 def _prod(a, b):

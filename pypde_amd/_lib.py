@@ -73,6 +73,13 @@ class RbcRhs(ctypes.Structure):
                 ("N1", ctypes.c_int), ("batch", ctypes.c_int)]
 
 
+class MultiInfo(ctypes.Structure):
+    """Mirror of `pypde_multi_info`."""
+
+    _fields_ = [("kind", ctypes.c_int), ("count", ctypes.c_int), ("variant", ctypes.c_int),
+                ("flags", ctypes.c_int), ("total", ctypes.c_longlong)]
+
+
 class Recurrence(ctypes.Structure):
     """Mirror of `pypde_recurrence`."""
 
@@ -132,6 +139,18 @@ _SIGS = {
                                       _i], _i),
     "pypde_plan_add_rhs_banded": ([_vp, ctypes.c_void_p], _i),
     "pypde_rbc_rhs_banded": ([ctypes.c_void_p, _vp], _i),
+    "pypde_multi_table_bytes": ([_i], ctypes.c_longlong),
+    "pypde_gemm_multi_pack": ([ctypes.c_void_p, _i, _i, _vp, ctypes.c_longlong, ctypes.c_void_p],
+                              _i),
+    "pypde_dgemm_multi": ([_vp, ctypes.c_void_p, _vp], _i),
+    "pypde_recurrence_multi_pack": ([ctypes.c_void_p, ctypes.c_void_p, _i, _vp,
+                                     ctypes.c_longlong, ctypes.c_void_p], _i),
+    "pypde_recurrence_multi": ([_vp, ctypes.c_void_p, _vp], _i),
+    "pypde_products_multi_pack": ([ctypes.c_void_p] * 10 + [_i, _vp, ctypes.c_longlong,
+                                                              ctypes.c_void_p], _i),
+    "pypde_products_multi": ([_vp, ctypes.c_void_p, _vp], _i),
+    "pypde_rhs_multi_pack": ([ctypes.c_void_p, _i, _vp, ctypes.c_longlong, ctypes.c_void_p], _i),
+    "pypde_rhs_multi": ([_vp, ctypes.c_void_p, _vp], _i),
     "pypde_plan_size": ([_vp], _i),
     "pypde_plan_region_bytes": ([_vp, _i], ctypes.c_longlong),
     "pypde_plan_run": ([_vp, ctypes.c_void_p, _i, _vp], _i),

@@ -27,6 +27,8 @@
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
+#include <cstring>
+
 #include <algorithm>
 
 namespace pypde {
@@ -454,13 +456,10 @@ struct LaneTab {
 };
 
 template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
-__global__ void __launch_bounds__(64) lane_chunk_kernel(RecArgs a)
+__device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
 {
     constexpr int NX = PRE ? C + 2 : C;
     const int lane = threadIdx.x;
-    // (batch entry, line); along axis 0 neighbouring lines share every cache
-    // line they touch, so they run on one XCD
-    const long long gl = AXIS == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : blockIdx.x;
     const int b = (int)(gl / a.nlines);
     const int line = (int)(gl - (long long)b * a.nlines);
     const int n = a.n;
@@ -601,6 +600,47 @@ __global__ void __launch_bounds__(64) lane_chunk_kernel(RecArgs a)
 }
 
 template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+__global__ void __launch_bounds__(64) lane_chunk_kernel(RecArgs a)
+{
+    // (batch entry, line); along axis 0 neighbouring lines share every cache
+    // line they touch, so they run on one XCD
+    const long long gl = AXIS == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : blockIdx.x;
+    lane_chunk_line<AXIS, C, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, gl);
+}
+
+// Lane-chunk solves of many members (the same table form, each its own line
+// geometry and tables) in one launch, the arguments in a device table: block
+// t runs line t - start[i] of member i.  Lines shorter than 64 C leave their
+// upper lanes idle, as in a single launch with the same C.
+constexpr int kMaxRecMulti = 64;
+struct RecMulti {
+    int count, total;
+    int start[kMaxRecMulti + 1];
+    RecArgs a[kMaxRecMulti];
+};
+
+template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+__global__ void __launch_bounds__(64) lane_chunk_multi(const RecMulti* __restrict__ mm)
+{
+    const int t = AXIS == 0 ? xcd_remap((int)blockIdx.x, mm->total) : (int)blockIdx.x;
+    int lo = 0, hi = mm->count - 1;
+    while (lo < hi) {  // the last member whose range starts at or before t
+        const int mid = (lo + hi + 1) >> 1;
+        if (mm->start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const RecArgs a = mm->a[lo];
+    lane_chunk_line<AXIS, C, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, t - mm->start[lo]);
+}
+
+template <int AXIS, int C, bool LINEMAJOR>
+static void launch_lane_multi_t(const RecMulti* mm, int total, hipStream_t s)
+{
+    hipLaunchKernelGGL((lane_chunk_multi<AXIS, C, false, true, false, true, true, LINEMAJOR>),
+                       dim3((unsigned)total), dim3(64), 0, s, mm);
+}
+
+template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
 static int launch_lane_t(const RecArgs& a, hipStream_t s)
 {
     const long long blocks = (long long)a.nlines * a.batch;
@@ -625,9 +665,97 @@ static int launch_lane(const RecArgs& a, hipStream_t s)
     return launch_lane_t<1, 8, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR>(a, s);
 }
 
+long long recurrence_multi_bytes() { return (long long)sizeof(RecMulti); }
+
+// flags of a recurrence multi table: bit 0 axis, bit 1 line-major per-line
+// tables, bits 2.. log2 of the chunk C (2, 4 or 8)
+int recurrence_multi_pack(const pypde_lines* g, const pypde_recurrence* r, int count, void* table,
+                          long long table_bytes, pypde_multi_info* info)
+{
+    PYPDE_REQUIRE(g && r && table && info, "null argument");
+    PYPDE_REQUIRE(table_bytes >= (long long)sizeof(RecMulti), "recurrence table needs %zu bytes",
+                  sizeof(RecMulti));
+    PYPDE_REQUIRE(count >= 1 && count <= kMaxRecMulti, "multi recurrence: count %d not in 1..%d",
+                  count, kMaxRecMulti);
+    RecMulti* mm = static_cast<RecMulti*>(table);
+    memset(mm, 0, sizeof(RecMulti));
+    mm->count = count;
+    const int axis = g[0].axis;
+    const int linemajor = r[0].per_line == 2;
+    int nmax = 0;
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const pypde_lines& gi = g[i];
+        const pypde_recurrence& ri = r[i];
+        PYPDE_REQUIRE(gi.src && gi.dst, "member %d: null src/dst pointer", i);
+        PYPDE_REQUIRE(gi.axis == axis, "member %d: another axis", i);
+        PYPDE_REQUIRE((ri.per_line == 2) == (linemajor != 0) && ri.per_line != 1,
+                      "member %d: tables must be shared/per entry (per_line 0) or line-major "
+                      "(per_line 2) like member 0", i);
+        PYPDE_REQUIRE(!ri.pre && ri.LM && !ri.F && ri.R && ri.MU && ri.NU,
+                      "member %d: multi launches support the (LM, R, MU, NU) form only", i);
+        PYPDE_REQUIRE(gi.len_out >= 3 && gi.len_in == gi.len_out && gi.nlines >= 0 && gi.batch >= 0,
+                      "member %d: invalid line geometry", i);
+        PYPDE_REQUIRE(gi.len_out <= 512, "member %d: line length %d exceeds 512", i, gi.len_out);
+        mm->a[i] = RecArgs{gi.src, gi.dst, gi.src_ld, gi.dst_ld, gi.src_bs, gi.dst_bs, gi.len_in,
+                           gi.len_out, gi.nlines, gi.batch, gi.axis, ri.F, ri.LM, ri.R, ri.MU,
+                           ri.NU, ri.pre, ri.entry_stride, ri.per_line};
+        mm->start[i] = (int)total;
+        total += (long long)gi.nlines * gi.batch;
+        PYPDE_REQUIRE(total < (1LL << 31), "multi recurrence too large");
+        if (gi.len_out > nmax) nmax = gi.len_out;
+    }
+    mm->start[count] = (int)total;
+    mm->total = (int)total;
+    const int lc = nmax <= 128 ? 1 : (nmax <= 256 ? 2 : 3);
+    info->kind = PYPDE_MULTI_RECURRENCE;
+    info->count = count;
+    info->variant = 2;
+    info->flags = axis | (linemajor << 1) | (lc << 2);
+    info->total = total;
+    return OK;
+}
+
+int recurrence_multi(const void* table, const pypde_multi_info* info, hipStream_t s)
+{
+    PYPDE_REQUIRE(table && info, "null argument");
+    PYPDE_REQUIRE(info->kind == PYPDE_MULTI_RECURRENCE, "not a recurrence table");
+    if (info->total == 0) return OK;
+    const RecMulti* mm = static_cast<const RecMulti*>(table);
+    const int axis = info->flags & 1, lm = (info->flags >> 1) & 1, lc = info->flags >> 2;
+    const int total = (int)info->total;
+    PYPDE_REQUIRE(lc >= 1 && lc <= 3, "recurrence table: bad chunk code %d", lc);
+#define PYPDE_LANE_MULTI(AX, CC)                                                   \
+    (lm ? launch_lane_multi_t<AX, CC, true>(mm, total, s)                         \
+        : launch_lane_multi_t<AX, CC, false>(mm, total, s))
+    if (axis == 0) {
+        if (lc == 1) PYPDE_LANE_MULTI(0, 2);
+        else if (lc == 2) PYPDE_LANE_MULTI(0, 4);
+        else PYPDE_LANE_MULTI(0, 8);
+    } else {
+        if (lc == 1) PYPDE_LANE_MULTI(1, 2);
+        else if (lc == 2) PYPDE_LANE_MULTI(1, 4);
+        else PYPDE_LANE_MULTI(1, 8);
+    }
+#undef PYPDE_LANE_MULTI
+    return check_launch("pypde_recurrence_multi");
+}
+
 }  // namespace pypde
 
 using namespace pypde;
+
+extern "C" int pypde_recurrence_multi_pack(const pypde_lines* g, const pypde_recurrence* r,
+                                           int count, void* table, long long table_bytes,
+                                           pypde_multi_info* info)
+{
+    return recurrence_multi_pack(g, r, count, table, table_bytes, info);
+}
+
+extern "C" int pypde_recurrence_multi(const void* table, const pypde_multi_info* info, void* stream)
+{
+    return recurrence_multi(table, info, as_stream(stream));
+}
 
 extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurrence* r, void* stream)
 {

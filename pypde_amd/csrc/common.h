@@ -47,6 +47,11 @@ __device__ __forceinline__ int xcd_remap(int pid, int total)
     return x * q + (x < r ? x : r) + local;
 }
 
+// Bytes of the device tables of the multi launches (one per translation unit).
+long long gemm_multi_bytes();
+long long recurrence_multi_bytes();
+long long products_multi_bytes();
+
 }  // namespace pypde
 
 #define PYPDE_REQUIRE(cond, ...)                 \

@@ -38,6 +38,8 @@
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
+#include <cstring>
+
 namespace pypde {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -334,6 +336,72 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_group(GemmGroup gg)
     int bx, by, bz;
     const int i = group_tile(gg, bx, by, bz);
     dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK, PAD>(gg.a[i], bx, by, bz);
+}
+
+// Many independent GEMMs of one LDS-tiled kernel form in one launch, their
+// arguments in a table in device memory (up to kMaxMulti members, beyond what
+// kernel arguments hold): a heterogeneous batch runs each operator of the step
+// for every size class in one launch.  Block t runs tile t - start[i] of the
+// member i found by binary search; members share the kernel template, so each
+// computes exactly what its own launch of that variant computes.
+constexpr int kMaxMulti = 64;
+struct GemmMulti {
+    int count, total;
+    int start[kMaxMulti + 1];
+    int tx[kMaxMulti], ty[kMaxMulti];
+    GemmArgs a[kMaxMulti];
+};
+
+__device__ __forceinline__ int multi_tile(const GemmMulti* __restrict__ mm, int& bx, int& by, int& bz)
+{
+    const int t = xcd_remap(blockIdx.x, mm->total);
+    int lo = 0, hi = mm->count - 1;
+    while (lo < hi) {  // the last member whose range starts at or before t
+        const int mid = (lo + hi + 1) >> 1;
+        if (mm->start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int local = t - mm->start[lo];
+    bx = local % mm->tx[lo];
+    const int r = local / mm->tx[lo];
+    by = r % mm->ty[lo];
+    bz = r / mm->ty[lo];
+    return lo;
+}
+
+template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK, int PAD>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_mfma_multi(const GemmMulti* __restrict__ mm)
+{
+    int bx, by, bz;
+    const int i = multi_tile(mm, bx, by, bz);
+    const GemmArgs g = mm->a[i];
+    dgemm_mfma_body<BM, BN, WGM, WGN, PIPE, TA, TB, BK, PAD>(g, bx, by, bz);
+}
+
+template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16, int PAD = 1>
+static void launch_gemm_multi(bool ta, bool tb, const GemmMulti* mm, int total, hipStream_t s)
+{
+    dim3 grid(total);
+    dim3 block(64 * WGM * WGN);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_mfma_multi<BM, BN, WGM, WGN, PIPE, false, false, BK, PAD>), grid, block, 0, s, mm);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_mfma_multi<BM, BN, WGM, WGN, PIPE, false, true, BK, PAD>), grid, block, 0, s, mm);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_mfma_multi<BM, BN, WGM, WGN, PIPE, true, false, BK, PAD>), grid, block, 0, s, mm);
+    else hipLaunchKernelGGL((dgemm_mfma_multi<BM, BN, WGM, WGN, PIPE, true, true, BK, PAD>), grid, block, 0, s, mm);
+}
+
+// Block tile (rows, columns) of the LDS-tiled variants; 0 for the others.
+static void multi_tile_shape(int v, int& bm, int& bn)
+{
+    bm = bn = 0;
+    switch (v) {
+    case 1: case 25: case 37: bm = 64; bn = 64; break;
+    case 3: case 4: case 26: bm = 128; bn = 128; break;
+    case 5: case 7: case 38: bm = 128; bn = 64; break;
+    case 6: case 24: bm = 64; bn = 128; break;
+    case 2: case 27: case 28: case 29: case 30: case 31: case 32: case 33: case 34: case 35:
+    case 36: case 39: case 40: case 41: case 42: bm = 32; bn = 32; break;
+    default: break;
+    }
 }
 
 template <int BM, int BN, int WGM, int WGN, int PIPE, int BK = 16, int PAD = 1>
@@ -1336,6 +1404,92 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     return check_launch("pypde_dgemm_grouped");
 }
 
+long long gemm_multi_bytes() { return (long long)sizeof(GemmMulti); }
+
+int gemm_multi_pack(const pypde_gemm_desc* d, int count, int variant, void* table,
+                    long long table_bytes, pypde_multi_info* info)
+{
+    PYPDE_REQUIRE(d && table && info, "null argument");
+    PYPDE_REQUIRE(table_bytes >= (long long)sizeof(GemmMulti), "GEMM table needs %zu bytes",
+                  sizeof(GemmMulti));
+    PYPDE_REQUIRE(count >= 1 && count <= kMaxMulti, "multi GEMM: count %d not in 1..%d", count,
+                  kMaxMulti);
+    int bm, bn;
+    multi_tile_shape(variant, bm, bn);
+    PYPDE_REQUIRE(bm > 0, "multi GEMM: variant %d is not an LDS-tiled variant", variant);
+    GemmMulti* mm = static_cast<GemmMulti*>(table);
+    memset(mm, 0, sizeof(GemmMulti));
+    mm->count = count;
+    const int ta = d[0].transA, tb = d[0].transB;
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const pypde_gemm_desc& e = d[i];
+        PYPDE_REQUIRE(e.fold == 0, "multi GEMM: member %d is folded (not supported)", i);
+        PYPDE_REQUIRE(e.transA == ta && e.transB == tb, "multi GEMM: member %d has other transposes", i);
+        PYPDE_REQUIRE(e.m >= 0 && e.n >= 0 && e.k >= 0 && e.batch >= 0, "negative GEMM size (member %d)", i);
+        PYPDE_REQUIRE(!e.alpha_vec || e.alpha_div >= 1, "alpha_div must be >= 1");
+        PYPDE_REQUIRE(!e.gamma_vec || e.gamma_div >= 1, "gamma_div must be >= 1");
+        const bool live = e.m > 0 && e.n > 0 && e.batch > 0;
+        if (live) PYPDE_REQUIRE(e.A && e.B && e.C, "null GEMM operand (member %d)", i);
+        mm->a[i] = args_of(&e);
+        mm->tx[i] = (e.n + bn - 1) / bn;
+        mm->ty[i] = (e.m + bm - 1) / bm;
+        mm->start[i] = (int)total;
+        if (live) total += (long long)mm->tx[i] * mm->ty[i] * e.batch;
+        PYPDE_REQUIRE(total < (1LL << 31), "multi GEMM too large");
+    }
+    mm->start[count] = (int)total;
+    mm->total = (int)total;
+    info->kind = PYPDE_MULTI_GEMM;
+    info->count = count;
+    info->variant = variant;
+    info->flags = (ta ? 1 : 0) | (tb ? 2 : 0);
+    info->total = total;
+    return OK;
+}
+
+int dgemm_multi(const void* table, const pypde_multi_info* info, hipStream_t stream)
+{
+    PYPDE_REQUIRE(table && info, "null argument");
+    PYPDE_REQUIRE(info->kind == PYPDE_MULTI_GEMM, "not a GEMM table");
+    if (info->total == 0) return OK;
+    const GemmMulti* mm = static_cast<const GemmMulti*>(table);
+    const bool ta = info->flags & 1, tb = info->flags & 2;
+    const int total = (int)info->total;
+    switch (info->variant) {
+    case 1: launch_gemm_multi<64, 64, 2, 2, false>(ta, tb, mm, total, stream); break;
+    case 2: launch_gemm_multi<32, 32, 2, 2, false>(ta, tb, mm, total, stream); break;
+    case 3: launch_gemm_multi<128, 128, 4, 2, false>(ta, tb, mm, total, stream); break;
+    case 4: launch_gemm_multi<128, 128, 4, 2, true>(ta, tb, mm, total, stream); break;
+    case 5: launch_gemm_multi<128, 64, 4, 2, false>(ta, tb, mm, total, stream); break;
+    case 6: launch_gemm_multi<64, 128, 2, 4, false>(ta, tb, mm, total, stream); break;
+    case 7: launch_gemm_multi<128, 64, 4, 2, 2>(ta, tb, mm, total, stream); break;
+    case 24: launch_gemm_multi<64, 128, 2, 4, 2>(ta, tb, mm, total, stream); break;
+    case 25: launch_gemm_multi<64, 64, 2, 2, 2>(ta, tb, mm, total, stream); break;
+    case 26: launch_gemm_multi<128, 128, 4, 2, 2>(ta, tb, mm, total, stream); break;
+    case 27: launch_gemm_multi<32, 32, 2, 2, 2>(ta, tb, mm, total, stream); break;
+    case 28: launch_gemm_multi<32, 32, 2, 2, 1>(ta, tb, mm, total, stream); break;
+    case 29: launch_gemm_multi<32, 32, 2, 1, 0>(ta, tb, mm, total, stream); break;
+    case 30: launch_gemm_multi<32, 32, 1, 1, 0>(ta, tb, mm, total, stream); break;
+    case 31: launch_gemm_multi<32, 32, 2, 1, 2>(ta, tb, mm, total, stream); break;
+    case 32: launch_gemm_multi<32, 32, 1, 1, 2>(ta, tb, mm, total, stream); break;
+    case 33: launch_gemm_multi<32, 32, 2, 2, 0, 32>(ta, tb, mm, total, stream); break;
+    case 34: launch_gemm_multi<32, 32, 2, 2, 2, 32>(ta, tb, mm, total, stream); break;
+    case 35: launch_gemm_multi<32, 32, 2, 2, 0, 64>(ta, tb, mm, total, stream); break;
+    case 36: launch_gemm_multi<32, 32, 2, 2, 2, 64>(ta, tb, mm, total, stream); break;
+    case 37: launch_gemm_multi<64, 64, 2, 2, 0, 32>(ta, tb, mm, total, stream); break;
+    case 38: launch_gemm_multi<128, 64, 4, 2, 0, 32>(ta, tb, mm, total, stream); break;
+    case 39: launch_gemm_multi<32, 32, 2, 2, 0, 16, 2>(ta, tb, mm, total, stream); break;
+    case 40: launch_gemm_multi<32, 32, 2, 2, 2, 16, 2>(ta, tb, mm, total, stream); break;
+    case 41: launch_gemm_multi<32, 32, 2, 2, 0, 32, 2>(ta, tb, mm, total, stream); break;
+    case 42: launch_gemm_multi<32, 32, 2, 2, 2, 32, 2>(ta, tb, mm, total, stream); break;
+    default:
+        set_error("multi GEMM: variant %d is not an LDS-tiled variant", info->variant);
+        return ERR_ARG;
+    }
+    return check_launch("pypde_dgemm_multi");
+}
+
 }  // namespace pypde
 
 extern "C" int pypde_dgemm(int transA, int transB, int m, int n, int k, double alpha,
@@ -1356,4 +1510,15 @@ extern "C" int pypde_dgemm_batched(const pypde_gemm_desc* d, void* stream)
 extern "C" int pypde_dgemm_grouped(const pypde_gemm_desc* descs, int count, void* stream)
 {
     return pypde::dgemm_grouped(descs, count, pypde::as_stream(stream));
+}
+
+extern "C" int pypde_gemm_multi_pack(const pypde_gemm_desc* descs, int count, int variant,
+                                     void* table, long long table_bytes, pypde_multi_info* info)
+{
+    return pypde::gemm_multi_pack(descs, count, variant, table, table_bytes, info);
+}
+
+extern "C" int pypde_dgemm_multi(const void* table, const pypde_multi_info* info, void* stream)
+{
+    return pypde::dgemm_multi(table, info, pypde::as_stream(stream));
 }

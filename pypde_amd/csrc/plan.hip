@@ -48,7 +48,7 @@ namespace pypde {
 
 // Shift p by the move of the region that contains it (none: unchanged).
 template <typename T>
-static inline T* relocate(const pypde_plan* pl, const std::vector<intptr_t>& delta, T* p)
+static inline T* relocate(const pypde_plan* pl, const intptr_t* delta, T* p)
 {
     if (!p) return p;
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
@@ -59,7 +59,7 @@ static inline T* relocate(const pypde_plan* pl, const std::vector<intptr_t>& del
     return p;
 }
 
-static void relocate_desc(const pypde_plan* pl, const std::vector<intptr_t>& dl, pypde_gemm_desc& d)
+static void relocate_desc(const pypde_plan* pl, const intptr_t* dl, pypde_gemm_desc& d)
 {
     d.A = relocate(pl, dl, d.A);
     d.B = relocate(pl, dl, d.B);
@@ -72,7 +72,7 @@ static void relocate_desc(const pypde_plan* pl, const std::vector<intptr_t>& dl,
     d.B1 = relocate(pl, dl, d.B1);
 }
 
-static int run_op(const pypde_plan* pl, const std::vector<intptr_t>& dl, bool moved,
+static int run_op(const pypde_plan* pl, const intptr_t* dl, bool moved,
                   const PlanOp& op, void* stream)
 {
     switch (op.kind) {
@@ -294,8 +294,9 @@ extern "C" int pypde_plan_run(const pypde_plan* p, const void* const* bases, int
 {
     PYPDE_REQUIRE(p, "null plan");
     PYPDE_REQUIRE(n_steps >= 0, "negative step count");
+    // per-region moves on the stack: a run allocates nothing, host or device
     const size_t nr = p->base.size();
-    std::vector<intptr_t> delta(nr, 0);
+    intptr_t delta[PYPDE_PLAN_MAX_REGIONS] = {};
     bool moved = false;
     if (bases) {
         for (size_t r = 0; r < nr; ++r) {

@@ -2,6 +2,8 @@
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
+#include <cstring>
+
 namespace pypde {
 
 #pragma clang fp contract(off)
@@ -48,16 +50,16 @@ __global__ void rbc_products_kernel(const double* __restrict__ Y, const double* 
 // P(l) + P(L1-1-l) and block 1 P(l) - P(L1-1-l).  With Yold (the previous RK3
 // stage, same layout), ratio times the products of the current gradients with
 // the old velocities is added (the c-term of rbc2d.py:261-283).
-__global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const double* __restrict__ Yold,
-                                          const double* __restrict__ dtbc, double* __restrict__ P,
-                                          int L0, int L1, double sv, double sd, double ratio, int B)
+__device__ __forceinline__ void products_yfold_at(const double* __restrict__ Y,
+                                                  const double* __restrict__ Yold,
+                                                  const double* __restrict__ dtbc,
+                                                  double* __restrict__ P, int L0, int L1, double sv,
+                                                  double sd, double ratio, long long t)
 {
     const int H = L1 / 2;
     const long long per = (long long)L0 * H;
     const long long fs = 6LL * per;
-    const long long total = per * B;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
+    {
         const long long b = t / per;
         const long long r = t - b * per;
         const int i = (int)(r / H), l = (int)(r - (long long)i * H);
@@ -103,6 +105,50 @@ __global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const do
         Pb[4 * per] = pt1 + pt2;
         Pb[5 * per] = pt1 - pt2;
     }
+}
+
+__global__ void rbc_products_yfold_kernel(const double* __restrict__ Y, const double* __restrict__ Yold,
+                                          const double* __restrict__ dtbc, double* __restrict__ P,
+                                          int L0, int L1, double sv, double sd, double ratio, int B)
+{
+    const long long total = (long long)L0 * (L1 / 2) * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x)
+        products_yfold_at(Y, Yold, dtbc, P, L0, L1, sv, sd, ratio, t);
+}
+
+// The y-folded products of many members (size classes) in one launch: block
+// range [start[i], start[i+1]) serves member i, one element per thread.
+constexpr int kMaxProdMulti = 64;
+struct ProdMember {
+    const double* Y;
+    const double* Yold;
+    const double* dtbc;
+    double* P;
+    int L0, L1, batch;
+    double sv, sd, ratio;
+};
+struct ProdMulti {
+    int count, total;
+    int start[kMaxProdMulti + 1];
+    ProdMember m[kMaxProdMulti];
+};
+
+long long products_multi_bytes() { return (long long)sizeof(ProdMulti); }
+
+__global__ void rbc_products_yfold_multi(const ProdMulti* __restrict__ mm)
+{
+    const int blk = blockIdx.x;
+    int lo = 0, hi = mm->count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mm->start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const ProdMember q = mm->m[lo];
+    const long long t = (long long)(blk - mm->start[lo]) * blockDim.x + threadIdx.x;
+    if (t < (long long)q.L0 * (q.L1 / 2) * q.batch)
+        products_yfold_at(q.Y, q.Yold, q.dtbc, q.P, q.L0, q.L1, q.sv, q.sd, q.ratio, t);
 }
 
 // Adjoint of rbc_products_kernel: given the cotangents Pb of the three
@@ -277,6 +323,54 @@ extern "C" int pypde_rbc_products_yfold(const double* Y, const double* Yold, con
     hipLaunchKernelGGL(rbc_products_yfold_kernel, dim3(grid_for((long long)L0 * (L1 / 2) * batch)), dim3(256), 0,
                        as_stream(stream), Y, Yold, dtbc, P, L0, L1, sign_value, sign_dy, ratio, batch);
     return check_launch("pypde_rbc_products_yfold");
+}
+
+extern "C" int pypde_products_multi_pack(const double* const* Y, const double* const* Yold,
+                                         const double* const* dtbc, double* const* P,
+                                         const int* L0, const int* L1, const double* sign_value,
+                                         const double* sign_dy, const double* ratio,
+                                         const int* batch, int count, void* table,
+                                         long long table_bytes, pypde_multi_info* info)
+{
+    PYPDE_REQUIRE(Y && Yold && dtbc && P && L0 && L1 && sign_value && sign_dy && ratio && batch &&
+                      table && info,
+                  "null argument");
+    PYPDE_REQUIRE(table_bytes >= (long long)sizeof(ProdMulti), "products table needs %zu bytes",
+                  sizeof(ProdMulti));
+    PYPDE_REQUIRE(count >= 1 && count <= kMaxProdMulti, "multi products: count %d not in 1..%d",
+                  count, kMaxProdMulti);
+    ProdMulti* mm = static_cast<ProdMulti*>(table);
+    memset(mm, 0, sizeof(ProdMulti));
+    mm->count = count;
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        PYPDE_REQUIRE(Y[i] && dtbc[i] && P[i], "member %d: null pointer", i);
+        PYPDE_REQUIRE(L0[i] > 0 && L1[i] > 0 && L1[i] % 2 == 0 && batch[i] >= 0,
+                      "member %d: invalid product geometry (L1 must be even)", i);
+        mm->m[i] = ProdMember{Y[i], Yold[i], dtbc[i], P[i], L0[i], L1[i], batch[i],
+                              sign_value[i], sign_dy[i], ratio[i]};
+        mm->start[i] = (int)total;
+        total += ((long long)L0[i] * (L1[i] / 2) * batch[i] + 255) / 256;
+        PYPDE_REQUIRE(total < (1LL << 31), "multi products too large");
+    }
+    mm->start[count] = (int)total;
+    mm->total = (int)total;
+    info->kind = PYPDE_MULTI_PRODUCTS;
+    info->count = count;
+    info->variant = 0;
+    info->flags = 0;
+    info->total = total;
+    return OK;
+}
+
+extern "C" int pypde_products_multi(const void* table, const pypde_multi_info* info, void* stream)
+{
+    PYPDE_REQUIRE(table && info, "null argument");
+    PYPDE_REQUIRE(info->kind == PYPDE_MULTI_PRODUCTS, "not a products table");
+    if (info->total == 0) return OK;
+    hipLaunchKernelGGL(rbc_products_yfold_multi, dim3((unsigned)info->total), dim3(256), 0,
+                       as_stream(stream), static_cast<const ProdMulti*>(table));
+    return check_launch("pypde_products_multi");
 }
 
 extern "C" int pypde_rbc_products_vjp(const double* Y, const double* dtbc, const double* Pb,

@@ -18,6 +18,8 @@
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
+#include <cstring>
+
 namespace pypde {
 
 #pragma clang fp contract(off)
@@ -60,21 +62,21 @@ __device__ __forceinline__ double band2(const double* __restrict__ in, long long
     return acc;
 }
 
-__global__ void rbc_rhs_banded_kernel(const double* __restrict__ X, const double* __restrict__ pres,
-                                      const double* __restrict__ cst, double* __restrict__ R,
-                                      const double* __restrict__ mdta, const double* __restrict__ dta,
-                                      const double* __restrict__ fdiff, const double* __restrict__ tx,
-                                      const double* __restrict__ ty, int M0, int M1, int N0, int N1,
-                                      int B)
+__device__ __forceinline__ void rhs_banded_at(const double* __restrict__ X, const double* __restrict__ pres,
+                                              const double* __restrict__ cst, double* __restrict__ R,
+                                              const double* __restrict__ mdta,
+                                              const double* __restrict__ dta,
+                                              const double* __restrict__ fdiff,
+                                              const double* __restrict__ tx,
+                                              const double* __restrict__ ty, int M0, int M1, int N0,
+                                              int N1, long long t)
 {
     const long long per = (long long)M0 * M1;
-    const long long total = 3 * per * B;
     const int o135[3] = {1, 3, 5};
     const int o024[3] = {0, 2, 4};
     const int o4[4] = {-2, 0, 2, 4};
     const int o02[2] = {0, 2};
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
+    {
         const long long e = t / per;  // (instance, field) entry
         const long long r = t - e * per;
         const int b = (int)(e / 3), f = (int)(e - 3LL * b);
@@ -107,11 +109,49 @@ __global__ void rbc_rhs_banded_kernel(const double* __restrict__ X, const double
     }
 }
 
+__global__ void rbc_rhs_banded_kernel(const double* __restrict__ X, const double* __restrict__ pres,
+                                      const double* __restrict__ cst, double* __restrict__ R,
+                                      const double* __restrict__ mdta, const double* __restrict__ dta,
+                                      const double* __restrict__ fdiff, const double* __restrict__ tx,
+                                      const double* __restrict__ ty, int M0, int M1, int N0, int N1,
+                                      int B)
+{
+    const long long total = 3LL * M0 * M1 * B;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x)
+        rhs_banded_at(X, pres, cst, R, mdta, dta, fdiff, tx, ty, M0, M1, N0, N1, t);
+}
+
+// The banded right-hand sides of many members (size classes) in one launch:
+// block range [start[i], start[i+1]) serves member i, one output per thread.
+constexpr int kMaxRhsMulti = 64;
+struct RhsMulti {
+    int count, total;
+    int start[kMaxRhsMulti + 1];
+    pypde_rbc_rhs m[kMaxRhsMulti];
+};
+
+__global__ void rbc_rhs_banded_multi(const RhsMulti* __restrict__ mm)
+{
+    const int blk = blockIdx.x;
+    int lo = 0, hi = mm->count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mm->start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const pypde_rbc_rhs a = mm->m[lo];
+    const long long t = (long long)(blk - mm->start[lo]) * blockDim.x + threadIdx.x;
+    if (t < 3LL * a.M0 * a.M1 * a.batch)
+        rhs_banded_at(a.X, a.pres, a.cst, a.R, a.mdta, a.dta, a.fdiff, a.tx, a.ty, a.M0, a.M1,
+                      a.N0, a.N1, t);
+}
+
 }  // namespace pypde
 
 using namespace pypde;
 
-extern "C" int pypde_rbc_rhs_banded(const pypde_rbc_rhs* a, void* stream)
+static int check_rhs(const pypde_rbc_rhs* a)
 {
     PYPDE_REQUIRE(a, "null argument");
     PYPDE_REQUIRE(a->X && a->pres && a->R && a->mdta && a->dta && a->tx && a->ty,
@@ -121,6 +161,64 @@ extern "C" int pypde_rbc_rhs_banded(const pypde_rbc_rhs* a, void* stream)
                   "invalid geometry: M0 %d M1 %d N0 %d N1 %d batch %d", a->M0, a->M1, a->N0,
                   a->N1, a->batch);
     PYPDE_REQUIRE(a->R != a->X, "R must not alias X (neighbouring entries are read)");
+    return OK;
+}
+
+extern "C" int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* table,
+                                    long long table_bytes, pypde_multi_info* info)
+{
+    PYPDE_REQUIRE(args && table && info, "null argument");
+    PYPDE_REQUIRE(table_bytes >= (long long)sizeof(RhsMulti), "right-hand-side table needs %zu bytes",
+                  sizeof(RhsMulti));
+    PYPDE_REQUIRE(count >= 1 && count <= kMaxRhsMulti, "multi right-hand side: count %d not in 1..%d",
+                  count, kMaxRhsMulti);
+    RhsMulti* mm = static_cast<RhsMulti*>(table);
+    memset(mm, 0, sizeof(RhsMulti));
+    mm->count = count;
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const int rc = check_rhs(&args[i]);
+        if (rc != OK) return rc;
+        mm->m[i] = args[i];
+        mm->start[i] = (int)total;
+        total += (3LL * args[i].M0 * args[i].M1 * args[i].batch + 255) / 256;
+        PYPDE_REQUIRE(total < (1LL << 31), "multi right-hand side too large");
+    }
+    mm->start[count] = (int)total;
+    mm->total = (int)total;
+    info->kind = PYPDE_MULTI_RHS;
+    info->count = count;
+    info->variant = 0;
+    info->flags = 0;
+    info->total = total;
+    return OK;
+}
+
+extern "C" int pypde_rhs_multi(const void* table, const pypde_multi_info* info, void* stream)
+{
+    PYPDE_REQUIRE(table && info, "null argument");
+    PYPDE_REQUIRE(info->kind == PYPDE_MULTI_RHS, "not a right-hand-side table");
+    if (info->total == 0) return OK;
+    hipLaunchKernelGGL(rbc_rhs_banded_multi, dim3((unsigned)info->total), dim3(256), 0,
+                       as_stream(stream), static_cast<const RhsMulti*>(table));
+    return check_launch("pypde_rhs_multi");
+}
+
+extern "C" long long pypde_multi_table_bytes(int kind)
+{
+    switch (kind) {
+    case PYPDE_MULTI_GEMM: return gemm_multi_bytes();
+    case PYPDE_MULTI_RECURRENCE: return recurrence_multi_bytes();
+    case PYPDE_MULTI_PRODUCTS: return products_multi_bytes();
+    case PYPDE_MULTI_RHS: return (long long)sizeof(RhsMulti);
+    default: return -1;
+    }
+}
+
+extern "C" int pypde_rbc_rhs_banded(const pypde_rbc_rhs* a, void* stream)
+{
+    const int rc = check_rhs(a);
+    if (rc != OK) return rc;
     if (a->batch == 0) return OK;
     const long long total = 3LL * a->M0 * a->M1 * a->batch;
     long long blocks = (total + 255) / 256;

@@ -26,3 +26,25 @@ def rel(a, b):
     b = np.asarray(b, dtype=float)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def oracle_runs(make, nsteps):
+    """(ref, dense): the oracle solver made by make() after nsteps updates, and
+    the same run with the oracle's dense-product transforms (same arithmetic,
+    different rounding; its distance from ref is the rounding floor)."""
+    from oracle import pypde_oracle as o
+    ref = make()
+    for _ in range(nsteps):
+        ref.update()
+    with o.dense_transforms():
+        dense = make()
+        for _ in range(nsteps):
+            dense.update()
+    return ref, dense
+
+
+def floor_tol(ref, dense, field, base):
+    """Tolerance for a field: base, or twice the rounding floor where that is
+    larger."""
+    a, b = getattr(dense, field).vhat, getattr(ref, field).vhat
+    return max(base, 2.0 * rel(a, b))

@@ -37,6 +37,9 @@ def test_c3_trajectory_512():
     e_ke = np.abs(ke - ref["c3_ke"]) / np.abs(ref["c3_ke"])
     print("c3: max |dNu_plate| %.2e, max rel dNu_vol %.2e, max rel dKE (n>=50) %.2e"
           % (e_nu.max(), e_nv[1:].max(), e_ke[50:].max()))
+    for lo, hi in ((0, 10), (10, 100), (100, 500), (500, 1000), (1000, 1500), (1500, 2000)):
+        print("c3 steps %4d-%4d: max |dNu_plate| %.2e, rel %.2e" %
+              (lo, hi, e_nu[lo:hi].max(), (e_nu[lo:hi] / np.abs(ref["c3_nu_plate"][lo:hi])).max()))
     assert e_nu.max() < 5e-9
     assert e_nv[1:].max() < 1e-10
     assert e_ke[50:].max() < 1e-10

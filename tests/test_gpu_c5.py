@@ -6,7 +6,8 @@ noise 1e-3 from seed 0), by the hand-written adjoint of the fused step:
   100 steps with recomputed segments, through the torch.autograd.Function
   TimeAveragedNusselt; against central differences with eps = 1e-6 along 8
   unit directions drawn from numpy default_rng(5), 1e-6 relative to each
-  directional derivative, and the eps = 1e-4 quotient as a convergence check;
+  directional derivative; the eps = 1e-4 quotient, whose O(eps^2) truncation
+  error is a few 1e-6 here, within 1e-5;
 * J_res = 0.5 ||(Phi_dt(X) - X)/dt||^2 at a state 400 steps into the run
   (rbc2d_base.py:321-341 restated, pressure held fixed): its temperature
   gradient against central differences along the same kind of directions,
@@ -79,7 +80,7 @@ def test_c5_nusselt_gradient_2000_steps_fd(c5):
         print("direction %d: adjoint %.12e fd(1e-6) %.12e (rel %.2e) fd(1e-4) %.12e (rel %.2e)"
               % (k, an, f6, e6, f4, e4))
     assert worst6 <= 1e-6, worst6
-    assert worst4 <= 1e-6, worst4
+    assert worst4 <= 1e-5, worst4
 
 
 def test_c5_residual_gradient_fd_and_adjoint_descent(c5):

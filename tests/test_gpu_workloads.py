@@ -8,10 +8,11 @@ are chosen as in bench.py):
   per output (reference: chebyshev.py:67-93,287-337, plans.py:54-74,184-224);
 * the 256x256 Euler step of the headline metric (rbc, Ra=1e8, dt=1e-3) with
   its tuned launch set and graph replays, against the oracle's update
-  (navier/rbc2d.py:396-434), 1e-12 after 10 steps;
+  (navier/rbc2d.py:396-434) after 12 steps: 1e-12, or twice the distance
+  between two exact FP64 restatements where that is larger;
 * the mixed-size batch: the 256 instances of bench.c4_specs in one graph
   launch per step; one instance of every size class against the oracle after
-  5 steps (1e-11), then 500 steps: finite states, plate Nusselt numbers near
+  5 steps (1e-11, or twice the rounding floor), then 500 steps: finite states, plate Nusselt numbers near
   the conductive value, and sampled instances equal to single-instance runs.
 """
 import os
@@ -74,36 +75,55 @@ def test_c2_chain_512_fields_matches_oracle():
 
 
 def test_plain_256_step_with_tuned_launches_matches_oracle():
+    """Tolerance: 1e-12, or twice the rounding floor where that is larger.  At
+    this size the temperature's Helmholtz operator (lambda = dt kappa = 1e-7)
+    is close to the pseudo-inverse of D^2, so its solve amplifies rounding of
+    the right-hand side by about N^2: two exact FP64 restatements of the
+    reference (the oracle with scipy's DCT and with dense-product transforms)
+    already differ by about 1e-10 in T after 12 steps from this state."""
     import torch
     import bench
     from pypde_amd.navier import NavierStokes
     from oracle import pypde_oracle as o
+    from conftest import floor_tol, oracle_runs
     N = 256
     kw = dict(shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None)
     ns = NavierStokes(case="rbc", dealias=True, integrator="eu", beta=1.0, aspect=1.0, **kw)
-    ref = o.NavierStokes(case="rbc", **kw)
     # the bench's initial state, plus a velocity so that the products matter
     noise = bench.seeded_noise((N, N), seed=0)
     ns.set_state(T=ns.T.forward(noise))
     ns.set_velocity(amplitude=0.05)
-    ref.T.vhat = host(ns.T.vhat).copy()
-    ref.U.vhat = host(ns.U.vhat).copy()
-    ref.V.vhat = host(ns.V.vhat).copy()
+    init = {f: host(getattr(ns, f).vhat).copy() for f in "TUV"}
+
+    def make():
+        r = o.NavierStokes(case="rbc", **kw)
+        for f in "TUV":
+            getattr(r, f).vhat = init[f].copy()
+        return r
     ns.prepare_graphs()
     print("launch variants:", ns.engine.variants())
     nsteps = ns.GRAPH_BLOCK + 2   # one 10-step graph replay and two 1-step replays
     ns.update_many(nsteps)
     torch.cuda.synchronize()
-    for _ in range(nsteps):
-        ref.update()
+    ref, dense = oracle_runs(make, nsteps)
     for f in ("T", "U", "V", "pres"):
         e = rel(host(getattr(ns, f).vhat), getattr(ref, f).vhat)
-        print("256x256 %s after %d steps: %.2e" % (f, nsteps, e))
-        assert e < 1e-12, (f, e)
-    nu = ns.eval_Nu()
-    nr = ref.eval_Nu()
-    assert abs(nu[0] - nr[0]) <= 1e-10 * abs(nr[0])
-    assert abs(nu[1] - nr[1]) <= 1e-10 * abs(nr[1])
+        tol = floor_tol(ref, dense, f, 1e-12)
+        print("256x256 %s after %d steps: %.2e (floor %.2e)"
+              % (f, nsteps, e, rel(getattr(dense, f).vhat, getattr(ref, f).vhat)))
+        assert e < tol, (f, e, tol)
+    # Nusselt numbers: the plate value takes d/dy at the walls, which amplifies
+    # the rounding of the high Chebyshev modes by about N^2; its floor is the
+    # larger of the two restatements' distance and the distance between the two
+    # evaluations of the reference's own final state
+    nu, nr = ns.eval_Nu(), ref.eval_Nu()
+    with o.dense_transforms():
+        nd, nre = dense.eval_Nu(), ref.eval_Nu()
+    for k in range(2):
+        floor = max(abs(nd[k] - nr[k]), abs(nre[k] - nr[k]))
+        tol = max(1e-10 * abs(nr[k]), 4.0 * floor)
+        print("256x256 Nu[%d]: %.3e (floor %.3e)" % (k, abs(nu[k] - nr[k]), floor))
+        assert abs(nu[k] - nr[k]) <= tol, (k, nu[k], nr[k], tol)
 
 
 @pytest.fixture(scope="module")
@@ -119,8 +139,11 @@ def c4_batch():
 
 
 def test_c4_one_instance_per_size_class_matches_oracle(c4_batch):
+    """Tolerance: 1e-11, or twice the rounding floor of the instance where that
+    is larger (see the 256x256 test)."""
     import torch
     from oracle import pypde_oracle as o
+    from conftest import floor_tol, oracle_runs
     specs, seeds, hb = c4_batch
     assert len(hb.shapes) == 16
     first = {}
@@ -131,15 +154,18 @@ def test_c4_one_instance_per_size_class_matches_oracle(c4_batch):
     torch.cuda.synchronize()
     for shape, i in sorted(first.items()):
         s = specs[i]
-        r = o.NavierStokes(case="rbc", shape=shape, ra=s["ra"], pr=s["pr"], dt=s["dt"],
-                           tsave=None)
-        o.seeded_initial_state(r, seed=seeds[i], amplitude=1e-3)
-        for _ in range(nsteps):
-            r.update()
+
+        def make(s=s, shape=shape, i=i):
+            r = o.NavierStokes(case="rbc", shape=shape, ra=s["ra"], pr=s["pr"], dt=s["dt"],
+                               tsave=None)
+            return o.seeded_initial_state(r, seed=seeds[i], amplitude=1e-3)
+        r, dense = oracle_runs(make, nsteps)
         st = hb.state(i)
         for f in ("U", "V", "T", "pres"):
             e = rel(host(st[f]), getattr(r, f).vhat)
-            assert e < 1e-11, (shape, i, f, e)
+            tol = floor_tol(r, dense, f, 1e-11)
+            print("c4 %s instance %d %s: %.2e (tol %.2e)" % (shape, i, f, e, tol))
+            assert e < tol, (shape, i, f, e, tol)
 
 
 def test_c4_500_steps_properties(c4_batch):

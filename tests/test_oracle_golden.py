@@ -81,3 +81,19 @@ def test_oracle_adjoint_descent_step_matches_reference(golden):
         assert rel(getattr(nsa, f).vhat, golden["adj_out_" + f]) < 1e-14, f
     assert rel(nsa.pres.vhat, golden["adj_out_pres"]) < 1e-14
     assert rel(nsa.NS.pres.vhat, golden["adj_ns_pres"]) < 1e-14
+
+
+def test_dense_transform_oracle_matches_reference_single_steps(golden):
+    """The oracle with dense-product transforms (the rounding-floor variant)
+    still reproduces the reference's single steps: it is the same arithmetic
+    up to rounding."""
+    from oracle import pypde_oracle as o
+    pre = "step_rbc_eu_"
+    with o.dense_transforms():
+        ns = o.NavierStokes(case="rbc", shape=(20, 16), ra=1e5, pr=1.0, dt=0.01, tsave=None)
+        for f in ("T", "U", "V", "P", "pres"):
+            getattr(ns, f).vhat = np.array(golden[pre + "in_" + f])
+        ns.update()
+    for f in ("T", "U", "V", "pres"):
+        a, b = getattr(ns, f).vhat, golden[pre + "out1_" + f]
+        assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b), f
