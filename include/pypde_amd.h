@@ -444,6 +444,21 @@ int pypde_plan_run(const pypde_plan* p, const void* const* bases, int n_steps, v
 int pypde_rbc_step(const pypde_plan* plan, double* X, double* pres, double* P, void* workspace,
                    int n_steps, void* stream);
 
+/* pypde_rbc_step without a caller workspace: the plan's workspace region is
+ * moved to a device array the library reserves, PYPDE_SCRATCH_BYTES (48 MiB)
+ * on every GPU it runs on, held until the process ends.  Calls no allocator
+ * (no hipMalloc / hipFree, host or device).  Rejects a plan whose workspace
+ * is larger (pypde_plan_region_bytes(plan, 3) > pypde_scratch_bytes()): a
+ * single 256 x 256 instance fits.  All calls on one GPU share its array, so
+ * calls that overlap in time on one GPU (two streams, two host threads) race;
+ * calls in one stream do not.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * allocation). */
+#define PYPDE_SCRATCH_BYTES (48ull << 20)
+long long pypde_scratch_bytes(void);
+int pypde_rbc_step_noalloc(const pypde_plan* plan, double* X, double* pres, double* P,
+                           int n_steps, void* stream);
 
 #ifdef __cplusplus
 }

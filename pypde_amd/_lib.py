@@ -155,6 +155,8 @@ _SIGS = {
     "pypde_plan_region_bytes": ([_vp, _i], ctypes.c_longlong),
     "pypde_plan_run": ([_vp, ctypes.c_void_p, _i, _vp], _i),
     "pypde_rbc_step": ([_vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
+    "pypde_scratch_bytes": ([], ctypes.c_longlong),
+    "pypde_rbc_step_noalloc": ([_vp, _vp, _vp, _vp, _i, _vp], _i),
 }
 
 for _name, (_args, _res) in _SIGS.items():

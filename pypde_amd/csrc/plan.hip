@@ -11,6 +11,8 @@
 #include "common.h"
 #include "../../include/pypde_amd.h"
 
+#include <atomic>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -318,6 +320,65 @@ extern "C" int pypde_plan_run(const pypde_plan* p, const void* const* bases, int
         }
     }
     return OK;
+}
+
+// ---- the step without a caller workspace ---------------------------------
+// Device memory that the library reserves for callers that cannot allocate:
+// one array per GPU, held until the process ends.  Host code never uses its
+// name as an address: the device address comes from hipGetSymbolAddress, once
+// per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[PYPDE_SCRATCH_BYTES];
+
+namespace pypde {
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::atomic<int> g_scratch_ready[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch` (resolved on the first call on
+// that device; later calls read the kept pointer).
+static int device_scratch(void** out)
+{
+    int dev = -1;
+    PYPDE_HIP(hipGetDevice(&dev));
+    PYPDE_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d: at most %d devices are supported",
+                  dev, kMaxDevices);
+    if (!g_scratch_ready[dev].load(std::memory_order_acquire)) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ready[dev].load(std::memory_order_relaxed)) {
+            void* ptr = nullptr;
+            PYPDE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            PYPDE_REQUIRE(ptr, "hipGetSymbolAddress returned no address for the scratch array");
+            g_scratch_ptr[dev] = ptr;
+            g_scratch_ready[dev].store(1, std::memory_order_release);
+        }
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+}  // namespace pypde
+
+extern "C" long long pypde_scratch_bytes(void) { return (long long)PYPDE_SCRATCH_BYTES; }
+
+extern "C" int pypde_rbc_step_noalloc(const pypde_plan* p, double* X, double* pres, double* P,
+                                      int n_steps, void* stream)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PYPDE_REQUIRE(p->base.size() == 4,
+                  "an RBC step plan has the regions X, pres, P and workspace (got %zu regions)",
+                  p->base.size());
+    PYPDE_REQUIRE(p->bytes[3] <= (long long)PYPDE_SCRATCH_BYTES,
+                  "the plan's workspace needs %lld bytes; the library's scratch array holds %lld "
+                  "(use pypde_rbc_step with a caller workspace)",
+                  p->bytes[3], (long long)PYPDE_SCRATCH_BYTES);
+    PYPDE_REQUIRE(X && pres && P, "null state pointer");
+    void* ws = nullptr;
+    const int rc = device_scratch(&ws);
+    if (rc != OK) return rc;
+    // every workspace entry the step reads is written earlier in the same
+    // step, so the contents left by an earlier call need no clearing
+    const void* bases[4] = {X, pres, P, ws};
+    return pypde_plan_run(p, bases, n_steps, stream);
 }
 
 extern "C" int pypde_rbc_step(const pypde_plan* p, double* X, double* pres, double* P,
