@@ -24,7 +24,9 @@ Also reported on the same line:
   variants       the same 256x256 step with RK3 (three stages) and with beta = 0.5
   hetero         256 instances of mixed sizes (N per axis in {64, 96, 128, 192}) and
                  Ra, one graph launch per step, 500 steps: aggregate instance-steps/s and
-                 the ratio to looping the plain single-instance step over them
+                 the ratio to looping the plain single-instance step over them; the
+                 persistent form (one launch per stage for all 16 size classes), and
+                 the stream-branch form (one branch per class) as hetero_branches
   microbench     512 fields of 256x256: transforms, ChebDirichlet stencil maps,
                  Helmholtz right-hand side and four-diagonal solves, per-kernel HBM fractions
   adjoint        gradient of the time-averaged plate Nusselt number with respect
@@ -234,7 +236,7 @@ def c4_specs(n=256, seed=7):
             for a, b, r in zip(nx, ny, ras)]
 
 
-def time_hetero(steps=500, warm=5, n=256, seed=7):
+def time_hetero(steps=500, warm=5, n=256, seed=7, persistent=True):
     """256 independent instances, N drawn per axis from {64, 96, 128, 192},
     Ra log-uniform in [1e4, 1e7], Pr = 1, dt = clip(0.01 sqrt(1e5/Ra), 1e-3, 1e-2),
     instance i seeded with noise 1e-3 from seed 1000 + i; one graph launch per
@@ -245,7 +247,7 @@ def time_hetero(steps=500, warm=5, n=256, seed=7):
     from pypde_amd.engine import RBCBatch
     specs = c4_specs(n, seed)
     t0 = time.perf_counter()
-    hb = HeteroBatch(specs, case="rbc")
+    hb = HeteroBatch(specs, case="rbc", persistent=persistent)
     setup = time.perf_counter() - t0
     hb.seed([1000 + i for i in range(n)], amplitude=1e-3)
     hb.capture(1)
@@ -270,7 +272,9 @@ def time_hetero(steps=500, warm=5, n=256, seed=7):
         del e
     loop_sec_per_step = sum(per_shape[tuple(s["shape"])] for s in specs)
     agg = n * steps / sec
-    return dict(instances=n, size_classes=len(hb.shapes), steps=steps, sec=round(sec, 3),
+    return dict(form="persistent" if persistent else "stream branches",
+                launches_per_step=len(hb.multi) if persistent else None,
+                instances=n, size_classes=len(hb.shapes), steps=steps, sec=round(sec, 3),
                 inst_steps_per_s=round(agg, 1), ms_per_step=round(sec / steps * 1e3, 3),
                 plain_loop_inst_steps_per_s=round(n / loop_sec_per_step, 1),
                 ratio_vs_plain_loop=round(agg / (n / loop_sec_per_step), 2),
@@ -549,9 +553,14 @@ def main():
     ap.add_argument("--tune-db", metavar="PATH",
                     help="tune every workload of this benchmark from scratch and write the "
                          "kernel decisions to PATH (the database pypde_amd/tune_db.json)")
+    ap.add_argument("--tune-db-update", metavar="PATH",
+                    help="as --tune-db, but keep the stored decisions and tune only the "
+                         "shapes they do not cover")
     args = ap.parse_args()
     if args.tune_db:
         os.environ["PYPDE_TUNE_DB"] = ""  # measure, do not load stored decisions
+    elif args.tune_db_update:
+        args.tune_db = args.tune_db_update
     if args.gpus != 1:
         print("bench.py: pypde_amd runs one instance set on one GPU (got --gpus %d); "
               "multi-GPU execution is not implemented." % args.gpus, file=sys.stderr)
@@ -572,18 +581,26 @@ def main():
     if args.tune_db:
         from pypde_amd import autotune
         args.steps, args.warmup = 20, 2
-        time_plain(args)
-        time_batched(args)
-        time_hetero()
-        time_variant(steps=5, integrator="rk3")
-        time_variant(steps=5, beta=0.5)
-        time_variant(steps=5, N=512)
-        microbench()
+
+        def stage(name, fn, *a, **kw):
+            t0 = time.perf_counter()
+            fn(*a, **kw)
+            print("tuned %s (%.1f s, %d decisions)" % (name, time.perf_counter() - t0,
+                                                       len(autotune.cache())),
+                  file=sys.stderr, flush=True)
+        stage("plain", time_plain, args)
+        stage("batched", time_batched, args)
+        stage("hetero", time_hetero)
+        stage("hetero_branches", time_hetero, persistent=False)
+        stage("rk3", time_variant, steps=5, integrator="rk3")
+        stage("beta", time_variant, steps=5, beta=0.5)
+        stage("512", time_variant, steps=5, N=512)
+        stage("microbench", microbench)
         for shape, ra in (((20, 16), 1e4), ((64, 64), 1e4), ((128, 128), 5e4)):
-            time_steady(shape, ra, pre=20, tol=1e-3)
-        time_adjoint_descent(256, steps=2)
-        time_adjoint(256, 20, ckpt=10)
-        time_adjoint(128, 20, ckpt=10)
+            stage("steady %s" % (shape,), time_steady, shape, ra, pre=20, tol=1e-3)
+        stage("adjoint descent", time_adjoint_descent, 256, steps=2)
+        stage("adjoint 256", time_adjoint, 256, 20, ckpt=10)
+        stage("adjoint 128", time_adjoint, 128, 20, ckpt=10)
         n = autotune.save_db(args.tune_db)
         print(json.dumps({"tune_db": args.tune_db, "entries": n}))
         return
@@ -604,6 +621,7 @@ def main():
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
         extra["hetero"] = time_hetero()
+        extra["hetero_branches"] = time_hetero(persistent=False)
         extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5),
                              time_variant(N=512)]
         extra["microbench"] = microbench()

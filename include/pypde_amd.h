@@ -411,8 +411,8 @@ int pypde_plan_create(int nregions, const void* const* bases, const long long* b
 int pypde_plan_destroy(pypde_plan* p);
 /* Recording: the arguments of pypde_dgemm_batched, pypde_dgemm_grouped,
  * pypde_recurrence_solve, pypde_rbc_products, pypde_rbc_products_yfold,
- * pypde_splitk_reduce and pypde_rbc_rhs_banded, without the stream.  Arguments
- * are checked when run. */
+ * pypde_splitk_reduce, pypde_rbc_rhs_banded, pypde_fdma_solve and
+ * pypde_poisson_solve, without the stream.  Arguments are checked when run. */
 int pypde_plan_add_gemm(pypde_plan* p, const pypde_gemm_desc* d);
 int pypde_plan_add_grouped(pypde_plan* p, const pypde_gemm_desc* descs, int count);
 int pypde_plan_add_recurrence(pypde_plan* p, const pypde_lines* g, const pypde_recurrence* r);
@@ -425,6 +425,10 @@ int pypde_plan_add_splitk_reduce(pypde_plan* p, const double* W, int S, const do
                                  const double* X, int xm, int xr, double* C, long long mn,
                                  int entries);
 int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* args);
+int pypde_plan_add_fdma(pypde_plan* p, const pypde_lines* g, const double* l, const double* d,
+                        const double* u1, const double* u2, long long table_bs);
+int pypde_plan_add_poisson(pypde_plan* p, const pypde_lines* g, const double* tab,
+                           const int* sing, int transpose);
 /* Number of recorded launches (-1 for a null plan); bytes of a region (-1 if none). */
 int pypde_plan_size(const pypde_plan* p);
 long long pypde_plan_region_bytes(const pypde_plan* p, int region);

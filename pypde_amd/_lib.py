@@ -138,6 +138,8 @@ _SIGS = {
     "pypde_plan_add_splitk_reduce": ([_vp, _vp, _i, _vp, _vp, _i, _i, _vp, ctypes.c_longlong,
                                       _i], _i),
     "pypde_plan_add_rhs_banded": ([_vp, ctypes.c_void_p], _i),
+    "pypde_plan_add_fdma": ([_vp, _lp, _vp, _vp, _vp, _vp, ctypes.c_longlong], _i),
+    "pypde_plan_add_poisson": ([_vp, _lp, _vp, _vp, _i], _i),
     "pypde_rbc_rhs_banded": ([ctypes.c_void_p, _vp], _i),
     "pypde_multi_table_bytes": ([_i], ctypes.c_longlong),
     "pypde_gemm_multi_pack": ([ctypes.c_void_p, _i, _i, _vp, ctypes.c_longlong, ctypes.c_void_p],

@@ -10,10 +10,45 @@ end of every step, so classes of small fields overlap on the device instead of
 running one after another.  No reference counterpart: the reference steps one
 NavierStokes object at a time (navier/rbc2d.py:396-434).
 """
+import ctypes
+
 import numpy as np
 import torch
 
-from .engine import RBCBatch
+from ._lib import Lines, MultiInfo, RbcRhs, Recurrence, check, lib
+from .engine import _GROUPS, RBCBatch, _Call, _Gemm, _RecCall
+
+# the LDS-tiled GEMM variants a multi launch can run (one for all members)
+MULTI_GEMM_VARIANTS = (1, 2, 3, 5, 6, 7, 24, 25, 26, 27, 28, 33, 34, 35, 36, 37, 38)
+_KIND = {"gemm": 1, "recurrence": 2, "products": 3, "rhs": 4}
+
+
+class _Multi:
+    """One multi launch of the C-ABI (pypde_*_multi): the members' arguments
+    packed on the host, copied once into a device table, launched in one go."""
+
+    def __init__(self, name, kind, pack):
+        self.name, self.kind = name, kind
+        self.info = MultiInfo()
+        nbytes = lib.pypde_multi_table_bytes(_KIND[kind])
+        self.host = (ctypes.c_ubyte * nbytes)()
+        self.pack = pack
+        self.table = None
+        self.repack()
+
+    def repack(self, *args):
+        check(self.pack(self.host, len(self.host), ctypes.byref(self.info), *args), self.name)
+        host = torch.frombuffer(bytearray(self.host), dtype=torch.uint8)
+        dev = torch.empty(host.numel(), dtype=torch.uint8, device="cuda")
+        dev.copy_(host)
+        torch.cuda.synchronize()
+        self.table = dev  # the launch reads it: kept alive with the launch
+
+    def __call__(self, stream):
+        fn = {"gemm": lib.pypde_dgemm_multi, "recurrence": lib.pypde_recurrence_multi,
+              "products": lib.pypde_products_multi, "rhs": lib.pypde_rhs_multi}[self.kind]
+        check(fn(ctypes.c_void_p(self.table.data_ptr()), ctypes.byref(self.info), stream),
+              self.name)
 
 
 def default_dt(ra):
@@ -23,9 +58,14 @@ def default_dt(ra):
 
 
 class HeteroBatch:
-    """specs: sequence of dicts with keys shape=(N0, N1), ra, pr, dt."""
+    """specs: sequence of dicts with keys shape=(N0, N1), ra, pr, dt.
 
-    def __init__(self, specs, case="rbc", dealias=True, aspect=1.0):
+    persistent=True: every operator of the step runs once for all size classes
+    (the multi launches of the C-ABI, one per stage: about 13 per Euler step
+    whatever the number of classes); False: one engine step per class on its
+    own stream branch of the captured graph."""
+
+    def __init__(self, specs, case="rbc", dealias=True, aspect=1.0, persistent=False):
         self.specs = [dict(s) for s in specs]
         if not self.specs:
             raise ValueError("HeteroBatch needs at least one instance")
@@ -43,13 +83,17 @@ class HeteroBatch:
             idx = by_shape[shape]
             params = [dict(ra=self.specs[i]["ra"], pr=self.specs[i].get("pr", 1.0),
                            dt=self.specs[i]["dt"]) for i in idx]
+            # the persistent form takes each class's launches one by one (no
+            # grouped or K-split launches inside a class)
             self.classes.append(RBCBatch(shape, params, case=case, dealias=dealias,
-                                         aspect=aspect))
+                                         aspect=aspect, group_launches=not persistent))
             for slot, i in enumerate(idx):
                 self.where[i] = (c, slot)
         self.graph = None
         self.graph_steps = 0
         self._streams = None
+        self.persistent = bool(persistent)
+        self.multi = self._build_multi() if self.persistent else None
 
     def __len__(self):
         return len(self.specs)
@@ -99,8 +143,129 @@ class HeteroBatch:
                     out[i] = vals[slot]
         return out
 
+    # ---- one launch per stage for every class -----------------------------
+    def _build_multi(self):
+        """The step as multi launches: position p of every class's launch list
+        (the same operator of the step) becomes one launch, and neighbouring
+        GEMM positions that the engine may group share it."""
+        lists = [e.launches for e in self.classes]
+        names = [l.name for l in lists[0]]
+        for L in lists[1:]:
+            if [l.name for l in L] != names:
+                raise RuntimeError("size classes issue different launch sequences")
+        mergeable = set()
+        for grp in _GROUPS:
+            for a, b in zip(grp, grp[1:]):
+                mergeable.add((a, b))
+        base = lambda name: name.split("[", 1)[0]   # without the RK3 stage tag
+        saved = [[t.clone() for t in (e.X, e.pres, e.P)] for e in self.classes]
+        out, p = [], 0
+        while p < len(names):
+            first = lists[0][p]
+            if isinstance(first, _Gemm):
+                q = p + 1
+                while (q < len(names) and isinstance(lists[0][q], _Gemm)
+                       and (base(names[q - 1]), base(names[q])) in mergeable
+                       and (self._plain(lists[0][q]).desc.transA,
+                            self._plain(lists[0][q]).desc.transB) ==
+                       (self._plain(first).desc.transA, self._plain(first).desc.transB)):
+                    q += 1
+                descs = [self._plain(L[k]).desc for k in range(p, q) for L in lists]
+                out.append(self._multi_gemm("+".join(names[p:q]), descs))
+                p = q
+                continue
+            members = [L[p] for L in lists]
+            if isinstance(first, _RecCall):
+                out.append(self._multi_rec(names[p], members))
+            elif isinstance(first, _Call) and first.fn.__name__ == "pypde_rbc_products_yfold":
+                out.append(self._multi_products(names[p], members))
+            elif isinstance(first, _Call) and first.fn.__name__ == "pypde_rbc_rhs_banded":
+                out.append(self._multi_rhs(names[p], members))
+            else:
+                raise RuntimeError("launch %s has no multi form" % names[p])
+            p += 1
+        torch.cuda.synchronize()
+        for e, sv in zip(self.classes, saved):
+            for t, v in zip((e.X, e.pres, e.P), sv):
+                t.copy_(v)
+        torch.cuda.synchronize()
+        return out
+
+    @staticmethod
+    def _plain(launch):
+        """The unfolded form of a GEMM launch (multi launches run plain tiles)."""
+        if launch.desc.fold != 0:
+            if launch.alt is None:
+                raise RuntimeError("%s has no unfolded form" % launch.name)
+            return launch.alt
+        return launch
+
+    def _multi_gemm(self, name, descs):
+        from . import autotune
+        from ._lib import GemmDesc
+        arr = (GemmDesc * len(descs))(*descs)
+        state = {"v": 2}
+
+        def pack(host, nb, info):
+            return lib.pypde_gemm_multi_pack(ctypes.byref(arr), len(descs), state["v"], host, nb,
+                                             info)
+        m = _Multi(name, "gemm", pack)
+        m._arr = arr
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        tables = {}
+
+        def run(v):
+            if v not in tables:
+                state["v"] = v
+                m.repack()
+                tables[v] = (m.table, MultiInfo.from_buffer_copy(m.info))
+            t, info = tables[v]
+            check(lib.pypde_dgemm_multi(ctypes.c_void_p(t.data_ptr()), ctypes.byref(info), stream),
+                  name)
+        key = ("multi_gemm",) + tuple(autotune.gemm_key(d) for d in descs)
+        best = autotune.choose(key, run, MULTI_GEMM_VARIANTS, default=2)
+        state["v"] = best
+        m.repack()
+        m.variant = best
+        return m
+
+    def _multi_rec(self, name, members):
+        n = len(members)
+        gs = (Lines * n)(*[l.g for l in members])
+        rs = (Recurrence * n)(*[l.structs[2] for l in members])
+        m = _Multi(name, "recurrence", lambda host, nb, info: lib.pypde_recurrence_multi_pack(
+            ctypes.byref(gs), ctypes.byref(rs), n, host, nb, info))
+        m._keep = (gs, rs)
+        return m
+
+    def _multi_products(self, name, members):
+        n = len(members)
+        cols = list(zip(*[l.args for l in members]))   # Y, Yold, dtbc, P, L0, L1, sv, sd, ratio, B
+        vp = ctypes.c_void_p
+        arrs = [(vp * n)(*[a.value if a is not None else None for a in cols[i]]) for i in range(4)]
+        ints = [(ctypes.c_int * n)(*cols[i]) for i in (4, 5)]
+        dbls = [(ctypes.c_double * n)(*cols[i]) for i in (6, 7, 8)]
+        bat = (ctypes.c_int * n)(*cols[9])
+        m = _Multi(name, "products", lambda host, nb, info: lib.pypde_products_multi_pack(
+            *arrs, *ints, *dbls, bat, n, host, nb, info))
+        m._keep = (arrs, ints, dbls, bat)
+        return m
+
+    def _multi_rhs(self, name, members):
+        n = len(members)
+        args = (RbcRhs * n)(*[l.args[0]._obj for l in members])
+        m = _Multi(name, "rhs", lambda host, nb, info: lib.pypde_rhs_multi_pack(
+            ctypes.byref(args), n, host, nb, info))
+        m._keep = args
+        return m
+
     # ---- stepping ---------------------------------------------------------
     def step_eager(self):
+        if self.persistent:
+            s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            for launch in self.multi:
+                launch(s)
+            return
         for e in self.classes:
             e.step_eager()
 
@@ -114,8 +279,15 @@ class HeteroBatch:
             for t, v in zip((e.X, e.pres, e.P), sv):
                 t.copy_(v)
         torch.cuda.synchronize()
-        self._streams = [torch.cuda.Stream() for _ in self.classes]
         g = torch.cuda.CUDAGraph()
+        if self.persistent:
+            with torch.cuda.graph(g):
+                for _ in range(nsteps):
+                    self.step_eager()
+            self.graph = g
+            self.graph_steps = nsteps
+            return
+        self._streams = [torch.cuda.Stream() for _ in self.classes]
         with torch.cuda.graph(g):
             cap = torch.cuda.current_stream()
             for _ in range(nsteps):

@@ -19,7 +19,7 @@
 namespace pypde {
 
 enum PlanOpKind { OP_GEMM, OP_GROUPED, OP_RECURRENCE, OP_PRODUCTS, OP_PRODUCTS_YFOLD, OP_REDUCE,
-                  OP_RHS_BANDED };
+                  OP_RHS_BANDED, OP_FDMA, OP_POISSON };
 
 struct PlanOp {
     PlanOpKind kind;
@@ -27,6 +27,10 @@ struct PlanOp {
     pypde_lines lines;                   // OP_RECURRENCE
     pypde_recurrence rec;
     pypde_rbc_rhs rhs;                   // OP_RHS_BANDED
+    const double* tab[4];                // OP_FDMA: l, d, u1, u2; OP_POISSON: tab, (sing)
+    const int* sing;
+    long long table_bs;
+    int transpose;
     // products: Y, Yold (yfold only), dtbc, P; reduce: W, D, X, C
     const double* p0;
     const double* p1;
@@ -129,6 +133,21 @@ static int run_op(const pypde_plan* pl, const intptr_t* dl, bool moved,
         a.tx = relocate(pl, dl, a.tx);
         a.ty = relocate(pl, dl, a.ty);
         return pypde_rbc_rhs_banded(&a, stream);
+    }
+    case OP_FDMA: {
+        pypde_lines g = op.lines;
+        g.src = relocate(pl, dl, g.src);
+        g.dst = relocate(pl, dl, g.dst);
+        return pypde_fdma_solve(&g, relocate(pl, dl, op.tab[0]), relocate(pl, dl, op.tab[1]),
+                                relocate(pl, dl, op.tab[2]), relocate(pl, dl, op.tab[3]),
+                                op.table_bs, stream);
+    }
+    case OP_POISSON: {
+        pypde_lines g = op.lines;
+        g.src = relocate(pl, dl, g.src);
+        g.dst = relocate(pl, dl, g.dst);
+        return pypde_poisson_solve(&g, relocate(pl, dl, op.tab[0]), relocate(pl, dl, op.sing),
+                                   op.transpose, stream);
     }
     case OP_REDUCE:
         return pypde_splitk_reduce(relocate(pl, dl, op.p0), op.S, relocate(pl, dl, op.p1),
@@ -277,6 +296,35 @@ extern "C" int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* arg
     PlanOp op{};
     op.kind = OP_RHS_BANDED;
     op.rhs = *args;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_fdma(pypde_plan* p, const pypde_lines* g, const double* l,
+                                   const double* d, const double* u1, const double* u2,
+                                   long long table_bs)
+{
+    PYPDE_REQUIRE(p && g, "null argument");
+    PlanOp op{};
+    op.kind = OP_FDMA;
+    op.lines = *g;
+    op.tab[0] = l;
+    op.tab[1] = d;
+    op.tab[2] = u1;
+    op.tab[3] = u2;
+    op.table_bs = table_bs;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_poisson(pypde_plan* p, const pypde_lines* g, const double* tab,
+                                      const int* sing, int transpose)
+{
+    PYPDE_REQUIRE(p && g, "null argument");
+    PlanOp op{};
+    op.kind = OP_POISSON;
+    op.lines = *g;
+    op.tab[0] = tab;
+    op.sing = sing;
+    op.transpose = transpose;
     return add_op(p, std::move(op));
 }
 

@@ -290,7 +290,9 @@ class Plan:
     _ADD = {"pypde_rbc_products": "pypde_plan_add_products",
             "pypde_rbc_products_yfold": "pypde_plan_add_products_yfold",
             "pypde_splitk_reduce": "pypde_plan_add_splitk_reduce",
-            "pypde_rbc_rhs_banded": "pypde_plan_add_rhs_banded"}
+            "pypde_rbc_rhs_banded": "pypde_plan_add_rhs_banded",
+            "pypde_fdma_solve": "pypde_plan_add_fdma",
+            "pypde_poisson_solve": "pypde_plan_add_poisson"}
 
     def __init__(self, nregions, bases, sizes):
         h = ctypes.c_void_p()
@@ -396,8 +398,12 @@ class RBCBatch:
     """
 
     def __init__(self, shape, params, case="rbc", dealias=True, aspect=1.0, beta=1.0,
-                 integrator="eu", group_launches=True):
+                 integrator="eu", group_launches=True, exact_solves=False):
         self.group_launches = bool(group_launches)
+        # the Helmholtz and Poisson line solves as the LDS-panel kernels that
+        # keep the reference's LU statement order (pypde_fdma_solve,
+        # pypde_poisson_solve), instead of the faster register recurrences
+        self.exact_solves = bool(exact_solves)
         if case not in ("rbc", "linear", "zero"):
             raise ValueError("Specified case is not available: ", ["rbc", "linear", "zero"])
         if integrator not in ("eu", "rk3"):
@@ -765,8 +771,17 @@ class RBCBatch:
         gy = _lines(p(X), p(X), M1, sX, M1, M0, 3 * B, 1)
         rx = {v: _rec(p(self.st_rhx[st]), M0, 4 * M0, v) for v in REC_VARIANTS}
         ry = {v: _rec(p(self.st_rhy[st]), M1, 4 * M1, v) for v in REC_VARIANTS}
-        L.append(_RecCall("helmholtz_x" + tag, gx, rx, _rec_default(3 * B * M1)))
-        L.append(_RecCall("helmholtz_y" + tag, gy, ry, _rec_default(3 * B * M0)))
+        if self.exact_solves:
+            for name, g, tab, M in (("helmholtz_x", gx, self.st_hx[st], M0),
+                                    ("helmholtz_y", gy, self.st_hy[st], M1)):
+                self._keep.append(g)
+                t = tab.data_ptr()
+                L.append(_Call(name + tag, lib.pypde_fdma_solve, ctypes.byref(g),
+                               ctypes.c_void_p(t), ctypes.c_void_p(t + 8 * M),
+                               ctypes.c_void_p(t + 16 * M), ctypes.c_void_p(t + 24 * M), 4 * M))
+        else:
+            L.append(_RecCall("helmholtz_x" + tag, gx, rx, _rec_default(3 * B * M1)))
+            L.append(_RecCall("helmholtz_y" + tag, gy, ry, _rec_default(3 * B * M0)))
         # 11  W2_f = X*_f [C ; H_y C]_f^T  (f = U with C_y, V with Cd_y): the y factors
         #     of the divergence and of the Poisson right-hand side in one product
         NH = N1 + M1
@@ -788,7 +803,13 @@ class RBCBatch:
             r.LM, r.R, r.MU, r.NU = (tabs[i].data_ptr() for i in range(4))
             r.per_line, r.entry_stride, r.variant = layout, 0, v
             rp[v] = r
-        L.append(_RecCall("poisson_lines" + tag, gp, rp, _rec_default(B * M1)))
+        if self.exact_solves:
+            self._keep.append(gp)
+            L.append(_Call("poisson_lines" + tag, lib.pypde_poisson_solve, ctypes.byref(gp),
+                           ctypes.c_void_p(self.ptab.data_ptr()),
+                           ctypes.c_void_p(self.psing.data_ptr()), 0))
+        else:
+            L.append(_RecCall("poisson_lines" + tag, gp, rp, _rec_default(B * M1)))
         # 16  P = RP Q_y^T with the gauge P[0,0] = 0 (GEMM epilogue), and
         # 17  W1 = P [S; T S; T D S/s]_y^T computed from RP directly (same launch)
         NW = N1 + 2 * M1

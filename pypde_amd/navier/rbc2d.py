@@ -28,10 +28,13 @@ class NavierStokes(NavierStokesBase, Integrator):
 
     avail_cases = ["rbc", "linear", "zero"]
 
-    def __init__(self, case="rbc", use_graph=True, **kwargs):
+    def __init__(self, case="rbc", use_graph=True, exact_solves=False, **kwargs):
         if case not in self.avail_cases:
             raise ValueError("Specified case is not available: ", self.avail_cases)
         self.case = case
+        # line solves as the kernels that keep the reference's LU statement
+        # order (slower than the default register recurrences)
+        self.exact_solves = exact_solves
         NavierStokesBase.__init__(self, **kwargs)
         Integrator.__init__(self)
         side = "CN" if case == "rbc" else "CD"
@@ -63,7 +66,8 @@ class NavierStokes(NavierStokesBase, Integrator):
             engine = RBCBatch(self.shape, [dict(ra=self.ra, pr=self.pr, dt=self.dt, nu=self.nu,
                                                 kappa=self.kappa)],
                               case=self.case, dealias=self.dealias, aspect=self.aspect,
-                              beta=self.beta, integrator=self.integrator)
+                              beta=self.beta, integrator=self.integrator,
+                              exact_solves=self.exact_solves)
         except NotImplementedError:
             # RK3 / beta != 1 with odd dealiased lengths: the op-by-op path
             engine = None

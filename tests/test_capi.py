@@ -102,3 +102,41 @@ def test_engine_launch_records_without_gpu():
     a = _lib.RbcRhs()
     plan.add(_Call("b", lib.pypde_rbc_rhs_banded, ctypes.byref(a)))
     assert plan.size() == 7
+
+
+def test_multi_launch_packing_without_gpu():
+    """The multi launches' host packing (no kernel runs): member ranges, the
+    work total and the argument checks."""
+    from pypde_amd import _lib
+    lib = _lib.lib
+    info = _lib.MultiInfo()
+    nb = lib.pypde_multi_table_bytes(1)
+    assert nb > 0 and lib.pypde_multi_table_bytes(9) == -1
+    host = (ctypes.c_ubyte * nb)()
+    descs = (_lib.GemmDesc * 3)()
+    for i, (m, n, b) in enumerate(((64, 64, 2), (0, 64, 1), (33, 65, 1))):
+        d = descs[i]
+        d.m, d.n, d.k, d.batch = m, n, 16, b
+        d.A = d.B = d.C = 4096
+        d.lda = d.ldb = d.ldc = 128
+    # 32 x 32 tiles: 2*2*2 + 0 + 2*3*1 = 14
+    assert lib.pypde_gemm_multi_pack(descs, 3, 2, host, nb, ctypes.byref(info)) == 0
+    assert (info.kind, info.count, info.variant, info.total) == (1, 3, 2, 14)
+    # a split-K variant has no multi form; a folded member is rejected
+    assert lib.pypde_gemm_multi_pack(descs, 3, 9, host, nb, ctypes.byref(info)) != 0
+    descs[2].fold = 1
+    assert lib.pypde_gemm_multi_pack(descs, 3, 2, host, nb, ctypes.byref(info)) != 0
+    assert b"folded" in lib.pypde_last_error()
+    # recurrences: lines per member, chunk code from the longest line
+    g = (_lib.Lines * 2)()
+    r = (_lib.Recurrence * 2)()
+    for i, (n, nl, b) in enumerate(((190, 64, 3), (62, 30, 2))):
+        g[i].src = g[i].dst = 4096
+        g[i].len_in = g[i].len_out = n
+        g[i].nlines, g[i].batch, g[i].axis = nl, b, 0
+        g[i].src_ld = g[i].dst_ld = nl
+        r[i].LM = r[i].R = r[i].MU = r[i].NU = 8192
+    nb = lib.pypde_multi_table_bytes(2)
+    host = (ctypes.c_ubyte * nb)()
+    assert lib.pypde_recurrence_multi_pack(g, r, 2, host, nb, ctypes.byref(info)) == 0
+    assert info.total == 64 * 3 + 30 * 2 and info.flags == (0 | (0 << 1) | (2 << 2))

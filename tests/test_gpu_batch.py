@@ -57,3 +57,63 @@ def test_default_dt_rule():
     assert default_dt(1e7) == pytest.approx(1e-3)
     assert default_dt(1e8) == pytest.approx(1e-3)
     assert default_dt(1e4) == pytest.approx(1e-2)
+
+
+def test_persistent_hetero_batch_matches_oracle_per_instance():
+    """The persistent form (every operator of the step one multi launch for all
+    size classes) against the oracle's single-instance runs, as above."""
+    from pypde_amd.batch import HeteroBatch, default_dt
+    from oracle import pypde_oracle as o
+    specs = [dict(shape=(20, 16), ra=1e4, pr=1.0), dict(shape=(24, 20), ra=3e5, pr=1.0),
+             dict(shape=(16, 24), ra=2e6, pr=0.7), dict(shape=(20, 16), ra=5e5, pr=1.0),
+             dict(shape=(24, 20), ra=1e5, pr=1.0)]
+    for s in specs:
+        s["dt"] = default_dt(s["ra"])
+    hb = HeteroBatch(specs, case="rbc", persistent=True)
+    assert len(hb.classes) == 3 and len(hb.multi) == 13
+    seeds = [100 + i for i in range(len(specs))]
+    hb.seed(seeds, amplitude=0.05)
+    hb.capture(1)
+    nsteps = 5
+    hb.step(nsteps)
+    for i, (s, sd) in enumerate(zip(specs, seeds)):
+        r = o.NavierStokes(case="rbc", shape=s["shape"], ra=s["ra"], pr=s["pr"], dt=s["dt"],
+                           tsave=None)
+        o.seeded_initial_state(r, seed=sd, amplitude=0.05)
+        for _ in range(nsteps):
+            r.update()
+        st = hb.state(i)
+        for f in ("U", "V", "T"):
+            e = rel(host(st[f]), getattr(r, f).vhat)
+            assert e < 1e-11, (i, f, e)
+        assert rel(host(st["pres"]), r.pres.vhat) < 1e-11, i
+
+
+def test_persistent_c4_batch_matches_branch_form():
+    """The 256-instance mixed-size batch (N per axis from {64, 96, 128, 192}):
+    13 launches per step for its 16 size classes, and after 5 steps every
+    instance equal to the stream-branch form within 1e-10 (the two forms may
+    run different GEMM tiles)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    from pypde_amd.batch import HeteroBatch
+    specs = bench.c4_specs()
+    seeds = [1000 + i for i in range(len(specs))]
+    out = []
+    for persistent in (True, False):
+        hb = HeteroBatch(specs, case="rbc", persistent=persistent)
+        if persistent:
+            assert len(hb.classes) == 16 and len(hb.multi) == 13
+        hb.seed(seeds, amplitude=1e-3)
+        hb.capture(1)
+        hb.step(5)
+        out.append([{f: host(t) for f, t in hb.state(i).items()} for i in range(len(specs))])
+        del hb
+    worst = 0.0
+    for a, b in zip(*out):
+        for f in ("U", "V", "T", "pres"):
+            worst = max(worst, rel(a[f], b[f]))
+    print("persistent vs branch form, worst relative difference %.2e" % worst)
+    assert worst < 1e-10

@@ -20,11 +20,14 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def test_c3_trajectory_512():
+@pytest.mark.parametrize("exact", [False, True])
+def test_c3_trajectory_512(exact):
     from pypde_amd.navier import NavierStokes
     ref = np.load(os.path.join(HERE, "golden", "rbc2d_c3_512.npz"))
     N, nsteps = 512, 2000
-    ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None)
+    ns = NavierStokes(case="rbc", shape=(N, N), ra=1e8, pr=1.0, dt=1e-3, tsave=None,
+                      exact_solves=exact)
+    print("exact line solves:", exact)
     rng = np.random.default_rng(0)
     ns.set_state(T=ns.T.forward(rng.uniform(-1e-3, 1e-3, size=(N, N))))
     nu, nv, ke = np.empty(nsteps), np.empty(nsteps), np.empty(nsteps)

@@ -97,3 +97,23 @@ def test_dense_transform_oracle_matches_reference_single_steps(golden):
     for f in ("T", "U", "V", "pres"):
         a, b = getattr(ns, f).vhat, golden[pre + "out1_" + f]
         assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b), f
+
+
+def test_c3_rounding_floor_fixture():
+    """The c3 trajectory (512x512, Ra=1e8, 2000 steps) of the oracle with
+    dense-product transforms (tests/golden/make_c3_floor.py) against the
+    reference's own run: two exact FP64 restatements differ by this much, the
+    floor on which the GPU series is judged.  The plate Nusselt number, a wall
+    derivative that amplifies the rounding of the high modes by about N^2, is
+    the most sensitive: up to about 1e-10 absolute throughout the run (4e-11
+    over the first 10 steps), while Nu_vol and KE stay within 1e-11."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    a = np.load(os.path.join(here, "rbc2d_c3_dense_oracle.npz"))
+    r = np.load(os.path.join(here, "rbc2d_c3_512.npz"))
+    assert a["nu_plate"].shape == r["c3_nu_plate"].shape == (2000,)
+    d = np.abs(a["nu_plate"] - r["c3_nu_plate"])
+    dv = np.abs(a["nu_vol"] - r["c3_nu_vol"]) / np.abs(r["c3_nu_vol"])
+    dk = np.abs(a["ke"] - r["c3_ke"]) / np.abs(r["c3_ke"])
+    assert 5e-11 < d.max() < 2e-10
+    assert dv[1:].max() < 1e-11 and dk[50:].max() < 1e-11
