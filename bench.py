@@ -312,10 +312,11 @@ class C2Chain:
         self.c, self.nf, self.N = c, nf, N
         ch = pypde_amd.Base(N, "CH")
         cd = pypde_amd.Base(N, "CD")
-        # parity-folded transforms (the cosine matrices are mirror-symmetric)
-        opB = ops.MirrorOperator(ch.backward_matrix())
-        opF = ops.MirrorOperator(ch.forward_matrix())
-        assert opB.mode and opF.mode
+        # the transforms in the kernel form tuned per shape: the parity-folded
+        # MFMA product or the LDS-staged cosine sum (ops.ChebTransform)
+        opB = ops.ChebTransform("bwd", N, ch.backward_matrix())
+        opF = ops.ChebTransform("fwd", N, ch.forward_matrix())
+        assert opB.mirror.mode and opF.mirror.mode
         s = cd.stencil_vector()
         Bm, _, Ax = hholtz_matrices(cd, 0.01, 1.0)
         rhs_x, rhs_y = PlanRHS(Bm, 2, 0), PlanRHS(Bm, 2, 1)
@@ -348,6 +349,14 @@ class C2Chain:
             ("K12_fdma_y", lambda: lhs_y.solve(b["x"]), 8 * 2 * M * M, None),
         ]
         self.bytes_per_field = sum(st[2] for st in self.stages)
+        self.transforms = (opB, opF)
+
+    def forms(self):
+        """The kernel form of each transform stage (tuned per shape)."""
+        opB, opF = self.transforms
+        b = self.bufs
+        return {"K1_backward_x": opB.form(self.c, 0), "K2_backward_y": opB.form(b["u1"], 1),
+                "K3_forward_x": opF.form(b["u2"], 0), "K4_forward_y": opF.form(b["c1"], 1)}
 
     def prepare(self, name):
         """Set-up that is not part of a kernel: the in-place solves start from
@@ -404,6 +413,9 @@ def microbench(nf=512, reps=10):
     for v in out.values():
         v["copy_frac"] = round(v["GBps"] / copy["GBps"], 4)
     chain_gbs = total_bytes / (total_us * 1e-6) / 1e9
+    forms = chain.forms()
+    for k, f in forms.items():
+        out[k]["form"] = "mfma" if f == "mfma" else "lds_dct/%d" % f
     return dict(fields=nf, N=N, kernels=out, chain_us=round(total_us, 1),
                 chain_GBps=round(chain_gbs, 1),
                 chain_hbm_frac=round(chain_gbs / HBM_PEAK_GBS, 4),

@@ -210,6 +210,20 @@ int pypde_twodma_solve(const pypde_lines* g, const double* d, const double* u, v
 int pypde_pdma_solve(const pypde_lines* g, const double* a, const double* b, const double* d,
                      const double* e, const double* f, void* stream);
 
+/* Chebyshev transform along an axis as an LDS-staged cosine sum on the FP64
+ * vector units, the alternative to the dense MFMA contraction (pypde_dgemm with
+ * the transform matrix; the autotuner picks one per shape):
+ *   dst[o] = a[o] * sum_i cos(pi o i / P) * b[i] * src[i],  o <= P (len_out = P + 1),
+ *   i < len_in <= P + 1,
+ * with ctab[m] = cos(pi m / P) for m < 2P.  a = 1, b[i] = (-1)^i is the backward
+ * transform of chebyshev.py:78-83; a[o] = w_o (-1)^o / (2P), b = (1, 2, .., 2, 1)
+ * the forward transform with the mass weights of chebyshev.py:67-76,143-149.
+ * Output pairs (o, P - o) come from one pass over even and odd i.  Not in place.
+ * variant: 1, 2, 3 (0 = 2) choose the register tiling (output pairs per wave,
+ * lines per lane). */
+int pypde_cheb_dct(const pypde_lines* g, int P, const double* a, const double* b,
+                   const double* ctab, int variant, void* stream);
+
 /* ---- register-resident recurrences (the hot path) ---------------------- */
 /* Banded line solve in one normalised form over positions i of each line:
  *   pre-stencil (pre != NULL)  y_i = u_i + pre_i u_{i+2}     (len_in = len_out + 2)
@@ -318,6 +332,9 @@ typedef struct pypde_rbc_rhs {
     const double* tx;
     const double* ty;
     int M0, M1, N0, N1, batch;
+    int rows_per_thread; /* kernel form: 1 one output per thread; 4 or 8 (0: 4) a strip
+                            of rows per thread sharing the y sums of its input rows
+                            (the same sums in the same order: bit-identical) */
 } pypde_rbc_rhs;
 
 int pypde_rbc_rhs_banded(const pypde_rbc_rhs* args, void* stream);

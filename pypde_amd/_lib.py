@@ -70,7 +70,7 @@ class RbcRhs(ctypes.Structure):
                 ("R", ctypes.c_void_p), ("mdta", ctypes.c_void_p), ("dta", ctypes.c_void_p),
                 ("fdiff", ctypes.c_void_p), ("tx", ctypes.c_void_p), ("ty", ctypes.c_void_p),
                 ("M0", ctypes.c_int), ("M1", ctypes.c_int), ("N0", ctypes.c_int),
-                ("N1", ctypes.c_int), ("batch", ctypes.c_int)]
+                ("N1", ctypes.c_int), ("batch", ctypes.c_int), ("rows_per_thread", ctypes.c_int)]
 
 
 class MultiInfo(ctypes.Structure):
@@ -153,6 +153,7 @@ _SIGS = {
     "pypde_products_multi": ([_vp, ctypes.c_void_p, _vp], _i),
     "pypde_rhs_multi_pack": ([ctypes.c_void_p, _i, _vp, ctypes.c_longlong, ctypes.c_void_p], _i),
     "pypde_rhs_multi": ([_vp, ctypes.c_void_p, _vp], _i),
+    "pypde_cheb_dct": ([_lp, _i, _vp, _vp, _vp, _i, _vp], _i),
     "pypde_plan_size": ([_vp], _i),
     "pypde_plan_region_bytes": ([_vp, _i], ctypes.c_longlong),
     "pypde_plan_run": ([_vp, ctypes.c_void_p, _i, _vp], _i),

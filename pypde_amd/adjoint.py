@@ -330,9 +330,15 @@ class NusseltFunctional:
         scale = Lz / 2.0
         self.a = w @ Gx                                   # [M0]
         self.b = (ev @ (Dy @ Gy)) / scale                 # [M1]
-        # constant part from the lifted boundary field Tbc
+        # constant part from the lifted boundary field Tbc, from its Chebyshev
+        # coefficients by the FFT (the dense forward transform's rounding in the
+        # high modes of this O(1) profile would reach the wall derivative)
+        from .navier.rbc2d_base import cheb_coeffs_fft
         tbc = e.Tbc.v.detach().cpu().numpy()
-        self.c0 = float(w @ (tbc @ (Dy.T @ ev)) / scale)
+        dy = cheb_coeffs_fft(tbc) @ deriv_matrix(N1).T / scale        # d/dy coefficients
+        walls = dy @ np.stack([(-1.0) ** np.arange(N1), np.ones(N1)], axis=1)  # y = -1, +1
+        vals = cheb_backward_matrix(N0, N0) @ walls                    # along x on the grid
+        self.c0 = float(-0.5 * Lz * (w @ vals).sum())
         self.ta = torch.as_tensor(self.a, dtype=ops.DTYPE, device=e.X.device)
         self.tb = torch.as_tensor(self.b, dtype=ops.DTYPE, device=e.X.device)
         self.gmat = torch.outer(self.ta, self.tb)         # dNu/dT  [M0, M1]

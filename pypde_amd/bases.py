@@ -183,6 +183,8 @@ class Chebyshev(MetaBase):
         if not mass:
             raise NotImplementedError("forward_fft(mass=False) is not provided")
         f = ops.to_dev(f)
+        if n_coef is None or n_coef == self.N:
+            return self._transform("fwd").apply(f, axis)
         return self._mirror(("fwd", n_coef), self.forward_matrix(n_coef)).apply(f, axis)
 
     def backward_fft(self, c, axis=0):
@@ -191,7 +193,17 @@ class Chebyshev(MetaBase):
         n = ops._len(c, axis)
         if n > self.N:
             raise AssertionError("Shape mismatch (%d) (%d)" % (n, self.N))
+        if n == self.N:
+            return self._transform("bwd").apply(c, axis)
         return self._mirror(("bwd", n), self.backward_matrix(n)).apply(c, axis)
+
+    def _transform(self, kind):
+        """Cached ops.ChebTransform (square transform, kernel form tuned per shape)."""
+        op = self._mops.get(("cheb", kind))
+        if op is None:
+            mat = self.forward_matrix() if kind == "fwd" else self.backward_matrix()
+            op = self._mops[("cheb", kind)] = ops.ChebTransform(kind, self.N, mat)
+        return op
 
     def derivative(self, fhat, deriv, out_cheby=True, axis=0, scale_pow=1.0):
         fhat = ops.to_dev(fhat)

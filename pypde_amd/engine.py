@@ -1013,6 +1013,16 @@ class RBCBatch:
                 launch.set_variant(autotune.choose(launch.key(), run, REC_VARIANTS,
                                                    default=launch.variant))
         for launch in self.launches:
+            if isinstance(launch, _Call) and launch.fn.__name__ == "pypde_rbc_rhs_banded":
+                a = launch.args[0]._obj
+
+                def run(v, launch=launch, a=a):
+                    a.rows_per_thread = v
+                    launch(s)
+                a.rows_per_thread = autotune.choose(
+                    ("rhs_banded", self.B, self.M0, self.M1, self.diffusion), run, (1, 4, 8),
+                    default=4)
+        for launch in self.launches:
             if isinstance(launch, _GemmGroup):
                 self._tune_group(launch, s, launch_fn)
         for i, launch in enumerate(self.launches):

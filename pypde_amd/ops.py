@@ -299,6 +299,83 @@ def bgemm(A, B, transA=False, transB=False, alpha=1.0, alpha_vec=None, beta=0.0,
     return out
 
 
+class ChebTransform:
+    """The square Chebyshev transform of N points along an axis ("bwd": values
+    from coefficients, chebyshev.py:78-83; "fwd": coefficients from values with
+    the mass weights, :67-76,143-149) in two kernel forms: the parity-folded
+    MFMA product with the transform matrix (MirrorOperator), and the
+    LDS-staged cosine sum on the vector units (pypde_cheb_dct, variants 1-3).
+    apply() runs the form the autotuner timed fastest for the call's shape
+    (the decision is stored with the others), so the choice is made per N,
+    axis and batch from measurements on the device."""
+
+    FORMS = ("mfma", 1, 2, 3)
+
+    def __init__(self, kind, N, matrix):
+        if kind not in ("bwd", "fwd"):
+            raise ValueError("kind must be 'bwd' or 'fwd'")
+        M = np.asarray(matrix.detach().cpu().numpy() if isinstance(matrix, torch.Tensor)
+                       else matrix, dtype=np.float64)
+        if M.shape != (N, N):
+            raise ValueError("a square N x N transform matrix is required")
+        self.kind, self.N, self.P = kind, N, N - 1
+        self.mirror = MirrorOperator(M)
+        P = self.P
+        k = np.arange(N)
+        sign = (-1.0) ** k
+        if kind == "bwd":
+            a, b = np.ones(N), sign
+        else:
+            w = np.full(N, 2.0)
+            w[0] = w[-1] = 1.0
+            a, b = w * sign / (2.0 * P), w
+        m = np.arange(2 * P)
+        ctab = np.cos(np.pi * m / P)
+        # the factorised form must be the matrix
+        C = ctab[(k[:, None] * k[None, :]) % (2 * P)]
+        err = np.abs(a[:, None] * C * b[None, :] - M).max()
+        if err > 1e-13 * np.abs(M).max():
+            raise ValueError("matrix is not the %s Chebyshev transform (%.3e)" % (kind, err))
+        self.a, self.b, self.ctab = to_dev(a), to_dev(b), to_dev(ctab)
+
+    def _dct(self, x3, o3, axis, v):
+        g = _geom(x3, o3, axis, self.N, self.N)
+        check(lib.pypde_cheb_dct(ctypes.byref(g), self.P, ptr(self.a), ptr(self.b), ptr(self.ctab),
+                                 v, stream_ptr()), "pypde_cheb_dct")
+
+    def form(self, x, axis):
+        """The kernel form used for x along axis (tuned on first use)."""
+        from . import autotune
+        key = ("cheb_transform", self.kind, self.N, axis, tuple(x.shape))
+        known = autotune.cached(key, None)
+        if known is not None:
+            return known
+        out = torch.empty(_out_shape(x, axis, self.N), dtype=DTYPE, device=x.device)
+        x3, o3 = _as3(x), _as3(out)
+
+        def run(f):
+            if f == "mfma":
+                self.mirror.apply(x, axis, out=out)
+            else:
+                self._dct(x3, o3, axis, f)
+        return autotune.choose(key, run, self.FORMS, default="mfma")
+
+    def apply(self, x, axis=0, out=None, form=None):
+        _check_axis(x, axis)
+        if _len(x, axis) != self.N:
+            raise AssertionError("Shape mismatch: transform has %d points, input axis has %d"
+                                 % (self.N, _len(x, axis)))
+        f = self.form(x, axis) if form is None else form
+        if f == "mfma" or x.stride(-1) != 1:
+            return self.mirror.apply(x, axis, out=out)
+        if out is None:
+            out = torch.empty(_out_shape(x, axis, self.N), dtype=DTYPE, device=x.device)
+        if out.data_ptr() == x.data_ptr():
+            raise ValueError("the transform cannot run in place")
+        self._dct(_as3(x), _as3(out), axis, f)
+        return out
+
+
 def launch_grouped(descs, tag):
     """Independent GEMM descriptors with the same transposes in one
     pypde_dgemm_grouped launch (at most four), the kernel variant tuned per

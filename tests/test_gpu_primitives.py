@@ -536,3 +536,28 @@ def test_split_fold_variants_reject_input_folds(amd):
                       p + 8 * 32 * 64, 8, 0)
         d.fold, d.fold_len, d.mirror_sign, d.mirror_div, d.variant = fold, 8, 1.0, 1, v
         assert lib.pypde_dgemm_batched(ctypes.byref(d), ops.stream_ptr()) != 0
+
+
+@pytest.mark.parametrize("N", [9, 24, 64, 65, 128, 256, 512])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_lds_dct_every_form_matches_oracle(amd, N, axis):
+    """The LDS-staged cosine-sum transform (pypde_cheb_dct) in each of its
+    kernel forms, and the MFMA form, against the oracle's scipy DCT-I
+    (chebyshev.py:67-93), backward and forward, on ragged batches along both
+    axes (odd and even N: with and without a middle output pair)."""
+    pypde_amd, ops, torch = amd
+    from oracle import pypde_oracle as o
+    rng = np.random.default_rng(N + 31 * axis)
+    b = pypde_amd.Base(N, "CH")
+    ob = o.Basis(N, "CH")
+    other = 37
+    shape = (3, N, other) if axis == 0 else (3, other, N)
+    x = rng.standard_normal(shape)
+    for kind in ("bwd", "fwd"):
+        op = b._transform(kind)
+        fn = ob._cheb_backward if kind == "bwd" else ob._cheb_forward
+        ref = np.stack([o._along(fn, x[e], axis) for e in range(3)])
+        for form in op.FORMS:
+            y = host(op.apply(ops.to_dev(x), axis, form=form))
+            e = rel(y, ref)
+            assert e < 1e-13, (kind, form, e)
