@@ -131,15 +131,18 @@ def launch_cost(name, launch, eng):
     if isinstance(launch, (_Gemm, _GemmGroup)):
         return "mfma", launch.flops
     B, N0, N1, M0, M1, L0, L1 = eng.B, eng.N0, eng.N1, eng.M0, eng.M1, eng.L0, eng.L1
+    prod = (3 * 3 * L0 * L1 + 3 * L0 * L1) * B + L0 * L1
+    rhs = (3 * M0 * M1 + N0 * N1 + 3 * M0 * M1 + 3 * M0 * M1) * B
     el = {
         # read 9 blocks + the lifted term, write 3 products
-        "products": (3 * 3 * L0 * L1 + 3 * L0 * L1) * B + L0 * L1,
+        "products": prod,
+        "products+rhs_banded": prod + rhs,
         "helmholtz_x": 2 * 3 * M0 * M1 * B,
         "helmholtz_y": 2 * 3 * M0 * M1 * B,
         # in/out plus the per-line factor tables, counted once per launch
         "poisson_lines": 2 * M0 * M1 * B + 4 * M0 * M1,
         # read X, pres and the constant terms, write R
-        "rhs_banded": (3 * M0 * M1 + N0 * N1 + 3 * M0 * M1 + 3 * M0 * M1) * B,
+        "rhs_banded": rhs,
     }.get(name, 0)
     return "hbm", 8.0 * el
 
@@ -423,6 +426,47 @@ def microbench(nf=512, reps=10):
                 device_copy=dict(copy, kernel="torch.mul(x, 1.0, out=y), 512 x 256 x 256 float64"))
 
 
+def transform_forms(sizes=(64, 128, 256, 512), reps=10):
+    """The square Chebyshev transforms in both kernel forms (the parity-folded
+    MFMA product and the LDS-staged cosine sums, ops.ChebTransform) at each N,
+    over batches of about 32 M elements (512 fields of 256 x 256), both axes,
+    backward and forward: microseconds per launch (median of reps, HIP events)
+    and the form the tuner keeps."""
+    import torch
+    import pypde_amd
+    from pypde_amd import autotune, ops
+    out = {}
+    for N in sizes:
+        nf = max(1, (512 * 256 * 256) // (N * N))
+        x = torch.as_tensor(c2_input(nf, N, seed=2)).to(ops.device())
+        ch = pypde_amd.Base(N, "CH")
+        res = {}
+        for kind in ("bwd", "fwd"):
+            op = ch._transform(kind)
+            for axis in (0, 1):
+                y = torch.empty_like(x)
+                t = {}
+                for form in op.FORMS:
+                    op.apply(x, axis, out=y, form=form)
+                    torch.cuda.synchronize()
+                    ts = []
+                    for _ in range(reps):
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        e0.record()
+                        op.apply(x, axis, out=y, form=form)
+                        e1.record()
+                        torch.cuda.synchronize()
+                        ts.append(e0.elapsed_time(e1) * 1e3)
+                    t["mfma" if form == "mfma" else "lds_dct/%d" % form] = round(float(np.median(ts)), 2)
+                chosen = op.form(x, axis)
+                res["%s_axis%d" % (kind, axis)] = dict(
+                    us=t, chosen="mfma" if chosen == "mfma" else "lds_dct/%d" % chosen,
+                    hbm_frac_best=round(16.0 * x.numel() / (min(t.values()) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
+        out[str(N)] = dict(fields=nf, forms=res)
+        del x
+    return out
+
+
 def time_variant(steps=100, N=256, **kw):
     """Steps/s of the fused N x N step (Ra=1e8, dt=1e-3) for another integrator /
     beta or size (graph replay)."""
@@ -533,6 +577,84 @@ def cpu_baseline(steps=6, warm=1):
                        (steps, warm), sec=round(sec, 2), host=host_cpu())
 
 
+def cpu_table(threads=(1, None)):
+    """The oracle on the host at every workload (one thread, then every core
+    through threadpoolctl; scipy's FFTs stay single-threaded as the
+    reference's): 64x64 (Ra=1e5, dt=0.01) over its full 1000 steps; the
+    512 x 256x256 transform/solve chain on a sample of fields, scaled to 512;
+    512x512 (Ra=1e8, dt=1e-3) 10 timed steps after 2; the mixed-size batch on
+    one instance per size class, scaled to the 256 instances; 128x128
+    (Ra=1e6, dt=1e-3) 20 steps after 2; 256x256 (Ra=1e8, dt=1e-3) 20 steps
+    after 5."""
+    from threadpoolctl import threadpool_limits
+    from oracle import pypde_oracle as o
+
+    def steps_rate(shape, ra, dt, steps, warm, seed=0):
+        ns = o.NavierStokes(case="rbc", shape=shape, ra=ra, pr=1.0, dt=dt, tsave=None)
+        o.seeded_initial_state(ns, seed=seed)
+        for _ in range(warm):
+            ns.update()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ns.update()
+        return steps / (time.perf_counter() - t0)
+
+    def chain_rate(nsample=16, nf=512, N=256):
+        c = c2_input(nsample, N, seed=1)
+        ch, cd = o.Basis(N, "CH"), o.Basis(N, "CD")
+        Bm, _, Ax = o.hholtz_matrices(cd, 0.01, 1.0)
+        along = o._along
+        mx, my, fx, fy = o.MatAxis(Bm, 0), o.MatAxis(Bm, 1), o.FdmaAxis(Ax, 0), o.FdmaAxis(Ax, 1)
+        t0 = time.perf_counter()
+        for f in range(nsample):
+            u1 = along(ch._cheb_backward, c[f], 0)
+            u2 = along(ch._cheb_backward, u1, 1)
+            c1 = along(ch._cheb_forward, u2, 0)
+            c2 = along(ch._cheb_forward, c1, 1)
+            g1 = along(cd.from_chebyshev, c2, 0)
+            g2 = along(cd.from_chebyshev, g1, 1)
+            along(cd.to_chebyshev, along(cd.to_chebyshev, g2, 0), 1)
+            fy(fx(my(mx(c2))))
+        return nsample / (time.perf_counter() - t0)
+
+    out = {}
+    for th in threads:
+        label = "1_thread" if th == 1 else "all_cores"
+        r = {}
+        with threadpool_limits(limits=th):
+            t0 = time.perf_counter()
+            r["c1_64x64_1000_steps"] = dict(steps_per_s=round(steps_rate((64, 64), 1e5, 0.01, 1000, 0), 2))
+            print("cpu table %s: c1 done (%.0f s)" % (label, time.perf_counter() - t0), file=sys.stderr, flush=True)
+            r["c2_chain_fields_per_s"] = dict(
+                value=round(chain_rate(), 2), sample="16 of the 512 fields, scaled",
+                batch_s=None)
+            r["c2_chain_fields_per_s"]["batch_s"] = round(512 / r["c2_chain_fields_per_s"]["value"], 2)
+            print("cpu table %s: c2 done" % label, file=sys.stderr, flush=True)
+            r["c3_512x512"] = dict(steps_per_s=round(steps_rate((512, 512), 1e8, 1e-3, 10, 2), 3),
+                                   sample="10 timed steps after 2")
+            print("cpu table %s: c3 done" % label, file=sys.stderr, flush=True)
+            specs = c4_specs()
+            classes = {}
+            for i, sp in enumerate(specs):
+                classes.setdefault(sp["shape"], []).append(i)
+            sec_per_step = {}
+            for shape, idx in classes.items():
+                sp = specs[idx[0]]
+                sec_per_step[shape] = 1.0 / steps_rate(shape, sp["ra"], sp["dt"], 4, 1, seed=1000 + idx[0])
+            total = sum(sec_per_step[sp["shape"]] for sp in specs)
+            r["c4_mixed_256"] = dict(instance_steps_per_s=round(len(specs) / total, 2),
+                                     sample="one instance per size class (%d classes), 4 steps "
+                                            "after 1, scaled to the 256 instances" % len(classes))
+            print("cpu table %s: c4 done" % label, file=sys.stderr, flush=True)
+            r["c5_128x128_forward"] = dict(steps_per_s=round(steps_rate((128, 128), 1e6, 1e-3, 20, 2), 2))
+            r["metric_256x256"] = dict(steps_per_s=round(steps_rate((256, 256), 1e8, 1e-3, 20, 5), 3))
+            r["sec"] = round(time.perf_counter() - t0, 1)
+            print("cpu table %s: done (%.0f s)" % (label, r["sec"]), file=sys.stderr, flush=True)
+        out[label] = r
+    out["host"] = host_cpu()
+    return out
+
+
 def host_cpu():
     """Model name and logical CPU count of the host the CPU baseline ran on."""
     model = None
@@ -556,6 +678,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="time the oracle on the host at every workload (one thread and all cores)")
+    ap.add_argument("--transforms", action="store_true",
+                    help="time both kernel forms of the square Chebyshev transforms at "
+                         "N = 64, 128, 256, 512")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the adjoint gradient at 256x256 (200 steps) only")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -587,6 +714,12 @@ def main():
     if args.micro:
         print(json.dumps({"microbench": microbench()}))
         return
+    if args.transforms:
+        print(json.dumps({"transforms": transform_forms()}))
+        return
+    if args.cpu_table:
+        print(json.dumps({"cpu_table": cpu_table()}))
+        return
     if args.adjoint:
         print(json.dumps({"adjoint": time_adjoint(256, 200, ckpt=100)}))
         return
@@ -608,6 +741,7 @@ def main():
         stage("beta", time_variant, steps=5, beta=0.5)
         stage("512", time_variant, steps=5, N=512)
         stage("microbench", microbench)
+        stage("transforms", transform_forms)
         for shape, ra in (((20, 16), 1e4), ((64, 64), 1e4), ((128, 128), 5e4)):
             stage("steady %s" % (shape,), time_steady, shape, ra, pre=20, tol=1e-3)
         stage("adjoint descent", time_adjoint_descent, 256, steps=2)
@@ -628,25 +762,39 @@ def main():
                 kernels = len(l.members) if isinstance(l, _GemmGroup) and l.split else 1
                 order.append([l.name, kernels])
         print(json.dumps({"launch_order": order}))
+    t_start = time.perf_counter()
+
+    def done(what):
+        print("bench: %s done (%.0f s)" % (what, time.perf_counter() - t_start), file=sys.stderr,
+              flush=True)
+    done("plain")
     extra = {}
     if not args.no_extra:
         b = time_batched(args)
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
+        done("batched")
         extra["hetero"] = time_hetero()
         extra["hetero_branches"] = time_hetero(persistent=False)
+        done("hetero")
         extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5),
                              time_variant(N=512)]
+        done("variants")
         extra["microbench"] = microbench()
+        extra["transforms"] = transform_forms()
+        done("microbench")
         extra["steady"] = [time_steady((20, 16), 1e4), time_steady((64, 64), 1e4),
                            time_steady((128, 128), 5e4)]
+        done("steady")
         extra["adjoint_descent"] = time_adjoint_descent(256)
         # checkpoints every 100 steps (the configured run), and every state kept
         # in HBM (400 MB at 256x256 over 200 steps), which removes the recomputation
         extra["adjoint"] = {"256x256": time_adjoint(256, 200), "128x128": time_adjoint(128, 2000),
                             "256x256_all_states": time_adjoint(256, 200, ckpt=1)}
+        done("adjoint")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(steps=args.cpu_steps)
+        done("cpu baseline")
     line = {
         "metric": "RBC2D FP64 time-steps/sec @256x256 (plain vs batched vs adjoint), % HBM roofline",
         "value": round(plain["steps_per_s"], 2),

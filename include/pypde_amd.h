@@ -265,9 +265,18 @@ typedef struct pypde_recurrence {
 int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurrence* r, void* stream);
 
 /* ---- pointwise --------------------------------------------------------- */
-/* y = alpha*x + beta*y over n elements. */
+/* y = alpha*x + beta*y over n elements (beta == 0: y is only written, as in
+ * BLAS, so it may hold anything on entry). */
 int pypde_axpby(long long n, double alpha, const double* x, double beta, double* y,
                 void* stream);
+
+/* dst[b][j*dst_ld + i] = src[b][i*src_ld + j] for i < rows, j < cols, b < batch
+ * (batch strides src_bs, dst_bs in elements; not in place).  The reference
+ * transposes by numpy's swapaxes (pypde/solver/plans.py:40-51); here it lets
+ * the solves along rows run as solves along columns. */
+int pypde_transpose(const double* src, long long src_ld, long long src_bs, double* dst,
+                    long long dst_ld, long long dst_bs, int rows, int cols, int batch,
+                    void* stream);
 
 /* ---- fused RBC2D step helpers ------------------------------------------ */
 /* Dealiased convective products of the U, V, T fields, with the lifted
@@ -338,6 +347,15 @@ typedef struct pypde_rbc_rhs {
 } pypde_rbc_rhs;
 
 int pypde_rbc_rhs_banded(const pypde_rbc_rhs* args, void* stream);
+
+/* pypde_rbc_products_yfold and pypde_rbc_rhs_banded (one output per thread:
+ * rhs->rows_per_thread must be 1) in one launch.  The two are independent --
+ * the products read the dealiased fields, the right-hand side the state -- so
+ * merged they overlap on the device; each output is computed exactly as by
+ * its own entry point.  Same reference interfaces as those two. */
+int pypde_rbc_products_rhs(const double* Y, const double* Yold, const double* dtbc, double* P,
+                           int L0, int L1, double sign_value, double sign_dy, double ratio,
+                           int batch, const pypde_rbc_rhs* rhs, void* stream);
 
 /* Discrete adjoint of the products with the c-term of the three-stage scheme:
  * the forward products are P_f = grad_f . (u + ratio u_old) (plus the lifted
@@ -428,8 +446,8 @@ int pypde_plan_create(int nregions, const void* const* bases, const long long* b
 int pypde_plan_destroy(pypde_plan* p);
 /* Recording: the arguments of pypde_dgemm_batched, pypde_dgemm_grouped,
  * pypde_recurrence_solve, pypde_rbc_products, pypde_rbc_products_yfold,
- * pypde_splitk_reduce, pypde_rbc_rhs_banded, pypde_fdma_solve and
- * pypde_poisson_solve, without the stream.  Arguments are checked when run. */
+ * pypde_splitk_reduce, pypde_rbc_rhs_banded, pypde_rbc_products_rhs, pypde_fdma_solve
+ * and pypde_poisson_solve, without the stream.  Arguments are checked when run. */
 int pypde_plan_add_gemm(pypde_plan* p, const pypde_gemm_desc* d);
 int pypde_plan_add_grouped(pypde_plan* p, const pypde_gemm_desc* descs, int count);
 int pypde_plan_add_recurrence(pypde_plan* p, const pypde_lines* g, const pypde_recurrence* r);
@@ -442,10 +460,33 @@ int pypde_plan_add_splitk_reduce(pypde_plan* p, const double* W, int S, const do
                                  const double* X, int xm, int xr, double* C, long long mn,
                                  int entries);
 int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* args);
+int pypde_plan_add_products_rhs(pypde_plan* p, const double* Y, const double* Yold,
+                                const double* dtbc, double* P, int L0, int L1,
+                                double sign_value, double sign_dy, double ratio, int batch,
+                                const pypde_rbc_rhs* rhs);
 int pypde_plan_add_fdma(pypde_plan* p, const pypde_lines* g, const double* l, const double* d,
                         const double* u1, const double* u2, long long table_bs);
 int pypde_plan_add_poisson(pypde_plan* p, const pypde_lines* g, const double* tab,
                            const int* sing, int transpose);
+/* A multi launch (the device table stays where it is; its members' pointers are
+ * not moved by a run). */
+int pypde_plan_add_multi(pypde_plan* p, const void* table, const pypde_multi_info* info);
+/* The pointwise launches of the step's adjoint: the arguments of pypde_axpby,
+ * pypde_transpose, pypde_scaled_sub, pypde_set_entries, pypde_rbc_products_vjp
+ * and pypde_rbc_products_vjp_c, without the stream. */
+int pypde_plan_add_axpby(pypde_plan* p, long long n, double alpha, const double* x, double beta,
+                         double* y);
+int pypde_plan_add_transpose(pypde_plan* p, const double* src, long long src_ld, long long src_bs,
+                             double* dst, long long dst_ld, long long dst_bs, int rows, int cols,
+                             int batch);
+int pypde_plan_add_scaled_sub(pypde_plan* p, long long per, int batch, const double* coef,
+                              double c2, const double* x, double* y);
+int pypde_plan_add_set_entries(pypde_plan* p, double* ptr, long long stride, int n, double value);
+int pypde_plan_add_products_vjp(pypde_plan* p, const double* Y, const double* dtbc,
+                                const double* Pb, double* Yb, int L0, int L1, int batch);
+int pypde_plan_add_products_vjp_c(pypde_plan* p, const double* Y, const double* Yold,
+                                  const double* dtbc, const double* Pb, double* Yb, double* Yob,
+                                  double ratio, int L0, int L1, int batch);
 /* Number of recorded launches (-1 for a null plan); bytes of a region (-1 if none). */
 int pypde_plan_size(const pypde_plan* p);
 long long pypde_plan_region_bytes(const pypde_plan* p, int region);
@@ -464,6 +505,31 @@ int pypde_plan_run(const pypde_plan* p, const void* const* bases, int n_steps, v
  * (navier/rbc2d.py:396-434, pypde/solver/integrator.py:14-41). */
 int pypde_rbc_step(const pypde_plan* plan, double* X, double* pres, double* P, void* workspace,
                    int n_steps, void* stream);
+
+/* n_steps of a heterogeneous batch (instances of several shapes, each size
+ * class with its own state and workspace) by a plan of multi launches, one per
+ * operator of the step for all classes, recorded on the classes' buffers:
+ * runs them in place there.  Same reference interface as pypde_rbc_step. */
+int pypde_rbc_step_batched(const pypde_plan* plan, int n_steps, void* stream);
+
+/* The vector-Jacobian product of one time step (the discrete adjoint: the
+ * step's launches reversed, with transposed operators), by a plan whose
+ * regions are, in order,
+ *   0  X     [B, 3, M0, M1]  the step's input state (read only)
+ *   1  pres  [B, N0, N1]     its pressure (read only)
+ *   2  Xb    [B, 3, M0, M1]  in: cotangent of the step's output X; out: of its input X
+ *   3  pb    [B, N0, N1]     in: cotangent of the output pres; out: of the input pres
+ *   4 ... workspace regions, carved one after the other from `workspace`
+ *         (each start rounded up to 256 bytes; pypde_rbc_step_vjp_workspace_bytes
+ *         gives the total; any contents).
+ * Euler and the three-stage scheme (whose stage states the plan recomputes in
+ * the workspace), any implicit fraction.  No reference counterpart: the
+ * reference differentiates nothing (navier/rbc2d_adj.py:321-356 is an
+ * adjoint-descent iteration of a continuous adjoint); this is the gradient
+ * of objectives of NavierStokes.update trajectories (navier/rbc2d.py:396-434). */
+long long pypde_rbc_step_vjp_workspace_bytes(const pypde_plan* plan);
+int pypde_rbc_step_vjp(const pypde_plan* plan, const double* X, const double* pres, double* Xb,
+                       double* pb, void* workspace, void* stream);
 
 /* pypde_rbc_step without a caller workspace: the plan's workspace region is
  * moved to a device array the library reserves, PYPDE_SCRATCH_BYTES (48 MiB)

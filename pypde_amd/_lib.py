@@ -141,6 +141,10 @@ _SIGS = {
     "pypde_plan_add_fdma": ([_vp, _lp, _vp, _vp, _vp, _vp, ctypes.c_longlong], _i),
     "pypde_plan_add_poisson": ([_vp, _lp, _vp, _vp, _i], _i),
     "pypde_rbc_rhs_banded": ([ctypes.c_void_p, _vp], _i),
+    "pypde_rbc_products_rhs": ([_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, ctypes.c_void_p,
+                                _vp], _i),
+    "pypde_plan_add_products_rhs": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i,
+                                     ctypes.c_void_p], _i),
     "pypde_multi_table_bytes": ([_i], ctypes.c_longlong),
     "pypde_gemm_multi_pack": ([ctypes.c_void_p, _i, _i, _vp, ctypes.c_longlong, ctypes.c_void_p],
                               _i),
@@ -154,6 +158,17 @@ _SIGS = {
     "pypde_rhs_multi_pack": ([ctypes.c_void_p, _i, _vp, ctypes.c_longlong, ctypes.c_void_p], _i),
     "pypde_rhs_multi": ([_vp, ctypes.c_void_p, _vp], _i),
     "pypde_cheb_dct": ([_lp, _i, _vp, _vp, _vp, _i, _vp], _i),
+    "pypde_plan_add_multi": ([_vp, _vp, ctypes.c_void_p], _i),
+    "pypde_rbc_step_batched": ([_vp, _i, _vp], _i),
+    "pypde_transpose": ([_vp, _ll, _ll, _vp, _ll, _ll, _i, _i, _i, _vp], _i),
+    "pypde_plan_add_axpby": ([_vp, _ll, _d, _vp, _d, _vp], _i),
+    "pypde_plan_add_transpose": ([_vp, _vp, _ll, _ll, _vp, _ll, _ll, _i, _i, _i], _i),
+    "pypde_plan_add_scaled_sub": ([_vp, _ll, _i, _vp, _d, _vp, _vp], _i),
+    "pypde_plan_add_set_entries": ([_vp, _vp, _ll, _i, _d], _i),
+    "pypde_plan_add_products_vjp": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _i], _i),
+    "pypde_plan_add_products_vjp_c": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _i, _i], _i),
+    "pypde_rbc_step_vjp_workspace_bytes": ([_vp], _ll),
+    "pypde_rbc_step_vjp": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pypde_plan_size": ([_vp], _i),
     "pypde_plan_region_bytes": ([_vp, _i], ctypes.c_longlong),
     "pypde_plan_run": ([_vp, ctypes.c_void_p, _i, _vp], _i),

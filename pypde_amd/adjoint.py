@@ -56,24 +56,50 @@ class StepAdjoint:
     pres_s, X_(s-1)) to (X_(s+1), pres_(s+1)) -- the previous stage's state
     enters through the velocities of the c-term (rbc2d.py:261-283) -- and the
     step's VJP runs the stage VJPs in reverse, adding each c-term cotangent to
-    the stage before."""
+    the stage before.
+
+    The VJP runs on buffers of its own: the step input (X, pres) and the
+    cotangents (Xb, pb, transformed in place) in `io`, every intermediate in
+    one workspace arena, and for the three-stage scheme the engine's state
+    and workspace, on which the stage states are recomputed.  Its launches
+    are recorded once (after one op-by-op pass that tunes every kernel
+    choice); vjp() replays them, and build_plan() records them into a step
+    plan of the C-ABI (pypde_rbc_step_vjp)."""
 
     def __init__(self, eng: RBCBatch):
         self.e = eng
         e = eng
-        B, N0, N1, M0, M1, L0, L1 = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1
+        B, N0, N1, M0, M1, L0, L1, K = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1, e.K
         dev = e.X.device
         f64 = dict(dtype=ops.DTYPE, device=dev)
-        self.Xb_buf = torch.zeros((B, 3, M0, M1), **f64)
-        self.Y1 = torch.zeros((B, 3, 2 * L0, M1), **f64)
-        self.Y2 = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
-        self.Y2b = torch.zeros((B, 3, 2 * L0, 2 * L1), **f64)
+        NW = N1 + 2 * M1
+        self.io = dict(X=torch.zeros((B, 3, M0, M1), **f64), pres=torch.zeros((B, N0, N1), **f64),
+                       Xb=torch.zeros((B, 3, M0, M1), **f64), pb=torch.zeros((B, N0, N1), **f64))
+        shapes = [("Wb", (B, M0, NW)), ("divb", (B, N0, N1)), ("Pb", (B, M0, M1)),
+                  ("XPb", (B, M0, M1)), ("Q1b", (B, M0, N1)), ("Wdb", (B, 2 * M0, N1)),
+                  ("Rb", (B, 3, M0, M1)), ("Rt", (3 * B, M1, M0)), ("Zb", (B, 3, K, M1)),
+                  ("Prb", (B, 3, L0, L1)), ("Y1", (B, 3, 2 * L0, M1)),
+                  ("Y2", (B, 3, 2 * L0, 2 * L1)), ("Y2b", (B, 3, 2 * L0, 2 * L1)),
+                  ("y1b", (B, 3, 2 * L0, M1))]
         if e.nstage > 1:
-            # the previous stage's fields and their cotangents (c-term)
-            self.Y2p = torch.zeros_like(self.Y2)
-            self.Y2pb = torch.zeros_like(self.Y2)
+            # the previous stage's fields and their cotangents (c-term), the
+            # recomputed stage inputs, and the c-term cotangents (two, used in turn)
+            shapes += [("Y2p", (B, 3, 2 * L0, 2 * L1)), ("Y2pb", (B, 3, 2 * L0, 2 * L1))]
+            for st in range(1, e.nstage):
+                shapes += [("Xs%d" % st, (B, 3, M0, M1)), ("ps%d" % st, (B, N0, N1))]
+            shapes += [("cb0", (B, 3, M0, M1)), ("cb1", (B, 3, M0, M1))]
+        total = sum(int(np.prod(sh)) for _, sh in shapes)
+        self.ws = torch.zeros(total, **f64)
+        self.v, at = {}, 0
+        for name, sh in shapes:
+            n = int(np.prod(sh))
+            self.v[name] = self.ws[at:at + n].view(sh)
+            at += n
+        self.Y1, self.Y2, self.Y2b = self.v["Y1"], self.v["Y2"], self.v["Y2b"]
         self.t_mnub = torch.as_tensor(-e.nu * e.beta, **f64)
         self.graph = None
+        self.launches = None
+        self.plan = None
         # transposed solves on reversed lines (four-diagonal recurrences): the
         # Helmholtz tables per stage (their lambda is dt a_s beta nu), the
         # Poisson lines once
@@ -119,11 +145,9 @@ class StepAdjoint:
             r.variant = v
             return r
 
-        stream = ops.stream_ptr()
-
         def run(g, v):
-            check(lib.pypde_recurrence_solve(ctypes.byref(g), ctypes.byref(struct(v)), stream),
-                  "transposed line solve")
+            ops.call("transposed line solve", lib.pypde_recurrence_solve, ctypes.byref(g),
+                     ctypes.byref(struct(v)))
         if variant is not None:
             run(geom(x), variant)
             return
@@ -131,7 +155,10 @@ class StepAdjoint:
         if autotune.enabled() and not autotune._capturing() and key not in autotune.cache():
             scratch = torch.zeros_like(x)  # kept alive while the candidates run on it
             gs = geom(scratch)
-            autotune.choose(key, lambda v: run(gs, v), (1, 2, 3), default=2)
+            stream = ops.stream_ptr()
+            autotune.choose(key, lambda v: check(lib.pypde_recurrence_solve(
+                ctypes.byref(gs), ctypes.byref(struct(v)), stream), "transposed line solve"),
+                (1, 2, 3), default=2)
             torch.cuda.synchronize()
             del scratch
         run(geom(x), autotune.cache().get(key, 2))
@@ -143,7 +170,6 @@ class StepAdjoint:
         e = self.e
         Y2 = self.Y2 if Y2 is None else Y2
         B3, M0, M1, L0, L1 = 3 * e.B, e.M0, e.M1, e.L0, e.L1
-        X = X.contiguous()
         _fields_gemm(0, 0, 2 * L0, M1, M0, B3, e.GGx.data_ptr(), M0, ops._bi(2 * L0 * M0, m1=3),
                      X.data_ptr(), M1, ops._bi(M0 * M1), self.Y1.data_ptr(), M1,
                      ops._bi(2 * L0 * M1))
@@ -156,7 +182,7 @@ class StepAdjoint:
         e = self.e
         B3, M0, M1, L0, L1 = 3 * e.B, e.M0, e.M1, e.L0, e.L1
         bi = ops._bi
-        y1b = torch.empty((e.B, 3, 2 * L0, M1), dtype=ops.DTYPE, device=out.device)
+        y1b = self.v["y1b"]
         _fields_gemm(0, 0, 2 * L0, M1, 2 * L1, B3, Y2b.data_ptr(), 2 * L1, bi(4 * L0 * L1),
                      e.GGy.data_ptr(), M1, bi(0), y1b.data_ptr(), M1, bi(2 * L0 * M1))
         _fields_gemm(1, 0, M0, M1, 2 * L0, B3, e.GGx.data_ptr(), M0, bi(2 * L0 * M0, m1=3),
@@ -164,44 +190,55 @@ class StepAdjoint:
                      beta=1.0 if accumulate else 0.0, D=out.data_ptr() if accumulate else None,
                      iD=bi(M0 * M1))
 
-    def vjp(self, X, pres, Xb, pb):
-        """Cotangents of the step input (X, pres) from those of its output.  For
-        the three-stage scheme the stage inputs are recomputed from (X, pres)
-        by the engine's stage launches (on its own buffers)."""
-        e = self.e
+    # ---- the launch sequence ---------------------------------------------
+    def _copy(self, src, dst):
+        ops.call("copy", lib.pypde_axpby, src.numel(), 1.0, _vp(src), 0.0, _vp(dst))
+
+    def _ops(self):
+        """The step VJP op by op on the adjoint's buffers: io["Xb"], io["pb"]
+        become the cotangents of io["X"], io["pres"].  For the three-stage scheme
+        the stage inputs are first recomputed from (X, pres) by the engine's
+        stage launches, on the engine's buffers."""
+        e, io, v = self.e, self.io, self.v
+        Xb, pb = io["Xb"], io["pb"]
         if e.nstage == 1:
-            Xin, pin, _ = self._vjp_stage(0, X, pres, None, Xb, pb)
-            return Xin, pin
-        states = [(X, pres)]
-        saved = [t.clone() for t in (e.X, e.pres, e.P)]
-        e.X.copy_(X)
-        e.pres.copy_(pres)
+            self._vjp_stage(0, io["X"], io["pres"], None, Xb, pb, None)
+            return
+        s = ops.stream_ptr()
+        self._copy(io["X"], e.X)
+        self._copy(io["pres"], e.pres)
+        states = [(io["X"], io["pres"])]
         for st in range(e.nstage - 1):
-            e.run_stage(st)
-            states.append((e.X.clone(), e.pres.clone()))
-        for t, v in zip((e.X, e.pres, e.P), saved):
-            t.copy_(v)
+            for f in e.launches:
+                if e.stage_of(f) == st:
+                    f(s)
+                    ops.record(f)
+            self._copy(e.X, v["Xs%d" % (st + 1)])
+            self._copy(e.pres, v["ps%d" % (st + 1)])
+            states.append((v["Xs%d" % (st + 1)], v["ps%d" % (st + 1)]))
         carry = None   # c-term cotangent of the stage input, from the stage after
         for st in range(e.nstage - 1, -1, -1):
             Xs, ps = states[st]
             Xprev = states[st - 1][0] if st > 0 else None
-            Xb, pb, cb = self._vjp_stage(st, Xs, ps, Xprev, Xb, pb)
+            cb = v["cb%d" % (st % 2)]
+            has_c = self._vjp_stage(st, Xs, ps, Xprev, Xb, pb, cb)
             if carry is not None:
-                Xb = Xb + carry
-            carry = cb
-        return Xb, pb
+                ops.call("c-term cotangent", lib.pypde_axpby, Xb.numel(), 1.0, _vp(carry), 1.0,
+                         _vp(Xb))
+            carry = cb if has_c else None
 
-    def _vjp_stage(self, st, X, pres, Xprev, Xb, pb):
-        """VJP of stage st: cotangents of its input (X, pres) and, for a stage
-        with a c-term, of the previous stage's state Xprev (else None)."""
-        e = self.e
+    def _vjp_stage(self, st, X, pres, Xprev, Xb, pb, cb):
+        """VJP of stage st, in place on the cotangents (Xb, pb) of its output,
+        which become those of its input (X, pres); for a stage with a c-term,
+        cb receives the cotangent of the previous stage's state Xprev (returns
+        whether it did)."""
+        e, v = self.e, self.v
         B, N0, N1, M0, M1, L0, L1, K = e.B, e.N0, e.N1, e.M0, e.M1, e.L0, e.L1, e.K
-        f64 = dict(dtype=ops.DTYPE, device=X.device)
         g = ops.bgemm
+        bi = ops._bi
         cterm = Xprev is not None and float(e.kc[st]) != 0.0
         # 21  velocity correction: W1b = -A_c0^T Ub, W2b = -A_c1^T Vb
-        NW = N1 + 2 * M1
-        Wb = torch.empty((B, M0, NW), **f64)
+        Wb = v["Wb"]
         ops.launch_grouped([
             g(e.Snx, pb, transA=True, alpha_vec=e.st_idta[st], out=Wb[:, :, :N1],
               defer=True)[1],  # 20
@@ -209,49 +246,57 @@ class StepAdjoint:
               defer=True)[1],
             g(e.Acorr[1], Xb[:, 1], transA=True, alpha=-1.0, out=Wb[:, :, N1 + M1:],
               defer=True)[1]], "vjp_correction")
-        divb = torch.zeros((B, N0, N1), **f64)
-        check(lib.pypde_scaled_sub(N0 * N1, B, _vp(e.t_nu), e.beta, _vp(pb), _vp(divb),
-                                   ops.stream_ptr()), "pres_div_vjp")
+        # 19  div^ = -nu beta pres^ (written whole: the axpby with beta 0 reads no divb)
+        divb = v["divb"]
+        ops.call("pres_div_vjp", lib.pypde_axpby, divb.numel(), 0.0, _vp(pb), 0.0, _vp(divb))
+        ops.call("pres_div_vjp", lib.pypde_scaled_sub, N0 * N1, B, _vp(e.t_nu), e.beta, _vp(pb),
+                 _vp(divb))
         # 18  P^ = W^ Wy
-        Pb = g(Wb, e.Wy)
+        Pb = g(Wb, e.Wy, out=v["Pb"])
         # 17  gauge
-        check(lib.pypde_set_entries(_vp(Pb), M0 * M1, B, 0.0, ops.stream_ptr()), "gauge_vjp")
+        ops.call("gauge_vjp", lib.pypde_set_entries, _vp(Pb), M0 * M1, B, 0.0)
         # 16  XP^ = P^ Qy ; 15 transposed Poisson line solves ; 14 Q1^ = RP^ Hy
-        XPb = g(Pb, e.Qy)
+        XPb = g(Pb, e.Qy, out=v["XPb"])
         self._reversed_solve(XPb, M0, M1, B, self.prt, True)
-        Q1b = g(XPb, e.Hy)
+        Q1b = g(XPb, e.Hy, out=v["Q1b"])
         # 13  div^ += Bx^T Q1^
         g(e.Bx, Q1b, transA=True, beta=1.0, D=divb, out=divb)
         # 12  Wd^ = Cx^T div^   [B, 2M0, N1]
-        Wdb = g(e.Cx, divb, transA=True)
+        Wdb = g(e.Cx, divb, transA=True, out=v["Wdb"])
         # 11  U*^ = Ub + Wd^_U C_y, V*^ = Vb + Wd^_V Cd_y, T*^ = Tb
-        Rb = torch.empty((B, 3, M0, M1), **f64)
+        Rb = v["Rb"]
         ops.launch_grouped([
             g(Wdb[:, :M0], e.Cy[0], beta=1.0, D=Xb[:, 0], out=Rb[:, 0], defer=True)[1],
             g(Wdb[:, M0:], e.Cy[1], beta=1.0, D=Xb[:, 1], out=Rb[:, 1], defer=True)[1]],
             "vjp_div")
-        Rb[:, 2].copy_(Xb[:, 2])
         # 9-10  transposed Helmholtz ADI: R^ = Hx^-T X*^ Hy^-1, both as
-        # four-diagonal recurrences over reversed lines (tables of J H^T J)
-        R3 = Rb.reshape(3 * B, M0, M1)
-        # along y: on the transposed fields, so that the lines are columns
-        Rt = R3.transpose(1, 2).contiguous()
-        self._reversed_solve(Rt, M1, M0, 3 * B, self.hrt_y_st[st], False)
-        R3.copy_(Rt.transpose(1, 2))
+        # four-diagonal recurrences over reversed lines (tables of J H^T J);
+        # along y on the transposed fields, so that the lines are columns (the
+        # transposes take U*^, V*^ from Rb and T*^ = Tb from Xb)
+        Rt = v["Rt"].view(B, 3, M1, M0)
+        ops.call("vjp_transpose", lib.pypde_transpose, _vp(Rb), M1, 3 * M0 * M1, _vp(Rt), M0,
+                 3 * M0 * M1, M0, M1, B)
+        ops.call("vjp_transpose", lib.pypde_transpose, ctypes.c_void_p(Rb[:, 1].data_ptr()), M1,
+                 3 * M0 * M1, ctypes.c_void_p(Rt[:, 1].data_ptr()), M0, 3 * M0 * M1, M0, M1, B)
+        ops.call("vjp_transpose", lib.pypde_transpose, ctypes.c_void_p(Xb[:, 2].data_ptr()), M1,
+                 3 * M0 * M1, ctypes.c_void_p(Rt[:, 2].data_ptr()), M0, 3 * M0 * M1, M0, M1, B)
+        self._reversed_solve(v["Rt"], M1, M0, 3 * B, self.hrt_y_st[st], False)
+        ops.call("vjp_transpose", lib.pypde_transpose, _vp(v["Rt"]), M0, M0 * M1, _vp(Rb), M1,
+                 M0 * M1, M1, M0, 3 * B)
+        R3 = Rb.view(3 * B, M0, M1)
         self._reversed_solve(R3, M0, M1, 3 * B, self.hrt_st[st], False)
         # 7  Z^_f = A_f^T R^_f ; 8 buoyancy Z^_T[2] += dt a BST_x^T R^_V
-        Zb = torch.empty((B, 3, K, M1), **f64)
-        bi = ops._bi
+        Zb = v["Zb"]
         _fields_gemm(1, 0, K, M1, M0, 3 * B, e.Arhs.data_ptr(), K, bi(M0 * K, m1=3),
                      Rb.data_ptr(), M1, bi(M0 * M1), Zb.data_ptr(), M1, bi(K * M1))
         zt = Zb[:, 2, L0 + N0:L0 + N0 + M0]
         g(e.BSTx, Rb[:, 1], transA=True, alpha_vec=e.st_dta[st], beta=1.0, D=zt, out=zt)
-        # 6  X^ (input) = Z^_f[2] BS_y  and  4  P^r_f = -dt b Z^_f[0] BF_y (one launch)
-        Xin = torch.empty((B, 3, M0, M1), **f64)
-        Prb = torch.empty((B, 3, L0, L1), **f64)
+        # 6  X^ (input) = Z^_f[2] BS_y  and  4  P^r_f = -dt b Z^_f[0] BF_y (one
+        #    launch); the input cotangent overwrites Xb, no longer read
+        Prb = v["Prb"]
         ops.launch_grouped([
             _fields_gemm(0, 0, M0, M1, M1, 3 * B, Zb.data_ptr() + 8 * (L0 + N0) * M1, M1,
-                         bi(K * M1), e.BSy.data_ptr(), M1, bi(0), Xin.data_ptr(), M1,
+                         bi(K * M1), e.BSy.data_ptr(), M1, bi(0), Xb.data_ptr(), M1,
                          bi(M0 * M1), defer=True),
             _fields_gemm(0, 0, L0, L1, M1, 3 * B, Zb.data_ptr(), M1, bi(K * M1),
                          e.BFy.data_ptr(), L1, bi(0), Prb.data_ptr(), L1, bi(L0 * L1),
@@ -262,37 +307,85 @@ class StepAdjoint:
             for blk in range(2):
                 _fields_gemm(0, 0, M0, M1, M1, 3 * B,
                              Zb.data_ptr() + 8 * (L0 + N0 + (1 + blk) * M0) * M1, M1,
-                             bi(K * M1), e.DFy[blk].data_ptr(), M1, bi(0), Xin.data_ptr(), M1,
+                             bi(K * M1), e.DFy[blk].data_ptr(), M1, bi(0), Xb.data_ptr(), M1,
                              bi(M0 * M1), alpha_vec=e.st_fdiff[st], alpha_div=1, beta=1.0,
-                             D=Xin.data_ptr(), iD=bi(M0 * M1))
-        # 5  pres^ (input) = pres^ - dt a (Z^_U[1] PBy_0 + Z^_V[1] PBy_1)
-        pin = pb.clone()
-        g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.st_mdta[st], beta=1.0, D=pin, out=pin)
-        g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.st_mdta[st], beta=1.0, D=pin, out=pin)
+                             D=Xb.data_ptr(), iD=bi(M0 * M1))
+        # 5  pres^ (input) = pres^ - dt a (Z^_U[1] PBy_0 + Z^_V[1] PBy_1), in place
+        g(Zb[:, 0, L0:L0 + N0], e.PBy[0], alpha_vec=e.st_mdta[st], beta=1.0, D=pb, out=pb)
+        g(Zb[:, 1, L0:L0 + N0], e.PBy[1], alpha_vec=e.st_mdta[st], beta=1.0, D=pb, out=pb)
         # 3  products (with the c-term: the previous stage's velocities too)
         self.forward_fields(X)
-        Xprev_b = None
         if cterm:
-            self.forward_fields(Xprev, self.Y2p)
+            self.forward_fields(Xprev, v["Y2p"])
             ratio = float(e.kc[st]) / float(e.kb[st])
-            check(lib.pypde_rbc_products_vjp_c(_vp(self.Y2), _vp(self.Y2p), _vp(e.dTbcdz1),
-                                               _vp(Prb), _vp(self.Y2b), _vp(self.Y2pb), ratio,
-                                               L0, L1, B, ops.stream_ptr()), "products_vjp_c")
-            Xprev_b = torch.empty((B, 3, M0, M1), **f64)
-            self.fields_vjp(self.Y2pb, Xprev_b, accumulate=False)
+            ops.call("products_vjp_c", lib.pypde_rbc_products_vjp_c, _vp(self.Y2), _vp(v["Y2p"]),
+                     _vp(e.dTbcdz1), _vp(Prb), _vp(self.Y2b), _vp(v["Y2pb"]), ratio, L0, L1, B)
+            self.fields_vjp(v["Y2pb"], cb, accumulate=False)
         else:
-            check(lib.pypde_rbc_products_vjp(_vp(self.Y2), _vp(e.dTbcdz1), _vp(Prb),
-                                             _vp(self.Y2b), L0, L1, B, ops.stream_ptr()),
-                  "products_vjp")
+            ops.call("products_vjp", lib.pypde_rbc_products_vjp, _vp(self.Y2), _vp(e.dTbcdz1),
+                     _vp(Prb), _vp(self.Y2b), L0, L1, B)
         # 2  Y1^ = Y2^ GGy ; 1 X^ += GG_f^T Y1^
-        self.fields_vjp(self.Y2b, Xin, accumulate=True)
-        return Xin, pin, Xprev_b
+        self.fields_vjp(self.Y2b, Xb, accumulate=True)
+        return cterm
+
+    def _build(self):
+        """Record the launch sequence (after one op-by-op pass that tunes every
+        kernel choice on the adjoint's buffers)."""
+        if self.launches is None:
+            saved = [t.clone() for t in (self.e.X, self.e.pres, self.e.P)]
+            self._ops()
+            with ops.record_launches() as L:
+                self._ops()
+            for t, s in zip((self.e.X, self.e.pres, self.e.P), saved):
+                t.copy_(s)
+            self.launches = L
+        return self.launches
+
+    def build_plan(self):
+        """The step VJP as a step plan of the C-ABI: regions X, pres, Xb, pb,
+        the adjoint's workspace and, for the three-stage scheme, the engine's
+        X, pres, P and workspace, on which the stage states are recomputed
+        (pypde_rbc_step_vjp carves the regions from 4 on out of one caller
+        workspace)."""
+        from .engine import Plan
+        if self.plan is None:
+            e, io = self.e, self.io
+            regions = [(io[k].data_ptr(), 8 * io[k].numel()) for k in ("X", "pres", "Xb", "pb")]
+            regions.append((self.ws.data_ptr(), 8 * self.ws.numel()))
+            if e.nstage > 1:
+                regions += [(t.data_ptr(), 8 * t.numel()) for t in (e.X, e.pres, e.P)]
+                regions.append((e.ws.data_ptr(), e.workspace_bytes()))
+            n = len(regions)
+            bases = (ctypes.c_void_p * n)(*[b for b, _ in regions])
+            sizes = (ctypes.c_longlong * n)(*[z for _, z in regions])
+            plan = Plan(n, bases, sizes)
+            for l in self._build():
+                plan.add(l)
+            self.plan = plan
+        return self.plan
+
+    def vjp(self, X, pres, Xb, pb):
+        """Cotangents of the step input (X, pres) from those of its output (the
+        recorded launches on the adjoint's buffers; new tensors).  The engine's
+        own state is left as it was."""
+        e, io = self.e, self.io
+        self._build()
+        saved = [t.clone() for t in (e.X, e.pres, e.P)] if e.nstage > 1 else None
+        for k, t in zip(("X", "pres", "Xb", "pb"), (X, pres, Xb, pb)):
+            io[k].copy_(t)
+        s = ops.stream_ptr()
+        for f in self.launches:
+            f(s)
+        if saved is not None:
+            for t, v in zip((e.X, e.pres, e.P), saved):
+                t.copy_(v)
+        return io["Xb"].clone(), io["pb"].clone()
 
     def vjp_replay(self, X, pres, Xb, pb):
         """vjp() replayed from a HIP graph (captured on the first call, after one
-        op-by-op call that tunes every kernel choice).  The same launches in the
-        same order, so the results are vjp()'s bit for bit.  The returned tensors
-        are the graph's output buffers: valid until the next call."""
+        call that records the launches).  The same launches in the same order,
+        so the results are vjp()'s bit for bit.  The returned tensors are the
+        graph's output buffers: valid until the next call."""
         if self.graph is None:
             self.gin = [t.clone() for t in (X, pres, Xb, pb)]
             self.vjp(*self.gin)

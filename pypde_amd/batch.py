@@ -259,12 +259,26 @@ class HeteroBatch:
         m._keep = args
         return m
 
+    def build_plan(self):
+        """The persistent step as a step plan of the C-ABI (its multi launches,
+        run in place on the classes' buffers by pypde_rbc_step_batched)."""
+        from .engine import Plan
+        if not self.persistent:
+            raise RuntimeError("only the persistent form has a step plan")
+        if getattr(self, "plan", None) is None:
+            plan = Plan(0, None, None)
+            for m in self.multi:
+                check(lib.pypde_plan_add_multi(plan.handle, ctypes.c_void_p(m.table.data_ptr()),
+                                               ctypes.byref(m.info)), m.name)
+            self.plan = plan
+        return self.plan
+
     # ---- stepping ---------------------------------------------------------
     def step_eager(self):
         if self.persistent:
             s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-            for launch in self.multi:
-                launch(s)
+            check(lib.pypde_rbc_step_batched(self.build_plan().handle, 1, s),
+                  "pypde_rbc_step_batched")
             return
         for e in self.classes:
             e.step_eager()

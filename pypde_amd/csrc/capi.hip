@@ -23,7 +23,7 @@ __global__ void axpby_kernel(long long n, double alpha, const double* __restrict
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
-        y[i] = alpha * x[i] + beta * y[i];
+        y[i] = beta == 0.0 ? alpha * x[i] : alpha * x[i] + beta * y[i];   // beta 0: y not read
 }
 
 }  // namespace pypde

@@ -19,7 +19,8 @@
 namespace pypde {
 
 enum PlanOpKind { OP_GEMM, OP_GROUPED, OP_RECURRENCE, OP_PRODUCTS, OP_PRODUCTS_YFOLD, OP_REDUCE,
-                  OP_RHS_BANDED, OP_FDMA, OP_POISSON };
+                  OP_RHS_BANDED, OP_FDMA, OP_POISSON, OP_MULTI, OP_AXPBY, OP_TRANSPOSE,
+                  OP_SCALED_SUB, OP_SET_ENTRIES, OP_PRODUCTS_VJP, OP_PRODUCTS_RHS };
 
 struct PlanOp {
     PlanOpKind kind;
@@ -31,15 +32,23 @@ struct PlanOp {
     const int* sing;
     long long table_bs;
     int transpose;
-    // products: Y, Yold (yfold only), dtbc, P; reduce: W, D, X, C
+    const void* multi_table;             // OP_MULTI: device argument table and its info
+    pypde_multi_info multi;
+    // products: Y, Yold (yfold only), dtbc, P; reduce: W, D, X, C;
+    // products VJP: Y, Yold, dtbc, Pb, Yb, Yob; axpby / scaled_sub: x, y (coef in p2);
+    // transpose: src, dst; set_entries: p3
     const double* p0;
     const double* p1;
     const double* p2;
     double* p3;
+    double* p4;
+    double* p5;
     int L0, L1, batch;
     double sign_value, sign_dy, ratio;
     int S, xm, xr, entries;
     long long mn;
+    long long ld0, ld1, bs0, bs1;        // transpose strides
+    double alpha, beta;                  // axpby; scaled_sub c2 in alpha; set_entries value
 };
 
 }  // namespace pypde
@@ -120,8 +129,8 @@ static int run_op(const pypde_plan* pl, const intptr_t* dl, bool moved,
                                         relocate(pl, dl, op.p2), relocate(pl, dl, op.p3), op.L0,
                                         op.L1, op.sign_value, op.sign_dy, op.ratio, op.batch,
                                         stream);
-    case OP_RHS_BANDED: {
-        if (!moved) return pypde_rbc_rhs_banded(&op.rhs, stream);
+    case OP_RHS_BANDED:
+    case OP_PRODUCTS_RHS: {
         pypde_rbc_rhs a = op.rhs;
         a.X = relocate(pl, dl, a.X);
         a.pres = relocate(pl, dl, a.pres);
@@ -132,7 +141,11 @@ static int run_op(const pypde_plan* pl, const intptr_t* dl, bool moved,
         a.fdiff = relocate(pl, dl, a.fdiff);
         a.tx = relocate(pl, dl, a.tx);
         a.ty = relocate(pl, dl, a.ty);
-        return pypde_rbc_rhs_banded(&a, stream);
+        if (op.kind == OP_RHS_BANDED) return pypde_rbc_rhs_banded(&a, stream);
+        return pypde_rbc_products_rhs(relocate(pl, dl, op.p0), relocate(pl, dl, op.p1),
+                                      relocate(pl, dl, op.p2), relocate(pl, dl, op.p3), op.L0,
+                                      op.L1, op.sign_value, op.sign_dy, op.ratio, op.batch, &a,
+                                      stream);
     }
     case OP_FDMA: {
         pypde_lines g = op.lines;
@@ -149,6 +162,37 @@ static int run_op(const pypde_plan* pl, const intptr_t* dl, bool moved,
         return pypde_poisson_solve(&g, relocate(pl, dl, op.tab[0]), relocate(pl, dl, op.sing),
                                    op.transpose, stream);
     }
+    case OP_MULTI:
+        // the members' pointers live in the device table: used as recorded
+        switch (op.multi.kind) {
+        case PYPDE_MULTI_GEMM: return pypde_dgemm_multi(op.multi_table, &op.multi, stream);
+        case PYPDE_MULTI_RECURRENCE: return pypde_recurrence_multi(op.multi_table, &op.multi, stream);
+        case PYPDE_MULTI_PRODUCTS: return pypde_products_multi(op.multi_table, &op.multi, stream);
+        case PYPDE_MULTI_RHS: return pypde_rhs_multi(op.multi_table, &op.multi, stream);
+        default: break;
+        }
+        set_error("step plan: unknown multi launch kind %d", op.multi.kind);
+        return ERR_ARG;
+    case OP_AXPBY:
+        return pypde_axpby(op.mn, op.alpha, relocate(pl, dl, op.p0), op.beta,
+                           relocate(pl, dl, op.p3), stream);
+    case OP_TRANSPOSE:
+        return pypde_transpose(relocate(pl, dl, op.p0), op.ld0, op.bs0, relocate(pl, dl, op.p3),
+                               op.ld1, op.bs1, op.L0, op.L1, op.batch, stream);
+    case OP_SCALED_SUB:
+        return pypde_scaled_sub(op.mn, op.batch, relocate(pl, dl, op.p2), op.alpha,
+                                relocate(pl, dl, op.p0), relocate(pl, dl, op.p3), stream);
+    case OP_SET_ENTRIES:
+        return pypde_set_entries(relocate(pl, dl, op.p3), op.mn, op.entries, op.alpha, stream);
+    case OP_PRODUCTS_VJP:
+        if (!op.p1)
+            return pypde_rbc_products_vjp(relocate(pl, dl, op.p0), relocate(pl, dl, op.p2),
+                                          relocate(pl, dl, op.p3), relocate(pl, dl, op.p4), op.L0,
+                                          op.L1, op.batch, stream);
+        return pypde_rbc_products_vjp_c(relocate(pl, dl, op.p0), relocate(pl, dl, op.p1),
+                                        relocate(pl, dl, op.p2), relocate(pl, dl, op.p3),
+                                        relocate(pl, dl, op.p4), relocate(pl, dl, op.p5), op.ratio,
+                                        op.L0, op.L1, op.batch, stream);
     case OP_REDUCE:
         return pypde_splitk_reduce(relocate(pl, dl, op.p0), op.S, relocate(pl, dl, op.p1),
                                    relocate(pl, dl, op.p2), op.xm, op.xr, relocate(pl, dl, op.p3),
@@ -299,6 +343,28 @@ extern "C" int pypde_plan_add_rhs_banded(pypde_plan* p, const pypde_rbc_rhs* arg
     return add_op(p, std::move(op));
 }
 
+extern "C" int pypde_plan_add_products_rhs(pypde_plan* p, const double* Y, const double* Yold,
+                                           const double* dtbc, double* P, int L0, int L1,
+                                           double sign_value, double sign_dy, double ratio,
+                                           int batch, const pypde_rbc_rhs* rhs)
+{
+    PYPDE_REQUIRE(p && rhs, "null argument");
+    PlanOp op{};
+    op.kind = OP_PRODUCTS_RHS;
+    op.p0 = Y;
+    op.p1 = Yold;
+    op.p2 = dtbc;
+    op.p3 = P;
+    op.L0 = L0;
+    op.L1 = L1;
+    op.sign_value = sign_value;
+    op.sign_dy = sign_dy;
+    op.ratio = ratio;
+    op.batch = batch;
+    op.rhs = *rhs;
+    return add_op(p, std::move(op));
+}
+
 extern "C" int pypde_plan_add_fdma(pypde_plan* p, const pypde_lines* g, const double* l,
                                    const double* d, const double* u1, const double* u2,
                                    long long table_bs)
@@ -326,6 +392,121 @@ extern "C" int pypde_plan_add_poisson(pypde_plan* p, const pypde_lines* g, const
     op.sing = sing;
     op.transpose = transpose;
     return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_multi(pypde_plan* p, const void* table, const pypde_multi_info* info)
+{
+    PYPDE_REQUIRE(p && table && info, "null argument");
+    PlanOp op{};
+    op.kind = OP_MULTI;
+    op.multi_table = table;
+    op.multi = *info;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_axpby(pypde_plan* p, long long n, double alpha, const double* x,
+                                    double beta, double* y)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_AXPBY;
+    op.mn = n;
+    op.alpha = alpha;
+    op.beta = beta;
+    op.p0 = x;
+    op.p3 = y;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_transpose(pypde_plan* p, const double* src, long long src_ld,
+                                        long long src_bs, double* dst, long long dst_ld,
+                                        long long dst_bs, int rows, int cols, int batch)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_TRANSPOSE;
+    op.p0 = src;
+    op.p3 = dst;
+    op.ld0 = src_ld;
+    op.bs0 = src_bs;
+    op.ld1 = dst_ld;
+    op.bs1 = dst_bs;
+    op.L0 = rows;
+    op.L1 = cols;
+    op.batch = batch;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_scaled_sub(pypde_plan* p, long long per, int batch,
+                                         const double* coef, double c2, const double* x, double* y)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_SCALED_SUB;
+    op.mn = per;
+    op.batch = batch;
+    op.p2 = coef;
+    op.alpha = c2;
+    op.p0 = x;
+    op.p3 = y;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_set_entries(pypde_plan* p, double* ptr, long long stride, int n,
+                                          double value)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_SET_ENTRIES;
+    op.p3 = ptr;
+    op.mn = stride;
+    op.entries = n;
+    op.alpha = value;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_products_vjp(pypde_plan* p, const double* Y, const double* dtbc,
+                                           const double* Pb, double* Yb, int L0, int L1, int batch)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PlanOp op{};
+    op.kind = OP_PRODUCTS_VJP;
+    op.p0 = Y;
+    op.p1 = nullptr;
+    op.p2 = dtbc;
+    op.p3 = const_cast<double*>(Pb);
+    op.p4 = Yb;
+    op.L0 = L0;
+    op.L1 = L1;
+    op.batch = batch;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_plan_add_products_vjp_c(pypde_plan* p, const double* Y, const double* Yold,
+                                             const double* dtbc, const double* Pb, double* Yb,
+                                             double* Yob, double ratio, int L0, int L1, int batch)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PYPDE_REQUIRE(Yold, "the c-term form needs the previous stage's fields");
+    PlanOp op{};
+    op.kind = OP_PRODUCTS_VJP;
+    op.p0 = Y;
+    op.p1 = Yold;
+    op.p2 = dtbc;
+    op.p3 = const_cast<double*>(Pb);
+    op.p4 = Yb;
+    op.p5 = Yob;
+    op.ratio = ratio;
+    op.L0 = L0;
+    op.L1 = L1;
+    op.batch = batch;
+    return add_op(p, std::move(op));
+}
+
+extern "C" int pypde_rbc_step_batched(const pypde_plan* p, int n_steps, void* stream)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    return pypde_plan_run(p, nullptr, n_steps, stream);
 }
 
 extern "C" int pypde_plan_size(const pypde_plan* p)
@@ -427,6 +608,37 @@ extern "C" int pypde_rbc_step_noalloc(const pypde_plan* p, double* X, double* pr
     // step, so the contents left by an earlier call need no clearing
     const void* bases[4] = {X, pres, P, ws};
     return pypde_plan_run(p, bases, n_steps, stream);
+}
+
+namespace pypde {
+constexpr int kVjpFixedRegions = 4;   // X, pres, Xb, pb
+static long long align256(long long b) { return (b + 255) / 256 * 256; }
+}  // namespace pypde
+
+extern "C" long long pypde_rbc_step_vjp_workspace_bytes(const pypde_plan* p)
+{
+    if (!p || (int)p->base.size() <= kVjpFixedRegions) return -1;
+    long long total = 0;
+    for (size_t r = kVjpFixedRegions; r < p->base.size(); ++r) total += align256(p->bytes[r]);
+    return total;
+}
+
+extern "C" int pypde_rbc_step_vjp(const pypde_plan* p, const double* X, const double* pres,
+                                  double* Xb, double* pb, void* workspace, void* stream)
+{
+    PYPDE_REQUIRE(p, "null plan");
+    PYPDE_REQUIRE((int)p->base.size() > kVjpFixedRegions,
+                  "a step VJP plan has the regions X, pres, Xb, pb and at least one workspace "
+                  "region (got %zu regions)", p->base.size());
+    PYPDE_REQUIRE(X && pres && Xb && pb && workspace, "null state, cotangent or workspace pointer");
+    const void* bases[PYPDE_PLAN_MAX_REGIONS] = {X, pres, Xb, pb};
+    uintptr_t at = (uintptr_t)workspace;
+    PYPDE_REQUIRE(at % 256 == 0, "the workspace must be 256-byte aligned");
+    for (size_t r = kVjpFixedRegions; r < p->base.size(); ++r) {
+        bases[r] = (const void*)at;
+        at += (uintptr_t)align256(p->bytes[r]);
+    }
+    return pypde_plan_run(p, bases, 1, stream);
 }
 
 extern "C" int pypde_rbc_step(const pypde_plan* p, double* X, double* pres, double* P,

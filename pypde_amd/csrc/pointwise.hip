@@ -1,6 +1,7 @@
 // Pointwise kernels of the fused RBC2D step.
 #include "common.h"
 #include "../../include/pypde_amd.h"
+#include "rhs_impl.h"
 
 #include <cstring>
 
@@ -289,12 +290,64 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ W, int S, const 
     }
 }
 
+// dst[b][j][i] = src[b][i][j] over i < rows, j < cols: 32 x 32 tiles through
+// LDS (one padding column, so the column reads hit distinct banks); loads and
+// stores each run along rows of 32 consecutive doubles.
+constexpr int kTrTile = 32;
+
+__global__ void __launch_bounds__(256) transpose_kernel(const double* __restrict__ src,
+                                                        long long src_ld, long long src_bs,
+                                                        double* __restrict__ dst, long long dst_ld,
+                                                        long long dst_bs, int rows, int cols)
+{
+    __shared__ double t[kTrTile][kTrTile + 1];
+    const int b = blockIdx.z;
+    const int i0 = blockIdx.y * kTrTile, j0 = blockIdx.x * kTrTile;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    const double* s = src + (long long)b * src_bs;
+    double* d = dst + (long long)b * dst_bs;
+#pragma unroll
+    for (int r = ty; r < kTrTile; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < rows && j < cols) t[r][tx] = s[(long long)i * src_ld + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < kTrTile; r += 8) {
+        const int j = j0 + r, i = i0 + tx;
+        if (i < rows && j < cols) d[(long long)j * dst_ld + i] = t[tx][r];
+    }
+}
+
 static unsigned grid_for(long long n)
 {
     long long b = (n + 255) / 256;
     if (b > 8192) b = 8192;
     if (b < 1) b = 1;
     return (unsigned)b;
+}
+
+// The banded right-hand side (blocks [0, nbr), one output per thread) and the
+// y-folded products (blocks [nbr, nbr + nbp)) in one launch: the two read
+// disjoint inputs (the state; the dealiased fields) and write disjoint outputs,
+// so they overlap on the device instead of running one after the other.  The
+// right-hand side, the longer of the two, gets the first blocks.  Each output
+// is computed exactly as by its own kernel.
+__global__ void __launch_bounds__(256) rbc_products_rhs_kernel(ProdMember q, pypde_rbc_rhs a,
+                                                               unsigned nbr, unsigned nbp)
+{
+    if (blockIdx.x < nbr) {
+        const long long total = 3LL * a.M0 * a.M1 * a.batch;
+        for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total;
+             t += (long long)nbr * 256)
+            rhs_banded_at(a.X, a.pres, a.cst, a.R, a.mdta, a.dta, a.fdiff, a.tx, a.ty, a.M0,
+                          a.M1, a.N0, a.N1, t);
+        return;
+    }
+    const long long total = (long long)q.L0 * (q.L1 / 2) * q.batch;
+    for (long long t = (long long)(blockIdx.x - nbr) * 256 + threadIdx.x; t < total;
+         t += (long long)nbp * 256)
+        products_yfold_at(q.Y, q.Yold, q.dtbc, q.P, q.L0, q.L1, q.sv, q.sd, q.ratio, t);
 }
 
 }  // namespace pypde
@@ -323,6 +376,36 @@ extern "C" int pypde_rbc_products_yfold(const double* Y, const double* Yold, con
     hipLaunchKernelGGL(rbc_products_yfold_kernel, dim3(grid_for((long long)L0 * (L1 / 2) * batch)), dim3(256), 0,
                        as_stream(stream), Y, Yold, dtbc, P, L0, L1, sign_value, sign_dy, ratio, batch);
     return check_launch("pypde_rbc_products_yfold");
+}
+
+extern "C" int pypde_rbc_products_rhs(const double* Y, const double* Yold, const double* dtbc,
+                                      double* P, int L0, int L1, double sign_value,
+                                      double sign_dy, double ratio, int batch,
+                                      const pypde_rbc_rhs* rhs, void* stream)
+{
+    PYPDE_REQUIRE(Y && dtbc && P && rhs, "null pointer");
+    PYPDE_REQUIRE(L0 > 0 && L1 > 0 && L1 % 2 == 0 && batch >= 0,
+                  "invalid geometry (L1 must be even: the y-folded layout)");
+    PYPDE_REQUIRE(rhs->X && rhs->pres && rhs->R && rhs->mdta && rhs->dta && rhs->tx && rhs->ty,
+                  "null right-hand-side pointer (X, pres, R, mdta, dta, tx and ty are required)");
+    PYPDE_REQUIRE(rhs->M0 >= 1 && rhs->M1 >= 1 && rhs->N0 == rhs->M0 + 2 &&
+                      rhs->N1 == rhs->M1 + 2 && rhs->batch >= 0,
+                  "invalid right-hand-side geometry: M0 %d M1 %d N0 %d N1 %d batch %d", rhs->M0,
+                  rhs->M1, rhs->N0, rhs->N1, rhs->batch);
+    PYPDE_REQUIRE(rhs->R != rhs->X, "R must not alias X (neighbouring entries are read)");
+    PYPDE_REQUIRE(rhs->rows_per_thread == 1,
+                  "the merged launch computes one right-hand-side output per thread "
+                  "(rows_per_thread must be 1, got %d)", rhs->rows_per_thread);
+    const long long tr = 3LL * rhs->M0 * rhs->M1 * rhs->batch;
+    const long long tp = (long long)L0 * (L1 / 2) * batch;
+    if (tr + tp == 0) return OK;
+    long long nbr = (tr + 255) / 256;
+    if (nbr > 16384) nbr = 16384;
+    const unsigned nbp = tp ? grid_for(tp) : 0;
+    const ProdMember q{Y, Yold, dtbc, P, L0, L1, batch, sign_value, sign_dy, ratio};
+    hipLaunchKernelGGL(rbc_products_rhs_kernel, dim3((unsigned)nbr + nbp), dim3(256), 0,
+                       as_stream(stream), q, *rhs, (unsigned)nbr, nbp);
+    return check_launch("pypde_rbc_products_rhs");
 }
 
 extern "C" int pypde_products_multi_pack(const double* const* Y, const double* const* Yold,
@@ -406,6 +489,25 @@ extern "C" int pypde_scaled_sub(long long per, int batch, const double* coef, do
     hipLaunchKernelGGL(scaled_sub_kernel, dim3(grid_for(per * batch)), dim3(256), 0,
                        as_stream(stream), per, batch, coef, c2, x, y);
     return check_launch("pypde_scaled_sub");
+}
+
+extern "C" int pypde_transpose(const double* src, long long src_ld, long long src_bs, double* dst,
+                               long long dst_ld, long long dst_bs, int rows, int cols, int batch,
+                               void* stream)
+{
+    PYPDE_REQUIRE(src && dst, "null pointer");
+    PYPDE_REQUIRE(src != dst, "the transpose cannot run in place");
+    PYPDE_REQUIRE(rows >= 0 && cols >= 0 && batch >= 0, "negative size");
+    if ((long long)rows * cols * batch == 0) return OK;
+    PYPDE_REQUIRE(src_ld >= cols && dst_ld >= rows,
+                  "leading dimensions too small (src_ld %lld < %d or dst_ld %lld < %d)", src_ld,
+                  cols, dst_ld, rows);
+    PYPDE_REQUIRE(batch < 65536, "batch must be below 65536 (got %d)", batch);
+    const dim3 grid((cols + kTrTile - 1) / kTrTile, (rows + kTrTile - 1) / kTrTile, batch);
+    PYPDE_REQUIRE(grid.y < 65536, "too many rows (%d)", rows);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, as_stream(stream), src, src_ld, src_bs,
+                       dst, dst_ld, dst_bs, rows, cols);
+    return check_launch("pypde_transpose");
 }
 
 extern "C" int pypde_set_entries(double* p, long long stride, int n, double value, void* stream)

@@ -291,8 +291,16 @@ class Plan:
             "pypde_rbc_products_yfold": "pypde_plan_add_products_yfold",
             "pypde_splitk_reduce": "pypde_plan_add_splitk_reduce",
             "pypde_rbc_rhs_banded": "pypde_plan_add_rhs_banded",
+            "pypde_rbc_products_rhs": "pypde_plan_add_products_rhs",
             "pypde_fdma_solve": "pypde_plan_add_fdma",
-            "pypde_poisson_solve": "pypde_plan_add_poisson"}
+            "pypde_poisson_solve": "pypde_plan_add_poisson",
+            "pypde_recurrence_solve": "pypde_plan_add_recurrence",
+            "pypde_axpby": "pypde_plan_add_axpby",
+            "pypde_transpose": "pypde_plan_add_transpose",
+            "pypde_scaled_sub": "pypde_plan_add_scaled_sub",
+            "pypde_set_entries": "pypde_plan_add_set_entries",
+            "pypde_rbc_products_vjp": "pypde_plan_add_products_vjp",
+            "pypde_rbc_products_vjp_c": "pypde_plan_add_products_vjp_c"}
 
     def __init__(self, nregions, bases, sizes):
         h = ctypes.c_void_p()
@@ -300,9 +308,23 @@ class Plan:
         self.handle = h
 
     def add(self, launch):
-        """Record one launch of the step (a _Gemm, _GemmGroup, _RecCall or _Call)."""
+        """Record one launch (a _Gemm, _GemmGroup, _RecCall, _Call, _SplitRhs or
+        an ops.Recorded)."""
         h = self.handle
-        if isinstance(launch, _Gemm):
+        if isinstance(launch, _SplitRhs):
+            self.add(launch.group)
+            self.add(launch.reduce)
+        elif isinstance(launch, ops.Recorded):
+            if launch.kind == "gemm":
+                check(lib.pypde_plan_add_gemm(h, ctypes.byref(launch.arr)), launch.name)
+            elif launch.kind == "group":
+                check(lib.pypde_plan_add_grouped(h, ctypes.byref(launch.arr), launch.n),
+                      launch.name)
+            elif launch.fn.__name__ in Plan._ADD:
+                check(getattr(lib, Plan._ADD[launch.fn.__name__])(h, *launch.args), launch.name)
+            else:
+                raise TypeError("launch %r cannot be recorded in a step plan" % launch.name)
+        elif isinstance(launch, _Gemm):
             check(lib.pypde_plan_add_gemm(h, ctypes.byref(launch.desc)), launch.name)
         elif isinstance(launch, _GemmGroup):
             if launch.split:
@@ -1022,6 +1044,8 @@ class RBCBatch:
                 a.rows_per_thread = autotune.choose(
                     ("rhs_banded", self.B, self.M0, self.M1, self.diffusion), run, (1, 4, 8),
                     default=4)
+        if self.group_launches:
+            self._merge_products_rhs(s)
         for launch in self.launches:
             if isinstance(launch, _GemmGroup):
                 self._tune_group(launch, s, launch_fn)
@@ -1044,6 +1068,35 @@ class RBCBatch:
         for t, v in zip((self.X, self.pres, self.P), saved):
             t.copy_(v)
         torch.cuda.synchronize()
+
+    def _merge_products_rhs(self, s):
+        """Per stage, the y-folded products and the banded right-hand side
+        (one output per thread) in one launch (pypde_rbc_products_rhs) where
+        the tuner times that faster than the two launches; the merged launch
+        takes the products' place (the right-hand side reads only the
+        stage-start state, which no launch in between changes)."""
+        from . import autotune
+        L = self.launches
+        for st in range(self.nstage):
+            tag = "" if self.nstage == 1 else "[%d]" % st
+            names = {l.name: i for i, l in enumerate(L)}
+            ip, ir = names.get("products" + tag), names.get("rhs_banded" + tag)
+            if ip is None or ir is None or ip > ir:
+                continue
+            prod, rhs = L[ip], L[ir]
+            if prod.fn.__name__ != "pypde_rbc_products_yfold" or rhs.args[0]._obj.rows_per_thread != 1:
+                continue
+            merged = _Call("products+rhs_banded" + tag, lib.pypde_rbc_products_rhs,
+                           *(prod.args + rhs.args))
+            forms = ((prod, rhs), (merged,))
+
+            def run(c):
+                for f in forms[c]:
+                    f(s)
+            if autotune.choose(("products_rhs", self.B, self.L0, self.L1, self.M0, self.M1,
+                                self.diffusion, st > 0), run, (0, 1), default=1) == 1:
+                L[ip] = merged
+                del L[ir]
 
     def _tune_group(self, launch, s, launch_fn):
         """Variant of a grouped launch, or separate launches when faster."""

@@ -90,14 +90,71 @@ def _bi(s1, d1=1, m1=0, s2=0, d2=1, m2=0):
     return b
 
 
+# ---------------------------------------------------------------------------
+# Launch recording.  While a recorder is active, the GEMM launches made here
+# (single descriptors and groups, each with the kernel variant the tuner chose)
+# and the library calls made through call() are appended to it as well, so
+# that a sequence of launches can be replayed without its Python logic or
+# recorded into a step plan of the C-ABI.
+_recorders = []
+
+
+class Recorded:
+    """One recorded launch: kind "gemm" (one descriptor), "group" (descriptors
+    of one grouped launch) or "call" (a library function with its arguments;
+    the stream is passed when run)."""
+
+    def __init__(self, kind, name, fn=None, args=(), descs=()):
+        from ._lib import GemmDesc
+        self.kind, self.name, self.fn, self.args = kind, name, fn, args
+        self.arr = (GemmDesc * max(1, len(descs)))(*[GemmDesc.from_buffer_copy(d) for d in descs])
+        self.n = len(descs)
+
+    def __call__(self, stream):
+        if self.kind == "gemm":
+            check(lib.pypde_dgemm_batched(ctypes.byref(self.arr), stream), self.name)
+        elif self.kind == "group":
+            check(lib.pypde_dgemm_grouped(ctypes.byref(self.arr), self.n, stream), self.name)
+        else:
+            check(self.fn(*self.args, stream), self.name)
+
+
+class record_launches:
+    """Context manager collecting the launches made inside it (a list of
+    Recorded, plus any launch objects passed to record())."""
+
+    def __enter__(self):
+        self.launches = []
+        _recorders.append(self.launches)
+        return self.launches
+
+    def __exit__(self, *exc):
+        _recorders.remove(self.launches)
+        return False
+
+
+def record(launch):
+    """Append a launch object (callable with a stream) to the active recorders."""
+    for r in _recorders:
+        r.append(launch)
+
+
+def call(name, fn, *args):
+    """fn(*args, stream) on the current stream, checked and recorded."""
+    check(fn(*args, stream_ptr()), name)
+    if _recorders:
+        record(Recorded("call", name, fn=fn, args=args))
+
+
 def _launch_desc(d):
     """Launch a GEMM descriptor with the autotuned kernel variant for its shape."""
     from . import autotune
-    from ._lib import GemmDesc
     s = stream_ptr()
     autotune.tune_gemm_desc(d, lambda dd: check(lib.pypde_dgemm_batched(ctypes.byref(dd), s),
                                                 "pypde_dgemm_batched"))
     check(lib.pypde_dgemm_batched(ctypes.byref(d), s), "pypde_dgemm_batched")
+    if _recorders:
+        record(Recorded("gemm", "gemm", descs=[d]))
 
 
 def _desc(ta, tb, m, n, k, batch, alpha, beta, A, lda, sA, B, ldb, sB, D, ldd, sD, C, ldc, sC,
@@ -395,8 +452,10 @@ def launch_grouped(descs, tag):
     # timed launch: such a group is not timed (stored choice, else variant 0)
     if any(autotune.aliased(d) for d in descs):
         run(autotune.cached(key, default=0))
-        return
-    run(autotune.choose(key, run, autotune.GEMM_CANDIDATES, default=0))
+    else:
+        run(autotune.choose(key, run, autotune.GEMM_CANDIDATES, default=0))
+    if _recorders:
+        record(Recorded("group", tag, descs=list(arr)))
 
 
 # ---------------------------------------------------------------------------

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import REPO
 
 
@@ -102,6 +104,35 @@ def test_engine_launch_records_without_gpu():
     a = _lib.RbcRhs()
     plan.add(_Call("b", lib.pypde_rbc_rhs_banded, ctypes.byref(a)))
     assert plan.size() == 7
+    # the adjoint's launches, as ops records them
+    from pypde_amd import ops
+    plan.add(ops.Recorded("gemm", "g", descs=[g.desc]))
+    plan.add(ops.Recorded("group", "gg", descs=[g.desc, g.desc]))
+    for fn, args in ((lib.pypde_axpby, (16, 1.0, vp(base), 0.0, vp(2 * base))),
+                     (lib.pypde_transpose, (vp(base), 4, 16, vp(2 * base), 4, 16, 4, 4, 1)),
+                     (lib.pypde_scaled_sub, (16, 1, vp(base), 0.5, vp(base), vp(2 * base))),
+                     (lib.pypde_set_entries, (vp(base), 16, 1, 0.0)),
+                     (lib.pypde_rbc_products_vjp, (vp(base), vp(base), vp(base), vp(2 * base),
+                                                   6, 6, 1)),
+                     (lib.pypde_rbc_products_vjp_c, (vp(base), vp(base), vp(base), vp(base),
+                                                     vp(2 * base), vp(3 * base), 0.5, 6, 6, 1)),
+                     (lib.pypde_recurrence_solve, (ctypes.byref(gl),
+                                                   ctypes.byref(_rec(3 * base, 4, 16, 2))))):
+        plan.add(ops.Recorded("call", fn.__name__, fn=fn, args=args))
+    assert plan.size() == 16
+    with pytest.raises(TypeError):
+        plan.add(ops.Recorded("call", "x", fn=lib.pypde_version, args=()))
+    # a VJP plan carves its workspace regions (4 on) from one buffer, 256-byte aligned
+    assert lib.pypde_rbc_step_vjp_workspace_bytes(plan.handle) == -1
+    bases6 = (ctypes.c_void_p * 6)(*[base * (i + 1) for i in range(6)])
+    sizes6 = (ctypes.c_longlong * 6)(base, base, base, base, 100, 300)
+    vplan = Plan(6, bases6, sizes6)
+    assert lib.pypde_rbc_step_vjp_workspace_bytes(vplan.handle) == 256 + 512
+    assert lib.pypde_rbc_step_vjp(vplan.handle, 1, 1, 1, 1, 255, None) != 0
+    assert b"aligned" in lib.pypde_last_error()
+    # a plan of four regions has no workspace region: rejected before anything runs
+    assert lib.pypde_rbc_step_vjp(plan.handle, 1, 1, 1, 1, 256, None) != 0
+    assert b"regions" in lib.pypde_last_error()
 
 
 def test_multi_launch_packing_without_gpu():

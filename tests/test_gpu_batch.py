@@ -117,3 +117,36 @@ def test_persistent_c4_batch_matches_branch_form():
             worst = max(worst, rel(a[f], b[f]))
     print("persistent vs branch form, worst relative difference %.2e" % worst)
     assert worst < 1e-10
+
+
+def test_persistent_batch_plan_through_capi_equals_its_launches():
+    """pypde_rbc_step_batched (the persistent step's multi launches recorded as
+    one step plan, run through ctypes) gives bit for bit what the same launches
+    give one by one, over three steps."""
+    import ctypes
+    import torch
+    from pypde_amd._lib import lib
+    from pypde_amd.batch import HeteroBatch, default_dt
+    specs = [dict(shape=(20, 16), ra=1e4, pr=1.0), dict(shape=(24, 20), ra=3e5, pr=1.0),
+             dict(shape=(16, 24), ra=2e6, pr=0.7)]
+    for s in specs:
+        s["dt"] = default_dt(s["ra"])
+    seeds = [7, 8, 9]
+    hb = HeteroBatch(specs, case="rbc", persistent=True)
+    hb.seed(seeds, amplitude=0.05)
+    plan = hb.build_plan()
+    assert lib.pypde_plan_size(plan.handle) == len(hb.multi) == 13
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.pypde_rbc_step_batched(plan.handle, 3, s) == 0
+    torch.cuda.synchronize()
+    via_plan = [{f: host(t) for f, t in hb.state(i).items()} for i in range(len(specs))]
+    hb.seed(seeds, amplitude=0.05)
+    for _ in range(3):
+        for m in hb.multi:
+            m(s)
+    torch.cuda.synchronize()
+    for i in range(len(specs)):
+        for f, a in via_plan[i].items():
+            assert np.array_equal(a, host(hb.state(i)[f])), (i, f)
+    assert lib.pypde_rbc_step_batched(None, 1, s) != 0
+    assert lib.pypde_rbc_step_batched(plan.handle, -1, s) != 0

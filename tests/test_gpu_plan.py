@@ -121,3 +121,42 @@ def test_batched_plan_steps_every_instance():
         torch.cuda.synchronize()
         assert rel(host(X[b]), host(e1.X[0])) < 1e-11, b
         assert rel(host(pres[b]), host(e1.pres[0])) < 1e-11, b
+
+
+@pytest.mark.parametrize("name", ["rbc_eu", "rbc_rk3", "rbc_eu_beta"])
+def test_step_vjp_plan_on_caller_buffers(name):
+    """pypde_rbc_step_vjp (the step's VJP recorded as a plan; three-stage
+    states recomputed inside its workspace) on other input and cotangent
+    arrays and a NaN-filled workspace gives bit for bit StepAdjoint.vjp, whose
+    launches it replays; the engine's own state is not touched."""
+    import torch
+    from pypde_amd._lib import lib
+    from pypde_amd.adjoint import StepAdjoint
+    eng = _engine(name)
+    adj = StepAdjoint(eng)
+    rng = np.random.default_rng(11)
+    dev = eng.X.device
+    rnd = lambda t, a: torch.as_tensor(rng.standard_normal(tuple(t.shape)) * a, device=dev)
+    X, pres = rnd(eng.X, 0.05), rnd(eng.pres, 0.01)
+    Xb, pb = rnd(eng.X, 1.0), rnd(eng.pres, 1.0)
+    ref_X, ref_p = adj.vjp(X, pres, Xb, pb)
+    eng_state = [t.clone() for t in (eng.X, eng.pres)]
+    plan = adj.build_plan()
+    wsb = lib.pypde_rbc_step_vjp_workspace_bytes(plan.handle)
+    assert wsb > 0
+    ws = torch.full((wsb // 8 + 32,), float("nan"), dtype=torch.float64, device=dev)
+    off = (-ws.data_ptr()) % 256 // 8
+    wsa = ws[off:]
+    Xc, pc, Xbc, pbc = X.clone(), pres.clone(), Xb.clone(), pb.clone()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib.pypde_rbc_step_vjp(plan.handle, vp(Xc), vp(pc), vp(Xbc), vp(pbc), vp(wsa), s) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(host(Xbc), host(ref_X)), name
+    assert np.array_equal(host(pbc), host(ref_p)), name
+    assert np.array_equal(host(Xc), host(X)) and np.array_equal(host(pc), host(pres))
+    for t, v in zip((eng.X, eng.pres), eng_state):
+        assert np.array_equal(host(t), host(v))
+    assert np.isfinite(host(Xbc)).all()
+    assert lib.pypde_rbc_step_vjp(plan.handle, vp(Xc), None, vp(Xbc), vp(pbc), vp(wsa), s) != 0
+    assert b"null" in lib.pypde_last_error()
