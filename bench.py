@@ -68,22 +68,18 @@ def seeded_noise(shape, seed=0, amplitude=1e-3):
 # (`python3 bench.py --copy-calibration`); profiles/hbm_traffic_pmc.md gives the
 # commands, the calibration and the per-launch table.
 TRAFFIC_BYTES = {
-    "xpass": 12623041,
-    "ypass_v+ypass_x": 23616773,
-    "products": 14242757,
-    "z_conv+z_pres+z_old": 17944500,
-    "rhs_part+buoyancy_part": 21316480,
-    "splitk_reduce": 6950337,
-    "helmholtz_x": 3658467,
-    "helmholtz_y": 3515566,
-    "div_y": 11538193,
-    "div_x+poisson_rhs": 12058217,
-    "poisson_lines": 3499868,
-    "poisson_qy+corr_y+corr_y0": 15151572,
-    "pres_p+correct": 10772144,
-    # the unsplit right-hand side, when the tuner keeps it (earlier measurement)
-    "rhs": 26877192,
-    "buoyancy": 5797532,
+    "xpass": 12615994,
+    "ypass_v+ypass_x": 23615960,
+    "products+rhs_banded": 27793527,
+    "z_conv": 10696155,
+    "rhs": 13369201,
+    "helmholtz_x": 3658500,
+    "helmholtz_y": 3514573,
+    "div_y": 11533202,
+    "div_x+poisson_rhs": 12059385,
+    "poisson_lines": 3499907,
+    "poisson_qy+corr_y+corr_y0": 15145603,
+    "pres_p+correct": 10773292,
 }
 
 
@@ -175,22 +171,24 @@ def time_plain(args):
         per.setdefault(name, [0.0, 0, launch])
         per[name][0] += e0.elapsed_time(e1) * 1e-3
         per[name][1] += 1
-    dom = max(per, key=lambda k: per[k][0])
-    tot, cnt, launch = per[dom]
+    # the dominant launch: the longest when each launch is replayed back to back
+    # from a HIP graph (as in the timed steps); the eager event times above also
+    # carry the host's launch gaps
     from pypde_amd.engine import _SplitRhs
-    if isinstance(launch, _SplitRhs):
-        # a K-split product: its GEMM kernel (the partial products and the
-        # buoyancy product in one launch) is the roofline kernel; the summing
-        # kernel after it is timed in the kernel table with the launch
-        launch = launch.group
-        dom = launch.name
-    avg = launch_graph_time(eng, launch)
+    graph_s = {}
+    for l in eng.launches:
+        for part in ([l.group, l.reduce] if isinstance(l, _SplitRhs) else [l]):
+            graph_s[part.name] = (launch_graph_time(eng, part), part)
+    dom = max(graph_s, key=lambda k: graph_s[k][0])
+    avg, launch = graph_s[dom]
     bound, amount = launch_cost(dom, launch, eng)
     if bound == "mfma":
         achieved, peak, unit = amount / avg / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
     else:
         achieved, peak, unit = amount / avg / 1e9, HBM_PEAK_GBS, "GB/s"
     kernel_table = {k: dict(avg_us=round(v[0] / v[1] * 1e6, 3), calls=v[1]) for k, v in per.items()}
+    for k, (t, _) in graph_s.items():
+        kernel_table.setdefault(k, {})["graph_us"] = round(t * 1e6, 3)
     nu = ns.eval_Nu()
     finite = bool(np.isfinite(nu[0]) and np.isfinite(nu[1]))
     return dict(sec=sec, steps_per_s=args.steps / sec, dominant=dom, avg_s=avg, bound=bound,
@@ -577,6 +575,109 @@ def cpu_baseline(steps=6, warm=1):
                        (steps, warm), sec=round(sec, 2), host=host_cpu())
 
 
+def gemm_variants():
+    """Every plain GEMM kernel variant on each plain GEMM launch of the 256x256
+    step (single descriptors and grouped launches; the folded x-pass is not a
+    plain GEMM), timed like the tuner does; for the y-pass also with its A
+    operand read contiguously (ska = 1: the same shape without the
+    every-other-column access).  The engine state is restored afterwards."""
+    import ctypes
+    import torch
+    from pypde_amd import autotune
+    from pypde_amd._lib import GemmDesc, check, lib
+    from pypde_amd.engine import _Gemm, _GemmGroup, _SplitRhs
+    from pypde_amd.navier import NavierStokes
+    ns = NavierStokes(case="rbc", shape=(256, 256), ra=1e8, pr=1.0, dt=1e-3, tsave=None)
+    eng = ns.engine
+    eng.step_eager()
+    torch.cuda.synchronize()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    saved = [t.clone() for t in (eng.X, eng.pres, eng.P)]
+
+    def sweep(descs):
+        arr = (GemmDesc * len(descs))(*descs)
+
+        def run(v):
+            for i in range(len(descs)):
+                arr[i].variant = v
+            if len(descs) == 1:
+                check(lib.pypde_dgemm_batched(ctypes.byref(arr), s), "variant sweep")
+            else:
+                check(lib.pypde_dgemm_grouped(ctypes.byref(arr), len(descs), s), "variant sweep")
+        return autotune.time_candidates(run, autotune.GEMM_CANDIDATES)
+    out = {}
+    for l in eng.launches:
+        for p in ([l.group] if isinstance(l, _SplitRhs) else [l]):
+            if isinstance(p, _GemmGroup):
+                descs = [GemmDesc.from_buffer_copy(m.desc) for m in p.members]
+            elif isinstance(p, _Gemm) and p.desc.fold == 0:
+                descs = [GemmDesc.from_buffer_copy(p.desc)]
+            else:
+                continue
+            t = sweep(descs)
+            ranked = sorted(t.items(), key=lambda kv: kv[1])
+            e = dict(in_step=descs[0].variant, us_in_step=round(t[descs[0].variant], 2),
+                     ranked=[[v, round(u, 2)] for v, u in ranked[:12]])
+            if any(d.ska == 2 for d in descs):
+                for d in descs:
+                    d.ska = 1
+                t1 = sweep(descs)
+                b1 = min(t1, key=t1.get)
+                e["contiguous_A"] = dict(fastest=b1, us=round(t1[b1], 2),
+                                         us_in_step_variant=round(t1[e["in_step"]], 2))
+            out[p.name] = e
+            print("variants: %s done" % p.name, file=sys.stderr, flush=True)
+    for t, v in zip((eng.X, eng.pres, eng.P), saved):
+        t.copy_(v)
+    torch.cuda.synchronize()
+    return out
+
+
+def hetero_launches():
+    """Per-launch times of the mixed-size batch (bench.c4_specs): each multi
+    launch of the persistent form, and each size class's launches in the
+    stream-branch form, timed alone (20 back-to-back launches, HIP events)."""
+    import ctypes
+    import torch
+    from pypde_amd.batch import HeteroBatch
+
+    def ev(fn, reps=20):
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        fn(st)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn(st)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / reps
+    specs = c4_specs()
+    seeds = [1000 + i for i in range(len(specs))]
+    hb = HeteroBatch(specs, case="rbc", persistent=True)
+    hb.seed(seeds, amplitude=1e-3)
+    saved = [[t.clone() for t in (e.X, e.pres, e.P)] for e in hb.classes]
+    pers = [[m.name, m.kind, round(ev(m), 2)] for m in hb.multi]
+    for e, sv in zip(hb.classes, saved):
+        for t, v in zip((e.X, e.pres, e.P), sv):
+            t.copy_(v)
+    out = dict(persistent=pers, persistent_sum_us=round(sum(r[2] for r in pers), 1))
+    del hb
+    hb = HeteroBatch(specs, case="rbc", persistent=False)
+    hb.seed(seeds, amplitude=1e-3)
+    per = {}
+    for e in hb.classes:
+        saved = [t.clone() for t in (e.X, e.pres, e.P)]
+        for l in e.launches:
+            per.setdefault(l.name, []).append(round(ev(l), 2))
+        for t, v in zip((e.X, e.pres, e.P), saved):
+            t.copy_(v)
+    out["branches"] = {k: dict(sum_us=round(sum(v), 1), per_class_us=v) for k, v in per.items()}
+    out["branches_sum_us"] = round(sum(sum(v) for v in per.values()), 1)
+    out["classes"] = [dict(shape=list(sh), instances=e.B) for sh, e in zip(hb.shapes, hb.classes)]
+    return out
+
+
 def cpu_table(threads=(1, None)):
     """The oracle on the host at every workload (one thread, then every core
     through threadpoolctl; scipy's FFTs stay single-threaded as the
@@ -678,6 +779,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip batched and microbench")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
+    ap.add_argument("--gemm-variants", action="store_true",
+                    help="time every plain GEMM variant on the 256x256 step's GEMM launches")
+    ap.add_argument("--hetero-launches", action="store_true",
+                    help="per-launch times of the mixed-size batch in both forms")
     ap.add_argument("--cpu-table", action="store_true",
                     help="time the oracle on the host at every workload (one thread and all cores)")
     ap.add_argument("--transforms", action="store_true",
@@ -719,6 +824,12 @@ def main():
         return
     if args.cpu_table:
         print(json.dumps({"cpu_table": cpu_table()}))
+        return
+    if args.gemm_variants:
+        print(json.dumps({"gemm_variants": gemm_variants()}))
+        return
+    if args.hetero_launches:
+        print(json.dumps({"hetero_launches": hetero_launches()}))
         return
     if args.adjoint:
         print(json.dumps({"adjoint": time_adjoint(256, 200, ckpt=100)}))

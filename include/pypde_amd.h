@@ -342,8 +342,9 @@ typedef struct pypde_rbc_rhs {
     const double* ty;
     int M0, M1, N0, N1, batch;
     int rows_per_thread; /* kernel form: 1 one output per thread; 4 or 8 (0: 4) a strip
-                            of rows per thread sharing the y sums of its input rows
-                            (the same sums in the same order: bit-identical) */
+                            of rows per thread sharing the y sums of its input rows;
+                            16 tiles of 16 rows x 64 columns per block with the y sums
+                            in LDS (the same sums in the same order: bit-identical) */
 } pypde_rbc_rhs;
 
 int pypde_rbc_rhs_banded(const pypde_rbc_rhs* args, void* stream);
@@ -419,7 +420,9 @@ int pypde_products_multi_pack(const double* const* Y, const double* const* Yold,
                               const double* ratio, const int* batch, int count, void* table,
                               long long table_bytes, pypde_multi_info* info);
 int pypde_products_multi(const void* table, const pypde_multi_info* info, void* stream);
-/* Banded right-hand sides (pypde_rbc_rhs_banded), one argument struct per member. */
+/* Banded right-hand sides (pypde_rbc_rhs_banded), one argument struct per member;
+ * one kernel form for all of them: every member's rows_per_thread must be the
+ * same (0 or 1: one output per thread; 4 or 8: row strips; 16: LDS tiles). */
 int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* table, long long table_bytes,
                          pypde_multi_info* info);
 int pypde_rhs_multi(const void* table, const pypde_multi_info* info, void* stream);

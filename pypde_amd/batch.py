@@ -252,11 +252,37 @@ class HeteroBatch:
         return m
 
     def _multi_rhs(self, name, members):
+        """The banded right-hand sides of every class in one launch, in the
+        kernel form (one output or a strip of 4 / 8 rows per thread, or LDS
+        tiles) the tuner times fastest for the whole set."""
+        from . import autotune
         n = len(members)
-        args = (RbcRhs * n)(*[l.args[0]._obj for l in members])
-        m = _Multi(name, "rhs", lambda host, nb, info: lib.pypde_rhs_multi_pack(
-            ctypes.byref(args), n, host, nb, info))
+        args = (RbcRhs * n)(*[RbcRhs.from_buffer_copy(l.args[0]._obj) for l in members])
+
+        def pack(host, nb, info):
+            return lib.pypde_rhs_multi_pack(ctypes.byref(args), n, host, nb, info)
+
+        def set_form(rs):
+            for a in args:
+                a.rows_per_thread = rs
+        set_form(1)
+        m = _Multi(name, "rhs", pack)
         m._keep = args
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        tables = {}
+
+        def run(rs):
+            if rs not in tables:
+                set_form(rs)
+                m.repack()
+                tables[rs] = (m.table, MultiInfo.from_buffer_copy(m.info))
+            t, info = tables[rs]
+            check(lib.pypde_rhs_multi(ctypes.c_void_p(t.data_ptr()), ctypes.byref(info), stream),
+                  name)
+        key = ("multi_rhs",) + tuple((a.M0, a.M1, a.batch, a.fdiff is not None) for a in args)
+        best = autotune.choose(key, run, (1, 4, 8, 16), default=1)
+        set_form(best)
+        m.repack()
         return m
 
     def build_plan(self):

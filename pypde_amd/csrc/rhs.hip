@@ -163,6 +163,129 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
     }
 }
 
+// Tile form: a block of 256 threads serves kRhsTI rows x 64 columns of one
+// (instance, field) entry.  First the y-direction sums of every input row the
+// tile needs -- rows i0-2 .. i0+kRhsTI+3 of X_f, and of X_T and of the pressure
+// where the field has those terms -- go to LDS, one column per lane, each formed
+// once; then every thread sums its outputs over the x diagonals from LDS.  About
+// six global loads per output instead of 25-60.  The same terms in the same
+// order as rhs_banded_at: bit-identical.
+constexpr int kRhsTI = 16, kRhsTJ = 64, kRhsRows = kRhsTI + 6;
+
+__device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long long e, int ti, int tj)
+{
+    __shared__ double ys[kRhsRows][kRhsTJ];   // (B S)_y sums of X_f
+    __shared__ double yt[kRhsRows][kRhsTJ];   // (B S)_y sums of X_T (buoyancy, V)
+    __shared__ double yp[kRhsRows][kRhsTJ];   // pressure sums (U: B_y, V: (B D)_y)
+    __shared__ double yd[kRhsRows][kRhsTJ];   // (B D2 S)_y sums of X_f (explicit diffusion)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int M0 = a.M0, M1 = a.M1, N0 = a.N0, N1 = a.N1;
+    const int b = (int)(e / 3), f = (int)(e - 3LL * b);
+    const int i0 = ti * kRhsTI, j = tj * kRhsTJ + lane;
+    const bool jin = j < M1;
+    const long long mm = (long long)M0 * M1;
+    const double* Xf = a.X + e * mm;
+    const double* pb = a.pres + (long long)b * N0 * N1;
+    const int o4[4] = {-2, 0, 2, 4};
+    const int o02[2] = {0, 2};
+    const int o024[3] = {0, 2, 4};
+    const int o135[3] = {1, 3, 5};
+    const int pfirst = f == 0 ? 1 : 0;   // first pressure row: U offsets 1,3,5; V 0,2,4
+    // y sums: LDS row q holds X row i0 - 2 + q and pressure row i0 + pfirst + q
+    for (int q = w; q < kRhsRows; q += 4) {
+        if (!jin) break;
+        const int r = i0 - 2 + q;
+        const bool rin = r >= 0 && r < M0;
+        ys[q][lane] = rin ? ysum<4>(Xf + (long long)r * M1, j, M1, o4, a.ty + TY_BS * M1, M1) : 0.0;
+        if (f == 1)
+            yt[q][lane] = rin ? ysum<4>(a.X + (e + 1) * mm + (long long)r * M1, j, M1, o4,
+                                        a.ty + TY_BS * M1, M1)
+                              : 0.0;
+        if (f < 2) {
+            const int rp = i0 + pfirst + q;
+            yp[q][lane] = rp < N0 ? (f == 0 ? ysum<3>(pb + (long long)rp * N1, j, N1, o024,
+                                                      a.ty + TY_B * M1, M1)
+                                            : ysum<3>(pb + (long long)rp * N1, j, N1, o135,
+                                                      a.ty + TY_BD * M1, M1))
+                                  : 0.0;
+        }
+        if (a.fdiff)
+            yd[q][lane] = rin ? ysum<2>(Xf + (long long)r * M1, j, M1, o02, a.ty + TY_BD2S * M1, M1)
+                              : 0.0;
+    }
+    __syncthreads();
+    if (!jin) return;
+    const double* tbs = a.tx + (long long)(f == 2 ? TX_BST : TX_BS) * M0;
+#pragma unroll
+    for (int k = 0; k < kRhsTI / 4; ++k) {
+        const int i = i0 + 4 * w + k;
+        if (i >= M0) break;
+        const int qi = i - i0 + 2;   // LDS row of X row i
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int r = i + o4[d];
+            if (r >= 0 && r < M0) acc = acc + tbs[d * M0 + i] * ys[qi + o4[d]][lane];
+        }
+        if (f < 2) {
+            const double* tp = a.tx + (long long)(f == 0 ? TX_BD : TX_B) * M0;
+            double g = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int off = f == 0 ? o135[d] : o024[d];
+                if (i + off < N0) g = g + tp[d * M0 + i] * yp[i - i0 + off - pfirst][lane];
+            }
+            acc = acc + a.mdta[b] * g;
+        }
+        if (f == 1) {
+            const double* tt = a.tx + (long long)TX_BST * M0;
+            double bt = 0.0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int r = i + o4[d];
+                if (r >= 0 && r < M0) bt = bt + tt[d * M0 + i] * yt[qi + o4[d]][lane];
+            }
+            acc = acc + a.dta[b] * bt;
+        }
+        if (a.fdiff) {
+            const double* td2 = a.tx + (long long)(f == 2 ? TX_BD2ST : TX_BD2S) * M0;
+            double dxx = 0.0, dyy = 0.0;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int r = i + o02[d];
+                if (r < M0) dxx = dxx + td2[d * M0 + i] * ys[qi + o02[d]][lane];
+            }
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int r = i + o4[d];
+                if (r >= 0 && r < M0) dyy = dyy + tbs[d * M0 + i] * yd[qi + o4[d]][lane];
+            }
+            acc = acc + a.fdiff[e] * (dxx + dyy);
+        }
+        const long long o = e * mm + (long long)i * M1 + j;
+        a.R[o] = a.cst ? a.cst[o] + acc : acc;
+    }
+}
+
+__device__ __forceinline__ long long rhs_tiles_per_entry(const pypde_rbc_rhs& a)
+{
+    return (long long)((a.M0 + kRhsTI - 1) / kRhsTI) * ((a.M1 + kRhsTJ - 1) / kRhsTJ);
+}
+
+__device__ __forceinline__ void rhs_tile_at(const pypde_rbc_rhs& a, long long t)
+{
+    const int ntj = (a.M1 + kRhsTJ - 1) / kRhsTJ;
+    const long long per = rhs_tiles_per_entry(a);
+    const long long e = t / per;
+    const int r = (int)(t - e * per);
+    rhs_banded_tile(a, e, r / ntj, r % ntj);
+}
+
+__global__ void __launch_bounds__(256) rbc_rhs_tile_kernel(pypde_rbc_rhs a)
+{
+    rhs_tile_at(a, blockIdx.x);
+}
+
 template <int RS>
 __global__ void __launch_bounds__(64) rbc_rhs_strip_kernel(pypde_rbc_rhs a)
 {
@@ -195,7 +318,10 @@ struct RhsMulti {
     pypde_rbc_rhs m[kMaxRhsMulti];
 };
 
-__global__ void rbc_rhs_banded_multi(const RhsMulti* __restrict__ mm)
+// RS = 1: one output per thread (rhs_banded_at); 4, 8: a strip of RS rows of
+// one column per thread (rhs_banded_strip); the same bits either way.
+template <int RS>
+__global__ void __launch_bounds__(256) rbc_rhs_banded_multi(const RhsMulti* __restrict__ mm)
 {
     const int blk = blockIdx.x;
     int lo = 0, hi = mm->count - 1;
@@ -205,10 +331,19 @@ __global__ void rbc_rhs_banded_multi(const RhsMulti* __restrict__ mm)
         else hi = mid - 1;
     }
     const pypde_rbc_rhs a = mm->m[lo];
+    if (RS == 16) {
+        rhs_tile_at(a, blk - mm->start[lo]);
+        return;
+    }
     const long long t = (long long)(blk - mm->start[lo]) * blockDim.x + threadIdx.x;
-    if (t < 3LL * a.M0 * a.M1 * a.batch)
-        rhs_banded_at(a.X, a.pres, a.cst, a.R, a.mdta, a.dta, a.fdiff, a.tx, a.ty, a.M0, a.M1,
-                      a.N0, a.N1, t);
+    if (RS == 1) {
+        if (t < 3LL * a.M0 * a.M1 * a.batch)
+            rhs_banded_at(a.X, a.pres, a.cst, a.R, a.mdta, a.dta, a.fdiff, a.tx, a.ty, a.M0,
+                          a.M1, a.N0, a.N1, t);
+    } else if (t < 3LL * a.batch * ((a.M0 + RS - 1) / RS) * a.M1) {
+        rhs_banded_strip<RS>(a.X, a.pres, a.cst, a.R, a.mdta, a.dta, a.fdiff, a.tx, a.ty, a.M0,
+                             a.M1, a.N0, a.N1, t);
+    }
 }
 
 }  // namespace pypde
@@ -236,6 +371,11 @@ extern "C" int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* 
                   sizeof(RhsMulti));
     PYPDE_REQUIRE(count >= 1 && count <= kMaxRhsMulti, "multi right-hand side: count %d not in 1..%d",
                   count, kMaxRhsMulti);
+    // one kernel form for all members: rows_per_thread 0 or 1 one output per
+    // thread, 4 or 8 a strip of that many rows
+    const int rs = args[0].rows_per_thread <= 1 ? 1 : args[0].rows_per_thread;
+    PYPDE_REQUIRE(rs == 1 || rs == 4 || rs == 8 || rs == 16,
+                  "rows_per_thread must be 0, 1, 4, 8 or 16 (got %d)", args[0].rows_per_thread);
     RhsMulti* mm = static_cast<RhsMulti*>(table);
     memset(mm, 0, sizeof(RhsMulti));
     mm->count = count;
@@ -243,16 +383,21 @@ extern "C" int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* 
     for (int i = 0; i < count; ++i) {
         const int rc = check_rhs(&args[i]);
         if (rc != OK) return rc;
+        const int ri = args[i].rows_per_thread <= 1 ? 1 : args[i].rows_per_thread;
+        PYPDE_REQUIRE(ri == rs, "member %d: every member needs the same rows_per_thread", i);
         mm->m[i] = args[i];
         mm->start[i] = (int)total;
-        total += (3LL * args[i].M0 * args[i].M1 * args[i].batch + 255) / 256;
+        if (rs == 16)
+            total += 3LL * args[i].batch * ((args[i].M0 + 15) / 16) * ((args[i].M1 + 63) / 64);
+        else
+            total += (3LL * args[i].batch * ((args[i].M0 + rs - 1) / rs) * args[i].M1 + 255) / 256;
         PYPDE_REQUIRE(total < (1LL << 31), "multi right-hand side too large");
     }
     mm->start[count] = (int)total;
     mm->total = (int)total;
     info->kind = PYPDE_MULTI_RHS;
     info->count = count;
-    info->variant = 0;
+    info->variant = rs;
     info->flags = 0;
     info->total = total;
     return OK;
@@ -263,8 +408,30 @@ extern "C" int pypde_rhs_multi(const void* table, const pypde_multi_info* info, 
     PYPDE_REQUIRE(table && info, "null argument");
     PYPDE_REQUIRE(info->kind == PYPDE_MULTI_RHS, "not a right-hand-side table");
     if (info->total == 0) return OK;
-    hipLaunchKernelGGL(rbc_rhs_banded_multi, dim3((unsigned)info->total), dim3(256), 0,
-                       as_stream(stream), static_cast<const RhsMulti*>(table));
+    const RhsMulti* mm = static_cast<const RhsMulti*>(table);
+    switch (info->variant) {
+    case 0:
+    case 1:
+        hipLaunchKernelGGL(rbc_rhs_banded_multi<1>, dim3((unsigned)info->total), dim3(256), 0,
+                           as_stream(stream), mm);
+        break;
+    case 4:
+        hipLaunchKernelGGL(rbc_rhs_banded_multi<4>, dim3((unsigned)info->total), dim3(256), 0,
+                           as_stream(stream), mm);
+        break;
+    case 8:
+        hipLaunchKernelGGL(rbc_rhs_banded_multi<8>, dim3((unsigned)info->total), dim3(256), 0,
+                           as_stream(stream), mm);
+        break;
+    case 16:
+        hipLaunchKernelGGL(rbc_rhs_banded_multi<16>, dim3((unsigned)info->total), dim3(256), 0,
+                           as_stream(stream), mm);
+        break;
+    default:
+        set_error("pypde_rhs_multi: unknown kernel form %d (the table's rows per thread)",
+                  info->variant);
+        return ERR_ARG;
+    }
     return check_launch("pypde_rhs_multi");
 }
 
@@ -285,8 +452,16 @@ extern "C" int pypde_rbc_rhs_banded(const pypde_rbc_rhs* a, void* stream)
     if (rc != OK) return rc;
     if (a->batch == 0) return OK;
     const int rs = a->rows_per_thread == 0 ? 4 : a->rows_per_thread;
-    PYPDE_REQUIRE(rs == 1 || rs == 4 || rs == 8, "rows_per_thread must be 0, 1, 4 or 8 (got %d)",
-                  a->rows_per_thread);
+    PYPDE_REQUIRE(rs == 1 || rs == 4 || rs == 8 || rs == 16,
+                  "rows_per_thread must be 0, 1, 4, 8 or 16 (got %d)", a->rows_per_thread);
+    if (rs == 16) {
+        const long long blocks =
+            3LL * a->batch * ((a->M0 + 15) / 16) * ((a->M1 + 63) / 64);
+        PYPDE_REQUIRE(blocks < (1LL << 31), "right-hand side too large");
+        hipLaunchKernelGGL(rbc_rhs_tile_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                           as_stream(stream), *a);
+        return check_launch("pypde_rbc_rhs_banded");
+    }
     if (rs == 1) {
         const long long total = 3LL * a->M0 * a->M1 * a->batch;
         long long blocks = (total + 255) / 256;

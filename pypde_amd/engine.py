@@ -1042,8 +1042,8 @@ class RBCBatch:
                     a.rows_per_thread = v
                     launch(s)
                 a.rows_per_thread = autotune.choose(
-                    ("rhs_banded", self.B, self.M0, self.M1, self.diffusion), run, (1, 4, 8),
-                    default=4)
+                    ("rhs_banded_form", self.B, self.M0, self.M1, self.diffusion), run,
+                    (1, 4, 8, 16), default=4)
         if self.group_launches:
             self._merge_products_rhs(s)
         for launch in self.launches:
