@@ -237,7 +237,7 @@ def c4_specs(n=256, seed=7):
             for a, b, r in zip(nx, ny, ras)]
 
 
-def time_hetero(steps=500, warm=5, n=256, seed=7, persistent=True):
+def time_hetero(steps=500, warm=5, n=256, seed=7, persistent=True, groups=1):
     """256 independent instances, N drawn per axis from {64, 96, 128, 192},
     Ra log-uniform in [1e4, 1e7], Pr = 1, dt = clip(0.01 sqrt(1e5/Ra), 1e-3, 1e-2),
     instance i seeded with noise 1e-3 from seed 1000 + i; one graph launch per
@@ -248,7 +248,7 @@ def time_hetero(steps=500, warm=5, n=256, seed=7, persistent=True):
     from pypde_amd.engine import RBCBatch
     specs = c4_specs(n, seed)
     t0 = time.perf_counter()
-    hb = HeteroBatch(specs, case="rbc", persistent=persistent)
+    hb = HeteroBatch(specs, case="rbc", persistent=persistent, groups=groups)
     setup = time.perf_counter() - t0
     hb.seed([1000 + i for i in range(n)], amplitude=1e-3)
     hb.capture(1)
@@ -273,7 +273,8 @@ def time_hetero(steps=500, warm=5, n=256, seed=7, persistent=True):
         del e
     loop_sec_per_step = sum(per_shape[tuple(s["shape"])] for s in specs)
     agg = n * steps / sec
-    return dict(form="persistent" if persistent else "stream branches",
+    return dict(form=("persistent" if groups == 1 else "persistent, %d groups" % groups)
+                if persistent else "stream branches",
                 launches_per_step=len(hb.multi) if persistent else None,
                 instances=n, size_classes=len(hb.shapes), steps=steps, sec=round(sec, 3),
                 inst_steps_per_s=round(agg, 1), ms_per_step=round(sec / steps * 1e3, 3),
@@ -633,6 +634,42 @@ def gemm_variants():
     return out
 
 
+def rhs_forms():
+    """The banded right-hand side of the step in each kernel form (one output
+    per thread, row strips of 4 and 8, LDS tiles), microseconds per launch
+    (20 back-to-back launches, HIP events), at the workloads' shapes."""
+    import ctypes
+    import torch
+    from pypde_amd._lib import RbcRhs, lib, check
+    from pypde_amd.engine import RBCBatch
+    out = {}
+    for shape, B in (((256, 256), 1), ((256, 256), 64), ((128, 128), 16), ((64, 64), 16),
+                     ((192, 192), 16), ((512, 512), 1)):
+        params = [dict(ra=1e6, pr=1.0, dt=1e-3)] * B
+        eng = RBCBatch(shape, params, case="rbc", group_launches=False)
+        a = None
+        for l in eng.launches:
+            for arg in getattr(l, "args", ()):
+                if isinstance(getattr(arg, "_obj", None), RbcRhs):
+                    a = RbcRhs.from_buffer_copy(arg._obj)
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        res = {}
+        for rs in (1, 4, 8, 16):
+            a.rows_per_thread = rs
+            check(lib.pypde_rbc_rhs_banded(ctypes.byref(a), st), "rhs form")
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                lib.pypde_rbc_rhs_banded(ctypes.byref(a), st)
+            e1.record()
+            torch.cuda.synchronize()
+            res[rs] = round(e0.elapsed_time(e1) * 1e3 / 20, 2)
+        out["%dx%d_B%d" % (shape[0], shape[1], B)] = res
+        del eng
+    return out
+
+
 def hetero_launches():
     """Per-launch times of the mixed-size batch (bench.c4_specs): each multi
     launch of the persistent form, and each size class's launches in the
@@ -781,6 +818,8 @@ def main():
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
     ap.add_argument("--gemm-variants", action="store_true",
                     help="time every plain GEMM variant on the 256x256 step's GEMM launches")
+    ap.add_argument("--rhs-forms", action="store_true",
+                    help="time the banded right-hand side's kernel forms at the workloads' shapes")
     ap.add_argument("--hetero-launches", action="store_true",
                     help="per-launch times of the mixed-size batch in both forms")
     ap.add_argument("--cpu-table", action="store_true",
@@ -828,6 +867,9 @@ def main():
     if args.gemm_variants:
         print(json.dumps({"gemm_variants": gemm_variants()}))
         return
+    if args.rhs_forms:
+        print(json.dumps({"rhs_forms": rhs_forms()}))
+        return
     if args.hetero_launches:
         print(json.dumps({"hetero_launches": hetero_launches()}))
         return
@@ -847,6 +889,8 @@ def main():
         stage("plain", time_plain, args)
         stage("batched", time_batched, args)
         stage("hetero", time_hetero)
+        stage("hetero_groups", time_hetero, groups=2)
+        stage("hetero_groups4", time_hetero, groups=4)
         stage("hetero_branches", time_hetero, persistent=False)
         stage("rk3", time_variant, steps=5, integrator="rk3")
         stage("beta", time_variant, steps=5, beta=0.5)
@@ -885,6 +929,7 @@ def main():
         extra["batched"] = dict(b, ratio_vs_plain=round(b["inst_steps_per_s"] / plain["steps_per_s"], 2))
         done("batched")
         extra["hetero"] = time_hetero()
+        extra["hetero_groups"] = [time_hetero(groups=g) for g in (2, 4)]
         extra["hetero_branches"] = time_hetero(persistent=False)
         done("hetero")
         extra["variants"] = [time_variant(integrator="rk3"), time_variant(beta=0.5),

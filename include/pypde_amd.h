@@ -402,7 +402,8 @@ typedef struct pypde_multi_info {
 
 long long pypde_multi_table_bytes(int kind);
 /* GEMM members: plain descriptors with the same transposes, run with one
- * LDS-tiled variant (1..7, 24..42). */
+ * LDS-tiled variant (1..7, 24..42); or folded descriptors with one fold mode
+ * and the same transposes, run with one folded variant (1..5). */
 int pypde_gemm_multi_pack(const pypde_gemm_desc* descs, int count, int variant, void* table,
                           long long table_bytes, pypde_multi_info* info);
 int pypde_dgemm_multi(const void* table, const pypde_multi_info* info, void* stream);

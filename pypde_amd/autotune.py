@@ -138,7 +138,8 @@ def load_db(path=None):
     for k, v in db.get("entries", {}).items():
         key = ast.literal_eval(k)
         if key not in _cache:
-            _cache[key] = v
+            # JSON stores tuple decisions (form, variant) as lists
+            _cache[key] = tuple(v) if isinstance(v, list) else v
             n += 1
     return n
 

@@ -63,9 +63,14 @@ class HeteroBatch:
     persistent=True: every operator of the step runs once for all size classes
     (the multi launches of the C-ABI, one per stage: about 13 per Euler step
     whatever the number of classes); False: one engine step per class on its
-    own stream branch of the captured graph."""
+    own stream branch of the captured graph.
 
-    def __init__(self, specs, case="rbc", dealias=True, aspect=1.0, persistent=False):
+    groups (persistent form): the size classes are split into this many groups
+    of about equal work, each with its own chain of multi launches; the captured
+    graph runs the chains on parallel branches (13 x groups nodes per step), so
+    that one group's latency-bound line solves overlap another's products."""
+
+    def __init__(self, specs, case="rbc", dealias=True, aspect=1.0, persistent=False, groups=1):
         self.specs = [dict(s) for s in specs]
         if not self.specs:
             raise ValueError("HeteroBatch needs at least one instance")
@@ -93,7 +98,12 @@ class HeteroBatch:
         self.graph_steps = 0
         self._streams = None
         self.persistent = bool(persistent)
-        self.multi = self._build_multi() if self.persistent else None
+        self.groups = None
+        self.multi = None
+        if self.persistent:
+            self.groups = self._partition(max(1, min(int(groups), len(self.classes))))
+            self.group_multi = [self._build_multi(g) for g in self.groups]
+            self.multi = [m for chain in self.group_multi for m in chain]
 
     def __len__(self):
         return len(self.specs)
@@ -144,11 +154,25 @@ class HeteroBatch:
         return out
 
     # ---- one launch per stage for every class -----------------------------
-    def _build_multi(self):
-        """The step as multi launches: position p of every class's launch list
-        (the same operator of the step) becomes one launch, and neighbouring
-        GEMM positions that the engine may group share it."""
-        lists = [e.launches for e in self.classes]
+    def _partition(self, n):
+        """Class indices in n groups of about equal work (largest first onto the
+        lightest group); the work of a class is its instances times its
+        dealiased grid times its Galerkin extent, the shape of the GEMM work."""
+        work = [e.B * e.L0 * e.L1 * (e.M0 + e.M1) for e in self.classes]
+        groups, load = [[] for _ in range(n)], [0] * n
+        for c in sorted(range(len(work)), key=lambda c: -work[c]):
+            g = load.index(min(load))
+            groups[g].append(c)
+            load[g] += work[c]
+        return [sorted(g) for g in groups]
+
+    def _build_multi(self, cls=None):
+        """The step as multi launches over the classes `cls` (all by default):
+        position p of every class's launch list (the same operator of the step)
+        becomes one launch, and neighbouring GEMM positions that the engine may
+        group share it."""
+        cls = list(range(len(self.classes))) if cls is None else list(cls)
+        lists = [self.classes[c].launches for c in cls]
         names = [l.name for l in lists[0]]
         for L in lists[1:]:
             if [l.name for l in L] != names:
@@ -158,7 +182,8 @@ class HeteroBatch:
             for a, b in zip(grp, grp[1:]):
                 mergeable.add((a, b))
         base = lambda name: name.split("[", 1)[0]   # without the RK3 stage tag
-        saved = [[t.clone() for t in (e.X, e.pres, e.P)] for e in self.classes]
+        saved = [[t.clone() for t in (self.classes[c].X, self.classes[c].pres, self.classes[c].P)]
+                 for c in cls]
         out, p = [], 0
         while p < len(names):
             first = lists[0][p]
@@ -171,7 +196,12 @@ class HeteroBatch:
                        (self._plain(first).desc.transA, self._plain(first).desc.transB)):
                     q += 1
                 descs = [self._plain(L[k]).desc for k in range(p, q) for L in lists]
-                out.append(self._multi_gemm("+".join(names[p:q]), descs))
+                folded = [self._folded(L[p]) for L in lists] if q == p + 1 else [None]
+                if all(f is not None for f in folded) and \
+                        len({f.desc.fold for f in folded}) == 1:
+                    out.append(self._multi_gemm(names[p], descs, [f.desc for f in folded]))
+                else:
+                    out.append(self._multi_gemm("+".join(names[p:q]), descs))
                 p = q
                 continue
             members = [L[p] for L in lists]
@@ -185,7 +215,8 @@ class HeteroBatch:
                 raise RuntimeError("launch %s has no multi form" % names[p])
             p += 1
         torch.cuda.synchronize()
-        for e, sv in zip(self.classes, saved):
+        for c, sv in zip(cls, saved):
+            e = self.classes[c]
             for t, v in zip((e.X, e.pres, e.P), sv):
                 t.copy_(v)
         torch.cuda.synchronize()
@@ -200,33 +231,53 @@ class HeteroBatch:
             return launch.alt
         return launch
 
-    def _multi_gemm(self, name, descs):
+    @staticmethod
+    def _folded(launch):
+        """The parity-folded form of a GEMM launch, if it has one."""
+        if launch.desc.fold != 0:
+            return launch
+        if launch.alt is not None and launch.alt.desc.fold != 0:
+            return launch.alt
+        return None
+
+    def _multi_gemm(self, name, descs, folded=None):
+        """One multi launch of the plain descriptors (variants MULTI_GEMM_VARIANTS)
+        or, when given, of their folded forms (variants 1-5: half the MFMA work
+        of a transform), whichever form and variant the tuner times fastest."""
         from . import autotune
         from ._lib import GemmDesc
-        arr = (GemmDesc * len(descs))(*descs)
-        state = {"v": 2}
+        arrs = {"plain": (GemmDesc * len(descs))(*descs)}
+        if folded is not None:
+            arrs["fold"] = (GemmDesc * len(folded))(*folded)
+        state = {"form": "plain", "v": 2}
 
         def pack(host, nb, info):
-            return lib.pypde_gemm_multi_pack(ctypes.byref(arr), len(descs), state["v"], host, nb,
+            arr = arrs[state["form"]]
+            return lib.pypde_gemm_multi_pack(ctypes.byref(arr), len(arr), state["v"], host, nb,
                                              info)
         m = _Multi(name, "gemm", pack)
-        m._arr = arr
+        m._arr = arrs
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         tables = {}
 
-        def run(v):
-            if v not in tables:
-                state["v"] = v
+        def run(c):
+            if c not in tables:
+                state["form"], state["v"] = c
                 m.repack()
-                tables[v] = (m.table, MultiInfo.from_buffer_copy(m.info))
-            t, info = tables[v]
+                tables[c] = (m.table, MultiInfo.from_buffer_copy(m.info))
+            t, info = tables[c]
             check(lib.pypde_dgemm_multi(ctypes.c_void_p(t.data_ptr()), ctypes.byref(info), stream),
                   name)
+        cands = [("plain", v) for v in MULTI_GEMM_VARIANTS]
         key = ("multi_gemm",) + tuple(autotune.gemm_key(d) for d in descs)
-        best = autotune.choose(key, run, MULTI_GEMM_VARIANTS, default=2)
-        state["v"] = best
+        if folded is not None:
+            cands += [("fold", v) for v in (1, 2, 3, 4, 5)]
+            key = ("multi_gemm_or_fold",) + key[1:] + tuple(autotune.gemm_key(d) for d in folded)
+        best = autotune.choose(key, run, cands, default=("plain", 2))
+        state["form"], state["v"] = best
         m.repack()
-        m.variant = best
+        m.variant = best[1]
+        m.form = best[0]
         return m
 
     def _multi_rec(self, name, members):
@@ -320,10 +371,27 @@ class HeteroBatch:
                 t.copy_(v)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        if self.persistent:
+        if self.persistent and len(self.group_multi) == 1:
             with torch.cuda.graph(g):
                 for _ in range(nsteps):
                     self.step_eager()
+            self.graph = g
+            self.graph_steps = nsteps
+            return
+        if self.persistent:
+            # one branch per group, each its chain of multi launches
+            self._streams = [torch.cuda.Stream() for _ in self.group_multi]
+            with torch.cuda.graph(g):
+                cap = torch.cuda.current_stream()
+                for _ in range(nsteps):
+                    for s, chain in zip(self._streams, self.group_multi):
+                        s.wait_stream(cap)
+                        with torch.cuda.stream(s):
+                            sp = ctypes.c_void_p(s.cuda_stream)
+                            for m in chain:
+                                m(sp)
+                    for s in self._streams:
+                        cap.wait_stream(s)
             self.graph = g
             self.graph_steps = nsteps
             return

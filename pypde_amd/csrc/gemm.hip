@@ -799,7 +799,8 @@ struct FoldArgs {
 };
 
 template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, FoldArgs f)
+__device__ __forceinline__ void dgemm_fold_body(const GemmArgs& g, const FoldArgs& f, int bx, int by,
+                                                int bz)
 {
     constexpr int BK = 16;
     constexpr int NT = 64 * WGM * WGN;
@@ -823,8 +824,6 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, 
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave / WGN, wn = wave % WGN;
-    int bx, by, bz;
-    grid_tile(bx, by, bz);
     const int m0 = by * BM;
     const int n0 = bx * BN;
     const long long oA = bofs(bz, g.iA), oB = bofs(bz, g.iB);
@@ -959,6 +958,43 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, 
                     if (x2 != x1) ep.store(gm, x2, sg * (e - o));
                 }
             }
+}
+
+template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, FoldArgs f)
+{
+    int bx, by, bz;
+    grid_tile(bx, by, bz);
+    dgemm_fold_body<BM, BN, WGM, WGN, MODE, TA, TB>(g, f, bx, by, bz);
+}
+
+// Folded GEMMs of one mode and tile form for many members in one launch (the
+// folded x-pass of every size class of a heterogeneous batch): the plain multi
+// table layout plus each member's fold arguments.
+struct FoldMulti {
+    int count, total;
+    int start[kMaxMulti + 1];
+    int tx[kMaxMulti], ty[kMaxMulti];
+    GemmArgs a[kMaxMulti];
+    FoldArgs f[kMaxMulti];
+};
+
+template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_multi(const FoldMulti* __restrict__ mm)
+{
+    const int t = xcd_remap(blockIdx.x, mm->total);
+    int lo = 0, hi = mm->count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mm->start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int local = t - mm->start[lo];
+    const int bx = local % mm->tx[lo];
+    const int r = local / mm->tx[lo];
+    const GemmArgs g = mm->a[lo];
+    const FoldArgs f = mm->f[lo];
+    dgemm_fold_body<BM, BN, WGM, WGN, MODE, TA, TB>(g, f, bx, r % mm->ty[lo], r / mm->ty[lo]);
 }
 
 // Output folds (modes 1, 2) split over waves: waves [0, KW) run the first
@@ -1404,16 +1440,114 @@ int dgemm_grouped(const pypde_gemm_desc* d, int count, hipStream_t stream)
     return check_launch("pypde_dgemm_grouped");
 }
 
-long long gemm_multi_bytes() { return (long long)sizeof(GemmMulti); }
+long long gemm_multi_bytes()
+{
+    return (long long)(sizeof(GemmMulti) > sizeof(FoldMulti) ? sizeof(GemmMulti) : sizeof(FoldMulti));
+}
+
+// Block tile of the folded variants 1-5 (0 for others).
+static void fold_tile_shape(int v, int& bm, int& bn)
+{
+    bm = bn = 0;
+    switch (v) {
+    case 1: bm = 64; bn = 64; break;
+    case 2: bm = 32; bn = 32; break;
+    case 3: bm = 128; bn = 128; break;
+    case 4: bm = 128; bn = 64; break;
+    case 5: bm = 64; bn = 128; break;
+    default: break;
+    }
+}
+
+static FoldArgs fold_args_of(const pypde_gemm_desc* d)
+{
+    FoldArgs f{d->A1, d->B1, d->lda1, d->ldb1, d->k1, d->ska < 1 ? 1 : d->ska,
+               d->skb < 1 ? 1 : d->skb, d->fold_len, d->fold_sign, d->mirror_sign,
+               d->mirror_div < 1 ? 1 : d->mirror_div};
+    return f;
+}
+
+// Folded members: one fold mode and transposes for all, variants 1-5.
+static int fold_multi_pack(const pypde_gemm_desc* d, int count, int variant, void* table,
+                           pypde_multi_info* info)
+{
+    int bm, bn;
+    fold_tile_shape(variant, bm, bn);
+    PYPDE_REQUIRE(bm > 0, "multi GEMM: folded members run variants 1-5 (got %d)", variant);
+    FoldMulti* mm = static_cast<FoldMulti*>(table);
+    memset(mm, 0, sizeof(FoldMulti));
+    mm->count = count;
+    const int ta = d[0].transA, tb = d[0].transB, mode = d[0].fold;
+    PYPDE_REQUIRE(mode >= 1 && mode <= 4, "multi GEMM: fold mode %d", mode);
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const pypde_gemm_desc& e = d[i];
+        PYPDE_REQUIRE(e.fold == mode, "multi GEMM: member %d has another fold mode", i);
+        PYPDE_REQUIRE(e.transA == ta && e.transB == tb, "multi GEMM: member %d has other transposes", i);
+        PYPDE_REQUIRE(e.m >= 0 && e.n >= 0 && e.k >= 0 && e.batch >= 0, "negative GEMM size (member %d)", i);
+        PYPDE_REQUIRE(!e.alpha_vec || e.alpha_div >= 1, "alpha_div must be >= 1");
+        PYPDE_REQUIRE(!e.gamma_vec || e.gamma_div >= 1, "gamma_div must be >= 1");
+        PYPDE_REQUIRE(e.fold_len >= 1, "member %d: fold_len must be >= 1", i);
+        const bool live = e.m > 0 && e.n > 0 && e.batch > 0;
+        if (live) PYPDE_REQUIRE(e.A && e.B && e.C, "null GEMM operand (member %d)", i);
+        if (mode <= 2) {
+            PYPDE_REQUIRE(!live || (e.A1 && e.B1 && e.k1 >= 0),
+                          "member %d: output fold needs the second product (A1, B1, k1)", i);
+            const long long half = (e.fold_len + 1) / 2;
+            PYPDE_REQUIRE((mode == 1 ? e.m : e.n) % half == 0,
+                          "member %d: the folded extent is not a multiple of %lld", i, half);
+        } else {
+            PYPDE_REQUIRE(e.fold_len % 2 == 0 && 2LL * e.k == e.fold_len,
+                          "member %d: k must be half of an even fold_len", i);
+        }
+        mm->a[i] = args_of(&e);
+        mm->f[i] = fold_args_of(&e);
+        mm->tx[i] = (e.n + bn - 1) / bn;
+        mm->ty[i] = (e.m + bm - 1) / bm;
+        mm->start[i] = (int)total;
+        if (live) total += (long long)mm->tx[i] * mm->ty[i] * e.batch;
+        PYPDE_REQUIRE(total < (1LL << 31), "multi GEMM too large");
+    }
+    mm->start[count] = (int)total;
+    mm->total = (int)total;
+    info->kind = PYPDE_MULTI_GEMM;
+    info->count = count;
+    info->variant = variant;
+    info->flags = (ta ? 1 : 0) | (tb ? 2 : 0) | (mode << 2);
+    info->total = total;
+    return OK;
+}
+
+template <int BM, int BN, int WGM, int WGN, int MODE>
+static void launch_fold_multi_m(bool ta, bool tb, const FoldMulti* mm, int total, hipStream_t s)
+{
+    dim3 grid(total), block(64 * WGM * WGN);
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, false, false>), grid, block, 0, s, mm);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, false, true>), grid, block, 0, s, mm);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, true, false>), grid, block, 0, s, mm);
+    else hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, true, true>), grid, block, 0, s, mm);
+}
+
+template <int BM, int BN, int WGM, int WGN>
+static void launch_fold_multi(int mode, bool ta, bool tb, const FoldMulti* mm, int total, hipStream_t s)
+{
+    switch (mode) {
+    case 1: launch_fold_multi_m<BM, BN, WGM, WGN, 1>(ta, tb, mm, total, s); break;
+    case 2: launch_fold_multi_m<BM, BN, WGM, WGN, 2>(ta, tb, mm, total, s); break;
+    case 3: launch_fold_multi_m<BM, BN, WGM, WGN, 3>(ta, tb, mm, total, s); break;
+    default: launch_fold_multi_m<BM, BN, WGM, WGN, 4>(ta, tb, mm, total, s); break;
+    }
+}
 
 int gemm_multi_pack(const pypde_gemm_desc* d, int count, int variant, void* table,
                     long long table_bytes, pypde_multi_info* info)
 {
     PYPDE_REQUIRE(d && table && info, "null argument");
-    PYPDE_REQUIRE(table_bytes >= (long long)sizeof(GemmMulti), "GEMM table needs %zu bytes",
-                  sizeof(GemmMulti));
+    PYPDE_REQUIRE(table_bytes >= gemm_multi_bytes(), "GEMM table needs %lld bytes",
+                  gemm_multi_bytes());
     PYPDE_REQUIRE(count >= 1 && count <= kMaxMulti, "multi GEMM: count %d not in 1..%d", count,
                   kMaxMulti);
+    if (d[0].fold != 0) return fold_multi_pack(d, count, variant, table, info);
     int bm, bn;
     multi_tile_shape(variant, bm, bn);
     PYPDE_REQUIRE(bm > 0, "multi GEMM: variant %d is not an LDS-tiled variant", variant);
@@ -1456,6 +1590,22 @@ int dgemm_multi(const void* table, const pypde_multi_info* info, hipStream_t str
     const GemmMulti* mm = static_cast<const GemmMulti*>(table);
     const bool ta = info->flags & 1, tb = info->flags & 2;
     const int total = (int)info->total;
+    const int mode = (info->flags >> 2) & 7;
+    if (mode) {
+        const FoldMulti* fm = static_cast<const FoldMulti*>(table);
+        PYPDE_REQUIRE(mode <= 4, "multi GEMM: fold mode %d", mode);
+        switch (info->variant) {
+        case 1: launch_fold_multi<64, 64, 4, 2>(mode, ta, tb, fm, total, stream); break;
+        case 2: launch_fold_multi<32, 32, 2, 2>(mode, ta, tb, fm, total, stream); break;
+        case 3: launch_fold_multi<128, 128, 4, 4>(mode, ta, tb, fm, total, stream); break;
+        case 4: launch_fold_multi<128, 64, 4, 2>(mode, ta, tb, fm, total, stream); break;
+        case 5: launch_fold_multi<64, 128, 2, 4>(mode, ta, tb, fm, total, stream); break;
+        default:
+            set_error("multi GEMM: folded members run variants 1-5 (got %d)", info->variant);
+            return ERR_ARG;
+        }
+        return check_launch("pypde_dgemm_multi (folded)");
+    }
     switch (info->variant) {
     case 1: launch_gemm_multi<64, 64, 2, 2, false>(ta, tb, mm, total, stream); break;
     case 2: launch_gemm_multi<32, 32, 2, 2, false>(ta, tb, mm, total, stream); break;

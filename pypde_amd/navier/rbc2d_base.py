@@ -178,9 +178,17 @@ class NavierStokesBase:
         return n
 
     def solve_steady_state(self, X0=None, dt=None, maxiter=300, disp=True, tol=1e-8,
-                           jac_options={"inner_maxiter": 30}):
+                           jac_options={"inner_maxiter": 30}, method="newton_krylov"):
         """Steady state by Jacobian-free Newton-Krylov on the GPU (the
         capability of rbc2d_base.py:259-291).
+
+        method="reference" is the reference's own call instead:
+        scipy.optimize.root(steady_fun, X0, args=(self, dt), method="krylov",
+        options={maxiter, disp, fatol=tol, jac_options}) on host vectors
+        (rbc2d_base.py:265-291), with its semantics: the unknown is [T, U, V]
+        alone, and every evaluation of steady_fun advances the pressure held in
+        the solver, so F depends on the call history.  It returns scipy's
+        result and leaves the solver state where the last evaluation left it.
 
         The unknown is [T, U, V, pres]: F = (Phi(z) - z)/tau, Phi the time-step
         map over tau = dt (iterate(dt) steps; dt=None: one step of self.dt)
@@ -195,6 +203,20 @@ class NavierStokesBase:
         Returns a result with x (the [T, U, V] vector), pres, success, status,
         message, nit, nfev, fun (F at x, [T, U, V] part), fnorm (max|F| over
         all of z) and history; the solver state is set to the solution."""
+        if method == "reference":
+            from scipy import optimize
+            if disp:
+                print("\nSolve steady state ...\n")
+            x0 = (self.vectorify() if X0 is None else ops.to_dev(X0).reshape(-1))
+            x0 = x0.detach().cpu().numpy()
+            options = {"maxiter": maxiter, "disp": disp, "fatol": tol,
+                       "jac_options": jac_options}
+
+            def fun(x, ns, dt_):
+                return self.steady_fun(x, ns, dt_).detach().cpu().numpy()
+            return optimize.root(fun, x0, args=(self, dt), method="krylov", options=options)
+        if method != "newton_krylov":
+            raise ValueError("method must be 'newton_krylov' or 'reference' (got %r)" % (method,))
         from ..steady import newton_krylov
         opts = dict(inner_maxiter=30, inner_m=100, rdiff=None)
         opts.update(jac_options or {})

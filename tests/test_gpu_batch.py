@@ -150,3 +150,27 @@ def test_persistent_batch_plan_through_capi_equals_its_launches():
             assert np.array_equal(a, host(hb.state(i)[f])), (i, f)
     assert lib.pypde_rbc_step_batched(None, 1, s) != 0
     assert lib.pypde_rbc_step_batched(plan.handle, -1, s) != 0
+
+
+def test_persistent_groups_match_single_chain():
+    """The persistent form with its size classes in two groups (two chains of
+    multi launches on parallel graph branches) gives what the single chain
+    gives, to rounding: each class runs the same operators, but the tuner may
+    choose another GEMM variant for a launch over fewer members."""
+    from pypde_amd.batch import HeteroBatch, default_dt
+    specs = [dict(shape=(20, 16), ra=1e4, pr=1.0), dict(shape=(24, 20), ra=3e5, pr=1.0),
+             dict(shape=(16, 24), ra=2e6, pr=0.7), dict(shape=(20, 16), ra=5e5, pr=1.0)]
+    for s in specs:
+        s["dt"] = default_dt(s["ra"])
+    seeds = [50 + i for i in range(len(specs))]
+    out = []
+    for groups in (1, 2):
+        hb = HeteroBatch(specs, case="rbc", persistent=True, groups=groups)
+        assert len(hb.group_multi) == groups and len(hb.multi) == 13 * groups
+        hb.seed(seeds, amplitude=0.05)
+        hb.capture(1)
+        hb.step(4)
+        out.append([{f: host(t) for f, t in hb.state(i).items()} for i in range(len(specs))])
+    for a, b in zip(*out):
+        for f in a:
+            assert np.allclose(a[f], b[f], rtol=0, atol=1e-11 * max(1.0, np.abs(a[f]).max())), f

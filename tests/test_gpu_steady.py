@@ -64,3 +64,31 @@ def test_solve_steady_state_reaches_reference_attractor(steady, dt):
     # a further step of the map does not move it
     ns.update()
     assert np.abs(host(ns.vectorify()) - x).max() < 1e-11 * ns.dt * 10
+
+
+def test_reference_shaped_solve_keeps_reference_semantics(steady):
+    """method="reference" is the reference's own call (scipy's root with
+    method="krylov" on steady_fun over [T, U, V], the pressure advancing in
+    the solver with every evaluation, rbc2d_base.py:265-291).  From the state
+    after 300 steps it does not reach the attractor within 20 iterations --
+    F keeps changing with the call history -- where the default Newton-Krylov
+    on [T, U, V, pres] converges (previous test); its first evaluation is
+    steady_fun at X0, the reference's value."""
+    ns = make()
+    X0 = steady["steady_X0"]
+    ns.set_state(pres=steady["steady_pres0"])
+    calls = []
+    orig = ns.steady_fun
+
+    def spy(X, NS=None, dt=None):
+        F = orig(X, NS, dt)
+        calls.append(host(F))
+        return F
+    ns.steady_fun = spy
+    sol = ns.solve_steady_state(X0=X0, maxiter=20, tol=1e-11, disp=False, method="reference")
+    assert np.abs(calls[0] - steady["steady_F0"]).max() < 1e-12 * np.abs(X0).max() / ns.dt
+    assert len(calls) > 20
+    assert not sol.success
+    assert np.abs(sol.fun).max() > 1e-6
+    with pytest.raises(ValueError):
+        ns.solve_steady_state(X0=X0, method="lgmres")
