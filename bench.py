@@ -634,6 +634,33 @@ def gemm_variants():
     return out
 
 
+def batched_launches(nb=64):
+    """Per-launch times of the batched 256x256 step (nb instances, the batch
+    of time_batched): each launch replayed 20 times back to back from a HIP
+    graph, with its bound and its achieved rate."""
+    import torch
+    from pypde_amd.engine import RBCBatch, _GemmGroup, _SplitRhs
+    N = 256
+    rng = np.random.default_rng(2024)
+    ras = 10.0 ** rng.uniform(4, 7, size=nb)
+    dts = np.clip(0.01 * np.sqrt(1e5 / ras), 1e-3, 1e-2)
+    eng = RBCBatch((N, N), [dict(ra=float(r), pr=1.0, dt=float(d)) for r, d in zip(ras, dts)],
+                   case="rbc")
+    eng.capture(1)
+    eng.step(2)
+    torch.cuda.synchronize()
+    out, total = [], 0.0
+    for l in eng.launches:
+        for part in ([l.group, l.reduce] if isinstance(l, _SplitRhs) else [l]):
+            t = launch_graph_time(eng, part)
+            bound, amount = launch_cost(part.name, part, eng)
+            rate = amount / t / (1e12 if bound == "mfma" else 1e9)
+            out.append(dict(launch=part.name, us=round(t * 1e6, 2), bound=bound,
+                            rate=round(rate, 2), unit="TFLOP/s" if bound == "mfma" else "GB/s"))
+            total += t
+    return dict(instances=nb, launches=out, sum_us=round(total * 1e6, 1))
+
+
 def rhs_forms():
     """The banded right-hand side of the step in each kernel form (one output
     per thread, row strips of 4 and 8, LDS tiles), microseconds per launch
@@ -818,6 +845,8 @@ def main():
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
     ap.add_argument("--gemm-variants", action="store_true",
                     help="time every plain GEMM variant on the 256x256 step's GEMM launches")
+    ap.add_argument("--batched-launches", action="store_true",
+                    help="per-launch times of the 64 x 256x256 batched step")
     ap.add_argument("--rhs-forms", action="store_true",
                     help="time the banded right-hand side's kernel forms at the workloads' shapes")
     ap.add_argument("--hetero-launches", action="store_true",
@@ -866,6 +895,9 @@ def main():
         return
     if args.gemm_variants:
         print(json.dumps({"gemm_variants": gemm_variants()}))
+        return
+    if args.batched_launches:
+        print(json.dumps({"batched_launches": batched_launches()}))
         return
     if args.rhs_forms:
         print(json.dumps({"rhs_forms": rhs_forms()}))

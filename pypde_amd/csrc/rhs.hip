@@ -56,13 +56,16 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
     const long long mm = (long long)M0 * M1;
     const double* Xf = X + e * mm;
     const double* tbs = tx + (long long)(f == 2 ? TX_BST : TX_BS) * M0;
+    // the column's (B S)_y coefficients, once for all the strip's rows
+    double cbs[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cbs[c] = ty[(TY_BS + c) * M1 + j];
     // y sums with (B S)_y of the rows i0-2 .. i0+RS+3 of X_f
     double ys[RS + 6];
 #pragma unroll
     for (int q = 0; q < RS + 6; ++q) {
         const int r = i0 - 2 + q;
-        ys[q] = (r >= 0 && r < M0) ? ysum<4>(Xf + (long long)r * M1, j, M1, o4, ty + TY_BS * M1, M1)
-                                   : 0.0;
+        ys[q] = (r >= 0 && r < M0) ? ysum_c<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
     }
     double acc[RS];
 #pragma unroll
@@ -80,12 +83,15 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
         // pressure rows i0+1 .. i0+RS+4 (U, offsets 1,3,5) or i0 .. i0+RS+3 (V, 0,2,4)
         const double* pb = pres + (long long)b * N0 * N1;
         const int first = f == 0 ? 1 : 0;
+        double cp[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cp[c] = ty[((f == 0 ? TY_B : TY_BD) + c) * M1 + j];
         double yp[RS + 4];
 #pragma unroll
         for (int q = 0; q < RS + 4; ++q) {
             const int r = i0 + first + q;
-            yp[q] = (r < N0) ? (f == 0 ? ysum<3>(pb + (long long)r * N1, j, N1, o024, ty + TY_B * M1, M1)
-                                       : ysum<3>(pb + (long long)r * N1, j, N1, o135, ty + TY_BD * M1, M1))
+            yp[q] = (r < N0) ? (f == 0 ? ysum_c<3>(pb + (long long)r * N1, j, N1, o024, cp)
+                                       : ysum_c<3>(pb + (long long)r * N1, j, N1, o135, cp))
                              : 0.0;
         }
         const double* tp = tx + (long long)(f == 0 ? TX_BD : TX_B) * M0;
@@ -109,7 +115,7 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
         for (int q = 0; q < RS + 6; ++q) {
             const int r = i0 - 2 + q;
-            yt[q] = (r >= 0 && r < M0) ? ysum<4>(XT + (long long)r * M1, j, M1, o4, ty + TY_BS * M1, M1)
+            yt[q] = (r >= 0 && r < M0) ? ysum_c<4>(XT + (long long)r * M1, j, M1, o4, cbs)
                                        : 0.0;
         }
         const double* tt = tx + (long long)TX_BST * M0;
@@ -167,9 +173,10 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 // (instance, field) entry.  First the y-direction sums of every input row the
 // tile needs -- rows i0-2 .. i0+kRhsTI+3 of X_f, and of X_T and of the pressure
 // where the field has those terms -- go to LDS, one column per lane, each formed
-// once; then every thread sums its outputs over the x diagonals from LDS.  About
-// six global loads per output instead of 25-60.  The same terms in the same
-// order as rhs_banded_at: bit-identical.
+// once (with the column's y coefficients held in registers); then every thread
+// sums its outputs over the x diagonals from LDS.  About 6-17 global loads per
+// output instead of 25-95.  The same terms in the same order as rhs_banded_at:
+// bit-identical.
 constexpr int kRhsTI = 16, kRhsTJ = 64, kRhsRows = kRhsTI + 6;
 
 __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long long e, int ti, int tj)
@@ -191,27 +198,33 @@ __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long lon
     const int o024[3] = {0, 2, 4};
     const int o135[3] = {1, 3, 5};
     const int pfirst = f == 0 ? 1 : 0;   // first pressure row: U offsets 1,3,5; V 0,2,4
-    // y sums: LDS row q holds X row i0 - 2 + q and pressure row i0 + pfirst + q
+    // this column's y coefficients, loaded once for all the tile's rows
+    double cbs[4], cd2[2], cp[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cbs[c] = jin ? a.ty[(TY_BS + c) * M1 + j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) cd2[c] = (jin && a.fdiff) ? a.ty[(TY_BD2S + c) * M1 + j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        cp[c] = (jin && f < 2) ? a.ty[((f == 0 ? TY_B : TY_BD) + c) * M1 + j] : 0.0;
+    // y sums (ysum's terms and order, the coefficients from registers): LDS row
+    // q holds X row i0 - 2 + q and pressure row i0 + pfirst + q
     for (int q = w; q < kRhsRows; q += 4) {
         if (!jin) break;
         const int r = i0 - 2 + q;
         const bool rin = r >= 0 && r < M0;
-        ys[q][lane] = rin ? ysum<4>(Xf + (long long)r * M1, j, M1, o4, a.ty + TY_BS * M1, M1) : 0.0;
+        ys[q][lane] = rin ? ysum_c<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
         if (f == 1)
-            yt[q][lane] = rin ? ysum<4>(a.X + (e + 1) * mm + (long long)r * M1, j, M1, o4,
-                                        a.ty + TY_BS * M1, M1)
+            yt[q][lane] = rin ? ysum_c<4>(a.X + (e + 1) * mm + (long long)r * M1, j, M1, o4, cbs)
                               : 0.0;
         if (f < 2) {
             const int rp = i0 + pfirst + q;
-            yp[q][lane] = rp < N0 ? (f == 0 ? ysum<3>(pb + (long long)rp * N1, j, N1, o024,
-                                                      a.ty + TY_B * M1, M1)
-                                            : ysum<3>(pb + (long long)rp * N1, j, N1, o135,
-                                                      a.ty + TY_BD * M1, M1))
+            yp[q][lane] = rp < N0 ? (f == 0 ? ysum_c<3>(pb + (long long)rp * N1, j, N1, o024, cp)
+                                            : ysum_c<3>(pb + (long long)rp * N1, j, N1, o135, cp))
                                   : 0.0;
         }
         if (a.fdiff)
-            yd[q][lane] = rin ? ysum<2>(Xf + (long long)r * M1, j, M1, o02, a.ty + TY_BD2S * M1, M1)
-                              : 0.0;
+            yd[q][lane] = rin ? ysum_c<2>(Xf + (long long)r * M1, j, M1, o02, cd2) : 0.0;
     }
     __syncthreads();
     if (!jin) return;

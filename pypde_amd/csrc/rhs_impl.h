@@ -30,6 +30,21 @@ __device__ __forceinline__ double ysum(const double* __restrict__ row, int j, in
     return s;
 }
 
+// ysum with the NY coefficients of column j already in registers
+template <int NY>
+__device__ __forceinline__ double ysum_c(const double* __restrict__ row, int j, int ncol,
+                                         const int (&oy)[NY], const double (&coef)[NY])
+{
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NY; ++b) {
+        const int c = j + oy[b];
+        const double v = (c >= 0 && c < ncol) ? row[c] : 0.0;
+        s = s + v * coef[b];
+    }
+    return s;
+}
+
 // sum_a tx[a][i] * ysum(row i + ox[a])
 template <int NX, int NY>
 __device__ __forceinline__ double band2(const double* __restrict__ in, long long ld, int nrow, int ncol,
