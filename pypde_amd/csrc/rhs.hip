@@ -181,10 +181,13 @@ constexpr int kRhsTI = 16, kRhsTJ = 64, kRhsRows = kRhsTI + 6;
 
 __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long long e, int ti, int tj)
 {
-    __shared__ double ys[kRhsRows][kRhsTJ];   // (B S)_y sums of X_f
-    __shared__ double yt[kRhsRows][kRhsTJ];   // (B S)_y sums of X_T (buoyancy, V)
-    __shared__ double yp[kRhsRows][kRhsTJ];   // pressure sums (U: B_y, V: (B D)_y)
-    __shared__ double yd[kRhsRows][kRhsTJ];   // (B D2 S)_y sums of X_f (explicit diffusion)
+    // LDS of the launch (rhs_tile_lds_bytes): three arrays, four with the
+    // explicit diffusion
+    extern __shared__ double rhs_lds[];
+    double (*ys)[kRhsTJ] = reinterpret_cast<double (*)[kRhsTJ]>(rhs_lds);   // (B S)_y sums of X_f
+    double (*yt)[kRhsTJ] = ys + kRhsRows;      // (B S)_y sums of X_T (buoyancy, V)
+    double (*yp)[kRhsTJ] = yt + kRhsRows;      // pressure sums (U: B_y, V: (B D)_y)
+    double (*yd)[kRhsTJ] = yp + kRhsRows;      // (B D2 S)_y sums of X_f (explicit diffusion)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int M0 = a.M0, M1 = a.M1, N0 = a.N0, N1 = a.N1;
     const int b = (int)(e / 3), f = (int)(e - 3LL * b);
@@ -209,8 +212,37 @@ __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long lon
         cp[c] = (jin && f < 2) ? a.ty[((f == 0 ? TY_B : TY_BD) + c) * M1 + j] : 0.0;
     // y sums (ysum's terms and order, the coefficients from registers): LDS row
     // q holds X row i0 - 2 + q and pressure row i0 + pfirst + q
-    for (int q = w; q < kRhsRows; q += 4) {
-        if (!jin) break;
+    // the constant terms and x coefficients of this thread's outputs (rows
+    // i0 + 4 w + k), fetched while the y sums form
+    const double* tbs = a.tx + (long long)(f == 2 ? TX_BST : TX_BS) * M0;
+    const double* tp = a.tx + (long long)(f == 0 ? TX_BD : TX_B) * M0;
+    const double* tt = a.tx + (long long)TX_BST * M0;
+    const double* td2 = a.tx + (long long)(f == 2 ? TX_BD2ST : TX_BD2S) * M0;
+    constexpr int KR = kRhsTI / 4;
+    double cv[KR], xbs[KR][4], xp[KR][3], xt[KR][4], xd[KR][2];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int i = i0 + 4 * w + k;
+        const bool ok = jin && i < M0;
+        cv[k] = (a.cst && ok) ? a.cst[e * mm + (long long)i * M1 + j] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) xbs[k][d] = ok ? tbs[d * M0 + i] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xp[k][d] = (ok && f < 2) ? tp[d * M0 + i] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) xt[k][d] = (ok && f == 1) ? tt[d * M0 + i] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 2; ++d) xd[k][d] = (ok && a.fdiff) ? td2[d * M0 + i] : 0.0;
+    }
+    const double mb = f < 2 ? a.mdta[b] : 0.0;
+    const double db = f == 1 ? a.dta[b] : 0.0;
+    const double fe = a.fdiff ? a.fdiff[e] : 0.0;
+    // unrolled over the wave's rows, so that the loads of all of them are in
+    // flight together
+#pragma unroll
+    for (int t = 0; t < (kRhsRows + 3) / 4; ++t) {
+        const int q = w + 4 * t;
+        if (q >= kRhsRows || !jin) continue;
         const int r = i0 - 2 + q;
         const bool rin = r >= 0 && r < M0;
         ys[q][lane] = rin ? ysum_c<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
@@ -228,9 +260,8 @@ __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long lon
     }
     __syncthreads();
     if (!jin) return;
-    const double* tbs = a.tx + (long long)(f == 2 ? TX_BST : TX_BS) * M0;
 #pragma unroll
-    for (int k = 0; k < kRhsTI / 4; ++k) {
+    for (int k = 0; k < KR; ++k) {
         const int i = i0 + 4 * w + k;
         if (i >= M0) break;
         const int qi = i - i0 + 2;   // LDS row of X row i
@@ -238,45 +269,42 @@ __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long lon
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int r = i + o4[d];
-            if (r >= 0 && r < M0) acc = acc + tbs[d * M0 + i] * ys[qi + o4[d]][lane];
+            if (r >= 0 && r < M0) acc = acc + xbs[k][d] * ys[qi + o4[d]][lane];
         }
         if (f < 2) {
-            const double* tp = a.tx + (long long)(f == 0 ? TX_BD : TX_B) * M0;
             double g = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int off = f == 0 ? o135[d] : o024[d];
-                if (i + off < N0) g = g + tp[d * M0 + i] * yp[i - i0 + off - pfirst][lane];
+                if (i + off < N0) g = g + xp[k][d] * yp[i - i0 + off - pfirst][lane];
             }
-            acc = acc + a.mdta[b] * g;
+            acc = acc + mb * g;
         }
         if (f == 1) {
-            const double* tt = a.tx + (long long)TX_BST * M0;
             double bt = 0.0;
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int r = i + o4[d];
-                if (r >= 0 && r < M0) bt = bt + tt[d * M0 + i] * yt[qi + o4[d]][lane];
+                if (r >= 0 && r < M0) bt = bt + xt[k][d] * yt[qi + o4[d]][lane];
             }
-            acc = acc + a.dta[b] * bt;
+            acc = acc + db * bt;
         }
         if (a.fdiff) {
-            const double* td2 = a.tx + (long long)(f == 2 ? TX_BD2ST : TX_BD2S) * M0;
             double dxx = 0.0, dyy = 0.0;
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const int r = i + o02[d];
-                if (r < M0) dxx = dxx + td2[d * M0 + i] * ys[qi + o02[d]][lane];
+                if (r < M0) dxx = dxx + xd[k][d] * ys[qi + o02[d]][lane];
             }
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int r = i + o4[d];
-                if (r >= 0 && r < M0) dyy = dyy + tbs[d * M0 + i] * yd[qi + o4[d]][lane];
+                if (r >= 0 && r < M0) dyy = dyy + xbs[k][d] * yd[qi + o4[d]][lane];
             }
-            acc = acc + a.fdiff[e] * (dxx + dyy);
+            acc = acc + fe * (dxx + dyy);
         }
         const long long o = e * mm + (long long)i * M1 + j;
-        a.R[o] = a.cst ? a.cst[o] + acc : acc;
+        a.R[o] = a.cst ? cv[k] + acc : acc;
     }
 }
 
@@ -292,6 +320,13 @@ __device__ __forceinline__ void rhs_tile_at(const pypde_rbc_rhs& a, long long t)
     const long long e = t / per;
     const int r = (int)(t - e * per);
     rhs_banded_tile(a, e, r / ntj, r % ntj);
+}
+
+// LDS bytes of a tile launch: the y-sum arrays it uses (the diffusion one only
+// when some member has the explicit diffusion)
+static size_t rhs_tile_lds_bytes(bool diffusion)
+{
+    return sizeof(double) * kRhsRows * kRhsTJ * (diffusion ? 4 : 3);
 }
 
 __global__ void __launch_bounds__(256) rbc_rhs_tile_kernel(pypde_rbc_rhs a)
@@ -408,10 +443,12 @@ extern "C" int pypde_rhs_multi_pack(const pypde_rbc_rhs* args, int count, void* 
     }
     mm->start[count] = (int)total;
     mm->total = (int)total;
+    bool diffusion = false;
+    for (int i = 0; i < count; ++i) diffusion = diffusion || args[i].fdiff != nullptr;
     info->kind = PYPDE_MULTI_RHS;
     info->count = count;
     info->variant = rs;
-    info->flags = 0;
+    info->flags = diffusion ? 1 : 0;   // bit 0: some member has the explicit diffusion
     info->total = total;
     return OK;
 }
@@ -437,7 +474,8 @@ extern "C" int pypde_rhs_multi(const void* table, const pypde_multi_info* info, 
                            as_stream(stream), mm);
         break;
     case 16:
-        hipLaunchKernelGGL(rbc_rhs_banded_multi<16>, dim3((unsigned)info->total), dim3(256), 0,
+        hipLaunchKernelGGL(rbc_rhs_banded_multi<16>, dim3((unsigned)info->total), dim3(256),
+                           rhs_tile_lds_bytes(info->flags & 1),
                            as_stream(stream), mm);
         break;
     default:
@@ -471,7 +509,8 @@ extern "C" int pypde_rbc_rhs_banded(const pypde_rbc_rhs* a, void* stream)
         const long long blocks =
             3LL * a->batch * ((a->M0 + 15) / 16) * ((a->M1 + 63) / 64);
         PYPDE_REQUIRE(blocks < (1LL << 31), "right-hand side too large");
-        hipLaunchKernelGGL(rbc_rhs_tile_kernel, dim3((unsigned)blocks), dim3(256), 0,
+        hipLaunchKernelGGL(rbc_rhs_tile_kernel, dim3((unsigned)blocks), dim3(256),
+                           rhs_tile_lds_bytes(a->fdiff != nullptr),
                            as_stream(stream), *a);
         return check_launch("pypde_rbc_rhs_banded");
     }
