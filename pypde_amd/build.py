@@ -31,6 +31,7 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=true",
     "-I" + os.path.join(REPO, "include"),
     "-Wno-unused-result",
 ]
