@@ -798,9 +798,24 @@ struct FoldArgs {
     int mirror_div;
 };
 
-template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
-__device__ __forceinline__ void dgemm_fold_body(const GemmArgs& g, const FoldArgs& f, int bx, int by,
-                                                int bz)
+// Folded GEMMs of one mode and tile form for many members in one launch (the
+// folded x-pass of every size class of a heterogeneous batch): the plain multi
+// table layout plus each member's fold arguments.
+struct FoldMulti {
+    int count, total;
+    int start[kMaxMulti + 1];
+    int tx[kMaxMulti], ty[kMaxMulti];
+    GemmArgs a[kMaxMulti];
+    FoldArgs f[kMaxMulti];
+};
+
+// MULTI = false: one folded GEMM, its tile from the launch grid (mm unused).
+// MULTI = true: the member and tile of this block from the table mm (g and f
+// unused).  One kernel body for both, so that the single form keeps its own
+// register allocation.
+template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB, bool MULTI>
+__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0, FoldArgs f0,
+                                                                   const FoldMulti* __restrict__ mm)
 {
     constexpr int BK = 16;
     constexpr int NT = 64 * WGM * WGN;
@@ -824,6 +839,29 @@ __device__ __forceinline__ void dgemm_fold_body(const GemmArgs& g, const FoldArg
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave / WGN, wn = wave % WGN;
+    int bx, by, bz;
+    GemmArgs g;
+    FoldArgs f;
+    if constexpr (MULTI) {
+        const int t = xcd_remap(blockIdx.x, mm->total);
+        int lo = 0, hi = mm->count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (mm->start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int local = t - mm->start[lo];
+        bx = local % mm->tx[lo];
+        const int r = local / mm->tx[lo];
+        by = r % mm->ty[lo];
+        bz = r / mm->ty[lo];
+        g = mm->a[lo];
+        f = mm->f[lo];
+    } else {
+        grid_tile(bx, by, bz);
+        g = g0;
+        f = f0;
+    }
     const int m0 = by * BM;
     const int n0 = bx * BN;
     const long long oA = bofs(bz, g.iA), oB = bofs(bz, g.iB);
@@ -958,43 +996,6 @@ __device__ __forceinline__ void dgemm_fold_body(const GemmArgs& g, const FoldArg
                     if (x2 != x1) ep.store(gm, x2, sg * (e - o));
                 }
             }
-}
-
-template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g, FoldArgs f)
-{
-    int bx, by, bz;
-    grid_tile(bx, by, bz);
-    dgemm_fold_body<BM, BN, WGM, WGN, MODE, TA, TB>(g, f, bx, by, bz);
-}
-
-// Folded GEMMs of one mode and tile form for many members in one launch (the
-// folded x-pass of every size class of a heterogeneous batch): the plain multi
-// table layout plus each member's fold arguments.
-struct FoldMulti {
-    int count, total;
-    int start[kMaxMulti + 1];
-    int tx[kMaxMulti], ty[kMaxMulti];
-    GemmArgs a[kMaxMulti];
-    FoldArgs f[kMaxMulti];
-};
-
-template <int BM, int BN, int WGM, int WGN, int MODE, bool TA, bool TB>
-__global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_multi(const FoldMulti* __restrict__ mm)
-{
-    const int t = xcd_remap(blockIdx.x, mm->total);
-    int lo = 0, hi = mm->count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (mm->start[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const int local = t - mm->start[lo];
-    const int bx = local % mm->tx[lo];
-    const int r = local / mm->tx[lo];
-    const GemmArgs g = mm->a[lo];
-    const FoldArgs f = mm->f[lo];
-    dgemm_fold_body<BM, BN, WGM, WGN, MODE, TA, TB>(g, f, bx, r % mm->ty[lo], r / mm->ty[lo]);
 }
 
 // Output folds (modes 1, 2) split over waves: waves [0, KW) run the first
@@ -1170,10 +1171,11 @@ static void launch_fold_m(bool ta, bool tb, const GemmArgs& g, const FoldArgs& f
 {
     dim3 grid((g.n + BN - 1) / BN, (g.m + BM - 1) / BM, batch);
     dim3 block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, false>), grid, block, 0, s, g, f);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, true>), grid, block, 0, s, g, f);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, false>), grid, block, 0, s, g, f);
-    else hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, true>), grid, block, 0, s, g, f);
+    const FoldMulti* none = nullptr;
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, false, false>), grid, block, 0, s, g, f, none);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, true, false>), grid, block, 0, s, g, f, none);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, false, false>), grid, block, 0, s, g, f, none);
+    else hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, true, false>), grid, block, 0, s, g, f, none);
 }
 
 template <int BM, int BN, int WGM, int WGN>
@@ -1522,10 +1524,12 @@ template <int BM, int BN, int WGM, int WGN, int MODE>
 static void launch_fold_multi_m(bool ta, bool tb, const FoldMulti* mm, int total, hipStream_t s)
 {
     dim3 grid(total), block(64 * WGM * WGN);
-    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, false, false>), grid, block, 0, s, mm);
-    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, false, true>), grid, block, 0, s, mm);
-    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, true, false>), grid, block, 0, s, mm);
-    else hipLaunchKernelGGL((dgemm_fold_multi<BM, BN, WGM, WGN, MODE, true, true>), grid, block, 0, s, mm);
+    const GemmArgs g{};
+    const FoldArgs f{};
+    if (!ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, false, true>), grid, block, 0, s, g, f, mm);
+    else if (!ta && tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, false, true, true>), grid, block, 0, s, g, f, mm);
+    else if (ta && !tb) hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, false, true>), grid, block, 0, s, g, f, mm);
+    else hipLaunchKernelGGL((dgemm_fold_kernel<BM, BN, WGM, WGN, MODE, true, true, true>), grid, block, 0, s, g, f, mm);
 }
 
 template <int BM, int BN, int WGM, int WGN>
