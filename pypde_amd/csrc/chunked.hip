@@ -221,20 +221,30 @@ __global__ void __launch_bounds__(MAXT, (rec_min_waves<CH, PRE, PERLINE, LPB>())
 
     // LDS: [present tables, n each (shared mode)] [transpose tiles (axis 1), reused
     // as the carry-exchange area P*12*LPB during the sweeps]
+    //
+    // Every global load below is unconditional: an element past the end of
+    // the input line or past the last line reads a clamped, valid address.
+    // Such values are never used (the sweeps skip positions >= n, and n_in >=
+    // n; lanes past the last line store nothing), so they are not even
+    // replaced by zeros: a load whose value is selected under a condition is
+    // moved by the compiler into a branch that waits for it before the next
+    // load issues, while unconditional loads are all in flight together
+    // (tables first, then the data, so that the table copy into LDS waits
+    // only for the tables).
     double* ltab = lds;
-    const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
-                     (a.NU != nullptr) + (a.pre != nullptr);
+    constexpr bool kPresent[6] = {HASF, FWD, HASR, true, HASNU, PRE};  // F, LM, R, MU, NU, pre
+    constexpr int ntab = HASF + FWD + HASR + 1 + HASNU + PRE;
     double* tiles = lds + (PERLINE ? 0 : ntab * n);
     double* xch = tiles;
+    // one table entry per thread (the block has at least n threads)
+    double tv[6];
     if (!PERLINE) {
         const long long eo = (long long)b * a.tbs;
         const double* src_t[6] = {a.F, a.LM, a.R, a.MU, a.NU, a.pre};
-        int slot = 0;
-        for (int t = 0; t < 6; ++t) {
-            if (!src_t[t]) continue;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) ltab[slot * n + i] = src_t[t][eo + i];
-            ++slot;
-        }
+        const int i = min((int)threadIdx.x, n - 1);
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+            if (kPresent[t]) tv[t] = src_t[t][eo + i];
     }
     Tab<PERLINE, HASF, FWD, HASR, HASNU> T;
     T.nlines = a.nlines;
@@ -243,63 +253,86 @@ __global__ void __launch_bounds__(MAXT, (rec_min_waves<CH, PRE, PERLINE, LPB>())
     T.g = PERLINE ? nullptr : ltab;
     constexpr int NX = PRE ? CH + 2 : CH;
     double v[NX];
+    const int lc = min(line, a.nlines - 1);  // clamped line of this lane
 
     if constexpr (AXIS == 0) {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int pos = base + k;
-            v[k] = (live && pos < a.n_in) ? src[(long long)pos * a.src_ld + line] : 0.0;
-        }
-    } else if constexpr (LPB == 64) {
-        double* tile = tiles + wv * 64 * TP;
-#pragma unroll
-        for (int t0 = 0; t0 < CH; t0 += TW) {
-#pragma unroll
-            for (int q = 0; q < TW; ++q) {
-                const int r = (lane / TW) + q * (64 / TW);
-                const int c = lane % TW;
-                const int pos = base + t0 + c;
-                const int ln = l0 + r;
-                tile[r * TP + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
-            }
-            wave_lds_sync();
-#pragma unroll
-            for (int c = 0; c < TW; ++c) v[t0 + c] = tile[lane * TP + c];
-            wave_lds_sync();
+            v[k] = src[(long long)min(pos, a.n_in - 1) * a.src_ld + lc];
         }
     } else {
-        // 32 lines x 32 positions per round; CH = 16: one round holds both
-        // half-waves' chunks, CH = 32: round h holds half-wave h's chunk
-        double* tile = tiles + wv * 32 * TP32;
-        const int wbase = wv * 2 * CH;
+        // positions of the lane's transposed tiles, loaded before any tile is
+        // written: LPB 64 -- CH / TW rounds of 64 lines x TW positions;
+        // LPB 32 -- CH / 16 rounds of 32 lines x 32 positions
+        constexpr int NR = LPB == 64 ? CH / TW : CH / 16;
+        constexpr int NQ = LPB == 64 ? TW : 16;
+        double tl[NR][NQ];
 #pragma unroll
-        for (int rr = 0; rr < CH / 16; ++rr) {
+        for (int rr = 0; rr < NR; ++rr)
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int r = (lane >> 5) + 2 * q;
-                const int c = lane & 31;
-                const int pos = wbase + rr * 32 + c;
+            for (int q = 0; q < NQ; ++q) {
+                const int r = LPB == 64 ? (lane / TW) + q * (64 / TW) : (lane >> 5) + 2 * q;
+                const int pos = LPB == 64 ? base + rr * TW + lane % TW : wv * 2 * CH + rr * 32 + (lane & 31);
                 const int ln = l0 + r;
-                tile[r * TP32 + c] = (ln < a.nlines && pos < a.n_in) ? src[(long long)ln * a.src_ld + pos] : 0.0;
+                tl[rr][q] = src[(long long)min(ln, a.nlines - 1) * a.src_ld + min(pos, a.n_in - 1)];
             }
-            wave_lds_sync();
-            if (CH == 16) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) v[k] = tile[li * TP32 + (lane >> 5) * 16 + k];
-            } else if ((lane >> 5) == rr) {
-#pragma unroll
-                for (int k = 0; k < CH && k < 32; ++k) v[k] = tile[li * TP32 + k];
-            }
-            wave_lds_sync();
-        }
-    }
-    if constexpr (AXIS == 1) {
         // the pre-stencil's two look-ahead positions: direct loads
 #pragma unroll
         for (int k = CH; k < NX; ++k) {
             const int pos = base + k;
-            v[k] = (live && pos < a.n_in) ? src[(long long)line * a.src_ld + pos] : 0.0;
+            v[k] = src[(long long)lc * a.src_ld + min(pos, a.n_in - 1)];
         }
+        if (!PERLINE) {
+            const int i = (int)threadIdx.x;
+            int slot = 0;
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                if (kPresent[t]) {
+                    if (i < n) ltab[slot * n + i] = tv[t];
+                    ++slot;
+                }
+        }
+        if constexpr (LPB == 64) {
+            double* tile = tiles + wv * 64 * TP;
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) tile[((lane / TW) + q * (64 / TW)) * TP + lane % TW] = tl[rr][q];
+                wave_lds_sync();
+#pragma unroll
+                for (int c = 0; c < TW; ++c) v[rr * TW + c] = tile[lane * TP + c];
+                wave_lds_sync();
+            }
+        } else {
+            // CH = 16: one round holds both half-waves' chunks, CH = 32: round
+            // rr holds half-wave rr's chunk
+            double* tile = tiles + wv * 32 * TP32;
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) tile[((lane >> 5) + 2 * q) * TP32 + (lane & 31)] = tl[rr][q];
+                wave_lds_sync();
+                if (CH == 16) {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) v[k] = tile[li * TP32 + (lane >> 5) * 16 + k];
+                } else if ((lane >> 5) == rr) {
+#pragma unroll
+                    for (int k = 0; k < CH && k < 32; ++k) v[k] = tile[li * TP32 + k];
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+    if (AXIS == 0 && !PERLINE) {
+        const int i = (int)threadIdx.x;
+        int slot = 0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+            if (kPresent[t]) {
+                if (i < n) ltab[slot * n + i] = tv[t];
+                ++slot;
+            }
     }
     // table copy visible; every wave is done with its tile before the area is
     // reused for the carry exchange
@@ -383,6 +416,7 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
     const long long blocks = (long long)bpe * a.batch;
     if (blocks == 0) return OK;
     PYPDE_REQUIRE(waves * 64 <= MAXT, "recurrence: %d waves exceed the launch bound", waves);
+    PYPDE_REQUIRE(waves * 64 >= a.n, "recurrence: %d threads cannot copy %d table entries", waves * 64, a.n);
     PYPDE_REQUIRE(blocks <= 0x7fffffffLL, "recurrence: %lld blocks exceed the grid", blocks);
     const int ntab = (a.F != nullptr) + (a.LM != nullptr) + (a.R != nullptr) + (a.MU != nullptr) +
                      (a.NU != nullptr) + (a.pre != nullptr);
@@ -449,10 +483,15 @@ static int launch_rec(const RecArgs& a, hipStream_t s, bool half)
 // from global memory (shared tables are a few KB and stay in cache; per-line
 // tables are read line-major, [nlines][n], so a wave reads its line's
 // positions contiguously).  No LDS and no block barrier: waves are independent.
-template <int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
+// The lane's table entries and inputs are fetched into registers before any
+// of them is used: positions past the line's end read a clamped, valid entry
+// (the last position of the line) whose value is then discarded, so every
+// load is unconditional and the wave keeps all of them in flight at once
+// instead of waiting for each behind a branch.
+template <int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU>
 struct LaneTab {
-    const double* t[6];   // F, LM, R, MU, NU, pre, already offset to this line / entry
-    __device__ __forceinline__ double operator()(int which, int pos) const { return t[which][pos]; }
+    double t[6][C];  // F, LM, R, MU, NU, pre at the lane's positions (present tables only)
+    __device__ __forceinline__ double operator()(int which, int k) const { return t[which][k]; }
 };
 
 template <int AXIS, int C, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool LINEMAJOR>
@@ -467,28 +506,33 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
     const double* __restrict__ src = a.src + (long long)b * a.src_bs;
     double* __restrict__ dst = a.dst + (long long)b * a.dst_bs;
 
-    LaneTab<C, PRE, FWD, HASF, HASR, HASNU, LINEMAJOR> T;
+    LaneTab<C, PRE, FWD, HASF, HASR, HASNU> T;
     {
         const double* tabs[6] = {a.F, a.LM, a.R, a.MU, a.NU, a.pre};
+        const bool present[6] = {HASF, FWD, HASR, true, HASNU, PRE};
         const long long off = (long long)b * a.tbs + (LINEMAJOR ? (long long)line * n : 0);
 #pragma unroll
-        for (int t = 0; t < 6; ++t) T.t[t] = tabs[t] ? tabs[t] + off : nullptr;
+        for (int t = 0; t < 6; ++t) {
+            if (!present[t]) continue;
+            const double* __restrict__ tp = tabs[t] + off;
+#pragma unroll
+            for (int k = 0; k < C; ++k) T.t[t][k] = tp[min(base + k, n - 1)];
+        }
     }
 
     double v[NX];
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
         const int pos = base + k;
-        double x = 0.0;
-        if (pos < a.n_in)
-            x = AXIS == 0 ? src[(long long)pos * a.src_ld + line] : src[(long long)line * a.src_ld + pos];
-        v[k] = x;
+        const int pc = min(pos, a.n_in - 1);
+        const double x = AXIS == 0 ? src[(long long)pc * a.src_ld + line] : src[(long long)line * a.src_ld + pc];
+        v[k] = pos < a.n_in ? x : 0.0;
     }
 
     if (PRE) {
 #pragma unroll
         for (int k = 0; k < C; ++k)
-            if (base + k < n) v[k] = v[k] + T(5, base + k) * v[k + 2];
+            if (base + k < n) v[k] = v[k] + T(5, k) * v[k + 2];
     }
 
     if (FWD) {
@@ -498,8 +542,8 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
         for (int k = 0; k < C; ++k) {
             const int p = k & 1;
             if (base + k < n) {
-                const double f = HASF ? T(0, base + k) : 1.0;
-                const double m = T(1, base + k);
+                const double f = HASF ? T(0, k) : 1.0;
+                const double m = T(1, k);
                 B[p] = f * v[k] + m * B[p];
                 A[p] = m * A[p];
             }
@@ -527,8 +571,8 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
         for (int k = 0; k < C; ++k) {
             const int p = k & 1;
             if (base + k < n) {
-                const double f = HASF ? T(0, base + k) : 1.0;
-                y[p] = f * v[k] + T(1, base + k) * y[p];
+                const double f = HASF ? T(0, k) : 1.0;
+                y[p] = f * v[k] + T(1, k) * y[p];
                 v[k] = y[p];
             }
         }
@@ -542,9 +586,9 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
     for (int k = C - 1; k >= 0; --k) {
         const int p = k & 1;
         if (base + k < n) {
-            const double r = HASR ? T(2, base + k) : 1.0;
-            const double mu = T(3, base + k);
-            const double nu = HASNU ? T(4, base + k) : 0.0;
+            const double r = HASR ? T(2, k) : 1.0;
+            const double mu = T(3, k);
+            const double nu = HASNU ? T(4, k) : 0.0;
             const double sv = r * v[k] + mu * s1[p] + nu * s2[p];
             const double al = mu * a1[p] + nu * a2[p];
             const double be = mu * b1[p] + nu * b2[p];
@@ -581,8 +625,8 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
     for (int k = C - 1; k >= 0; --k) {
         const int p = k & 1;
         if (base + k < n) {
-            const double r = HASR ? T(2, base + k) : 1.0;
-            const double z = r * v[k] + T(3, base + k) * Z1[p] + (HASNU ? T(4, base + k) * Z2[p] : 0.0);
+            const double r = HASR ? T(2, k) : 1.0;
+            const double z = r * v[k] + T(3, k) * Z1[p] + (HASNU ? T(4, k) * Z2[p] : 0.0);
             v[k] = z;
             Z2[p] = Z1[p];
             Z1[p] = z;
