@@ -93,7 +93,9 @@ struct GemmArgs {
 // Output update shared by every kernel family:
 //   C = (beta D - (g_e E) gamma) + alpha_e acc      (E term and D term optional)
 // in the order of the reference's pressure update (pres -= nu div beta, then
-// pres += ...), with the gauge C[0,0] = 0 applied last when requested.
+// pres += ...), with the gauge C[0,0] = 0 applied last when requested.  The
+// arithmetic is spelled out (one FMA for the E term, every other product and
+// sum rounded on its own) so that each kernel family rounds it the same way.
 struct Epilogue {
     double* C;
     const double* D;
@@ -116,20 +118,78 @@ struct Epilogue {
         beta_e_gamma = g.gamma;
         zero_first = g.zero_first != 0;
     }
-    __device__ __forceinline__ void store(int gm, int gn, double acc) const
+    // beta D - (g_e E) gamma at (gm, gn), for the terms present (HD, HE)
+    template <bool HD, bool HE>
+    __device__ __forceinline__ double addend(int gm, int gn) const
     {
-        double v = alpha * acc;
+#pragma clang fp contract(off)
         const long long cn = (long long)gn * csn;
-        if (D || E) {
-            double d = D ? beta * D[(long long)gm * ldd + cn] : 0.0;
-            if (E) d = d - (ge * E[(long long)gm * lde + cn]) * beta_e_gamma;
-            v = d + v;
-        }
+        double d = HD ? beta * D[(long long)gm * ldd + cn] : 0.0;
+        if (HE) d = __builtin_fma(-(ge * E[(long long)gm * lde + cn]), beta_e_gamma, d);
+        return d;
+    }
+    __device__ __forceinline__ void put(int gm, int gn, double acc, double add, bool has_add) const
+    {
+#pragma clang fp contract(off)
+        double v = alpha * acc;
+        if (has_add) v = add + v;
         if (zero_first && gm == 0 && gn == 0) v = 0.0;
-        C[(long long)gm * ldc + cn] = v;
+        C[(long long)gm * ldc + (long long)gn * csn] = v;
     }
     double beta_e_gamma;
 };
+
+// Stores the NE output elements a thread owns; elem(e, gm, gn, ok, v) gives
+// element e's position, whether it lies inside C, and its accumulated value.
+// With a D or E term the elements go in groups of four: the group's D / E
+// loads are issued first, at a clamped address (element (0, 0)) for elements
+// outside C, so that the wave waits once per group instead of once per element
+// behind a branch; groups of four keep the registers held by the loads few.
+constexpr int kEpiGroup = 4;  // elements per group of epilogue loads
+template <bool HD, bool HE, int NE, class Elem>
+__device__ __forceinline__ void epilogue_groups(const Epilogue& ep, Elem elem)
+{
+#pragma unroll
+    for (int c = 0; c < NE; c += kEpiGroup) {
+        double add[kEpiGroup];
+#pragma unroll
+        for (int u = 0; u < kEpiGroup; ++u) {
+            if (c + u >= NE) break;
+            int gm, gn;
+            bool ok;
+            double v;
+            elem(c + u, gm, gn, ok, v);
+            add[u] = ep.addend<HD, HE>(ok ? gm : 0, ok ? gn : 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kEpiGroup; ++u) {
+            if (c + u >= NE) break;
+            int gm, gn;
+            bool ok;
+            double v;
+            elem(c + u, gm, gn, ok, v);
+            if (ok) ep.put(gm, gn, v, add[u], true);
+        }
+    }
+}
+
+template <int NE, class Elem>
+__device__ __forceinline__ void epilogue_store(const Epilogue& ep, Elem elem)
+{
+    if (ep.D && ep.E) epilogue_groups<true, true, NE>(ep, elem);
+    else if (ep.D) epilogue_groups<true, false, NE>(ep, elem);
+    else if (ep.E) epilogue_groups<false, true, NE>(ep, elem);
+    else {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            int gm, gn;
+            bool ok;
+            double v;
+            elem(e, gm, gn, ok, v);
+            if (ok) ep.put(gm, gn, v, 0.0, false);
+        }
+    }
+}
 
 // GEMM tiles go through xcd_remap (common.h): the tiles that one XCD runs are
 // a contiguous range, so the operand panels they share stay in one L2.
@@ -310,16 +370,13 @@ __device__ __forceinline__ void dgemm_mfma_body(const GemmArgs& g, int bx, int b
     // Epilogue: for the f64 16x16x4 MFMA, accumulator item r of a lane holds
     // row (lane/16) + 4*r, column lane%16 of its 16x16 tile.
     const Epilogue ep(g, bz);
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
-                const int gn = n0 + wn * WN + j * 16 + (lane & 15);
-                if (gm < g.m && gn < g.n) ep.store(gm, gn, acc[i][j][r]);
-            }
+    epilogue_store<TM * TN * 4>(ep, [&](int e, int& gm, int& gn, bool& ok, double& v) {
+        const int i = e / (TN * 4), j = (e / 4) % TN, r = e % 4;
+        gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
+        gn = n0 + wn * WN + j * 16 + (lane & 15);
+        ok = gm < g.m && gn < g.n;
+        v = acc[i][j][r];
+    });
 }
 
 template <int BM, int BN, int WGM, int WGN, int PIPE, bool TA, bool TB, int BK, int PAD>
@@ -539,17 +596,20 @@ __device__ __forceinline__ void dgemm_splitk_body(const GemmArgs& g, int bx, int
 
     const Epilogue ep(g, bz);
     constexpr int E = TM * TN * 4 * 64;
-    for (int e = tid; e < E; e += 64 * KW) {
-        double v = 0.0;
+    // the tile's E items over 64 KW threads: item tid + q * 64 KW is a thread's q-th
+    epilogue_store<(E + 64 * KW - 1) / (64 * KW)>(ep, [&](int q, int& gm, int& gn, bool& ok, double& v) {
+        const int e = tid + q * 64 * KW;
+        const int ec = e < E ? e : 0;
+        v = 0.0;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) v += red[w * E + e];
-        const int l = e & 63;
-        const int r = (e >> 6) & 3;
-        const int t = e >> 8;
-        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
-        const int gn = n0 + (t % TN) * 16 + (l & 15);
-        if (gm < g.m && gn < g.n) ep.store(gm, gn, v);
-    }
+        for (int w = 0; w < KW; ++w) v += red[w * E + ec];
+        const int l = ec & 63;
+        const int r = (ec >> 6) & 3;
+        const int t = ec >> 8;
+        gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        gn = n0 + (t % TN) * 16 + (l & 15);
+        ok = e < E && gm < g.m && gn < g.n;
+    });
 }
 
 template <int TM, int TN, int KW, bool TA, bool TB>
@@ -715,17 +775,20 @@ __device__ __forceinline__ void dgemm_wsplit_body(const GemmArgs& g, int bx, int
     __syncthreads();
 
     const Epilogue ep(g, bz);
-    for (int e = tid; e < E; e += 64 * KW) {
-        double v = 0.0;
+    // the tile's E items over 64 KW threads: item tid + q * 64 KW is a thread's q-th
+    epilogue_store<(E + 64 * KW - 1) / (64 * KW)>(ep, [&](int q, int& gm, int& gn, bool& ok, double& v) {
+        const int e = tid + q * 64 * KW;
+        const int ec = e < E ? e : 0;
+        v = 0.0;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) v += lds[w * E + e];
-        const int l = e & 63;
-        const int r = (e >> 6) & 3;
-        const int t = e >> 8;
-        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
-        const int gn = n0 + (t % TN) * 16 + (l & 15);
-        if (gm < g.m && gn < g.n) ep.store(gm, gn, v);
-    }
+        for (int w = 0; w < KW; ++w) v += lds[w * E + ec];
+        const int l = ec & 63;
+        const int r = (ec >> 6) & 3;
+        const int t = ec >> 8;
+        gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        gn = n0 + (t % TN) * 16 + (l & 15);
+        ok = e < E && gm < g.m && gn < g.n;
+    });
 }
 
 template <int BM, int BN, int KW, bool TA, bool TB, int DEPTH>
@@ -877,6 +940,10 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0,
             for (int j = 0; j < TN; ++j) acc[a][i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
     double ra[LA], rb[LB];
+    // modes 3 / 4: the mirrored element of each A / B element and whether the
+    // element lies inside the operand
+    double ra2[LA] = {}, rb2[LB] = {};
+    bool ina[LA] = {}, inb[LB] = {};
     auto a_rc = [&](int q, int& r, int& c) {
         if (!TA) { c = tid % BK; r = tid / BK + q * (NT / BK); }
         else     { r = tid % BM; c = tid / BM + q * (NT / BM); }
@@ -893,18 +960,24 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0,
         const double* __restrict__ B = (p ? f.B1 : g.B) + oB;
         const long long lda = p ? f.lda1 : g.lda, ldb = p ? f.ldb1 : g.ldb;
         const int K = p ? f.k1 : g.k;
+        // The mirrored operand (modes 3, 4) needs two loads per element: both
+        // are issued unconditionally at clamped, valid indices, and the pair is
+        // summed (and zeroed outside the operand) only when the slab is stored
+        // to LDS, so that no use of a load in this phase makes the wave wait
+        // for it while the current slab's MFMAs could run.
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             int r, c;
             a_rc(q, r, c);
             const int gm = m0 + r, gk = k0 + c;
             double v = 0.0;
-            if (gm < g.m && gk < K) {
+            if (MODE == 3) {
+                ina[q] = (gm < g.m) & (gk < K);
+                const int gmc = min(gm, g.m - 1), gkc = min(gk, K - 1), mk = max(kf1 - gkc, 0);
+                v = TA ? A[(long long)gkc * lda + gmc] : A[(long long)gmc * lda + (long long)gkc * f.ska];
+                ra2[q] = TA ? A[(long long)mk * lda + gmc] : A[(long long)gmc * lda + (long long)mk * f.ska];
+            } else if (gm < g.m && gk < K) {
                 v = TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + (long long)gk * f.ska];
-                if (MODE == 3) {
-                    const int mk = kf1 - gk;
-                    v += se * (TA ? A[(long long)mk * lda + gm] : A[(long long)gm * lda + (long long)mk * f.ska]);
-                }
             }
             ra[q] = v;
         }
@@ -914,12 +987,13 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0,
             b_rc(q, r, c);
             const int gk = k0 + r, gn = n0 + c;
             double v = 0.0;
-            if (gk < K && gn < g.n) {
+            if (MODE == 4) {
+                inb[q] = (gk < K) & (gn < g.n);
+                const int gnc = min(gn, g.n - 1), gkc = min(gk, K - 1), mk = max(kf1 - gkc, 0);
+                v = TB ? B[(long long)gnc * ldb + (long long)gkc * f.skb] : B[(long long)gkc * ldb + gnc];
+                rb2[q] = TB ? B[(long long)gnc * ldb + (long long)mk * f.skb] : B[(long long)mk * ldb + gnc];
+            } else if (gk < K && gn < g.n) {
                 v = TB ? B[(long long)gn * ldb + (long long)gk * f.skb] : B[(long long)gk * ldb + gn];
-                if (MODE == 4) {
-                    const int mk = kf1 - gk;
-                    v += se * (TB ? B[(long long)gn * ldb + (long long)mk * f.skb] : B[(long long)mk * ldb + gn]);
-                }
             }
             rb[q] = v;
         }
@@ -929,13 +1003,15 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0,
         for (int q = 0; q < LA; ++q) {
             int r, c;
             a_rc(q, r, c);
-            As[c * PA + r] = ra[q];
+            const double sum = ra[q] + se * ra2[q];
+            As[c * PA + r] = MODE == 3 ? (ina[q] ? sum : 0.0) : ra[q];
         }
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             int r, c;
             b_rc(q, r, c);
-            Bs[r * PB + c] = rb[q];
+            const double sum = rb[q] + se * rb2[q];
+            Bs[r * PB + c] = MODE == 4 ? (inb[q] ? sum : 0.0) : rb[q];
         }
     };
     auto compute = [&](int a) {
@@ -969,34 +1045,32 @@ __global__ void __launch_bounds__(64 * WGM * WGN) dgemm_fold_kernel(GemmArgs g0,
     }
 
     const Epilogue ep(g, bz);
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
-                const int gn = n0 + wn * WN + j * 16 + (lane & 15);
-                if (gm >= g.m || gn >= g.n) continue;
-                if (!DUAL) {
-                    ep.store(gm, gn, acc[0][i][j][r]);
-                    continue;
-                }
-                const double e = acc[0][i][j][r], o = acc[NACC - 1][i][j][r];
-                const int Lb = f.fold_len, Lh = (Lb + 1) / 2;
-                const int x = (MODE == 1) ? gm : gn;
-                const int b = x / Lh, jj = x - b * Lh;
-                const int x1 = b * Lb + jj, x2 = b * Lb + Lb - 1 - jj;
-                const double sg = ((f.fold_sign >> b) & 1u) ? -1.0 : 1.0;
-                if (MODE == 1) {
-                    ep.store(x1, gn, e + o);
-                    if (x2 != x1) ep.store(x2, gn, sg * (e - o));
-                } else {
-                    ep.store(gm, x1, e + o);
-                    if (x2 != x1) ep.store(gm, x2, sg * (e - o));
-                }
-            }
+    // output elements: one per accumulator item, or (DUAL) the mirrored pair
+    // x1 = even + odd and x2 = sign (even - odd), x2 only when it differs
+    epilogue_store<TM * TN * 4 * (DUAL ? 2 : 1)>(ep, [&](int e, int& gm, int& gn, bool& ok, double& v) {
+        const int q = DUAL ? e >> 1 : e;
+        const int i = q / (TN * 4), j = (q / 4) % TN, r = q % 4;
+        gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
+        gn = n0 + wn * WN + j * 16 + (lane & 15);
+        ok = gm < g.m && gn < g.n;
+        if (!DUAL) {
+            v = acc[0][i][j][r];
+            return;
+        }
+        const double ev = acc[0][i][j][r], od = acc[NACC - 1][i][j][r];
+        const int Lb = f.fold_len, Lh = (Lb + 1) / 2;
+        const int x = ok ? ((MODE == 1) ? gm : gn) : 0;  // elements outside C are not stored
+        const int b = x / Lh, jj = x - b * Lh;
+        const int x1 = b * Lb + jj, x2 = b * Lb + Lb - 1 - jj;
+        const double sg = ((f.fold_sign >> b) & 1u) ? -1.0 : 1.0;
+        const int xo = (e & 1) ? x2 : x1;
+        if (MODE == 1) gm = xo;
+        else gn = xo;
+        ok = ok && (!(e & 1) || x2 != x1);
+        v = (e & 1) ? sg * (ev - od) : ev + od;
+    });
 }
+
 
 // Output folds (modes 1, 2) split over waves: waves [0, KW) run the first
 // product (even k) and waves [KW, 2 KW) the second (odd k), each wave a
@@ -1118,31 +1192,34 @@ __global__ void __launch_bounds__(128 * KW) dgemm_fold_wsplit_kernel(GemmArgs g,
 
     const Epilogue ep(g, bz);
     const int Lb = f.fold_len, Lh = (Lb + 1) / 2;
-    for (int e = tid; e < E; e += 64 * NW) {
-        double ev = 0.0, ov = 0.0;
+    // tile item tid + q * 64 NW of the E items gives the mirrored pair of
+    // output elements 2 q (x1) and 2 q + 1 (x2, only when it differs)
+    epilogue_store<2 * ((E + 64 * NW - 1) / (64 * NW))>(ep, [&](int o, int& gm, int& gn, bool& ok, double& v) {
+        const int q = o >> 1;
+        const int e = tid + q * 64 * NW;
+        const int ec = e < E ? e : 0;
+        double ev = 0.0, od = 0.0;
 #pragma unroll
         for (int w = 0; w < KW; ++w) {
-            ev += lds[w * E + e];
-            ov += lds[(KW + w) * E + e];
+            ev += lds[w * E + ec];
+            od += lds[(KW + w) * E + ec];
         }
-        const int l = e & 63;
-        const int r = (e >> 6) & 3;
-        const int t = e >> 8;
-        const int gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
-        const int gn = n0 + (t % TN) * 16 + (l & 15);
-        if (gm >= g.m || gn >= g.n) continue;
-        const int x = (MODE == 1) ? gm : gn;
+        const int l = ec & 63;
+        const int r = (ec >> 6) & 3;
+        const int t = ec >> 8;
+        gm = m0 + (t / TN) * 16 + (l >> 4) + 4 * r;
+        gn = n0 + (t % TN) * 16 + (l & 15);
+        ok = e < E && gm < g.m && gn < g.n;
+        const int x = ok ? ((MODE == 1) ? gm : gn) : 0;  // elements outside C are not stored
         const int b = x / Lh, jj = x - b * Lh;
         const int x1 = b * Lb + jj, x2 = b * Lb + Lb - 1 - jj;
         const double sg = ((f.fold_sign >> b) & 1u) ? -1.0 : 1.0;
-        if (MODE == 1) {
-            ep.store(x1, gn, ev + ov);
-            if (x2 != x1) ep.store(x2, gn, sg * (ev - ov));
-        } else {
-            ep.store(gm, x1, ev + ov);
-            if (x2 != x1) ep.store(gm, x2, sg * (ev - ov));
-        }
-    }
+        const int xo = (o & 1) ? x2 : x1;
+        if (MODE == 1) gm = xo;
+        else gn = xo;
+        ok = ok && (!(o & 1) || x2 != x1);
+        v = (o & 1) ? sg * (ev - od) : ev + od;
+    });
 }
 
 template <int BM, int BN, int KW>
