@@ -375,9 +375,11 @@ class C2Chain:
 
 def microbench(nf=512, reps=10):
     """Kernel chain of 512 fields at 256x256 (C2Chain; per-field algorithmic bytes
-    in its stage list); returns per-kernel time and HBM fraction."""
+    in its stage list); returns per-kernel time and HBM fraction.  Each launch is
+    timed with HIP events enqueued behind a millisecond of GPU work, so that the
+    events see the kernel and not the host's time to enqueue it."""
     import torch
-    from pypde_amd import ops
+    from pypde_amd import autotune, ops
     N = 256
     c = torch.as_tensor(c2_input(nf, N, seed=1)).to(ops.device())
     chain = C2Chain(c)
@@ -389,6 +391,7 @@ def microbench(nf=512, reps=10):
         ts = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            autotune.queue_delay()  # the GPU is busy while the host enqueues: events time the kernel
             e0.record()
             fn()
             e1.record()
@@ -451,6 +454,7 @@ def transform_forms(sizes=(64, 128, 256, 512), reps=10):
                     ts = []
                     for _ in range(reps):
                         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        autotune.queue_delay()
                         e0.record()
                         op.apply(x, axis, out=y, form=form)
                         e1.record()
@@ -669,6 +673,7 @@ def rhs_forms():
     import torch
     from pypde_amd._lib import RbcRhs, lib, check
     from pypde_amd.engine import RBCBatch
+    from pypde_amd import autotune
     out = {}
     for shape, B in (((256, 256), 1), ((256, 256), 64), ((128, 128), 16), ((64, 64), 16),
                      ((192, 192), 16), ((512, 512), 1)):
@@ -686,6 +691,7 @@ def rhs_forms():
             check(lib.pypde_rbc_rhs_banded(ctypes.byref(a), st), "rhs form")
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            autotune.queue_delay()
             e0.record()
             for _ in range(20):
                 lib.pypde_rbc_rhs_banded(ctypes.byref(a), st)
@@ -704,12 +710,14 @@ def hetero_launches():
     import ctypes
     import torch
     from pypde_amd.batch import HeteroBatch
+    from pypde_amd import autotune
 
     def ev(fn, reps=20):
         st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         fn(st)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        autotune.queue_delay()
         e0.record()
         for _ in range(reps):
             fn(st)

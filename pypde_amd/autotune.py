@@ -77,7 +77,7 @@ def _capturing():
 _delay_bufs = None
 
 
-def _queue_delay():
+def queue_delay():
     """Enqueue about a millisecond of GPU work (an FP64 matrix product) so that
     the launches queued behind it while it runs execute back to back: the events
     around them then time the GPU, not the host's launch rate, which for kernels
@@ -103,7 +103,7 @@ def time_candidates(run, candidates, reps=5, rounds=3):
         for c in cands:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            _queue_delay()
+            queue_delay()
             e0.record()
             for _ in range(reps):
                 run(c)

@@ -210,8 +210,11 @@ __global__ void __launch_bounds__(MAXT, (rec_min_waves<CH, PRE, PERLINE, LPB>())
     const int w = LPB == 64 ? wv : 2 * wv + (lane >> 5);       // chunk
     const int P = LPB == 64 ? (int)(blockDim.x >> 6) : 2 * (int)(blockDim.x >> 6);
     const int bpe = (a.nlines + LPB - 1) / LPB;
-    const int b = blockIdx.x / bpe;
-    const int l0 = (blockIdx.x - b * bpe) * LPB;
+    // along columns (axis 0) neighbouring blocks share the cache lines at the
+    // edges of their row segments: they run on one XCD (one L2)
+    const int blk = AXIS == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int b = blk / bpe;
+    const int l0 = (blk - b * bpe) * LPB;
     const int line = l0 + li;
     const bool live = line < a.nlines;
     const int base = w * CH;

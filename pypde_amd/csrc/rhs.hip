@@ -65,7 +65,8 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
     for (int q = 0; q < RS + 6; ++q) {
         const int r = i0 - 2 + q;
-        ys[q] = (r >= 0 && r < M0) ? ysum_c<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
+        const double v = ysum_c<4>(Xf + (long long)min(max(r, 0), M0 - 1) * M1, j, M1, o4, cbs);
+        ys[q] = ((r >= 0) & (r < M0)) ? v : 0.0;
     }
     double acc[RS];
 #pragma unroll
@@ -75,7 +76,8 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int r = i + o4[d];
-            if (i < M0 && r >= 0 && r < M0) a = a + tbs[d * M0 + i] * ys[k + d * 2];
+            const double sum = a + tbs[d * M0 + min(i, M0 - 1)] * ys[k + d * 2];
+            a = ((i < M0) & (r >= 0) & (r < M0)) ? sum : a;
         }
         acc[k] = a;
     }
@@ -90,9 +92,9 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
         for (int q = 0; q < RS + 4; ++q) {
             const int r = i0 + first + q;
-            yp[q] = (r < N0) ? (f == 0 ? ysum_c<3>(pb + (long long)r * N1, j, N1, o024, cp)
-                                       : ysum_c<3>(pb + (long long)r * N1, j, N1, o135, cp))
-                             : 0.0;
+            const double* prow = pb + (long long)min(r, N0 - 1) * N1;
+            const double v = f == 0 ? ysum_c<3>(prow, j, N1, o024, cp) : ysum_c<3>(prow, j, N1, o135, cp);
+            yp[q] = r < N0 ? v : 0.0;
         }
         const double* tp = tx + (long long)(f == 0 ? TX_BD : TX_B) * M0;
         const double m = mdta[b];
@@ -103,7 +105,8 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int r = i + (f == 0 ? o135[d] : o024[d]);
-                if (i < M0 && r < N0) g = g + tp[d * M0 + i] * yp[k + 2 * d];
+                const double sum = g + tp[d * M0 + min(i, M0 - 1)] * yp[k + 2 * d];
+            g = ((i < M0) & (r < N0)) ? sum : g;
             }
             acc[k] = acc[k] + m * g;
         }
@@ -115,8 +118,8 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
         for (int q = 0; q < RS + 6; ++q) {
             const int r = i0 - 2 + q;
-            yt[q] = (r >= 0 && r < M0) ? ysum_c<4>(XT + (long long)r * M1, j, M1, o4, cbs)
-                                       : 0.0;
+            const double v = ysum_c<4>(XT + (long long)min(max(r, 0), M0 - 1) * M1, j, M1, o4, cbs);
+            yt[q] = ((r >= 0) & (r < M0)) ? v : 0.0;
         }
         const double* tt = tx + (long long)TX_BST * M0;
         const double c = dta[b];
@@ -127,7 +130,8 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int r = i + o4[d];
-                if (i < M0 && r >= 0 && r < M0) bt = bt + tt[d * M0 + i] * yt[k + d * 2];
+                const double sum = bt + tt[d * M0 + min(i, M0 - 1)] * yt[k + d * 2];
+            bt = ((i < M0) & (r >= 0) & (r < M0)) ? sum : bt;
             }
             acc[k] = acc[k] + c * bt;
         }
@@ -137,8 +141,9 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
         for (int q = 0; q < RS + 6; ++q) {
             const int r = i0 - 2 + q;
-            yd[q] = (r >= 0 && r < M0) ? ysum<2>(Xf + (long long)r * M1, j, M1, o02, ty + TY_BD2S * M1, M1)
-                                       : 0.0;
+            const double v = ysum<2>(Xf + (long long)min(max(r, 0), M0 - 1) * M1, j, M1, o02,
+                                     ty + TY_BD2S * M1, M1);
+            yd[q] = ((r >= 0) & (r < M0)) ? v : 0.0;
         }
         const double* td2 = tx + (long long)(f == 2 ? TX_BD2ST : TX_BD2S) * M0;
         const double fe = fdiff[e];
@@ -149,12 +154,14 @@ __device__ __forceinline__ void rhs_banded_strip(const double* __restrict__ X,
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const int r = i + o02[d];
-                if (i < M0 && r < M0) dxx = dxx + td2[d * M0 + i] * ys[k + 2 + d * 2];
+                const double sum = dxx + td2[d * M0 + min(i, M0 - 1)] * ys[k + 2 + d * 2];
+            dxx = ((i < M0) & (r < M0)) ? sum : dxx;
             }
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int r = i + o4[d];
-                if (i < M0 && r >= 0 && r < M0) dyy = dyy + tbs[d * M0 + i] * yd[k + d * 2];
+                const double sum = dyy + tbs[d * M0 + min(i, M0 - 1)] * yd[k + d * 2];
+            dyy = ((i < M0) & (r >= 0) & (r < M0)) ? sum : dyy;
             }
             acc[k] = acc[k] + fe * (dxx + dyy);
         }
@@ -237,26 +244,27 @@ __device__ __forceinline__ void rhs_banded_tile(const pypde_rbc_rhs& a, long lon
     const double mb = f < 2 ? a.mdta[b] : 0.0;
     const double db = f == 1 ? a.dta[b] : 0.0;
     const double fe = a.fdiff ? a.fdiff[e] : 0.0;
-    // unrolled over the wave's rows, so that the loads of all of them are in
-    // flight together
+    // one input row per iteration, its loads together: with every row's loads
+    // in flight at once the tile takes the registers of two more waves and
+    // runs slower (265 against 208 us at 64 x 256x256)
 #pragma unroll
     for (int t = 0; t < (kRhsRows + 3) / 4; ++t) {
         const int q = w + 4 * t;
         if (q >= kRhsRows || !jin) continue;
         const int r = i0 - 2 + q;
         const bool rin = r >= 0 && r < M0;
-        ys[q][lane] = rin ? ysum_c<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
+        ys[q][lane] = rin ? ysum_c_inside<4>(Xf + (long long)r * M1, j, M1, o4, cbs) : 0.0;
         if (f == 1)
-            yt[q][lane] = rin ? ysum_c<4>(a.X + (e + 1) * mm + (long long)r * M1, j, M1, o4, cbs)
+            yt[q][lane] = rin ? ysum_c_inside<4>(a.X + (e + 1) * mm + (long long)r * M1, j, M1, o4, cbs)
                               : 0.0;
         if (f < 2) {
             const int rp = i0 + pfirst + q;
-            yp[q][lane] = rp < N0 ? (f == 0 ? ysum_c<3>(pb + (long long)rp * N1, j, N1, o024, cp)
-                                            : ysum_c<3>(pb + (long long)rp * N1, j, N1, o135, cp))
+            yp[q][lane] = rp < N0 ? (f == 0 ? ysum_c_inside<3>(pb + (long long)rp * N1, j, N1, o024, cp)
+                                            : ysum_c_inside<3>(pb + (long long)rp * N1, j, N1, o135, cp))
                                   : 0.0;
         }
         if (a.fdiff)
-            yd[q][lane] = rin ? ysum_c<2>(Xf + (long long)r * M1, j, M1, o02, cd2) : 0.0;
+            yd[q][lane] = rin ? ysum_c_inside<2>(Xf + (long long)r * M1, j, M1, o02, cd2) : 0.0;
     }
     __syncthreads();
     if (!jin) return;

@@ -15,6 +15,19 @@ namespace pypde {
 enum { TX_BD = 0, TX_B = 3, TX_BS = 6, TX_BST = 10, TX_BD2S = 14, TX_BD2ST = 16, TX_COUNT = 18 };
 enum { TY_B = 0, TY_BD = 3, TY_BS = 6, TY_BD2S = 10, TY_COUNT = 12 };
 
+// Every load below is unconditional: an element outside the array is read at
+// a clamped, valid index and its value replaced by zero (its term dropped)
+// with a select afterwards.  A load under a branch makes the wave wait for it
+// before it issues the next; the selects keep all of an output's loads in
+// flight together and give the same sums, term for term.
+
+// element c of a row of ncol, zero outside the row
+__device__ __forceinline__ double row_at(const double* __restrict__ row, int c, int ncol)
+{
+    const double v = row[min(max(c, 0), ncol - 1)];
+    return ((c >= 0) & (c < ncol)) ? v : 0.0;
+}
+
 // sum over the y diagonals (offsets oy[0..ny)) of row r of the input
 template <int NY>
 __device__ __forceinline__ double ysum(const double* __restrict__ row, int j, int ncol,
@@ -22,11 +35,7 @@ __device__ __forceinline__ double ysum(const double* __restrict__ row, int j, in
 {
     double s = 0.0;
 #pragma unroll
-    for (int b = 0; b < NY; ++b) {
-        const int c = j + oy[b];
-        const double v = (c >= 0 && c < ncol) ? row[c] : 0.0;
-        s = s + v * ty[b * M1 + j];
-    }
+    for (int b = 0; b < NY; ++b) s = s + row_at(row, j + oy[b], ncol) * ty[b * M1 + j];
     return s;
 }
 
@@ -34,6 +43,19 @@ __device__ __forceinline__ double ysum(const double* __restrict__ row, int j, in
 template <int NY>
 __device__ __forceinline__ double ysum_c(const double* __restrict__ row, int j, int ncol,
                                          const int (&oy)[NY], const double (&coef)[NY])
+{
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NY; ++b) s = s + row_at(row, j + oy[b], ncol) * coef[b];
+    return s;
+}
+
+// ysum_c for a row that is read once per tile: the loads of the row's
+// columns inside the row only (no clamped addresses to hold), which keeps the
+// tile form's registers at three waves per SIMD; the same terms and order
+template <int NY>
+__device__ __forceinline__ double ysum_c_inside(const double* __restrict__ row, int j, int ncol,
+                                                const int (&oy)[NY], const double (&coef)[NY])
 {
     double s = 0.0;
 #pragma unroll
@@ -45,7 +67,7 @@ __device__ __forceinline__ double ysum_c(const double* __restrict__ row, int j, 
     return s;
 }
 
-// sum_a tx[a][i] * ysum(row i + ox[a])
+// sum_a tx[a][i] * ysum(row i + ox[a]) over the rows inside the input
 template <int NX, int NY>
 __device__ __forceinline__ double band2(const double* __restrict__ in, long long ld, int nrow, int ncol,
                                         int i, int j, const int (&ox)[NX], const double* __restrict__ tx,
@@ -56,9 +78,10 @@ __device__ __forceinline__ double band2(const double* __restrict__ in, long long
 #pragma unroll
     for (int a = 0; a < NX; ++a) {
         const int r = i + ox[a];
-        if (r < 0 || r >= nrow) continue;
-        const double s = ysum<NY>(in + (long long)r * ld, j, ncol, oy, ty, M1);
-        acc = acc + tx[a * M0 + i] * s;
+        const int rc = min(max(r, 0), nrow - 1);
+        const double s = ysum<NY>(in + (long long)rc * ld, j, ncol, oy, ty, M1);
+        const double t = acc + tx[a * M0 + i] * s;
+        acc = ((r >= 0) & (r < nrow)) ? t : acc;
     }
     return acc;
 }
