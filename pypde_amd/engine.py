@@ -1072,9 +1072,10 @@ class RBCBatch:
     def _merge_products_rhs(self, s):
         """Per stage, the y-folded products and the banded right-hand side
         (one output per thread) in one launch (pypde_rbc_products_rhs) where
-        the tuner times that faster than the two launches; the merged launch
-        takes the products' place (the right-hand side reads only the
-        stage-start state, which no launch in between changes)."""
+        the tuner times that faster than the two launches, the right-hand side
+        in the form tuned for it alone; the merged launch takes the products'
+        place (the right-hand side reads only the stage-start state, which no
+        launch in between changes)."""
         from . import autotune
         L = self.launches
         for st in range(self.nstage):
@@ -1084,10 +1085,14 @@ class RBCBatch:
             if ip is None or ir is None or ip > ir:
                 continue
             prod, rhs = L[ip], L[ir]
-            if prod.fn.__name__ != "pypde_rbc_products_yfold" or rhs.args[0]._obj.rows_per_thread != 1:
+            if prod.fn.__name__ != "pypde_rbc_products_yfold":
                 continue
+            # the merged launch computes the right-hand side one output per thread
+            a1 = RbcRhs.from_buffer_copy(rhs.args[0]._obj)
+            a1.rows_per_thread = 1
+            self._keep.append(a1)
             merged = _Call("products+rhs_banded" + tag, lib.pypde_rbc_products_rhs,
-                           *(prod.args + rhs.args))
+                           *(prod.args + (ctypes.byref(a1),) + rhs.args[1:]))
             forms = ((prod, rhs), (merged,))
 
             def run(c):
