@@ -240,8 +240,10 @@ int pypde_cheb_dct(const pypde_lines* g, int P, const double* a, const double* b
  * up to 16 waves; for many lines), 2 = lane-chunk form (one wave per line, a
  * chunk of n/64 positions per lane, carries combined by wave scans; for few
  * lines), 3 = block form with 32 lines per block (half-waves hold neighbouring
- * chunks; half the LDS, so two blocks share a CU), 0 = chosen by the number of
- * lines (lane-chunk up to 2048 lines, else variant 1).
+ * chunks; half the LDS, so two blocks share a CU), 4 = variant 3 with 32
+ * positions per chunk at every length (half the chunks, so a shorter carry
+ * chain), 0 = chosen by the number of lines (lane-chunk up to 2048 lines,
+ * else variant 1).
  * per_line = 2 implies the lane-chunk form, per_line = 1 a block form.
  * Carries: the four-diagonal Helmholtz sweeps (Plan_fdma.solve, plans.py:184-224
  * -> fdma.f90:40-98), the Poisson line solves (Plan_Poisson.solve_fortran,

@@ -158,7 +158,7 @@ class StepAdjoint:
             stream = ops.stream_ptr()
             autotune.choose(key, lambda v: check(lib.pypde_recurrence_solve(
                 ctypes.byref(gs), ctypes.byref(struct(v)), stream), "transposed line solve"),
-                (1, 2, 3), default=2)
+                (1, 2, 3, 4), default=2)
             torch.cuda.synchronize()
             del scratch
         run(geom(x), autotune.cache().get(key, 2))

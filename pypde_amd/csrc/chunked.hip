@@ -451,13 +451,14 @@ static int launch_rec_ch(const RecArgs& a, hipStream_t s)
 }
 
 template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int LPB>
-static int launch_rec_lpb(const RecArgs& a, hipStream_t s)
+static int launch_rec_lpb(const RecArgs& a, hipStream_t s, bool long32)
 {
     PYPDE_REQUIRE(a.n >= 1 && a.n <= kMaxWaves * 32, "recurrence: line length %d exceeds %d", a.n,
                   kMaxWaves * 32);
     // 16 positions per chunk up to n = 256, except for the pre-stencil form in
-    // 64-line blocks (16 waves of it do not fit two blocks per CU)
-    const bool short16 = (!PRE || LPB == 32) && a.n <= kMaxWaves * 16;
+    // 64-line blocks (16 waves of it do not fit two blocks per CU) and when 32
+    // are asked for (long32: half the chunks, so half the carry chain)
+    const bool short16 = !long32 && (!PRE || LPB == 32) && a.n <= kMaxWaves * 16;
     if (a.axis == 0) {
         if (short16) return launch_rec_ch<0, 16, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
         return launch_rec_ch<0, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
@@ -466,12 +467,13 @@ static int launch_rec_lpb(const RecArgs& a, hipStream_t s)
     return launch_rec_ch<1, 32, PRE, FWD, HASF, HASR, HASNU, PERLINE, LPB>(a, s);
 }
 
-// Block form with 64 (half = false) or 32 (half = true) lines per block.
+// Block form with 64 (half = false) or 32 (half = true) lines per block;
+// long32: 32 positions per chunk at every length.
 template <bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE>
-static int launch_rec(const RecArgs& a, hipStream_t s, bool half)
+static int launch_rec(const RecArgs& a, hipStream_t s, bool half, bool long32)
 {
-    if (half) return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 32>(a, s);
-    return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 64>(a, s);
+    if (half) return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 32>(a, s, long32);
+    return launch_rec_lpb<PRE, FWD, HASF, HASR, HASNU, PERLINE, 64>(a, s, long32);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +818,7 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                   "recurrence: len_in must be len_out%s", pre ? " + 2" : "");
     PYPDE_REQUIRE(!(g->src == g->dst && pre), "pre-stencil cannot run in place");
     PYPDE_REQUIRE(r->per_line >= 0 && r->per_line <= 2, "per_line must be 0, 1 or 2");
-    PYPDE_REQUIRE(r->variant >= 0 && r->variant <= 3, "variant must be 0, 1, 2 or 3");
+    PYPDE_REQUIRE(r->variant >= 0 && r->variant <= 4, "variant must be 0, 1, 2, 3 or 4");
     RecArgs a{g->src, g->dst, g->src_ld, g->dst_ld, g->src_bs, g->dst_bs, g->len_in, g->len_out,
               g->nlines, g->batch, g->axis, r->F, r->LM, r->R, r->MU, r->NU, r->pre,
               r->entry_stride, r->per_line};
@@ -826,7 +828,7 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
     // lane-chunk form: asked for, required by line-major tables, or (variant 0)
     // chosen when the launch has few lines
     const long long lines = (long long)g->nlines * g->batch;
-    PYPDE_REQUIRE(!(r->variant == 3 && r->per_line == 2),
+    PYPDE_REQUIRE(!(r->variant >= 3 && r->per_line == 2),
                   "the block forms read per-line tables [n][nlines] (per_line = 1)");
     const bool lane = r->variant == 2 || r->per_line == 2 ||
                       (r->variant == 0 && r->per_line == 0 && lines <= 2048);
@@ -845,8 +847,9 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                   (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
         return ERR_UNSUP;
     }
-    // block form: 64 lines per block (variant 0 / 1) or 32 (variant 3)
-    const bool half = r->variant == 3;
+    // block form: 64 lines per block (variant 0 / 1) or 32 (variant 3; variant
+    // 4: 32 lines and 32 positions per chunk)
+    const bool half = r->variant >= 3, long32 = r->variant == 4;
     // the combinations used by the plans
     if (r->per_line) {
         PYPDE_REQUIRE(!pre && fwd && !hf && hr && hn,
@@ -855,12 +858,12 @@ extern "C" int pypde_recurrence_solve(const pypde_lines* g, const pypde_recurren
                       r->MU == r->R + (long long)g->len_out * g->nlines &&
                       r->NU == r->MU + (long long)g->len_out * g->nlines,
                       "per-line tables must be contiguous slots LM | R | MU | NU of [n][nlines]");
-        return launch_rec<false, true, false, true, true, true>(a, s, half);
+        return launch_rec<false, true, false, true, true, true>(a, s, half, long32);
     }
-    if (!pre && fwd && !hf && hr && hn) return launch_rec<false, true, false, true, true, false>(a, s, half);
-    if (pre && fwd && hf && !hr && !hn) return launch_rec<true, true, true, false, false, false>(a, s, half);
-    if (!pre && !fwd && !hf && hr && !hn) return launch_rec<false, false, false, true, false, false>(a, s, half);
-    if (!pre && fwd && hf && hr && hn) return launch_rec<false, true, true, true, true, false>(a, s, half);
+    if (!pre && fwd && !hf && hr && hn) return launch_rec<false, true, false, true, true, false>(a, s, half, long32);
+    if (pre && fwd && hf && !hr && !hn) return launch_rec<true, true, true, false, false, false>(a, s, half, long32);
+    if (!pre && !fwd && !hf && hr && !hn) return launch_rec<false, false, false, true, false, false>(a, s, half, long32);
+    if (!pre && fwd && hf && hr && hn) return launch_rec<false, true, true, true, true, false>(a, s, half, long32);
     set_error("pypde_recurrence_solve: unsupported table combination (pre=%d F=%d LM=%d R=%d NU=%d)",
               (int)pre, (int)hf, (int)fwd, (int)hr, (int)hn);
     return ERR_UNSUP;

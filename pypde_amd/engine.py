@@ -355,8 +355,8 @@ class Plan:
 
 
 # recurrence kernel forms: 1 block (64 lines per block), 2 lane-chunk,
-# 3 block with 32 lines per block
-REC_VARIANTS = (1, 2, 3)
+# 3 block with 32 lines per block, 4 the same with 32 positions per chunk
+REC_VARIANTS = (1, 2, 3, 4)
 
 
 class _RecCall(_Call):
@@ -820,7 +820,8 @@ class RBCBatch:
         # 15  per-line eigen-split solves along x
         gp = _lines(p(self.RP), p(self.RP), M1, sX, M0, M1, B, 0)
         rp = {}
-        for v, tabs, layout in ((1, self.prec, 1), (2, self.precT, 2), (3, self.prec, 1)):
+        for v, tabs, layout in ((1, self.prec, 1), (2, self.precT, 2), (3, self.prec, 1),
+                                (4, self.prec, 1)):
             r = Recurrence()
             r.LM, r.R, r.MU, r.NU = (tabs[i].data_ptr() for i in range(4))
             r.per_line, r.entry_stride, r.variant = layout, 0, v

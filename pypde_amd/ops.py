@@ -607,9 +607,9 @@ class RecTables:
 
     def struct(self, variant=0, lines=0):
         """The C struct for kernel variant 1 (block, 64 lines per block), 2
-        (lane-chunk) or 3 (block, 32 lines per block); 0 lets the library
-        choose, except for per-line tables, whose layout follows the variant
-        (lane-chunk up to 2048 lines)."""
+        (lane-chunk), 3 (block, 32 lines per block) or 4 (as 3, 32 positions
+        per chunk); 0 lets the library choose, except for per-line tables,
+        whose layout follows the variant (lane-chunk up to 2048 lines)."""
         from ._lib import Recurrence
         r = Recurrence()
         tabs = self.t
@@ -627,7 +627,7 @@ class RecTables:
         return r
 
 
-REC_CANDIDATES = (1, 2, 3)
+REC_CANDIDATES = (1, 2, 3, 4)
 
 
 def recurrence_solve(tabs, x, axis=0, out=None, lpb=0, variant=None):

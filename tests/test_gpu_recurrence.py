@@ -1,6 +1,6 @@
 """The kernel forms of pypde_recurrence_solve (block form with 64 lines per
 block, variant 1; lane-chunk form, variant 2; block form with 32 lines per
-block, variant 3) against a sequential evaluation of the normalised
+block, variant 3, and with 32 positions per chunk, variant 4) against a sequential evaluation of the normalised
 recurrence, for every table combination the plans use, both axes, line lengths
 around the chunk sizes of either form, per-entry and per-line tables (both
 layouts), in place and out of place."""
@@ -78,7 +78,7 @@ def run(ops, lib, Recurrence, x, out, axis, n_in, n, dev, variant, per_line, ent
 
 @pytest.mark.parametrize("form", sorted(FORMS))
 @pytest.mark.parametrize("axis", [0, 1])
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 def test_shared_and_per_entry_tables(form, axis, variant):
     import torch
     from pypde_amd import ops
@@ -122,7 +122,7 @@ def test_shared_and_per_entry_tables(form, axis, variant):
                 assert rel(ge, reference(xe, t1, n)) < 1e-13, (form, axis, variant, n, "inplace")
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 def test_per_line_tables(variant):
     """The Poisson form with tables per line: [n][nlines] for the block forms,
     line-major [nlines][n] for the lane-chunk form (axis 0, lines = columns)."""
@@ -133,7 +133,7 @@ def test_per_line_tables(variant):
     names = FORMS["fdma"]
     for n, nl in ((3, 5), (64, 64), (129, 70), (254, 254), (400, 33), (512, 9)):
         t = tables(rng, n, names, lead=(nl,))          # [nlines][n]
-        if variant in (1, 3):
+        if variant != 2:
             block = ops.to_dev(np.stack([t[k].T for k in names]))   # [4][n][nlines]
             layout = 1
         else:
@@ -157,6 +157,6 @@ def test_layout_and_variant_checks():
     x = torch.zeros((8, 4), dtype=torch.float64, device="cuda")
     t = torch.zeros((4, 8, 4), dtype=torch.float64, device="cuda")
     dev = {k: t[j] for j, k in enumerate(FORMS["fdma"])}
-    for per_line, variant in ((1, 2), (3, 0), (0, 5), (2, 3), (0, 4)):
+    for per_line, variant in ((1, 2), (3, 0), (0, 5), (2, 3), (2, 4), (0, -1)):
         with pytest.raises(PypdeError):
             run(ops, lib, Recurrence, x, x, 0, 8, 8, dev, variant, per_line)
