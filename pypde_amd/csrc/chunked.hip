@@ -482,12 +482,86 @@ static int launch_rec(const RecArgs& a, hipStream_t s, bool half, bool long32)
 // One wave per line; lane l holds positions [l*C, l*C + C) (C = 2, 4 or 8, so
 // n <= 64*C; C even keeps each lane's first position even, so the parity of
 // a position is the parity of its index in the chunk).  The chunk transfers of
-// the two sweeps are combined across the 64 lanes by Hillis-Steele scans over
-// shuffles: the forward chain's affine maps y -> A y + B upward, the backward
-// chain's 2-D maps (Z1, Z2) -> s + M (Z1, Z2) downward.  Tables come straight
+// the two sweeps are combined across the 64 lanes by scans: the forward
+// chain's affine maps y -> A y + B upward, the backward chain's 2-D maps
+// (Z1, Z2) -> s + M (Z1, Z2) downward, each as four DPP shift steps within the
+// rows of 16 lanes and one step across the rows with the rows' totals read
+// from single lanes (no LDS traffic, unlike shuffles).  Tables come straight
 // from global memory (shared tables are a few KB and stay in cache; per-line
 // tables are read line-major, [nlines][n], so a wave reads its line's
 // positions contiguously).  No LDS and no block barrier: waves are independent.
+
+// Cross-lane moves for the lane-chunk scans.  dpp_f64<CTRL>: a DPP move of a
+// double (two 32-bit moves); lanes whose source lies outside the pattern keep
+// `old`.  Controls used: row_shr:d (0x110 + d) and row_shl:d (0x100 + d) move
+// within rows of 16 lanes, wave_shr:1 (0x138) and wave_shl:1 (0x130) across the
+// wave.  readlane_f64: one lane's value, for every lane.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double old, double x)
+{
+    const unsigned long long ov = __builtin_bit_cast(unsigned long long, old);
+    const unsigned long long xv = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)ov, (int)(unsigned)xv, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(ov >> 32), (int)(unsigned)(xv >> 32), CTRL,
+                                               0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    const unsigned long long xv = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)xv, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(xv >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Forward chain maps y -> A y + B; (A, B) after (Ap, Bp): A Ap, A Bp + B.
+// One step of the inclusive scan within rows of 16 lanes: lane l composes the
+// map of lane l - D of its row (when there is one) before its own.
+template <int D>
+__device__ __forceinline__ void fwd_row_step(double& A, double& B, int lane)
+{
+    const double Au = dpp_f64<0x110 + D>(1.0, A);
+    const double Bu = dpp_f64<0x110 + D>(0.0, B);
+    const double nb = A * Bu + B, na = A * Au;
+    const bool in = (lane & 15) >= D;
+    B = in ? nb : B;
+    A = in ? na : A;
+}
+
+// Backward chain maps (Z1, Z2) -> s + M (Z1, Z2), M = [a1 b1; a2 b2]; a lane's
+// map applied after the map of the lane above it (its input is that lane's
+// output): own.s1 + own.a1 t.s1 + own.b1 t.s2, ...
+struct BwdMap {
+    double s1, s2, a1, b1, a2, b2;
+};
+
+__device__ __forceinline__ BwdMap bwd_compose(const BwdMap& x, const BwdMap& t)
+{
+    return BwdMap{x.s1 + x.a1 * t.s1 + x.b1 * t.s2, x.s2 + x.a2 * t.s1 + x.b2 * t.s2,
+                  x.a1 * t.a1 + x.b1 * t.a2,         x.a1 * t.b1 + x.b1 * t.b2,
+                  x.a2 * t.a1 + x.b2 * t.a2,         x.a2 * t.b1 + x.b2 * t.b2};
+}
+
+template <int D>
+__device__ __forceinline__ void bwd_row_step(BwdMap& m, int lane)
+{
+    const BwdMap t{dpp_f64<0x100 + D>(0.0, m.s1), dpp_f64<0x100 + D>(0.0, m.s2),
+                   dpp_f64<0x100 + D>(1.0, m.a1), dpp_f64<0x100 + D>(0.0, m.b1),
+                   dpp_f64<0x100 + D>(0.0, m.a2), dpp_f64<0x100 + D>(1.0, m.b2)};
+    const BwdMap c = bwd_compose(m, t);
+    const bool in = (lane & 15) + D < 16;
+    m.s1 = in ? c.s1 : m.s1; m.s2 = in ? c.s2 : m.s2;
+    m.a1 = in ? c.a1 : m.a1; m.b1 = in ? c.b1 : m.b1;
+    m.a2 = in ? c.a2 : m.a2; m.b2 = in ? c.b2 : m.b2;
+}
+
+__device__ __forceinline__ BwdMap bwd_readlane(const BwdMap& m, int l)
+{
+    return BwdMap{readlane_f64(m.s1, l), readlane_f64(m.s2, l), readlane_f64(m.a1, l),
+                  readlane_f64(m.b1, l), readlane_f64(m.a2, l), readlane_f64(m.b2, l)};
+}
+
 // The lane's table entries and inputs are fetched into registers before any
 // of them is used: positions past the line's end read a clamped, valid entry
 // (the last position of the line) whose value is then discarded, so every
@@ -553,24 +627,29 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
                 A[p] = m * A[p];
             }
         }
-        // inclusive scan upward: lane l composes its map after lanes < l
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const double Au = __shfl_up(A[p], d, 64);
-                const double Bu = __shfl_up(B[p], d, 64);
-                if (lane >= d) {
-                    B[p] = A[p] * Bu + B[p];
-                    A[p] = A[p] * Au;
-                }
-            }
-        }
+        // inclusive scan upward (lane l composes its map after lanes < l):
+        // within rows of 16 lanes by DPP shifts, then across the four rows
+        // with the rows' totals read from their last lanes (15, 31, 47)
         double y[2];
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const double yin = __shfl_up(B[p], 1, 64);
-            y[p] = lane == 0 ? 0.0 : yin;
+            fwd_row_step<1>(A[p], B[p], lane);
+            fwd_row_step<2>(A[p], B[p], lane);
+            fwd_row_step<4>(A[p], B[p], lane);
+            fwd_row_step<8>(A[p], B[p], lane);
+            const double A0 = readlane_f64(A[p], 15), B0 = readlane_f64(B[p], 15);
+            const double A1 = readlane_f64(A[p], 31), B1 = readlane_f64(B[p], 31);
+            const double A2 = readlane_f64(A[p], 47), B2 = readlane_f64(B[p], 47);
+            const double PA2 = A1 * A0, PB2 = A1 * B0 + B1;     // rows 0-1
+            const double PA3 = A2 * PA2, PB3 = A2 * PB2 + B2;   // rows 0-2
+            const int row = lane >> 4;
+            const double PA = row == 1 ? A0 : (row == 2 ? PA2 : PA3);
+            const double PB = row == 1 ? B0 : (row == 2 ? PB2 : PB3);
+            const double nb = A[p] * PB + B[p], na = A[p] * PA;
+            B[p] = row > 0 ? nb : B[p];
+            A[p] = row > 0 ? na : A[p];
+            // the chunk's incoming value: the lane below's outgoing one
+            y[p] = dpp_f64<0x138>(0.0, B[p]);
         }
 #pragma unroll
         for (int k = 0; k < C; ++k) {
@@ -601,30 +680,27 @@ __device__ __forceinline__ void lane_chunk_line(const RecArgs& a, long long gl)
             s1[p] = sv; a1[p] = al; b1[p] = be;
         }
     }
-    // inclusive scan downward: lane l's map applied after the maps of lanes > l
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const double t1 = __shfl_down(s1[p], d, 64), t2 = __shfl_down(s2[p], d, 64);
-            const double ta1 = __shfl_down(a1[p], d, 64), tb1 = __shfl_down(b1[p], d, 64);
-            const double ta2 = __shfl_down(a2[p], d, 64), tb2 = __shfl_down(b2[p], d, 64);
-            if (lane + d < 64) {
-                const double n1 = s1[p] + a1[p] * t1 + b1[p] * t2;
-                const double n2 = s2[p] + a2[p] * t1 + b2[p] * t2;
-                const double na1 = a1[p] * ta1 + b1[p] * ta2, nb1 = a1[p] * tb1 + b1[p] * tb2;
-                const double na2 = a2[p] * ta1 + b2[p] * ta2, nb2 = a2[p] * tb1 + b2[p] * tb2;
-                s1[p] = n1; s2[p] = n2;
-                a1[p] = na1; b1[p] = nb1; a2[p] = na2; b2[p] = nb2;
-            }
-        }
-    }
+    // inclusive scan downward (lane l's map applied after the maps of lanes >
+    // l): within rows of 16 lanes by DPP shifts, then across the rows with the
+    // rows' totals read from their first lanes (16, 32, 48)
     double Z1[2], Z2[2];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        const double z1 = __shfl_down(s1[p], 1, 64), z2 = __shfl_down(s2[p], 1, 64);
-        Z1[p] = lane == 63 ? 0.0 : z1;
-        Z2[p] = lane == 63 ? 0.0 : z2;
+        BwdMap m{s1[p], s2[p], a1[p], b1[p], a2[p], b2[p]};
+        bwd_row_step<1>(m, lane);
+        bwd_row_step<2>(m, lane);
+        bwd_row_step<4>(m, lane);
+        bwd_row_step<8>(m, lane);
+        const BwdMap R1 = bwd_readlane(m, 16), R2 = bwd_readlane(m, 32), R3 = bwd_readlane(m, 48);
+        const BwdMap Q1 = bwd_compose(R2, R3);   // rows 2-3
+        const BwdMap Q0 = bwd_compose(R1, Q1);   // rows 1-3
+        const int row = lane >> 4;
+        const BwdMap& Q = row == 0 ? Q0 : (row == 1 ? Q1 : R3);
+        const BwdMap c = bwd_compose(m, Q);
+        const bool up = row < 3;
+        // the chunk's incoming values: the lane above's outgoing ones
+        Z1[p] = dpp_f64<0x130>(0.0, up ? c.s1 : m.s1);
+        Z2[p] = dpp_f64<0x130>(0.0, up ? c.s2 : m.s2);
     }
 #pragma unroll
     for (int k = C - 1; k >= 0; --k) {
