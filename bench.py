@@ -6,6 +6,7 @@
     python bench.py --adjoint            # adjoint gradient at 256x256 only (one JSON line)
     python bench.py --copy-calibration   # 256 MiB device copies, for scaling PMC byte counters
     python bench.py --tune-db PATH       # tune every workload afresh, write the decisions
+    python bench.py --gemm-k-sweep       # y-pass-shaped GEMM against K (fixed cost per launch)
 
 Workload (value): the plain Euler IMEX step of navier.rbc2d.NavierStokes at
 256x256, rbc case, Ra=1e8, Pr=1, dt=1e-3, 3/2 dealiasing, seeded
@@ -580,6 +581,48 @@ def cpu_baseline(steps=6, warm=1):
                        (steps, warm), sec=round(sec, 2), host=host_cpu())
 
 
+def gemm_k_sweep(m=384, n=192, batch=18, ks=(16, 32, 64, 127, 254, 508), variants=(2, 1, 27, 33, 40)):
+    """Time of a plain strided-batch GEMM at the y-pass's shape (18 products of
+    384 x 192 over K, B transposed) against K, for a few kernel variants: the
+    intercept is the launch's fixed cost (dispatch, ramp, epilogue, tail), the
+    slope the cost per K slab.  Each launch timed by the tuner's timing (five
+    back-to-back launches behind a queued millisecond, best of three)."""
+    import ctypes
+    import torch
+    from pypde_amd import autotune
+    from pypde_amd._lib import BIdx, GemmDesc, check, lib
+
+    def bidx(s1):
+        b = BIdx()
+        b.s1, b.d1, b.m1, b.s2, b.d2, b.m2 = s1, 1, 0, 0, 1, 0
+        return b
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for k in ks:
+        A = torch.randn(batch, m, k, dtype=torch.float64, device="cuda")
+        Bm = torch.randn(batch, n, k, dtype=torch.float64, device="cuda")
+        C = torch.empty(batch, m, n, dtype=torch.float64, device="cuda")
+        d = GemmDesc()
+        d.transA, d.transB, d.m, d.n, d.k, d.batch = 0, 1, m, n, k, batch
+        d.alpha, d.beta = 1.0, 0.0
+        d.A, d.B, d.C = A.data_ptr(), Bm.data_ptr(), C.data_ptr()
+        d.lda, d.ldb, d.ldc = k, k, n
+        d.iA, d.iB, d.iC = bidx(m * k), bidx(n * k), bidx(m * n)
+        d.alpha_div = d.gamma_div = 1
+        d.ska = d.skb = d.csn = 1
+        rec = {}
+        for v in variants:
+            d.variant = v
+            t = autotune.time_candidates(
+                lambda _: check(lib.pypde_dgemm_batched(ctypes.byref(d), st), "gemm"), [0])
+            rec[str(v)] = round(t[0], 2)
+        ref = torch.bmm(A, Bm.transpose(1, 2).contiguous())
+        err = float((C - ref).abs().max() / ref.abs().max())
+        out[str(k)] = dict(us=rec, rel_err=err)
+        print("gemm K sweep: K=%d done" % k, file=sys.stderr, flush=True)
+    return out
+
+
 def gemm_variants():
     """Every plain GEMM kernel variant on each plain GEMM launch of the 256x256
     step (single descriptors and grouped launches; the folded x-pass is not a
@@ -853,6 +896,8 @@ def main():
     ap.add_argument("--micro", action="store_true", help="run the microbench only")
     ap.add_argument("--gemm-variants", action="store_true",
                     help="time every plain GEMM variant on the 256x256 step's GEMM launches")
+    ap.add_argument("--gemm-k-sweep", action="store_true",
+                    help="time a y-pass-shaped GEMM against K (fixed cost and cost per K slab)")
     ap.add_argument("--batched-launches", action="store_true",
                     help="per-launch times of the 64 x 256x256 batched step")
     ap.add_argument("--rhs-forms", action="store_true",
@@ -903,6 +948,9 @@ def main():
         return
     if args.gemm_variants:
         print(json.dumps({"gemm_variants": gemm_variants()}))
+        return
+    if args.gemm_k_sweep:
+        print(json.dumps({"gemm_k_sweep": gemm_k_sweep()}))
         return
     if args.batched_launches:
         print(json.dumps({"batched_launches": batched_launches()}))
