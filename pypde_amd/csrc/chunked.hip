@@ -131,6 +131,48 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
             if (PERLINE && (k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
     }
+    if constexpr (!HASNU) {
+        // backward with one super-diagonal: the chunk's lowest value of each
+        // parity as s + a Z, Z = z at the next chunk's lowest position of that
+        // parity (a product by a constant zero NU would not be folded away, so
+        // the two-term state below is not reused with NU = 0)
+        double s[2] = {0, 0}, am[2] = {1, 1};
+#pragma unroll
+        for (int k = CH - 1; k >= 0; --k) {
+            const int p = k & 1;
+            if (!GUARD || base + k < n) {
+                const double r = HASR ? T(2, base + k) : 1.0;
+                const double mu = T(3, base + k);
+                s[p] = r * v[k] + mu * s[p];
+                am[p] = mu * am[p];
+            }
+            // at most eight positions' table reads hoisted ahead of their use
+            if ((k & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            xch[(w * 4 + p) * LS + lane] = s[p];
+            xch[(w * 4 + 2 + p) * LS + lane] = am[p];
+        }
+        __syncthreads();
+        double Z[2] = {0.0, 0.0};
+        for (int u = P - 1; u > w; --u) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) Z[p] = xch[(u * 4 + p) * LS + lane] + xch[(u * 4 + 2 + p) * LS + lane] * Z[p];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = CH - 1; k >= 0; --k) {
+            const int p = k & 1;
+            if (!GUARD || base + k < n) {
+                const double r = HASR ? T(2, base + k) : 1.0;
+                Z[p] = r * v[k] + T(3, base + k) * Z[p];
+                v[k] = Z[p];
+            }
+            if ((k & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+        return;
+    }
     // backward: lowest two same-parity values as affine functions of the
     // incoming (Z1, Z2) = z at the next chunk's lowest positions
     double s1[2] = {0, 0}, a1[2] = {1, 1}, b1[2] = {0, 0};
@@ -183,9 +225,12 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
     }
 }
 
-// Occupancy hint: the pre-stencil (tri-diagonal projection) form runs best
-// capped at 128 VGPRs (4 waves per SIMD, two 8-wave blocks per CU); the other
-// forms spill under that cap and run best uncapped (measured at n = 254).
+// Occupancy hint (measured at n = 254, 512 lines of 256): the pre-stencil
+// (tri-diagonal projection) form with 16 positions per chunk runs within 128
+// VGPRs (4 waves per SIMD); with 32 positions it runs best capped at 168 (3
+// waves per SIMD: under 128 it spills, and along rows the spills cost more than
+// the extra wave gains); the four-diagonal and Poisson forms spill under 128
+// and run best uncapped, except the 32-line, 16-position shared-table form.
 //
 // LPB = lines per block.  64: lane = line, wave = chunk (P waves).  32: the two
 // half-waves hold two neighbouring chunks of the same 32 lines (ceil(P / 2)
@@ -196,7 +241,7 @@ __device__ __forceinline__ void sweeps(double (&v)[NX], const Tab<PERLINE, HASF,
 template <int CH, bool PRE, bool PERLINE, int LPB>
 constexpr int rec_min_waves()
 {
-    return (PRE || (LPB == 32 && CH == 16 && !PERLINE)) ? 4 : 1;
+    return (PRE && CH == 32) ? 3 : (PRE || (LPB == 32 && CH == 16 && !PERLINE)) ? 4 : 1;
 }
 
 template <int AXIS, int CH, bool PRE, bool FWD, bool HASF, bool HASR, bool HASNU, bool PERLINE, int MAXT,
@@ -425,7 +470,9 @@ static int launch_rec_t(const RecArgs& a, hipStream_t s)
                      (a.NU != nullptr) + (a.pre != nullptr);
     size_t tabs = PERLINE ? 0 : (size_t)ntab * a.n;
     size_t tiles = AXIS == 1 ? (size_t)waves * (LPB == 64 ? 64 * TP : 32 * TP32) : 0;
-    size_t xch = (size_t)chunks * 12 * LPB;
+    // carry exchange: 4 slots per chunk for the forward and one-super-diagonal
+    // backward sweeps, 12 for the two-super-diagonal backward sweep
+    size_t xch = (size_t)chunks * (HASNU ? 12 : 4) * LPB;
     const size_t shmem = (tabs + std::max(tiles, xch)) * sizeof(double);
     auto fn = recurrence_kernel<AXIS, CH, PRE, FWD, HASF, HASR, HASNU, PERLINE, MAXT, LPB>;
     static bool attr = false;
