@@ -611,31 +611,37 @@ __global__ void __launch_bounds__(256) banded_axis1_win_kernel(LineGeom g, BandS
 // runs over g.W rows, U at a time with their loads issued together.  Each
 // load of a wave is 64 consecutive entries of a row shifted by one band
 // offset, so all accesses coalesce and the shifted copies hit in cache.  The
-// sum is band_row's, term for term.
-template <int U, bool STENCIL>
+// sum is band_row's, term for term.  NDT: the number of diagonals when it is
+// known at compile time (3), else kMaxDiag.  Every load is unconditional: a
+// term that does not exist for this output (its column outside the input)
+// reads a clamped column and is left out of the sum, so the loads are not
+// placed behind per-lane branches that would wait for each in turn.
+template <int U, bool STENCIL, int NDT>
 __global__ void __launch_bounds__(256) banded_axis1_res_kernel(LineGeom g, BandSpec bs)
 {
-    constexpr int ND = STENCIL ? 2 : kMaxDiag;
+    constexpr int ND = STENCIL ? 2 : NDT;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.z;
     if (r >= g.len_out) return;
     double a[ND];
-    int off[ND];
+    int off[ND];  // column offset of the term's load, clamped into the input row
     bool use[ND];
 #pragma unroll
     for (int q = 0; q < ND; ++q) {
+        int k;
         if (STENCIL) {
             // term 0: s_{r-2} x_{r-2} (r >= 2); term 1: x_r (r < len_in)
-            off[q] = q == 0 ? -2 : 0;
+            k = r + (q == 0 ? -2 : 0);
             use[q] = q == 0 ? r >= 2 : r < g.len_in;
             a[q] = (q == 0 && use[q]) ? bs.vals[r - 2] : 1.0;
         } else {
-            const int k = r + bs.off[q];
-            off[q] = q < bs.nd ? bs.off[q] : 0;
-            use[q] = q < bs.nd && k >= 0 && k < g.len_in;
+            const bool present = NDT < kMaxDiag || q < bs.nd;
+            k = r + (present ? bs.off[q] : 0);
+            use[q] = present && k >= 0 && k < g.len_in;
             a[q] = use[q] ? bs.vals[(long long)q * g.len_out + r] : 0.0;
             use[q] = use[q] && a[q] != 0.0;
         }
+        off[q] = min(max(k, 0), g.len_in - 1) - r;
     }
     const int c0 = blockIdx.y * g.W;
     const int c1 = min(c0 + g.W, g.nlines);
@@ -647,7 +653,7 @@ __global__ void __launch_bounds__(256) banded_axis1_res_kernel(LineGeom g, BandS
         for (int u = 0; u < U; ++u) {
             const double* row = src + (long long)min(c + u, c1 - 1) * g.src_ld;
 #pragma unroll
-            for (int q = 0; q < ND; ++q) x[u][q] = use[q] ? row[off[q]] : 0.0;
+            for (int q = 0; q < ND; ++q) x[u][q] = row[off[q]];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -695,11 +701,14 @@ static int launch_banded(LineGeom g, const BandSpec& bs, int hint, void* stream,
         dim3 grid((g.len_out + 255) / 256, (g.nlines + g.W - 1) / g.W, g.batch);
         hipStream_t st = as_stream(stream);
         if (bs.stencil) {
-            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, true>), grid, dim3(256), 0, st, g, bs);
-            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, true>), grid, dim3(256), 0, st, g, bs);
+            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, true, 2>), grid, dim3(256), 0, st, g, bs);
+            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, true, 2>), grid, dim3(256), 0, st, g, bs);
+        } else if (bs.nd == 3) {
+            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, false, 3>), grid, dim3(256), 0, st, g, bs);
+            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, false, 3>), grid, dim3(256), 0, st, g, bs);
         } else {
-            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, false>), grid, dim3(256), 0, st, g, bs);
-            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, false>), grid, dim3(256), 0, st, g, bs);
+            if (u8) hipLaunchKernelGGL((banded_axis1_res_kernel<8, false, kMaxDiag>), grid, dim3(256), 0, st, g, bs);
+            else hipLaunchKernelGGL((banded_axis1_res_kernel<4, false, kMaxDiag>), grid, dim3(256), 0, st, g, bs);
         }
         return check_launch(name);
     }
