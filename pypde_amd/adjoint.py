@@ -118,7 +118,7 @@ class StepAdjoint:
         read backwards (the first row at the last one, row stride -nlines).
         tabs: per-entry tables [batch][LM | R | MU | NU][n], or per-line tables
         ([n][nlines] slots, line-major slots).  The kernel form is tuned per
-        shape on a scratch array; variant (1, 2 or 3) forces a form."""
+        shape on a scratch array; variant (1, 2, 3 or 4) forces a form."""
         from . import autotune
         from ._lib import Lines, Recurrence
 

@@ -225,7 +225,7 @@ def test_graph_replayed_sweep_is_bit_identical():
 
 
 @pytest.mark.parametrize("n", [22, 126, 254, 510])
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 def test_reversed_transposed_solves_match_dense(n, variant):
     """The adjoint's transposed line solves (negative row stride, the line read
     from its last row) in each kernel form, with per-entry Helmholtz tables and
